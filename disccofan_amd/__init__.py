@@ -1,0 +1,325 @@
+"""disccofan_amd -- MI355X-native component trees (max-tree / min-tree) behind disccofan's operator surface.
+
+Python host-side mirror of the reference's operator interface, over the C ABI of libdcf.so
+(include/dcf.h).  Names and argument meanings follow the reference:
+
+    tree_type       "max" | "min"                      (config.c tree_type, preprocessing.c:84-91)
+    connectivity    4, 8 (2D) / 6, 26 (3D)             (io.c:64-79 mapping)
+    attribute index 0..17 of AttribsArray              (attributes.c:1224-1248)
+    decision        0 Direct, 1 Min, 2 Max, 3 Subtractive (filtering.c:271-277)
+    operations      ("filter", attr, lambda[, decision]), ("extract", ...), ("check", attr),
+                    ("pattern", attr, "lvec"[, imscale]),
+                    ("pattern2D", attr1, "lvec1", s1, attr2, "lvec2", s2)   (config.c:297-433)
+
+Every call runs on the GPU through libdcf.so; there is no CPU fallback: if the library is missing
+or no GPU is visible, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libdcf.so")
+SYNTH_PATH = os.path.join(_HERE, "lib", "libdcf_synth.so")
+
+DCF_U8, DCF_U16, DCF_F32 = 0, 1, 2
+DCF_DEVICE_PTR = 0x1
+DCF_ALL_RULES = 0x2
+
+STATUS = {0: "DCF_OK", -1: "DCF_EINVAL", -2: "DCF_ENOMEM", -3: "DCF_EDEVICE", -4: "DCF_EUNSUPPORTED",
+          -5: "DCF_EIO", -6: "DCF_EGROUP"}
+
+ATTRIBUTES = ["area", "rectarea", "rectdiag", "xext", "yext", "zext", "xmean", "ymean", "zmean",
+              "xwmean", "ywmean", "zwmean", "flux", "inertiatrace", "elong", "flat", "spars", "ncomp"]
+DECISIONS = ["Direct", "Min", "Max", "Subtractive"]
+
+# exported symbols of libdcf.so (kept in sync with include/dcf.h)
+EXPORTS = [
+    "dcf_ctx_create", "dcf_ctx_destroy", "dcf_ctx_set_stream", "dcf_ctx_stream", "dcf_last_error",
+    "dcf_ctx_synchronize", "dcf_ctx_set_profiling", "dcf_kernel_stats", "dcf_kernel_stats_reset",
+    "dcf_kernel_name", "dcf_tree_build", "dcf_tree_free", "dcf_tree_accumulate", "dcf_tree_group",
+    "dcf_tree_parent", "dcf_tree_num_nodes", "dcf_filter", "dcf_check", "dcf_spectrum1d", "dcf_spectrum2d",
+    "dcf_lvec_read", "dcf_lvec_free", "dcf_attr_name", "dcf_attr_short_name", "dcf_attr_group",
+    "dcf_decision_name", "dcf_run_operations",
+]
+
+
+class DcfError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__("%s (%d): %s" % (STATUS.get(code, "?"), code, msg))
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load libdcf.so (built in-tree by disccofan_amd.build); raises when it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libdcf.so not built: run `python -m disccofan_amd.build` (%s missing)" % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    vp, i, u, d = C.c_void_p, C.c_int, C.c_uint, C.c_double
+    u64p = C.POINTER(C.c_uint64)
+    L.dcf_ctx_create.argtypes = [i, u, C.POINTER(vp)]
+    L.dcf_ctx_destroy.argtypes = [vp]
+    L.dcf_ctx_set_stream.argtypes = [vp, vp]
+    L.dcf_ctx_stream.restype = vp
+    L.dcf_ctx_stream.argtypes = [vp]
+    L.dcf_last_error.restype = C.c_char_p
+    L.dcf_last_error.argtypes = [vp]
+    L.dcf_ctx_synchronize.argtypes = [vp]
+    L.dcf_ctx_set_profiling.argtypes = [vp, i]
+    L.dcf_kernel_stats.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+    L.dcf_kernel_stats_reset.argtypes = [vp]
+    L.dcf_kernel_name.restype = C.c_char_p
+    L.dcf_kernel_name.argtypes = [vp, i]
+    L.dcf_tree_build.argtypes = [vp, vp, i, u64p, i, i, i, vp, u, C.POINTER(vp)]
+    L.dcf_tree_free.argtypes = [vp]
+    L.dcf_tree_accumulate.argtypes = [vp, i]
+    L.dcf_tree_group.argtypes = [vp]
+    L.dcf_tree_parent.argtypes = [vp, vp, u]
+    L.dcf_tree_num_nodes.argtypes = [vp, C.POINTER(C.c_int64)]
+    L.dcf_filter.argtypes = [vp, i, i, d, i, vp, i, u]
+    L.dcf_check.argtypes = [vp, i, vp, u]
+    L.dcf_spectrum1d.argtypes = [vp, i, vp, i, vp]
+    L.dcf_spectrum2d.argtypes = [vp, i, vp, i, i, vp, i, vp]
+    L.dcf_lvec_read.argtypes = [C.c_char_p, d, C.POINTER(C.POINTER(C.c_float)), C.POINTER(i)]
+    L.dcf_lvec_free.argtypes = [C.POINTER(C.c_float)]
+    L.dcf_attr_name.restype = C.c_char_p
+    L.dcf_attr_name.argtypes = [i]
+    L.dcf_attr_short_name.restype = C.c_char_p
+    L.dcf_attr_short_name.argtypes = [i]
+    L.dcf_attr_group.argtypes = [i]
+    L.dcf_decision_name.restype = C.c_char_p
+    L.dcf_decision_name.argtypes = [i]
+    L.dcf_run_operations.argtypes = [vp, vp, i, u64p, i, i, vp, C.c_char_p, vp, vp]
+    _lib = L
+    return L
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _dtype_code(dt) -> int:
+    dt = np.dtype(dt)
+    if dt == np.uint8:
+        return DCF_U8
+    if dt == np.uint16:
+        return DCF_U16
+    if dt == np.float32:
+        return DCF_F32
+    raise TypeError("image dtype must be uint8, uint16 or float32, got %s" % dt)
+
+
+def _dims_of(shape):
+    if len(shape) == 2:
+        return (shape[1], shape[0], 1)
+    if len(shape) == 3:
+        return (shape[2], shape[1], shape[0])
+    raise ValueError("2D (H,W) or 3D (D,H,W) image expected")
+
+
+def lvec_read(path: str, imscale: float = 1.0) -> np.ndarray:
+    """lambda_vector_read (lambdavec.c:35-84): lambda_0 = 0 prepended, values divided by imscale."""
+    L = lib()
+    p = C.POINTER(C.c_float)()
+    n = C.c_int()
+    rc = L.dcf_lvec_read(path.encode(), float(imscale), C.byref(p), C.byref(n))
+    if rc != 0:
+        raise DcfError(rc, "couldn't read the lvec file: %s" % path)
+    out = np.ctypeslib.as_array(p, shape=(n.value,)).copy()
+    L.dcf_lvec_free(p)
+    return out
+
+
+class Context:
+    """A GPU context (one HIP stream, a caching allocator and its trees)."""
+
+    def __init__(self, device: int = 0):
+        L = lib()
+        h = C.c_void_p()
+        rc = L.dcf_ctx_create(int(device), 0, C.byref(h))
+        if rc != 0:
+            raise DcfError(rc, "dcf_ctx_create(device=%d) failed (no GPU visible?)" % device)
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().dcf_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int):
+        if rc != 0:
+            raise DcfError(rc, lib().dcf_last_error(self._h).decode())
+
+    def set_stream(self, stream_handle: int | None):
+        self._check(lib().dcf_ctx_set_stream(self._h, C.c_void_p(stream_handle) if stream_handle else None))
+
+    def synchronize(self):
+        self._check(lib().dcf_ctx_synchronize(self._h))
+
+    def set_profiling(self, on: bool):
+        self._check(lib().dcf_ctx_set_profiling(self._h, 1 if on else 0))
+
+    def kernel_stats(self, name: str):
+        n = C.c_int64()
+        ms = C.c_double()
+        self._check(lib().dcf_kernel_stats(self._h, name.encode(), C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
+    def kernel_names(self):
+        out = []
+        i = 0
+        while True:
+            s = lib().dcf_kernel_name(self._h, i)
+            if not s:
+                return out
+            out.append(s.decode())
+            i += 1
+
+    def reset_stats(self):
+        self._check(lib().dcf_kernel_stats_reset(self._h))
+
+    def build(self, img, connectivity: int = 4, tree_type: str = "max", attr_idx: int = 0, pixdim=(1.0, 1.0, 1.0)):
+        """build_tree_with_attributes (flood.c:175) on a host numpy image."""
+        a = np.ascontiguousarray(img)
+        return self._build(_ptr(a), _dtype_code(a.dtype), _dims_of(a.shape), a.shape, connectivity, tree_type,
+                           attr_idx, pixdim, 0, keep=a)
+
+    def build_device(self, ptr: int, dtype, shape, connectivity: int = 4, tree_type: str = "max", attr_idx: int = 0,
+                     pixdim=(1.0, 1.0, 1.0)):
+        """Same on an image already resident in HBM (e.g. a torch tensor's data_ptr()); the image must
+        stay unchanged until the tree is freed."""
+        return self._build(C.c_void_p(ptr), _dtype_code(dtype), _dims_of(shape), tuple(shape), connectivity, tree_type,
+                           attr_idx, pixdim, DCF_DEVICE_PTR, keep=None)
+
+    def _build(self, p, code, dims, shape, connectivity, tree_type, attr_idx, pixdim, flags, keep):
+        if tree_type not in ("max", "min"):
+            raise ValueError("tree_type must be 'max' or 'min'")
+        d = (C.c_uint64 * 3)(*dims)
+        pd = np.asarray(pixdim, dtype=np.float32)
+        h = C.c_void_p()
+        rc = lib().dcf_tree_build(self._h, p, code, d, int(connectivity), 1 if tree_type == "min" else 0,
+                                  int(attr_idx), _ptr(pd), flags, C.byref(h))
+        self._check(rc)
+        return Tree(self, h, shape, code, keep)
+
+    def run_operations(self, img, operations, connectivity: int = 4, tree_type: str = "max", pixdim=(1.0, 1.0, 1.0)):
+        """loop_operations (disccofan.c:137-205): run the reference's operation tuples in order.
+        Returns one output per operation (image, check image, or spectrum)."""
+        a = np.ascontiguousarray(img)
+        if not isinstance(operations, str):
+            operations = ",".join(_tuple_text(t) for t in operations)
+        outs = {}
+        shape = a.shape
+
+        @C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int64)
+        def cb(user, idx, kind, data, count):
+            if kind in (0, 1):
+                arr = np.ctypeslib.as_array(C.cast(data, C.POINTER(C.c_float)), shape=(count,)).copy()
+                outs[idx] = arr.reshape(shape)
+            else:
+                outs[idx] = np.ctypeslib.as_array(C.cast(data, C.POINTER(C.c_double)), shape=(count,)).copy()
+
+        d = (C.c_uint64 * 3)(*_dims_of(a.shape))
+        pd = np.asarray(pixdim, dtype=np.float32)
+        rc = lib().dcf_run_operations(self._h, _ptr(a), _dtype_code(a.dtype), d, int(connectivity),
+                                      1 if tree_type == "min" else 0, _ptr(pd), operations.encode(),
+                                      C.cast(cb, C.c_void_p), None)
+        self._check(rc)
+        return [outs[i] for i in sorted(outs)]
+
+
+def _tuple_text(t) -> str:
+    parts = []
+    for v in t:
+        parts.append('"%s"' % v if isinstance(v, str) else repr(v))
+    return "(" + ",".join(parts) + ")"
+
+
+class Tree:
+    """A component tree resident on the GPU (dcf_tree)."""
+
+    def __init__(self, ctx: Context, h, shape, dtype_code, keep):
+        self.ctx = ctx
+        self._h = h
+        self.shape = tuple(shape)
+        self.n = int(np.prod(self.shape))
+        self.dtype_code = dtype_code
+        self._keep = keep
+
+    def free(self):
+        if getattr(self, "_h", None):
+            lib().dcf_tree_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    @property
+    def num_nodes(self) -> int:
+        k = C.c_int64()
+        self.ctx._check(lib().dcf_tree_num_nodes(self._h, C.byref(k)))
+        return k.value
+
+    @property
+    def group(self) -> int:
+        return int(lib().dcf_tree_group(self._h))
+
+    def accumulate(self, attr_idx: int):
+        """flood_attributes (flood.c:184) when attr_idx is in another group."""
+        self.ctx._check(lib().dcf_tree_accumulate(self._h, int(attr_idx)))
+
+    def parent(self) -> np.ndarray:
+        out = np.empty(self.n, dtype=np.int32)
+        self.ctx._check(lib().dcf_tree_parent(self._h, _ptr(out), 0))
+        return out
+
+    def filter(self, attr_idx: int, lam: float, decision: int = 0, extract: bool = False, out_dtype=np.float32,
+               all_rules: bool = False) -> np.ndarray:
+        out = np.empty(self.shape, dtype=out_dtype)
+        flags = DCF_ALL_RULES if all_rules else 0
+        self.ctx._check(lib().dcf_filter(self._h, int(attr_idx), int(decision), float(lam), 1 if extract else 0,
+                                         _ptr(out), _dtype_code(out_dtype), flags))
+        return out
+
+    def filter_device(self, attr_idx: int, lam: float, decision: int, out_ptr: int, out_dtype, extract: bool = False,
+                      all_rules: bool = False):
+        flags = DCF_DEVICE_PTR | (DCF_ALL_RULES if all_rules else 0)
+        self.ctx._check(lib().dcf_filter(self._h, int(attr_idx), int(decision), float(lam), 1 if extract else 0,
+                                         C.c_void_p(out_ptr), _dtype_code(out_dtype), flags))
+
+    def check(self, attr_idx: int) -> np.ndarray:
+        out = np.empty(self.shape, dtype=np.float32)
+        self.ctx._check(lib().dcf_check(self._h, int(attr_idx), _ptr(out), 0))
+        return out
+
+    def spectrum1d(self, attr_idx: int, lambdas) -> np.ndarray:
+        lv = np.ascontiguousarray(lambdas, dtype=np.float32)
+        out = np.empty(len(lv), dtype=np.float64)
+        self.ctx._check(lib().dcf_spectrum1d(self._h, int(attr_idx), _ptr(lv), len(lv), _ptr(out)))
+        return out
+
+    def spectrum2d(self, attr1: int, l1, attr2: int, l2) -> np.ndarray:
+        a = np.ascontiguousarray(l1, dtype=np.float32)
+        b = np.ascontiguousarray(l2, dtype=np.float32)
+        out = np.empty(len(a) * len(b), dtype=np.float64)
+        self.ctx._check(lib().dcf_spectrum2d(self._h, int(attr1), _ptr(a), len(a), int(attr2), _ptr(b), len(b),
+                                             _ptr(out)))
+        return out

@@ -1,0 +1,660 @@
+/*
+ * dcf_api.cpp -- the C ABI of libdcf: contexts, trees, operations and the operator driver.
+ *
+ * Host-side mirror of the reference's operator surface:
+ *   AttribsArray / Decisions tables         src/sources/attributes.c:1224-1248, filtering.c:271-277
+ *   connectivity mapping                     src/sources/io.c:64-79
+ *   min-tree inversion / output re-inversion src/sources/preprocessing.c:84-91, operation.c:23-28
+ *   operation tuples                         src/sources/config.c:297-433
+ *   operation loop with group reuse          src/sources/disccofan.c:137-205, operation.c:110-212
+ *   lambda-vector files                      src/sources/lambdavec.c:35-84
+ * Errors are status codes with a per-context message (the reference aborts through MPI_Abort).
+ */
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "dcf_internal.h"
+
+/* ------------------------------------------------------------------------------------------------ */
+/* tables                                                                                           */
+/* ------------------------------------------------------------------------------------------------ */
+static const char *kAttrName[DCF_NUM_ATTR] = {
+    "Area", "Area of minimal enclosing rectangle", "Square of diagonal of minimal enclosing rectangle", "X-extent",
+    "Y-extent", "Z-extent", "Mean X position", "Mean Y position", "Mean Z position", "Weighted Mean X position",
+    "Weighted Mean Y position", "Weighted Mean Z position", "Total flux", "Trace of Inertia Matrice", "Elongation",
+    "Flatness", "Sparseness", "Non-compactnesss"
+};
+static const char *kAttrShort[DCF_NUM_ATTR] = { "area", "rectarea", "rectdiag", "xext", "yext", "zext",
+                                                "xmean", "ymean", "zmean", "xwmean", "ywmean", "zwmean",
+                                                "flux", "inertiatrace", "elong", "flat", "spars", "ncomp" };
+static const char *kDecision[4] = { "Direct", "Min", "Max", "Subtractive" };
+
+extern "C" const char *dcf_attr_name(int i) { return (i >= 0 && i < DCF_NUM_ATTR) ? kAttrName[i] : nullptr; }
+extern "C" const char *dcf_attr_short_name(int i) { return (i >= 0 && i < DCF_NUM_ATTR) ? kAttrShort[i] : nullptr; }
+extern "C" const char *dcf_decision_name(int d) { return (d >= 0 && d < 4) ? kDecision[d] : nullptr; }
+
+int dcf_group_of(int idx)
+{
+    if (idx < 0 || idx >= DCF_NUM_ATTR) return -1;
+    if (idx == 0) return 0;
+    if (idx <= 5) return 1;
+    if (idx <= 8) return 2;
+    if (idx <= 12) return 3;
+    return 4;
+}
+
+extern "C" int dcf_attr_group(int i) { return dcf_group_of(i); }
+
+/* ------------------------------------------------------------------------------------------------ */
+/* errors, profiling, allocator                                                                     */
+/* ------------------------------------------------------------------------------------------------ */
+int dcf_set_error(dcf_ctx *ctx, int code, const char *fmt, ...)
+{
+    if (ctx) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        ctx->err = buf;
+    }
+    return code;
+}
+
+static hipEvent_t take_event(dcf_ctx *ctx)
+{
+    if (!ctx->event_pool.empty()) {
+        hipEvent_t e = ctx->event_pool.back();
+        ctx->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+void dcf_prof_begin(dcf_ctx *ctx, const char *name, hipEvent_t *a, hipEvent_t *b)
+{
+    (void)name;
+    *a = take_event(ctx);
+    *b = take_event(ctx);
+    if (*a) (void)hipEventRecord(*a, ctx->stream);
+}
+
+void dcf_prof_end(dcf_ctx *ctx, const char *name, hipEvent_t a, hipEvent_t b)
+{
+    if (!a || !b) return;
+    (void)hipEventRecord(b, ctx->stream);
+    ctx->pending.push_back({ name, a, b });
+}
+
+/* fold recorded event pairs into the per-kernel totals (called after a synchronisation) */
+static void prof_collect(dcf_ctx *ctx)
+{
+    for (auto &p : ctx->pending) {
+        float ms = 0;
+        if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            KernelStat &s = ctx->stats[p.name];
+            if (s.launches == 0) ctx->names.push_back(p.name);
+            s.launches++;
+            s.ms += ms;
+        }
+        ctx->event_pool.push_back(p.a);
+        ctx->event_pool.push_back(p.b);
+    }
+    ctx->pending.clear();
+}
+
+void *dcf_alloc(dcf_ctx *ctx, size_t bytes)
+{
+    if (bytes == 0) bytes = 16;
+    bytes = (bytes + 255) & ~(size_t)255;
+    auto it = ctx->free_blocks.lower_bound(bytes);
+    if (it != ctx->free_blocks.end() && it->first <= bytes * 2) {
+        void *p = it->second;
+        ctx->live_blocks[p] = it->first;
+        ctx->free_blocks.erase(it);
+        return p;
+    }
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        /* drop the cache and retry once */
+        (void)hipStreamSynchronize(ctx->stream);
+        for (auto &f : ctx->free_blocks) (void)hipFree(f.second);
+        ctx->free_blocks.clear();
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+    }
+    ctx->live_blocks[p] = bytes;
+    return p;
+}
+
+void dcf_release(dcf_ctx *ctx, void *p)
+{
+    if (!p) return;
+    auto it = ctx->live_blocks.find(p);
+    if (it == ctx->live_blocks.end()) return;
+    ctx->free_blocks.insert({ it->second, p });
+    ctx->live_blocks.erase(it);
+}
+
+/* ------------------------------------------------------------------------------------------------ */
+/* context                                                                                          */
+/* ------------------------------------------------------------------------------------------------ */
+extern "C" int dcf_ctx_create(int device, unsigned flags, dcf_ctx **out)
+{
+    (void)flags;
+    if (!out) return DCF_EINVAL;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return DCF_EDEVICE;
+    if (device < 0 || device >= ndev) return DCF_EINVAL;
+    if (hipSetDevice(device) != hipSuccess) return DCF_EDEVICE;
+    dcf_ctx *c = new dcf_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return DCF_EDEVICE;
+    }
+    c->stream = c->own;
+    const char *tx = getenv("DCF_TILE2D");
+    if (tx) {
+        int a = 0, b = 0;
+        if (sscanf(tx, "%dx%d", &a, &b) == 2 && a > 0 && b > 0 && a * b <= 4096) { c->tile_x = a; c->tile_y = b; }
+    }
+    const char *t3 = getenv("DCF_TILE3D");
+    if (t3) {
+        int a = 0, b = 0, d = 0;
+        if (sscanf(t3, "%dx%dx%d", &a, &b, &d) == 3 && a > 0 && b > 0 && d > 0 && a * b * d <= 4096) {
+            c->tile3_x = a; c->tile3_y = b; c->tile3_z = d;
+        }
+    }
+    *out = c;
+    return DCF_OK;
+}
+
+extern "C" int dcf_ctx_destroy(dcf_ctx *ctx)
+{
+    if (!ctx) return DCF_EINVAL;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    prof_collect(ctx);
+    for (auto &e : ctx->event_pool) (void)hipEventDestroy(e);
+    for (auto &f : ctx->free_blocks) (void)hipFree(f.second);
+    for (auto &l : ctx->live_blocks) (void)hipFree(l.first);
+    if (ctx->own) (void)hipStreamDestroy(ctx->own);
+    delete ctx;
+    return DCF_OK;
+}
+
+extern "C" int dcf_ctx_set_stream(dcf_ctx *ctx, void *stream)
+{
+    if (!ctx) return DCF_EINVAL;
+    ctx->stream = stream ? (hipStream_t)stream : ctx->own;
+    return DCF_OK;
+}
+
+extern "C" void *dcf_ctx_stream(dcf_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+extern "C" const char *dcf_last_error(dcf_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+extern "C" int dcf_ctx_synchronize(dcf_ctx *ctx)
+{
+    if (!ctx) return DCF_EINVAL;
+    DCF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return DCF_OK;
+}
+
+extern "C" int dcf_ctx_set_profiling(dcf_ctx *ctx, int on)
+{
+    if (!ctx) return DCF_EINVAL;
+    ctx->profiling = on ? 1 : 0;
+    return DCF_OK;
+}
+
+extern "C" int dcf_kernel_stats(dcf_ctx *ctx, const char *name, int64_t *launches, double *total_ms)
+{
+    if (!ctx || !name) return DCF_EINVAL;
+    (void)hipStreamSynchronize(ctx->stream);
+    prof_collect(ctx);
+    int64_t n = 0;
+    double ms = 0;
+    for (auto &s : ctx->stats)
+        if (strstr(s.first.c_str(), name)) { n += s.second.launches; ms += s.second.ms; }
+    if (launches) *launches = n;
+    if (total_ms) *total_ms = ms;
+    return DCF_OK;
+}
+
+extern "C" int dcf_kernel_stats_reset(dcf_ctx *ctx)
+{
+    if (!ctx) return DCF_EINVAL;
+    (void)hipStreamSynchronize(ctx->stream);
+    prof_collect(ctx);
+    ctx->stats.clear();
+    ctx->names.clear();
+    return DCF_OK;
+}
+
+extern "C" const char *dcf_kernel_name(dcf_ctx *ctx, int i)
+{
+    if (!ctx || i < 0) return nullptr;
+    (void)hipStreamSynchronize(ctx->stream);
+    prof_collect(ctx);
+    return i < (int)ctx->names.size() ? ctx->names[i].c_str() : nullptr;
+}
+
+/* ------------------------------------------------------------------------------------------------ */
+/* trees                                                                                            */
+/* ------------------------------------------------------------------------------------------------ */
+static size_t dtype_size(dcf_dtype d) { return d == DCF_U8 ? 1 : (d == DCF_U16 ? 2 : 4); }
+
+static void tree_release(dcf_tree *t)
+{
+    dcf_ctx *ctx = t->ctx;
+    if (t->own_img) dcf_release(ctx, t->img);
+    dcf_release(ctx, t->gmax);
+    dcf_release(ctx, t->link);
+    dcf_release(ctx, t->parent);
+    dcf_release(ctx, t->p2n);
+    dcf_release(ctx, t->npar);
+    dcf_release(ctx, t->ngrey);
+    dcf_release(ctx, t->counters);
+    dcf_free_attr(ctx, t->attr);
+}
+
+extern "C" int dcf_tree_free(dcf_tree *t)
+{
+    if (!t) return DCF_EINVAL;
+    (void)hipSetDevice(t->ctx->device);
+    tree_release(t);
+    delete t;
+    return DCF_OK;
+}
+
+extern "C" int dcf_tree_build(dcf_ctx *ctx, const void *img, dcf_dtype dtype, const uint64_t dims[3], int connectivity,
+                              int tree_type, int attr_idx, const float pixdim[3], unsigned flags, dcf_tree **out)
+{
+    if (!ctx) return DCF_EINVAL;
+    if (!out || !img || !dims) return dcf_set_error(ctx, DCF_EINVAL, "null argument");
+    *out = nullptr;
+    if (dtype != DCF_U8 && dtype != DCF_U16 && dtype != DCF_F32) return dcf_set_error(ctx, DCF_EINVAL, "bad dtype %d", (int)dtype);
+    if (dims[0] == 0 || dims[1] == 0 || dims[2] == 0) return dcf_set_error(ctx, DCF_EINVAL, "empty image");
+    if (dims[0] > 0x7FFFFFFFull || dims[1] > 0x7FFFFFFFull || dims[2] > 0x7FFFFFFFull)
+        return dcf_set_error(ctx, DCF_EINVAL, "dimension too large");
+    uint64_t n = dims[0] * dims[1] * dims[2];
+    if (n >= 0x7FFFFFFFull) return dcf_set_error(ctx, DCF_EINVAL, "image of %llu pixels exceeds 2^31-1", (unsigned long long)n);
+    if (tree_type != DCF_MAX_TREE && tree_type != DCF_MIN_TREE) return dcf_set_error(ctx, DCF_EINVAL, "bad tree type %d", tree_type);
+    if (attr_idx < 0 || attr_idx >= DCF_NUM_ATTR) return dcf_set_error(ctx, DCF_EINVAL, "bad attribute index %d", attr_idx);
+    /* check_connectivity (io.c:64-79) */
+    int conn = connectivity;
+    bool is3d = dims[2] > 1;
+    if (is3d && conn == 4) conn = 6;
+    else if (conn != 4 && conn != 6 && conn != 8 && conn != 26)
+        return dcf_set_error(ctx, DCF_EINVAL, "wrong connectivity %d (must be 4 or 8 for 2D, 6 or 26 for 3D)", connectivity);
+    else if (conn == 8 && is3d) conn = 26;
+    else if (conn == 26 && !is3d) conn = 8;
+    if (!is3d && conn == 6) conn = 4;
+    if (dtype == DCF_F32 && dcf_group_of(attr_idx) == 3)
+        return dcf_set_error(ctx, DCF_EUNSUPPORTED, "grey-weighted attributes on float images are not implemented");
+    if (hipSetDevice(ctx->device) != hipSuccess) return dcf_set_error(ctx, DCF_EDEVICE, "hipSetDevice");
+
+    dcf_tree *t = new dcf_tree();
+    t->ctx = ctx;
+    memcpy(t->dims, dims, sizeof t->dims);
+    t->n = (int64_t)n;
+    t->conn = conn;
+    t->tree_min = tree_type == DCF_MIN_TREE;
+    t->dtype = dtype;
+    if (pixdim) memcpy(t->pixdim, pixdim, sizeof t->pixdim);
+    int rc = DCF_OK;
+    size_t isz = dtype_size(dtype);
+    if (flags & DCF_DEVICE_PTR) {
+        t->img = (void *)img;
+        t->own_img = 0;
+    } else {
+        t->img = dcf_alloc(ctx, (size_t)n * isz);
+        t->own_img = 1;
+        if (!t->img) { rc = dcf_set_error(ctx, DCF_ENOMEM, "image copy"); goto fail; }
+        if (hipMemcpyAsync(t->img, img, (size_t)n * isz, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) {
+            rc = dcf_set_error(ctx, DCF_EDEVICE, "image upload");
+            goto fail;
+        }
+    }
+    t->gmax = (float *)dcf_alloc(ctx, 16);
+    t->counters = (uint32_t *)dcf_alloc(ctx, 64 * 4);
+    t->link = (uint64_t *)dcf_alloc(ctx, (size_t)n * 8);
+    t->parent = (int32_t *)dcf_alloc(ctx, (size_t)n * 4);
+    t->p2n = (uint32_t *)dcf_alloc(ctx, (size_t)n * 4);
+    if (!t->gmax || !t->counters || !t->link || !t->parent || !t->p2n) { rc = dcf_set_error(ctx, DCF_ENOMEM, "tree arrays"); goto fail; }
+    /* find_extremas (preprocessing.c:7-30): only a min-tree reads gmax */
+    if (t->tree_min && (rc = dcf_ops_minmax(t)) != DCF_OK) goto fail;
+    if ((rc = dcf_build_tree(t)) != DCF_OK) goto fail;
+    if ((rc = dcf_accumulate_group(t, dcf_group_of(attr_idx))) != DCF_OK) goto fail;
+    *out = t;
+    return DCF_OK;
+fail:
+    tree_release(t);
+    delete t;
+    return rc;
+}
+
+extern "C" int dcf_tree_accumulate(dcf_tree *t, int attr_idx)
+{
+    if (!t) return DCF_EINVAL;
+    int grp = dcf_group_of(attr_idx);
+    if (grp < 0) return dcf_set_error(t->ctx, DCF_EINVAL, "bad attribute index %d", attr_idx);
+    if (grp == t->grp && t->attr.area) return DCF_OK;
+    if (t->dtype == DCF_F32 && grp == 3)
+        return dcf_set_error(t->ctx, DCF_EUNSUPPORTED, "grey-weighted attributes on float images are not implemented");
+    (void)hipSetDevice(t->ctx->device);
+    return dcf_accumulate_group(t, grp);
+}
+
+extern "C" int dcf_tree_group(const dcf_tree *t) { return t ? t->grp : -1; }
+
+extern "C" int dcf_tree_num_nodes(dcf_tree *t, int64_t *k)
+{
+    if (!t || !k) return DCF_EINVAL;
+    *k = t->K;
+    return DCF_OK;
+}
+
+extern "C" int dcf_tree_parent(dcf_tree *t, int32_t *parent, unsigned flags)
+{
+    if (!t || !parent) return DCF_EINVAL;
+    dcf_ctx *ctx = t->ctx;
+    (void)hipSetDevice(ctx->device);
+    DCF_HIP(ctx, hipMemcpyAsync(parent, t->parent, (size_t)t->n * 4,
+                                (flags & DCF_DEVICE_PTR) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, ctx->stream));
+    if (!(flags & DCF_DEVICE_PTR)) DCF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return DCF_OK;
+}
+
+static int need_group(dcf_tree *t, int attr_idx)
+{
+    if (attr_idx < 0 || attr_idx >= DCF_NUM_ATTR) return dcf_set_error(t->ctx, DCF_EINVAL, "bad attribute index %d", attr_idx);
+    if (dcf_group_of(attr_idx) != t->grp)
+        return dcf_set_error(t->ctx, DCF_EGROUP, "attribute %d (group %d) not accumulated (tree has group %d)", attr_idx,
+                             dcf_group_of(attr_idx), t->grp);
+    return DCF_OK;
+}
+
+extern "C" int dcf_filter(dcf_tree *t, int attr_idx, int decision, double lambda, int extract, void *out, dcf_dtype out_dtype,
+                          unsigned flags)
+{
+    if (!t || !out) return DCF_EINVAL;
+    dcf_ctx *ctx = t->ctx;
+    int rc = need_group(t, attr_idx);
+    if (rc) return rc;
+    if (decision < 0 || decision > 3) return dcf_set_error(ctx, DCF_EINVAL, "bad decision index %d", decision);
+    if (out_dtype != DCF_U8 && out_dtype != DCF_U16 && out_dtype != DCF_F32) return dcf_set_error(ctx, DCF_EINVAL, "bad output dtype");
+    (void)hipSetDevice(ctx->device);
+    size_t bytes = (size_t)t->n * dtype_size(out_dtype);
+    void *dst = out;
+    if (!(flags & DCF_DEVICE_PTR)) {
+        dst = dcf_alloc(ctx, bytes);
+        if (!dst) return dcf_set_error(ctx, DCF_ENOMEM, "output staging");
+    }
+    rc = dcf_ops_filter(t, attr_idx, decision, lambda, extract, dst, out_dtype, flags);
+    if (rc == DCF_OK && !(flags & DCF_DEVICE_PTR)) {
+        if (hipMemcpyAsync(out, dst, bytes, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess)
+            rc = dcf_set_error(ctx, DCF_EDEVICE, "output download");
+    }
+    if (!(flags & DCF_DEVICE_PTR)) dcf_release(ctx, dst);
+    return rc;
+}
+
+extern "C" int dcf_check(dcf_tree *t, int attr_idx, float *out, unsigned flags)
+{
+    if (!t || !out) return DCF_EINVAL;
+    dcf_ctx *ctx = t->ctx;
+    int rc = need_group(t, attr_idx);
+    if (rc) return rc;
+    (void)hipSetDevice(ctx->device);
+    size_t bytes = (size_t)t->n * 4;
+    float *dst = out;
+    if (!(flags & DCF_DEVICE_PTR)) {
+        dst = (float *)dcf_alloc(ctx, bytes);
+        if (!dst) return dcf_set_error(ctx, DCF_ENOMEM, "output staging");
+    }
+    rc = dcf_ops_check(t, attr_idx, dst);
+    if (rc == DCF_OK && !(flags & DCF_DEVICE_PTR)) {
+        if (hipMemcpyAsync(out, dst, bytes, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess)
+            rc = dcf_set_error(ctx, DCF_EDEVICE, "output download");
+    }
+    if (!(flags & DCF_DEVICE_PTR)) dcf_release(ctx, dst);
+    return rc;
+}
+
+extern "C" int dcf_spectrum1d(dcf_tree *t, int attr_idx, const float *lambdas, int n, double *out)
+{
+    if (!t || !lambdas || !out || n <= 0) return t ? dcf_set_error(t->ctx, DCF_EINVAL, "bad spectrum arguments") : DCF_EINVAL;
+    int rc = need_group(t, attr_idx);
+    if (rc) return rc;
+    (void)hipSetDevice(t->ctx->device);
+    return dcf_ops_spectrum(t, attr_idx, lambdas, n, 0, nullptr, 1, 0, out);
+}
+
+extern "C" int dcf_spectrum2d(dcf_tree *t, int attr1, const float *l1, int n1, int attr2, const float *l2, int n2, double *out)
+{
+    if (!t || !l1 || !l2 || !out || n1 <= 0 || n2 <= 0) return t ? dcf_set_error(t->ctx, DCF_EINVAL, "bad spectrum arguments") : DCF_EINVAL;
+    dcf_ctx *ctx = t->ctx;
+    if (attr1 < 0 || attr1 >= DCF_NUM_ATTR || attr2 < 0 || attr2 >= DCF_NUM_ATTR)
+        return dcf_set_error(ctx, DCF_EINVAL, "bad attribute index");
+    /* operation.c:93-95: the area attribute must be the second axis */
+    if (attr1 == 0 && attr2 != 0)
+        return dcf_set_error(ctx, DCF_EINVAL, "invert the two attributes: the one with the highest index must be first");
+    int second_is_area = attr2 == 0 && attr1 != 0;
+    int rc = need_group(t, attr1);
+    if (rc) return rc;
+    if (!second_is_area && (rc = need_group(t, attr2)) != 0) return rc;
+    (void)hipSetDevice(ctx->device);
+    return dcf_ops_spectrum(t, attr1, l1, n1, attr2, l2, n2, second_is_area, out);
+}
+
+/* ------------------------------------------------------------------------------------------------ */
+/* lambda files (lambdavec.c:35-84)                                                                 */
+/* ------------------------------------------------------------------------------------------------ */
+extern "C" int dcf_lvec_read(const char *path, double imscale, float **lambdas, int *n)
+{
+    static const char *hdr = "#LambdaVector file. Keep this exact line in all similar lambda vector files\n";
+    if (!path || !lambdas || !n) return DCF_EINVAL;
+    *lambdas = nullptr;
+    *n = 0;
+    FILE *f = fopen(path, "r");
+    if (!f) return DCF_EIO;
+    /* the header is consumed the way a scanf literal would consume it */
+    const char *p = hdr;
+    int c = fgetc(f);
+    while (*p) {
+        if (*p == ' ' || *p == '\n') {
+            while (c == ' ' || c == '\t' || c == '\n' || c == '\r' || c == '\v' || c == '\f') c = fgetc(f);
+            p++;
+            continue;
+        }
+        if (c != (unsigned char)*p) break;
+        c = fgetc(f);
+        p++;
+    }
+    if (c != EOF) ungetc(c, f);
+    std::vector<float> v(1, 0.0f);
+    float x;
+    while (fscanf(f, "%f", &x) == 1) v.push_back(x);
+    fclose(f);
+    for (size_t i = 1; i < v.size(); i++) v[i] /= imscale;
+    float *out = (float *)malloc(v.size() * sizeof(float));
+    if (!out) return DCF_ENOMEM;
+    memcpy(out, v.data(), v.size() * sizeof(float));
+    *lambdas = out;
+    *n = (int)v.size();
+    return DCF_OK;
+}
+
+extern "C" void dcf_lvec_free(float *l) { free(l); }
+
+/* ------------------------------------------------------------------------------------------------ */
+/* operator driver: the reference's operation tuples (config.c:297-433), run like loop_operations  */
+/* ------------------------------------------------------------------------------------------------ */
+struct OpTuple {
+    std::vector<std::string> items; /* strings unquoted */
+    std::vector<int> is_str;
+};
+
+static int parse_tuples(const char *s, std::vector<OpTuple> &ops, std::string &err)
+{
+    size_t i = 0, L = strlen(s);
+    auto skip = [&]() { while (i < L && (s[i] == ' ' || s[i] == '\t' || s[i] == '\n' || s[i] == '\r' || s[i] == ',' || s[i] == ';')) i++; };
+    skip();
+    /* accept an optional outer pair of parentheses, as the CLI form ((tree,15)) (config.c:528) */
+    while (i < L) {
+        skip();
+        if (i >= L) break;
+        if (s[i] != '(') { err = "expected '('"; return -1; }
+        size_t j = i + 1;
+        while (j < L && (s[j] == ' ' || s[j] == '\t')) j++;
+        if (j < L && s[j] == '(') { i = j; continue; } /* nested: strip the outer level */
+        i++;
+        OpTuple t;
+        for (;;) {
+            while (i < L && (s[i] == ' ' || s[i] == '\t')) i++;
+            if (i >= L) { err = "unterminated tuple"; return -1; }
+            if (s[i] == ')') { i++; break; }
+            if (s[i] == ',') { i++; continue; }
+            if (s[i] == '"') {
+                size_t k = s[i + 1] ? i + 1 : i;
+                size_t e = k;
+                while (e < L && s[e] != '"') e++;
+                if (e >= L) { err = "unterminated string"; return -1; }
+                t.items.emplace_back(s + k, e - k);
+                t.is_str.push_back(1);
+                i = e + 1;
+            } else {
+                size_t e = i;
+                while (e < L && s[e] != ',' && s[e] != ')') e++;
+                std::string tok(s + i, e - i);
+                while (!tok.empty() && (tok.back() == ' ' || tok.back() == '\t')) tok.pop_back();
+                /* bare words (CLI form) are strings unless they parse as numbers */
+                char *endp = nullptr;
+                strtod(tok.c_str(), &endp);
+                t.items.push_back(tok);
+                t.is_str.push_back(!(endp && *endp == 0 && !tok.empty()));
+                i = e;
+            }
+        }
+        while (i < L && s[i] == ')') i++; /* close an outer level */
+        ops.push_back(t);
+    }
+    return 0;
+}
+
+struct Op {
+    std::string name;
+    int attr = 0, attr2 = 0, decision = 0;
+    double lambda = 0, imscale = 1, imscale2 = 1;
+    std::string lvec, lvec2;
+};
+
+static int to_ops(dcf_ctx *ctx, const std::vector<OpTuple> &tp, std::vector<Op> &ops)
+{
+    for (auto &t : tp) {
+        Op o;
+        if (t.items.size() < 2) return dcf_set_error(ctx, DCF_EINVAL, "an operation needs at least 2 parameters");
+        o.name = t.items[0];
+        o.attr = atoi(t.items[1].c_str());
+        if (o.attr < 0 || o.attr >= DCF_NUM_ATTR) return dcf_set_error(ctx, DCF_EINVAL, "bad attribute index %d", o.attr);
+        if (o.name == "filter" || o.name == "extract") {
+            if (t.items.size() < 3 || t.items.size() > 4) return dcf_set_error(ctx, DCF_EINVAL, "filter/extract take 3 or 4 parameters");
+            o.lambda = atof(t.items[2].c_str());
+            o.decision = t.items.size() > 3 ? atoi(t.items[3].c_str()) : 0;
+            if (o.decision < 0 || o.decision > 3) return dcf_set_error(ctx, DCF_EINVAL, "bad decision index %d", o.decision);
+        } else if (o.name == "pattern") {
+            if (t.items.size() < 3 || t.items.size() > 4) return dcf_set_error(ctx, DCF_EINVAL, "pattern takes 3 or 4 parameters");
+            o.lvec = t.items[2];
+            o.imscale = t.items.size() > 3 ? atof(t.items[3].c_str()) : 1.0;
+        } else if (o.name == "pattern2D") {
+            if (t.items.size() != 7) return dcf_set_error(ctx, DCF_EINVAL, "pattern2D takes 7 parameters");
+            o.lvec = t.items[2];
+            o.imscale = atof(t.items[3].c_str());
+            o.attr2 = atoi(t.items[4].c_str());
+            o.lvec2 = t.items[5];
+            o.imscale2 = atof(t.items[6].c_str());
+            if (o.attr2 < 0 || o.attr2 >= DCF_NUM_ATTR) return dcf_set_error(ctx, DCF_EINVAL, "bad attribute index %d", o.attr2);
+        } else if (o.name == "check") {
+            if (t.items.size() > 2) return dcf_set_error(ctx, DCF_EINVAL, "check takes 2 parameters");
+        } else if (o.name == "dmp" || o.name == "csl" || o.name == "tree") {
+            return dcf_set_error(ctx, DCF_EUNSUPPORTED, "operation \"%s\" is outside the GPU path", o.name.c_str());
+        } else {
+            return dcf_set_error(ctx, DCF_EINVAL, "operation \"%s\" not recognized", o.name.c_str());
+        }
+        ops.push_back(o);
+    }
+    return DCF_OK;
+}
+
+extern "C" int dcf_run_operations(dcf_ctx *ctx, const void *img, dcf_dtype dtype, const uint64_t dims[3], int connectivity,
+                                  int tree_type, const float pixdim[3], const char *opstr, dcf_output_fn cb, void *user)
+{
+    if (!ctx || !opstr) return DCF_EINVAL;
+    std::vector<OpTuple> tp;
+    std::string perr;
+    if (parse_tuples(opstr, tp, perr) != 0) return dcf_set_error(ctx, DCF_EINVAL, "operation list: %s", perr.c_str());
+    std::vector<Op> ops;
+    int rc = to_ops(ctx, tp, ops);
+    if (rc) return rc;
+    if (ops.empty()) return DCF_OK;
+    dcf_tree *t = nullptr;
+    const int64_t n = (int64_t)(dims[0] * dims[1] * dims[2]);
+    std::vector<float> fbuf;
+    for (size_t i = 0; i < ops.size() && rc == DCF_OK; i++) {
+        const Op &o = ops[i];
+        if (i == 0) {
+            rc = dcf_tree_build(ctx, img, dtype, dims, connectivity, tree_type, o.attr, pixdim, 0, &t);
+            if (rc) break;
+        } else if (dcf_group_of(o.attr) != dcf_group_of(ops[i - 1].attr)) {
+            rc = dcf_tree_accumulate(t, o.attr); /* disccofan.c:179-195 */
+            if (rc) break;
+        }
+        if (o.name == "filter" || o.name == "extract") {
+            fbuf.resize((size_t)n);
+            rc = dcf_filter(t, o.attr, o.decision, o.lambda, o.name == "extract", fbuf.data(), DCF_F32, 0);
+            if (rc == DCF_OK && cb) cb(user, (int)i, 0, fbuf.data(), n);
+        } else if (o.name == "check") {
+            fbuf.resize((size_t)n);
+            rc = dcf_check(t, o.attr, fbuf.data(), 0);
+            if (rc == DCF_OK && cb) cb(user, (int)i, 1, fbuf.data(), n);
+        } else if (o.name == "pattern") {
+            float *lv = nullptr;
+            int nl = 0;
+            rc = dcf_lvec_read(o.lvec.c_str(), o.imscale, &lv, &nl);
+            if (rc) { rc = dcf_set_error(ctx, rc, "couldn't read the lvec file: %s", o.lvec.c_str()); break; }
+            std::vector<double> sp((size_t)nl);
+            rc = dcf_spectrum1d(t, o.attr, lv, nl, sp.data());
+            dcf_lvec_free(lv);
+            if (rc == DCF_OK && cb) cb(user, (int)i, 2, sp.data(), nl);
+        } else if (o.name == "pattern2D") {
+            float *l1 = nullptr, *l2 = nullptr;
+            int n1 = 0, n2 = 0;
+            rc = dcf_lvec_read(o.lvec.c_str(), o.imscale, &l1, &n1);
+            if (rc) { rc = dcf_set_error(ctx, rc, "couldn't read the lvec file: %s", o.lvec.c_str()); break; }
+            rc = dcf_lvec_read(o.lvec2.c_str(), o.imscale2, &l2, &n2);
+            if (rc) { dcf_lvec_free(l1); rc = dcf_set_error(ctx, rc, "couldn't read the lvec file: %s", o.lvec2.c_str()); break; }
+            std::vector<double> sp((size_t)n1 * n2);
+            rc = dcf_spectrum2d(t, o.attr, l1, n1, o.attr2, l2, n2, sp.data());
+            dcf_lvec_free(l1);
+            dcf_lvec_free(l2);
+            if (rc == DCF_OK && cb) cb(user, (int)i, 3, sp.data(), (int64_t)n1 * n2);
+        }
+    }
+    if (t) dcf_tree_free(t);
+    return rc;
+}

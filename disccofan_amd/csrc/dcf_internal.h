@@ -1,0 +1,139 @@
+/*
+ * dcf_internal.h -- device data layout and host-side launchers of libdcf (not installed).
+ *
+ * HBM layout of one tree over N pixels (all arrays indexed by raster index p = x + y*W + z*W*H):
+ *   img        caller's image (u8/u16/f32), read in place; a min-tree reads gmax - v on the fly
+ *   link[N]    u64  (key(parent) << 32) | parent   build state of the zip merges
+ *   parent[N]  i32  canonical parent (levelroot form, root = -1)
+ *   p2n[N]     u32  node index of the pixel's levelroot
+ * and of the K nodes (levelroots, numbered in raster order of their levelroot pixel):
+ *   npar[K]    i32  parent node, -1 at the root
+ *   ngrey[K]   f32  grey value of the node (exact for integer images)
+ *   attribute fields (structure of arrays, exact integers): area u32, and per group
+ *     group 1 extent : min x/y/z, max x/y/z (max is exclusive, as init_extent_data)
+ *     group 2 mean   : sum x, y, z
+ *     group 3 wmean  : sum g, g*x, g*y, g*z
+ *     group 4 inertia: sum x, y, z, x^2, y^2, z^2, xy, yz, xz
+ */
+#ifndef DCF_INTERNAL_H
+#define DCF_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/dcf.h"
+
+#define DCF_MAX_SUMS 9
+
+struct DevAttr {
+    uint32_t *area = nullptr;
+    int64_t *s[DCF_MAX_SUMS] = {};
+    uint32_t *mn[3] = {};
+    uint32_t *mx[3] = {};
+    int nsum = 0;
+    int has_ext = 0;
+};
+
+/* what the kernels need to read a pixel's grey key and value */
+struct ImgView {
+    const void *img;
+    int dtype;        /* dcf_dtype */
+    int invert;       /* min-tree: value = gmax - v */
+    const float *gmax; /* device scalar (only read when invert) */
+};
+
+struct KernelStat {
+    int64_t launches = 0;
+    double ms = 0;
+};
+
+struct dcf_ctx {
+    int device = 0;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int profiling = 0;
+    struct Pending { std::string name; hipEvent_t a, b; };
+    std::multimap<size_t, void *> free_blocks;
+    std::map<void *, size_t> live_blocks;
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> event_pool;
+    std::map<std::string, KernelStat> stats;
+    std::vector<std::string> names;
+    int tile_x = 64, tile_y = 64, tile_z = 1;      /* 2D tile */
+    int tile3_x = 16, tile3_y = 16, tile3_z = 16;  /* 3D tile */
+};
+
+struct dcf_tree {
+    dcf_ctx *ctx = nullptr;
+    uint64_t dims[3] = { 1, 1, 1 };
+    int64_t n = 0;
+    int conn = 4;
+    int tree_min = 0;
+    dcf_dtype dtype = DCF_U8;
+    float pixdim[3] = { 1, 1, 1 };
+    int grp = 0;
+    /* device */
+    void *img = nullptr;  /* owned copy when the caller gave host memory */
+    int own_img = 0;
+    float *gmax = nullptr; /* device scalars: [0] max, [1] min */
+    uint64_t *link = nullptr;
+    int32_t *parent = nullptr;
+    uint32_t *p2n = nullptr;
+    int64_t K = 0;
+    int32_t *npar = nullptr;
+    float *ngrey = nullptr;
+    DevAttr attr;
+    uint32_t *counters = nullptr; /* 64 device words: per-round early-exit counters */
+    int max_rounds = 32;
+    float host_gmax = 0; /* filled lazily */
+    int host_gmax_valid = 0;
+};
+
+/* ---- error / launch helpers ------------------------------------------------------------------ */
+int dcf_set_error(dcf_ctx *ctx, int code, const char *fmt, ...);
+#define DCF_HIP(ctx, call)                                                                         \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return dcf_set_error((ctx), DCF_EDEVICE, "%s failed: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+void dcf_prof_begin(dcf_ctx *ctx, const char *name, hipEvent_t *a, hipEvent_t *b);
+void dcf_prof_end(dcf_ctx *ctx, const char *name, hipEvent_t a, hipEvent_t b);
+
+#define DCF_LAUNCH(ctx, name, kern, grid, block, shmem, ...)                                       \
+    do {                                                                                           \
+        hipEvent_t ea_ = nullptr, eb_ = nullptr;                                                   \
+        if ((ctx)->profiling) dcf_prof_begin((ctx), (name), &ea_, &eb_);                           \
+        hipLaunchKernelGGL(kern, (grid), (block), (shmem), (ctx)->stream, __VA_ARGS__);           \
+        hipError_t le_ = hipGetLastError();                                                        \
+        if ((ctx)->profiling) dcf_prof_end((ctx), (name), ea_, eb_);                               \
+        if (le_ != hipSuccess)                                                                     \
+            return dcf_set_error((ctx), DCF_EDEVICE, "launch %s: %s", (name), hipGetErrorString(le_)); \
+    } while (0)
+
+/* ---- caching device allocator (context-owned, stream-ordered reuse) --------------------------- */
+void *dcf_alloc(dcf_ctx *ctx, size_t bytes);
+void dcf_release(dcf_ctx *ctx, void *p);
+
+/* ---- attribute table (AttribsArray, attributes.c:1224-1248) ---------------------------------- */
+int dcf_group_of(int attr_idx);
+
+/* ---- launchers (dcf_build.hip) ---------------------------------------------------------------- */
+int dcf_build_tree(dcf_tree *t);              /* link -> parent -> nodes (no attributes) */
+int dcf_accumulate_group(dcf_tree *t, int grp); /* own sums + leaffix of group grp */
+
+/* ---- launchers (dcf_ops.hip) ------------------------------------------------------------------ */
+int dcf_ops_filter(dcf_tree *t, int attr_idx, int decision, double lambda, int extract, void *out_dev,
+                   dcf_dtype out_dtype, unsigned flags);
+int dcf_ops_check(dcf_tree *t, int attr_idx, float *out_dev);
+int dcf_ops_spectrum(dcf_tree *t, int attr1, const float *l1, int n1, int attr2, const float *l2, int n2,
+                     int second_is_area, double *out_host);
+int dcf_ops_minmax(dcf_tree *t);
+void dcf_free_attr(dcf_ctx *ctx, DevAttr &a);
+
+#endif

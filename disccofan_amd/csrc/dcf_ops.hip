@@ -1,0 +1,542 @@
+/*
+ * dcf_ops.hip -- attribute evaluation, filter rules, check images and pattern spectra on gfx950.
+ *
+ * Replaces tree_filtering / tree_filter_direct / tree_filter_subtractive / tree_attribute_check
+ * (src/sources/filtering.c:18-261), tree_pattern_spectrum[2d] (src/sources/pattern.c:27-203) and
+ * the attribute functions of AttribsArray (src/sources/attributes.c:47-874).
+ *
+ * The reference walks parent chains per pixel with a `reached` memo.  Here every rule is first
+ * solved per node (K nodes, coalesced), then one pass maps pixels to their node's value:
+ *   direct       out(n) = g(first ancestor-or-self a with attr(a) >= lambda), 0 if none
+ *                -> nearest-passing-ancestor by in-place pointer jumping
+ *   subtractive  out(root) = pass ? g : 0;  out(n) = out(par) + pass(n) * (g(n) - g(par))
+ *                -> rootfix sum by pointer doubling (ping-pong buffers)
+ *   For the area attribute (increasing: a parent's area is never smaller) every ancestor of a
+ *   passing node passes, so the subtractive closed form collapses to the direct one and the
+ *   cheaper direct kernels give the identical result (DCF_ALL_RULES forces the general kernels).
+ * Attribute values are evaluated in FP64 from exact integer moments with the reference's
+ * operation order, including its float evaluation of area_attribute (attributes.c:105: the
+ * ulong area is converted to float before the product), so bins and thresholds match bit for bit.
+ */
+#include "dcf_internal.h"
+
+struct AttrCtx {
+    DevAttr A;
+    float p0, p1, p2;
+};
+
+/* closed-form eigenvalues, ascending (replaces EISPACK rs, attributes.c:654,679) */
+__device__ static void eig2(double a, double b, double d, double ev[2])
+{
+    double tr = 0.5 * (a + d), df = 0.5 * (a - d);
+    double r = sqrt(df * df + b * b);
+    ev[0] = tr - r;
+    ev[1] = tr + r;
+}
+
+__device__ static void eig3(const double m[6], double ev[3])
+{
+    double a = m[0], b = m[1], c = m[2], d = m[3], e = m[4], f = m[5];
+    double p1 = d * d + e * e + f * f;
+    if (p1 == 0.0) {
+        double t0 = a, t1 = b, t2 = c, s;
+        if (t1 < t0) { s = t0; t0 = t1; t1 = s; }
+        if (t2 < t1) { s = t1; t1 = t2; t2 = s; }
+        if (t1 < t0) { s = t0; t0 = t1; t1 = s; }
+        ev[0] = t0; ev[1] = t1; ev[2] = t2;
+        return;
+    }
+    double q = (a + b + c) / 3.0;
+    double p2 = (a - q) * (a - q) + (b - q) * (b - q) + (c - q) * (c - q) + 2.0 * p1;
+    double p = sqrt(p2 / 6.0);
+    double B00 = (a - q) / p, B11 = (b - q) / p, B22 = (c - q) / p, B01 = d / p, B12 = e / p, B02 = f / p;
+    double detB = B00 * (B11 * B22 - B12 * B12) - B01 * (B01 * B22 - B12 * B02) + B02 * (B01 * B12 - B11 * B02);
+    double r = detB / 2.0;
+    double phi = r <= -1 ? M_PI / 3.0 : (r >= 1 ? 0.0 : acos(r) / 3.0);
+    double e1 = q + 2.0 * p * cos(phi);
+    double e3 = q + 2.0 * p * cos(phi + 2.0 * M_PI / 3.0);
+    double e2 = 3.0 * q - e1 - e3;
+    ev[0] = e3; ev[1] = e2; ev[2] = e1;
+}
+
+/* attribute `idx` of node k (AttribsArray[idx].attribute) */
+__device__ static double attr_value(const AttrCtx &c, int idx, int64_t k)
+{
+    const DevAttr &A = c.A;
+    const float p0 = c.p0, p1 = c.p1, p2 = c.p2;
+    uint32_t area = A.area[k];
+    switch (idx) {
+    case 0: return (double)(p0 * p1 * p2 * (float)area);
+    case 1: {
+        double v = (double)(A.mx[0][k] - A.mn[0][k]) * (double)(A.mx[1][k] - A.mn[1][k]) * (double)(A.mx[2][k] - A.mn[2][k]);
+        v *= (double)(p0 * p1 * p2);
+        return v;
+    }
+    case 2: {
+        double mnx = p0 * (double)A.mn[0][k], mny = p1 * (double)A.mn[1][k], mnz = p2 * (double)A.mn[2][k];
+        double mxx = p0 * (double)A.mx[0][k], mxy = p1 * (double)A.mx[1][k], mxz = p2 * (double)A.mx[2][k];
+        return (mxx - mnx) * (mxx - mnx) + (mxy - mny) * (mxy - mny) + (mxz - mnz) * (mxz - mnz);
+    }
+    case 3: return p0 * (double)(A.mx[0][k] - A.mn[0][k]);
+    case 4: return p1 * (double)(A.mx[1][k] - A.mn[1][k]);
+    case 5: return p2 * (double)(A.mx[2][k] - A.mn[2][k]);
+    case 6: return p0 * (double)A.s[0][k] / (double)area;
+    case 7: return p1 * (double)A.s[1][k] / (double)area;
+    case 8: return p2 * (double)A.s[2][k] / (double)area;
+    case 9: return (double)p0 * (double)A.s[1][k] / (double)A.s[0][k];
+    case 10: return (double)p1 * (double)A.s[2][k] / (double)A.s[0][k];
+    case 11: return (double)p2 * (double)A.s[3][k] / (double)A.s[0][k];
+    case 12: return (double)A.s[0][k];
+    default: break;
+    }
+    /* inertia group (inertia_arr, attributes.c:607-626) */
+    double in0 = (double)p0 * p1 * p2 * area;
+    double in1 = p0 * (double)A.s[0][k], in2 = p1 * (double)A.s[1][k], in3 = p2 * (double)A.s[2][k];
+    double in4 = (double)(p0 * p0) * (double)A.s[3][k];
+    double in5 = (double)(p1 * p1) * (double)A.s[4][k];
+    double in6 = (double)(p2 * p2) * (double)A.s[5][k];
+    double corr = in0 / 12;
+    double mxx = in4 - in1 * in1 / in0 + p0 * corr;
+    double myy = in5 - in2 * in2 / in0 + p1 * corr;
+    if (idx == 13 || idx == 17) {
+        if (in3 == 0) {
+            double tr = mxx + myy;
+            return idx == 13 ? tr : tr / (in0 * in0); /* pow(A, 2.0) is the correctly rounded square */
+        }
+        double mzz = in6 - in3 * in3 / in0 + p2 * corr;
+        double tr = mxx + myy + mzz;
+        return idx == 13 ? tr : tr / pow(in0, 5.0 / 3.0);
+    }
+    double in7 = (double)(p0 * p1) * (double)A.s[6][k];
+    double mxy = in7 - in1 * in2 / in0;
+    if (in3 == 0) {
+        if (idx == 15) return -1;
+        double ev[2];
+        eig2(mxx, mxy, myy, ev);
+        if (idx == 14) return ev[1] / ev[0];
+        double ax0 = sqrt(4 * ev[1] / in0), ax1 = sqrt(4 * ev[0] / in0);
+        return M_PI * ax0 * ax1 / in0;
+    }
+    double in8 = (double)(p1 * p2) * (double)A.s[7][k];
+    double in9 = (double)(p0 * p2) * (double)A.s[8][k];
+    double m6[6] = { mxx, myy, in6 - in3 * in3 / in0 + p2 * corr, mxy, in8 - in2 * in3 / in0, in9 - in1 * in3 / in0 };
+    double ev[3];
+    eig3(m6, ev);
+    if (idx == 14) return ev[2] / ev[1];
+    if (idx == 15) return ev[1] / ev[0];
+    double a0 = sqrt(20 * ev[2] / in0), a1 = sqrt(20 * ev[1] / in0), a2 = sqrt(20 * ev[0] / in0);
+    return M_PI * a0 * a1 * a2 / (6 * in0);
+}
+
+/* AttribsArray[idx].area of node k */
+__device__ static double attr_area(const AttrCtx &c, int grp, int64_t k)
+{
+    uint32_t area = c.A.area[k];
+    switch (grp) {
+    case 0: return (double)(c.p0 * c.p1 * c.p2 * (float)area);
+    case 2: return (double)(c.p0 * c.p1 * c.p2) * (double)area;
+    case 4: return (double)c.p0 * c.p1 * c.p2 * area;
+    default: return (double)area;
+    }
+}
+
+/* find_scale (pattern.c:7-24) */
+__device__ static int find_scale(const float *lv, int n, double a)
+{
+    int upper = n - 1, lower = 0, mid;
+    if (a >= (double)lv[upper]) return upper;
+    mid = (upper + lower) / 2;
+    while (mid != lower) {
+        if (a >= (double)lv[mid]) lower = mid;
+        else upper = mid;
+        mid = (upper + lower) / 2;
+    }
+    return lower;
+}
+
+static __global__ __launch_bounds__(256) void k_fill_u32(uint32_t *a, uint32_t v, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+/* ------------------------------------------------------------------------------------------------ */
+/* per-node kernels                                                                                 */
+/* ------------------------------------------------------------------------------------------------ */
+__global__ __launch_bounds__(256) void k_attr_eval(AttrCtx c, int idx, double *__restrict__ out, int64_t K)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < K) out[k] = attr_value(c, idx, k);
+}
+
+/* direct rule: J(k) = k if attr(k) >= lambda else parent; pass flags */
+__global__ __launch_bounds__(256) void k_direct_init(AttrCtx c, int idx, double lambda, const int32_t *__restrict__ npar,
+                                                     int32_t *__restrict__ J, uint8_t *__restrict__ pass, int64_t K)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    bool ok = attr_value(c, idx, k) >= lambda;
+    pass[k] = ok;
+    J[k] = ok ? (int32_t)k : npar[k];
+}
+
+/* in place: J(k) <- J(J(k)) while J(k) is a failing node; values only move up the path to the
+ * fixed target, so reads of concurrently updated entries are always valid */
+__global__ __launch_bounds__(256) void k_direct_jump(int32_t *__restrict__ J, const uint8_t *__restrict__ pass,
+                                                     uint32_t *__restrict__ active, int round, int64_t K)
+{
+    if (round > 0 && active[round - 1] == 0) return;
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool more = false;
+    if (k < K) {
+        int32_t j = J[k];
+        if (j >= 0 && !pass[j]) {
+            int32_t jj = J[j];
+            J[k] = jj;
+            more = jj >= 0 && !pass[jj];
+        }
+    }
+    unsigned long long b = __ballot(more);
+    if ((threadIdx.x & (warpSize - 1)) == 0 && b) atomicAdd(&active[round], (uint32_t)__popcll(b));
+}
+
+__global__ __launch_bounds__(256) void k_direct_value(const int32_t *__restrict__ J, const float *__restrict__ ngrey,
+                                                      float *__restrict__ val, int64_t K)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    int32_t j = J[k];
+    val[k] = j < 0 ? 0.0f : ngrey[j];
+}
+
+/* subtractive rule: F(k) = pass ? g(k) - g(par) : 0 (root: pass ? g : 0); out = rootfix sum */
+__global__ __launch_bounds__(256) void k_sub_init(AttrCtx c, int idx, double lambda, const int32_t *__restrict__ npar,
+                                                  const float *__restrict__ ngrey, double *__restrict__ F,
+                                                  int32_t *__restrict__ A, int64_t K)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    bool ok = attr_value(c, idx, k) >= lambda;
+    int32_t p = npar[k];
+    F[k] = ok ? (p >= 0 ? (double)ngrey[k] - (double)ngrey[p] : (double)ngrey[k]) : 0.0;
+    A[k] = p;
+}
+
+__global__ __launch_bounds__(256) void k_sub_round(const double *__restrict__ F, double *__restrict__ F2,
+                                                   const int32_t *__restrict__ A, int32_t *__restrict__ A2,
+                                                   uint32_t *__restrict__ active, int round, int64_t K)
+{
+    if (round > 0 && active[round - 1] == 0) return;
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int32_t a2 = -1;
+    if (k < K) {
+        int32_t a = A[k];
+        double f = F[k];
+        if (a >= 0) {
+            f += F[a];
+            a2 = A[a];
+        }
+        F2[k] = f;
+        A2[k] = a2;
+    }
+    unsigned long long b = __ballot(k < K && a2 >= 0);
+    if ((threadIdx.x & (warpSize - 1)) == 0 && b) atomicAdd(&active[round], (uint32_t)__popcll(b));
+}
+
+__global__ __launch_bounds__(256) void k_d2f(const double *__restrict__ F, float *__restrict__ val, int64_t K)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < K) val[k] = (float)F[k];
+}
+
+/* ------------------------------------------------------------------------------------------------ */
+/* per-pixel output                                                                                 */
+/* ------------------------------------------------------------------------------------------------ */
+__device__ __forceinline__ float px_grey_ops(const ImgView &iv, int64_t p)
+{
+    float v;
+    if (iv.dtype == DCF_U8) v = (float)((const uint8_t *)iv.img)[p];
+    else if (iv.dtype == DCF_U16) v = (float)((const uint16_t *)iv.img)[p];
+    else v = ((const float *)iv.img)[p];
+    return iv.invert ? *iv.gmax - v : v;
+}
+
+/* out[p] = post(val[p2n[p]]), post = extract (g - f) then min-tree re-inversion (gmax - f) */
+__global__ __launch_bounds__(256) void k_filter_out(ImgView iv, const uint32_t *__restrict__ p2n, const float *__restrict__ val,
+                                                    int extract, void *__restrict__ out, int out_dtype, int64_t n)
+{
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    float f = val[p2n[p]];
+    if (extract) f = px_grey_ops(iv, p) - f;
+    if (iv.invert) f = *iv.gmax - f;
+    if (out_dtype == DCF_F32) ((float *)out)[p] = f;
+    else if (out_dtype == DCF_U16) ((uint16_t *)out)[p] = (uint16_t)f;
+    else ((uint8_t *)out)[p] = (uint8_t)f;
+}
+
+__global__ __launch_bounds__(256) void k_check_out(const uint32_t *__restrict__ p2n, const double *__restrict__ av,
+                                                   float *__restrict__ out, int64_t n)
+{
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) out[p] = (float)av[p2n[p]];
+}
+
+/* ------------------------------------------------------------------------------------------------ */
+/* pattern spectra: LDS histogram per block, exact 64-bit integer bins for integer images          */
+/* ------------------------------------------------------------------------------------------------ */
+#define SPEC_LDS_BINS 2048
+
+__global__ __launch_bounds__(256) void k_spectrum(AttrCtx c, int grp, int attr1, const float *__restrict__ l1, int n1, int attr2,
+                                                  const float *__restrict__ l2, int n2, int second_is_area,
+                                                  const int32_t *__restrict__ npar, const float *__restrict__ ngrey, int exact,
+                                                  unsigned long long *__restrict__ hist_i, double *__restrict__ hist_d, int64_t K)
+{
+    __shared__ unsigned long long sh_i[SPEC_LDS_BINS];
+    __shared__ double sh_d[SPEC_LDS_BINS];
+    const int nb = n1 * n2;
+    const bool use_lds = nb <= SPEC_LDS_BINS;
+    if (use_lds) {
+        for (int i = threadIdx.x; i < nb; i += blockDim.x) { sh_i[i] = 0; sh_d[i] = 0; }
+        __syncthreads();
+    }
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < K; k += (int64_t)gridDim.x * blockDim.x) {
+        int32_t p = npar[k];
+        float dh = p >= 0 ? ngrey[k] - ngrey[p] : ngrey[k];
+        if (dh == 0) continue;
+        int s1 = find_scale(l1, n1, attr_value(c, attr1, k));
+        int bin = s1;
+        if (l2) {
+            double a2 = second_is_area ? attr_area(c, grp, k) : attr_value(c, attr2, k);
+            bin = s1 + find_scale(l2, n2, a2) * n1;
+        }
+        double ar = attr_area(c, grp, k);
+        if (exact) {
+            unsigned long long v = (unsigned long long)(long long)dh * (unsigned long long)(long long)ar;
+            if (use_lds) atomicAdd(&sh_i[bin], v);
+            else atomicAdd(&hist_i[bin], v);
+        } else {
+            double v = (double)dh * ar;
+            if (use_lds) atomicAdd(&sh_d[bin], v);
+            else atomicAdd(&hist_d[bin], v);
+        }
+    }
+    if (use_lds) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+            if (sh_i[i]) atomicAdd(&hist_i[i], sh_i[i]);
+            if (sh_d[i] != 0) atomicAdd(&hist_d[i], sh_d[i]);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------------------------------ */
+/* host side                                                                                        */
+/* ------------------------------------------------------------------------------------------------ */
+static inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+static ImgView view_of(dcf_tree *t)
+{
+    ImgView iv;
+    iv.img = t->img;
+    iv.dtype = t->dtype;
+    iv.invert = t->tree_min;
+    iv.gmax = t->gmax;
+    return iv;
+}
+
+static AttrCtx attr_ctx(dcf_tree *t)
+{
+    AttrCtx c;
+    c.A = t->attr;
+    c.p0 = t->pixdim[0];
+    c.p1 = t->pixdim[1];
+    c.p2 = t->pixdim[2];
+    return c;
+}
+
+/* node values of the rule (float per node) into val */
+static int rule_values(dcf_tree *t, int attr_idx, int decision, double lambda, unsigned flags, float *val)
+{
+    dcf_ctx *ctx = t->ctx;
+    const int64_t K = t->K;
+    AttrCtx c = attr_ctx(t);
+    bool general_sub = decision == DCF_SUBTRACTIVE && !(attr_idx == 0 && !(flags & DCF_ALL_RULES));
+    if (decision == DCF_DIRECT || (decision == DCF_SUBTRACTIVE && !general_sub)) {
+        int32_t *J = (int32_t *)dcf_alloc(ctx, (size_t)K * 4);
+        uint8_t *pass = (uint8_t *)dcf_alloc(ctx, (size_t)K);
+        if (!J || !pass) { dcf_release(ctx, J); dcf_release(ctx, pass); return dcf_set_error(ctx, DCF_ENOMEM, "filter scratch"); }
+        DCF_LAUNCH(ctx, "k_direct_init", k_direct_init, dim3(nblk(K, 256)), dim3(256), 0, c, attr_idx, lambda, t->npar, J, pass, K);
+        DCF_LAUNCH(ctx, "k_fill", k_fill_u32, dim3(1), dim3(64), 0, t->counters, 0u, (int64_t)64);
+        for (int r = 0; r < t->max_rounds; r++)
+            DCF_LAUNCH(ctx, "k_direct_jump", k_direct_jump, dim3(nblk(K, 256)), dim3(256), 0, J, pass, t->counters, r, K);
+        DCF_LAUNCH(ctx, "k_direct_value", k_direct_value, dim3(nblk(K, 256)), dim3(256), 0, J, t->ngrey, val, K);
+        dcf_release(ctx, J);
+        dcf_release(ctx, pass);
+        return DCF_OK;
+    }
+    if (decision == DCF_SUBTRACTIVE) {
+        double *F0 = (double *)dcf_alloc(ctx, (size_t)K * 8), *F1 = (double *)dcf_alloc(ctx, (size_t)K * 8);
+        int32_t *A0 = (int32_t *)dcf_alloc(ctx, (size_t)K * 4), *A1 = (int32_t *)dcf_alloc(ctx, (size_t)K * 4);
+        if (!F0 || !F1 || !A0 || !A1) {
+            dcf_release(ctx, F0); dcf_release(ctx, F1); dcf_release(ctx, A0); dcf_release(ctx, A1);
+            return dcf_set_error(ctx, DCF_ENOMEM, "filter scratch");
+        }
+        DCF_LAUNCH(ctx, "k_sub_init", k_sub_init, dim3(nblk(K, 256)), dim3(256), 0, c, attr_idx, lambda, t->npar, t->ngrey, F0, A0, K);
+        DCF_LAUNCH(ctx, "k_fill", k_fill_u32, dim3(1), dim3(64), 0, t->counters, 0u, (int64_t)64);
+        double *fi = F0, *fo = F1;
+        int32_t *ai = A0, *ao = A1;
+        for (int r = 0; r < t->max_rounds; r++) {
+            DCF_LAUNCH(ctx, "k_sub_round", k_sub_round, dim3(nblk(K, 256)), dim3(256), 0, fi, fo, ai, ao, t->counters, r, K);
+            double *tf = fi; fi = fo; fo = tf;
+            int32_t *ta = ai; ai = ao; ao = ta;
+        }
+        uint32_t act[64];
+        DCF_HIP(ctx, hipMemcpyAsync(act, t->counters, sizeof act, hipMemcpyDeviceToHost, ctx->stream));
+        DCF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        int ran = 0;
+        for (int r = 0; r < t->max_rounds; r++) { ran = r + 1; if (act[r] == 0) break; }
+        double *res = (ran & 1) ? F1 : F0;
+        DCF_LAUNCH(ctx, "k_d2f", k_d2f, dim3(nblk(K, 256)), dim3(256), 0, res, val, K);
+        dcf_release(ctx, F0); dcf_release(ctx, F1); dcf_release(ctx, A0); dcf_release(ctx, A1);
+        return DCF_OK;
+    }
+    return dcf_set_error(ctx, DCF_EUNSUPPORTED, "decision %d (%s) is not implemented on the GPU path", decision,
+                         dcf_decision_name(decision));
+}
+
+int dcf_ops_filter(dcf_tree *t, int attr_idx, int decision, double lambda, int extract, void *out_dev, dcf_dtype out_dtype,
+                   unsigned flags)
+{
+    dcf_ctx *ctx = t->ctx;
+    float *val = (float *)dcf_alloc(ctx, (size_t)t->K * 4);
+    if (!val) return dcf_set_error(ctx, DCF_ENOMEM, "filter values");
+    int rc = rule_values(t, attr_idx, decision, lambda, flags, val);
+    if (rc == DCF_OK) {
+        ImgView iv = view_of(t);
+        hipEvent_t ea = nullptr, eb = nullptr;
+        if (ctx->profiling) dcf_prof_begin(ctx, "k_filter_out", &ea, &eb);
+        hipLaunchKernelGGL(k_filter_out, dim3(nblk(t->n, 256)), dim3(256), 0, ctx->stream, iv, t->p2n, val, extract, out_dev,
+                           (int)out_dtype, t->n);
+        hipError_t le = hipGetLastError();
+        if (ctx->profiling) dcf_prof_end(ctx, "k_filter_out", ea, eb);
+        if (le != hipSuccess) rc = dcf_set_error(ctx, DCF_EDEVICE, "launch k_filter_out: %s", hipGetErrorString(le));
+    }
+    dcf_release(ctx, val);
+    return rc;
+}
+
+int dcf_ops_check(dcf_tree *t, int attr_idx, float *out_dev)
+{
+    dcf_ctx *ctx = t->ctx;
+    double *av = (double *)dcf_alloc(ctx, (size_t)t->K * 8);
+    if (!av) return dcf_set_error(ctx, DCF_ENOMEM, "check values");
+    AttrCtx c = attr_ctx(t);
+    DCF_LAUNCH(ctx, "k_attr_eval", k_attr_eval, dim3(nblk(t->K, 256)), dim3(256), 0, c, attr_idx, av, t->K);
+    DCF_LAUNCH(ctx, "k_check_out", k_check_out, dim3(nblk(t->n, 256)), dim3(256), 0, t->p2n, av, out_dev, t->n);
+    dcf_release(ctx, av);
+    return DCF_OK;
+}
+
+int dcf_ops_spectrum(dcf_tree *t, int attr1, const float *l1, int n1, int attr2, const float *l2, int n2, int second_is_area,
+                     double *out_host)
+{
+    dcf_ctx *ctx = t->ctx;
+    const int nb = n1 * (l2 ? n2 : 1);
+    bool unit = t->pixdim[0] == 1.0f && t->pixdim[1] == 1.0f && t->pixdim[2] == 1.0f;
+    int exact = (t->dtype != DCF_F32) && unit;
+    size_t bytes = (size_t)nb * 16 + (size_t)(n1 + (l2 ? n2 : 0)) * 4;
+    char *buf = (char *)dcf_alloc(ctx, bytes);
+    if (!buf) return dcf_set_error(ctx, DCF_ENOMEM, "spectrum bins");
+    unsigned long long *hi = (unsigned long long *)buf;
+    double *hd = (double *)(buf + (size_t)nb * 8);
+    float *dl1 = (float *)(buf + (size_t)nb * 16);
+    float *dl2 = l2 ? dl1 + n1 : nullptr;
+    DCF_LAUNCH(ctx, "k_fill", k_fill_u32, dim3(nblk((int64_t)nb * 4, 256)), dim3(256), 0, (uint32_t *)buf, 0u, (int64_t)nb * 4);
+    DCF_HIP(ctx, hipMemcpyAsync(dl1, l1, (size_t)n1 * 4, hipMemcpyHostToDevice, ctx->stream));
+    if (l2) DCF_HIP(ctx, hipMemcpyAsync(dl2, l2, (size_t)n2 * 4, hipMemcpyHostToDevice, ctx->stream));
+    AttrCtx c = attr_ctx(t);
+    int grp = dcf_group_of(attr1);
+    unsigned blocks = nblk(t->K, 256);
+    if (blocks > 4096) blocks = 4096;
+    if (blocks < 1) blocks = 1;
+    DCF_LAUNCH(ctx, "k_spectrum", k_spectrum, dim3(blocks), dim3(256), 0, c, grp, attr1, dl1, n1, attr2, dl2, l2 ? n2 : 1,
+               second_is_area, t->npar, t->ngrey, exact, hi, hd, t->K);
+    std::vector<unsigned long long> vi(nb);
+    std::vector<double> vd(nb);
+    DCF_HIP(ctx, hipMemcpyAsync(vi.data(), hi, (size_t)nb * 8, hipMemcpyDeviceToHost, ctx->stream));
+    DCF_HIP(ctx, hipMemcpyAsync(vd.data(), hd, (size_t)nb * 8, hipMemcpyDeviceToHost, ctx->stream));
+    DCF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < nb; i++) out_host[i] = exact ? (double)vi[i] : vd[i];
+    dcf_release(ctx, buf);
+    return DCF_OK;
+}
+
+/* ------------------------------------------------------------------------------------------------ */
+/* global extremes (find_extremas, preprocessing.c:7-17)                                            */
+/* ------------------------------------------------------------------------------------------------ */
+__device__ static void atomic_max_f(float *a, float v)
+{
+    unsigned int *ai = (unsigned int *)a;
+    unsigned int old = *ai;
+    while (__uint_as_float(old) < v) {
+        unsigned int prev = atomicCAS(ai, old, __float_as_uint(v));
+        if (prev == old) break;
+        old = prev;
+    }
+}
+
+__device__ static void atomic_min_f(float *a, float v)
+{
+    unsigned int *ai = (unsigned int *)a;
+    unsigned int old = *ai;
+    while (__uint_as_float(old) > v) {
+        unsigned int prev = atomicCAS(ai, old, __float_as_uint(v));
+        if (prev == old) break;
+        old = prev;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_minmax_init(const void *img, int dtype, float *mm)
+{
+    float v = dtype == DCF_U8 ? (float)((const uint8_t *)img)[0]
+                              : (dtype == DCF_U16 ? (float)((const uint16_t *)img)[0] : ((const float *)img)[0]);
+    mm[0] = v;
+    mm[1] = v;
+}
+
+__global__ __launch_bounds__(256) void k_minmax(const void *img, int dtype, float *mm, int64_t n)
+{
+    __shared__ float smx[256], smn[256];
+    float mx = -INFINITY, mn = INFINITY;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        float v = dtype == DCF_U8 ? (float)((const uint8_t *)img)[i]
+                                  : (dtype == DCF_U16 ? (float)((const uint16_t *)img)[i] : ((const float *)img)[i]);
+        mx = fmaxf(mx, v);
+        mn = fminf(mn, v);
+    }
+    smx[threadIdx.x] = mx;
+    smn[threadIdx.x] = mn;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            smx[threadIdx.x] = fmaxf(smx[threadIdx.x], smx[threadIdx.x + s]);
+            smn[threadIdx.x] = fminf(smn[threadIdx.x], smn[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomic_max_f(&mm[0], smx[0]);
+        atomic_min_f(&mm[1], smn[0]);
+    }
+}
+
+int dcf_ops_minmax(dcf_tree *t)
+{
+    dcf_ctx *ctx = t->ctx;
+    DCF_LAUNCH(ctx, "k_minmax", k_minmax_init, dim3(1), dim3(1), 0, (const void *)t->img, (int)t->dtype, t->gmax);
+    unsigned blocks = nblk(t->n, 256);
+    if (blocks > 2048) blocks = 2048;
+    DCF_LAUNCH(ctx, "k_minmax", k_minmax, dim3(blocks), dim3(256), 0, (const void *)t->img, (int)t->dtype, t->gmax, t->n);
+    return DCF_OK;
+}

@@ -1,0 +1,146 @@
+/*
+ * dcf.h -- C ABI of the MI355X component-tree library (libdcf.so).
+ *
+ * A drop-in for disccofan's component-tree path: build the max-tree or min-tree of a 2D/3D
+ * image, accumulate node attributes, apply the attribute filter rules and compute 1D/2D pattern
+ * spectra, all on one GPU.  Plain pointers and sizes only; every call returns a status code and
+ * no call keeps global state (a context owns a HIP stream and its trees).
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   build_tree_with_attributes(Arguments*, Node*)          src/sources/flood.c:175
+ *   flood_attributes(Node*)                                  src/sources/flood.c:184
+ *   tree_filtering(Node*, value*, ulong, int, int, double)   src/sources/filtering.c:18
+ *   tree_attribute_check(Node*, value*, ulong, ulong, ...)   src/sources/filtering.c:47
+ *   tree_pattern_spectrum(...)                               src/sources/pattern.c:27
+ *   tree_pattern_spectrum2d(...)                             src/sources/pattern.c:110
+ *   lambda_vector_read(char*, double)                        src/sources/lambdavec.c:35
+ *   AttribsArray[18] / Decisions[4] plugin tables            src/sources/attributes.c:1224, filtering.c:271
+ * and the operator semantics of perform_operation (src/sources/operation.c:110-212) and
+ * loop_operations (src/sources/disccofan.c:137-205) for the operations on this path.
+ *
+ * Pointers: by default image/output pointers are host memory; pass DCF_DEVICE_PTR in `flags`
+ * when they are device (HBM) pointers already resident on the context's device.
+ * Threading: contexts are independent; calls on one context must be serialised by the caller.
+ */
+#ifndef DCF_H
+#define DCF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dcf_ctx dcf_ctx;
+typedef struct dcf_tree dcf_tree;
+
+typedef enum { DCF_U8 = 0, DCF_U16 = 1, DCF_F32 = 2 } dcf_dtype;
+
+/* tree type, as the reference's tree_type string "max" / "min" (preprocessing.c:84-91) */
+enum { DCF_MAX_TREE = 0, DCF_MIN_TREE = 1 };
+
+/* decision index, as Decisions[] (filtering.c:271-277) */
+enum { DCF_DIRECT = 0, DCF_MIN = 1, DCF_MAX = 2, DCF_SUBTRACTIVE = 3 };
+
+/* status codes */
+enum {
+    DCF_OK = 0,
+    DCF_EINVAL = -1,       /* bad argument (connectivity, attribute or decision index, shape) */
+    DCF_ENOMEM = -2,       /* device or host allocation failed */
+    DCF_EDEVICE = -3,      /* HIP runtime error */
+    DCF_EUNSUPPORTED = -4, /* valid but not implemented on this path */
+    DCF_EIO = -5,          /* file could not be read */
+    DCF_EGROUP = -6        /* attribute not in the tree's accumulated group (call dcf_tree_accumulate) */
+};
+
+/* flags */
+#define DCF_DEVICE_PTR 0x1u     /* image / output pointer is a device pointer */
+#define DCF_ALL_RULES 0x2u      /* dcf_filter: always run the general rule kernels (no increasing-attribute shortcut) */
+
+/* number of attributes, as NUMATTR (types.h) */
+#define DCF_NUM_ATTR 18
+
+/* ---- context ---------------------------------------------------------------------------------- */
+int dcf_ctx_create(int device, unsigned flags, dcf_ctx **out);
+int dcf_ctx_destroy(dcf_ctx *ctx);
+/* Use an existing HIP stream (hipStream_t passed as void*) for every launch of this context;
+ * NULL restores the context's own stream. */
+int dcf_ctx_set_stream(dcf_ctx *ctx, void *stream);
+void *dcf_ctx_stream(dcf_ctx *ctx);
+const char *dcf_last_error(dcf_ctx *ctx);
+int dcf_ctx_synchronize(dcf_ctx *ctx);
+
+/* Per-kernel timing with HIP events on the context's stream (off by default).  When on, every
+ * launch is bracketed by two events; dcf_kernel_stats returns the launch count and the summed
+ * event time of kernels whose name contains `name`. */
+int dcf_ctx_set_profiling(dcf_ctx *ctx, int on);
+int dcf_kernel_stats(dcf_ctx *ctx, const char *name, int64_t *launches, double *total_ms);
+int dcf_kernel_stats_reset(dcf_ctx *ctx);
+/* name of the i-th distinct profiled kernel, or NULL past the end */
+const char *dcf_kernel_name(dcf_ctx *ctx, int i);
+
+/* ---- build (build_tree_with_attributes, flood.c:175) ------------------------------------------- *
+ * img: dims[0]*dims[1]*dims[2] values, x fastest (index = x + y*dims[0] + z*dims[0]*dims[1]).
+ * connectivity: 4 or 8 (2D), 6 or 26 (3D); mapped like check_connectivity (io.c:64-79).
+ * tree_type: DCF_MAX_TREE or DCF_MIN_TREE (min-tree = max-tree of gmax - g).
+ * attr_idx: the first operation's attribute; its group is accumulated (disccofan.c:145-148).
+ * pixdim: physical pixel size (NULL = 1,1,1), as data_properties.pixdim. */
+int dcf_tree_build(dcf_ctx *ctx, const void *img, dcf_dtype dtype, const uint64_t dims[3], int connectivity,
+                   int tree_type, int attr_idx, const float pixdim[3], unsigned flags, dcf_tree **out);
+int dcf_tree_free(dcf_tree *tree);
+
+/* Re-accumulate the attributes of attr_idx's group on the existing tree when the group differs
+ * from the accumulated one (flood_attributes, flood.c:184; group reuse disccofan.c:179-200). */
+int dcf_tree_accumulate(dcf_tree *tree, int attr_idx);
+int dcf_tree_group(const dcf_tree *tree);
+
+/* Canonical parent array: every pixel's levelroot, levelroots -> levelroot of the parent node,
+ * the root -> -1; levelroot = minimum raster index of the node's pixels at its grey level. */
+int dcf_tree_parent(dcf_tree *tree, int32_t *parent, unsigned flags);
+/* number of nodes (levelroots) */
+int dcf_tree_num_nodes(dcf_tree *tree, int64_t *num_nodes);
+
+/* ---- filter (tree_filtering, filtering.c:18; perform_filter_or_extract, operation.c:10-29) ------ *
+ * out: one value per pixel in out_dtype (DCF_F32 gives the reference's float output; DCF_U8/U16
+ * are exact for integer images of that range).  extract != 0 gives g - f; a min-tree's output is
+ * re-inverted gmax - f, as the reference. */
+int dcf_filter(dcf_tree *tree, int attr_idx, int decision, double lambda, int extract, void *out,
+               dcf_dtype out_dtype, unsigned flags);
+
+/* ---- check (tree_attribute_check, filtering.c:47-52): out[p] = (float) attr(levelroot(p)) ---- */
+int dcf_check(dcf_tree *tree, int attr_idx, float *out, unsigned flags);
+
+/* ---- spectra (pattern.c:27, 110), background = 1 (operation.c:82) ------------------------------ *
+ * lambdas: thresholds with lambda_0 = 0 included, as returned by dcf_lvec_read.
+ * 2D: bin index s1 + s2*n1; attr2 == 0 with attr1 != 0 means "area of attr1's group"
+ * (operation.c:97-98); attr1 == 0 with attr2 != 0 is refused (operation.c:93-95). */
+int dcf_spectrum1d(dcf_tree *tree, int attr_idx, const float *lambdas, int n, double *out);
+int dcf_spectrum2d(dcf_tree *tree, int attr1, const float *l1, int n1, int attr2, const float *l2, int n2,
+                   double *out);
+
+/* ---- lambda-vector files (lambda_vector_read, lambdavec.c:35-84) -------------------------------- *
+ * On success *lambdas (malloc'd, free with dcf_lvec_free) holds count+1 values, lambda_0 = 0. */
+int dcf_lvec_read(const char *path, double imscale, float **lambdas, int *n);
+void dcf_lvec_free(float *lambdas);
+
+/* ---- attribute / decision tables (AttribsArray, Decisions) -------------------------------------- */
+const char *dcf_attr_name(int attr_idx);
+const char *dcf_attr_short_name(int attr_idx);
+int dcf_attr_group(int attr_idx);
+const char *dcf_decision_name(int decision);
+
+/* ---- operator driver (loop_operations, disccofan.c:137-205) ------------------------------------- *
+ * ops: the reference's operation tuples, e.g. "(\"filter\",0,100),(\"pattern\",0,\"lvec.txt\",1)".
+ * Runs them in order on one image with attribute-group reuse; outputs are written by the callback
+ * (kind: 0 filtered/extracted image (float, n values), 1 check image (float), 2 1D spectrum
+ * (double, count values), 3 2D spectrum (double)).  Unsupported operations return
+ * DCF_EUNSUPPORTED before anything runs. */
+typedef void (*dcf_output_fn)(void *user, int op_index, int kind, const void *data, int64_t count);
+int dcf_run_operations(dcf_ctx *ctx, const void *img, dcf_dtype dtype, const uint64_t dims[3], int connectivity,
+                       int tree_type, const float pixdim[3], const char *ops, dcf_output_fn cb, void *user);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DCF_H */

@@ -1,0 +1,78 @@
+"""The C-ABI library loads on a machine without a GPU and exports every entry point of include/dcf.h.
+No compute call is made here; status-code behaviour without a device is checked where it is safe."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "dcf.h")
+
+
+def declared_functions():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    names = re.findall(r"\b(dcf_[a-z0-9_]+)\s*\(", txt)
+    return sorted(set(n for n in names if n != "dcf_output_fn"))
+
+
+def test_header_declares_the_path():
+    names = declared_functions()
+    for must in ("dcf_tree_build", "dcf_tree_accumulate", "dcf_filter", "dcf_check", "dcf_spectrum1d",
+                 "dcf_spectrum2d", "dcf_tree_parent", "dcf_lvec_read", "dcf_run_operations"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    import disccofan_amd as d
+    L = d.lib()
+    missing = [n for n in declared_functions() if not hasattr(L, n)]
+    assert not missing, missing
+    assert sorted(d.EXPORTS) == sorted(declared_functions())
+
+
+def test_tables_match_reference():
+    import disccofan_amd as d
+    L = d.lib()
+    shorts = [L.dcf_attr_short_name(i).decode() for i in range(18)]
+    assert shorts == d.ATTRIBUTES
+    assert [L.dcf_attr_group(i) for i in range(18)] == [0, 1, 1, 1, 1, 1, 2, 2, 2, 3, 3, 3, 3, 4, 4, 4, 4, 4]
+    assert [L.dcf_decision_name(i).decode() for i in range(4)] == d.DECISIONS
+    assert L.dcf_attr_name(18) is None and L.dcf_decision_name(4) is None
+
+
+def test_lvec_reader_host_only(tmp_path):
+    import numpy as np
+    import disccofan_amd as d
+    lv = d.lvec_read(os.path.join(ROOT, "tests", "golden", "lvec2.txt"))
+    assert lv[0] == 0 and np.allclose(lv[1:], [1, 1.5, 2, 2.5, 3, 5])
+    with pytest.raises(d.DcfError) as e:
+        d.lvec_read(str(tmp_path / "missing.txt"))
+    assert e.value.code == -5
+    # no header line: values are read from the start, like the reference's fscanf literal
+    p = tmp_path / "nohdr.txt"
+    p.write_text("10 20\n30\n")
+    assert np.array_equal(d.lvec_read(str(p), 2.0), np.array([0, 5, 10, 15], np.float32))
+
+
+def test_lvec_reader_matches_oracle(tmp_path):
+    import numpy as np
+    import disccofan_amd as d
+    from oracle.oracle import lvec_read as orc_read
+    p = tmp_path / "odd.txt"
+    p.write_text("#LambdaVector file. Keep this exact line in all similar lambda vector files\n0.5 1e3\n  7.25\n")
+    for s in (1.0, 3.0):
+        assert np.array_equal(d.lvec_read(str(p), s), orc_read(str(p), s))
+    q = tmp_path / "badhdr.txt"
+    q.write_text("#Lambda something else\n1 2 3\n")
+    assert np.array_equal(d.lvec_read(str(q)), orc_read(str(q)))
+
+
+def test_context_without_gpu_fails_loudly():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    import disccofan_amd as d
+    with pytest.raises(d.DcfError):
+        d.Context(0)

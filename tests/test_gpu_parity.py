@@ -1,0 +1,229 @@
+"""GPU (libdcf.so, HIP on gfx950) against the CPU oracle, bit for bit.
+
+Integer images: canonical parent arrays, check images, filtered images and spectra must be
+identical.  Every case prints its name/shape before running so a failure names it.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import disccofan_amd as dcf
+from disccofan_amd import synth
+from oracle.oracle import OracleTree, lvec_read
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = dcf.Context(0)
+    yield c
+    c.close()
+
+
+def assert_same(a, b, what):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    if not np.array_equal(a, b):
+        bad = np.nonzero(a.reshape(-1) != b.reshape(-1))[0]
+        raise AssertionError("%s differs at %d of %d positions, first %s: gpu %r oracle %r" % (
+            what, len(bad), a.size, bad[:5], a.reshape(-1)[bad[:5]], b.reshape(-1)[bad[:5]]))
+
+
+# ------------------------------------------------------------------------------------------------
+# reference golden files
+# ------------------------------------------------------------------------------------------------
+def test_lena_golden(ctx, lena):
+    img = lena["img"]
+    say("lena 512x512 u8 4-conn")
+    t = ctx.build(img, 4, "max", 0)
+    assert_same(t.check(0), lena["check_area4"].astype(np.float32), "check area 4-conn")
+    assert_same(t.filter(0, 200, 0), lena["filter_area200"].astype(np.float32), "filter area 200")
+    assert t.num_nodes == len(lena["tree_index"])
+    o = OracleTree(img, 4, "max", 0)
+    assert_same(t.parent(), o.parent().astype(np.int32), "parent")
+    t.accumulate(17)
+    assert_same(t.check(17), lena["check_ncomp4"], "check ncomp")
+    assert_same(t.filter(17, 0.2, 0), lena["filter_ncomp02"].astype(np.float32), "filter ncomp 0.2")
+    say("lena 512x512 u8 8-conn")
+    t8 = ctx.build(img, 8, "max", 0)
+    assert_same(t8.check(0), lena["check_area8"].astype(np.float32), "check area 8-conn")
+
+
+def test_lena_operations_driver(ctx, lena):
+    say("run_operations on lena")
+    lv = os.path.join(GOLDEN, "lvec.txt")
+    ops = [("filter", 0, 200), ("filter", 17, 0.2), ("check", 17), ("pattern", 0, lv, 1)]
+    outs = ctx.run_operations(lena["img"], ops, connectivity=4)
+    assert_same(outs[0], lena["filter_area200"].astype(np.float32), "op filter area")
+    assert_same(outs[1], lena["filter_ncomp02"].astype(np.float32), "op filter ncomp")
+    assert_same(outs[2], lena["check_ncomp4"], "op check ncomp")
+    o = OracleTree(lena["img"], 4, "max", 0)
+    assert_same(outs[3], o.spectrum1d(0, lvec_read(lv)), "op spectrum")
+
+
+# ------------------------------------------------------------------------------------------------
+# seeded random images, every connectivity and both tree types
+# ------------------------------------------------------------------------------------------------
+def _compare_all(ctx, img, conn, tree_type, lam_area=20, lam_nc=0.25, label=""):
+    say("case", label, img.shape, img.dtype, conn, tree_type)
+    o = OracleTree(img, conn, tree_type, 0)
+    t = ctx.build(img, conn, tree_type, 0)
+    assert_same(t.parent(), o.parent().astype(np.int32), "parent " + label)
+    assert_same(t.check(0), o.check(0), "area " + label)
+    for dec in (0, 3):
+        assert_same(t.filter(0, lam_area, dec), o.filter(0, lam_area, dec), "filter area d%d %s" % (dec, label))
+        assert_same(t.filter(0, lam_area, dec, all_rules=True), o.filter(0, lam_area, dec), "general d%d %s" % (dec, label))
+    assert_same(t.filter(0, lam_area, 0, extract=True), o.filter(0, lam_area, 0, extract=True), "extract " + label)
+    lv = synth.log_lambdas(16, 1, 2 ** 14)
+    assert_same(t.spectrum1d(0, lv), o.spectrum1d(0, lv), "spectrum " + label)
+    o.reaccumulate(17)
+    t.accumulate(17)
+    assert_same(t.check(17), o.check(17), "ncomp " + label)
+    assert_same(t.check(13), o.check(13), "trace " + label)
+    for dec in (0, 3):
+        assert_same(t.filter(17, lam_nc, dec), o.filter(17, lam_nc, dec), "filter ncomp d%d %s" % (dec, label))
+    l1 = synth.log_lambdas(8, 0.05, 2.0)
+    l2 = synth.log_lambdas(8, 1, 4096)
+    assert_same(t.spectrum2d(17, l1, 0, l2), o.spectrum2d(17, l1, 0, l2), "spectrum2d " + label)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_2d(ctx, seed):
+    rng = np.random.default_rng(100 + seed)
+    H, W = int(rng.integers(1, 150)), int(rng.integers(1, 150))
+    levels = [2, 7, 256, 65536][seed % 4]
+    dt = np.uint8 if levels <= 256 else np.uint16
+    img = rng.integers(0, levels, size=(H, W)).astype(dt)
+    for conn in (4, 8):
+        for tt in ("max", "min"):
+            _compare_all(ctx, img, conn, tt, label="rand2d s%d" % seed)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_random_3d(ctx, seed):
+    rng = np.random.default_rng(200 + seed)
+    D, H, W = int(rng.integers(2, 20)), int(rng.integers(1, 40)), int(rng.integers(1, 40))
+    levels = [3, 16, 256, 1000][seed % 4]
+    dt = np.uint8 if levels <= 256 else np.uint16
+    img = rng.integers(0, levels, size=(D, H, W)).astype(dt)
+    for conn in (6, 26):
+        for tt in ("max", "min"):
+            _compare_all(ctx, img, conn, tt, label="rand3d s%d" % seed)
+
+
+EDGE = {
+    "1x1": np.array([[7]], np.uint8),
+    "1xN": (np.arange(300) * 7 % 251).astype(np.uint8).reshape(1, 300),
+    "Nx1": (np.arange(300) * 13 % 253).astype(np.uint8).reshape(300, 1),
+    "constant": np.full((97, 131), 42, np.uint8),
+    "ramp": (np.arange(130 * 140) % 65536).astype(np.uint16).reshape(130, 140),
+    "checker": ((np.indices((66, 67)).sum(0) % 2) * 200).astype(np.uint8),
+    "tile_edges": np.random.default_rng(5).integers(0, 5, size=(129, 65)).astype(np.uint8),
+    "zeros_and_max": np.where(np.random.default_rng(6).random((70, 70)) < 0.5, 0, 255).astype(np.uint8),
+}
+
+
+@pytest.mark.parametrize("name", sorted(EDGE))
+def test_edge_cases(ctx, name):
+    img = EDGE[name]
+    for conn in (4, 8):
+        _compare_all(ctx, img, conn, "max", lam_area=3, label=name)
+    _compare_all(ctx, img, 4, "min", lam_area=3, label=name + " min")
+
+
+def test_edge_3d_thin(ctx):
+    rng = np.random.default_rng(9)
+    for shape in [(3, 1, 1), (17, 1, 33), (2, 40, 1), (33, 17, 18)]:
+        img = rng.integers(0, 9, size=shape).astype(np.uint8)
+        for conn in (6, 26):
+            _compare_all(ctx, img, conn, "max", lam_area=4, label="thin3d %s" % (shape,))
+
+
+# ------------------------------------------------------------------------------------------------
+# the benchmark workloads (reduced sizes here; full sizes in test_gpu_fullsize.py)
+# ------------------------------------------------------------------------------------------------
+def test_workload_disks_1024(ctx):
+    img = synth.disks(1024, 1024)
+    say("disks 1024^2 u8 4-conn max, sha", synth.sha256(img)[:16])
+    o = OracleTree(img, 4, "max", 0)
+    t = ctx.build(img, 4, "max", 0)
+    assert_same(t.parent(), o.parent().astype(np.int32), "parent")
+    assert_same(t.filter(0, 100, 0), o.filter(0, 100, 0), "filter")
+    lv = np.zeros(32, np.float32)
+    lv[1:] = 2.0 ** np.arange(31)
+    assert_same(t.spectrum1d(0, lv), o.spectrum1d(0, lv), "spectrum")
+
+
+@pytest.mark.parametrize("tree_type", ["max", "min"])
+def test_workload_vnoise_1024(ctx, tree_type):
+    img = synth.vnoise(1024, 1024)
+    say("vnoise 1024^2 u16 8-conn", tree_type)
+    o = OracleTree(img, 8, tree_type, 0, threads=8)
+    t = ctx.build(img, 8, tree_type, 0)
+    assert_same(t.parent(), o.parent().astype(np.int32), "parent")
+    assert_same(t.filter(0, 1000, 3, out_dtype=np.uint16), o.filter(0, 1000, 3).astype(np.uint16), "filter u16 out")
+    assert_same(t.filter(0, 1000, 3, all_rules=True), o.filter(0, 1000, 3), "filter general")
+    lv = synth.log_lambdas(64, 1, 2 ** 26)
+    assert_same(t.spectrum1d(0, lv), o.spectrum1d(0, lv), "spectrum")
+
+
+def test_workload_spheres_64(ctx):
+    img = synth.spheres(64, 64, 64)
+    say("spheres 64^3 u8 6-conn")
+    o = OracleTree(img, 6, "max", 0, threads=8)
+    t = ctx.build(img, 6, "max", 0)
+    assert_same(t.parent(), o.parent().astype(np.int32), "parent")
+    assert_same(t.filter(0, 500, 0), o.filter(0, 500, 0), "filter")
+    lv = synth.log_lambdas(48, 1, 2 ** 27)
+    assert_same(t.spectrum1d(0, lv), o.spectrum1d(0, lv), "spectrum")
+
+
+def test_workload_blobs_512(ctx):
+    img = synth.blobs(512, 512)
+    say("blobs 512^2 u16 8-conn inertia")
+    o = OracleTree(img, 8, "max", 13, threads=8)
+    t = ctx.build(img, 8, "max", 13)
+    assert_same(t.parent(), o.parent().astype(np.int32), "parent")
+    assert_same(t.check(13), o.check(13), "check trace")
+    assert_same(t.check(17), o.check(17), "check ncomp")
+    l1 = synth.log_lambdas(32, 0.16, 100.0)
+    l2 = synth.log_lambdas(32, 1, 2 ** 24)
+    assert_same(t.spectrum2d(17, l1, 0, l2), o.spectrum2d(17, l1, 0, l2), "pattern2D")
+
+
+# ------------------------------------------------------------------------------------------------
+# error behaviour (status codes, no device fault)
+# ------------------------------------------------------------------------------------------------
+def test_errors(ctx, tmp_path):
+    img = np.zeros((8, 8), np.uint8)
+    with pytest.raises(dcf.DcfError) as e:
+        ctx.build(img, 5)
+    assert e.value.code == -1
+    with pytest.raises(dcf.DcfError) as e:
+        ctx.build(img, 4, "max", 18)
+    assert e.value.code == -1
+    t = ctx.build(img, 4, "max", 0)
+    with pytest.raises(dcf.DcfError) as e:
+        t.check(17)
+    assert e.value.code == -6
+    with pytest.raises(dcf.DcfError) as e:
+        t.filter(0, 5, 7)
+    assert e.value.code == -1
+    with pytest.raises(dcf.DcfError) as e:
+        t.spectrum2d(0, [0, 1], 17, [0, 1])
+    assert e.value.code == -1
+    with pytest.raises(dcf.DcfError) as e:
+        ctx.run_operations(img, [("pattern", 0, str(tmp_path / "none.txt"))])
+    assert e.value.code == -5
+    with pytest.raises(dcf.DcfError) as e:
+        ctx.run_operations(img, [("dmp", 0, "x")])
+    assert e.value.code == -4
