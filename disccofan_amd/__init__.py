@@ -18,6 +18,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -149,9 +150,12 @@ class Context:
         if rc != 0:
             raise DcfError(rc, "dcf_ctx_create(device=%d) failed (no GPU visible?)" % device)
         self._h = h
+        self._trees = weakref.WeakSet()
 
     def close(self):
         if getattr(self, "_h", None):
+            for t in list(self._trees):
+                t.free()
             lib().dcf_ctx_destroy(self._h)
             self._h = None
 
@@ -215,7 +219,9 @@ class Context:
         rc = lib().dcf_tree_build(self._h, p, code, d, int(connectivity), 1 if tree_type == "min" else 0,
                                   int(attr_idx), _ptr(pd), flags, C.byref(h))
         self._check(rc)
-        return Tree(self, h, shape, code, keep)
+        t = Tree(self, h, shape, code, keep)
+        self._trees.add(t)
+        return t
 
     def run_operations(self, img, operations, connectivity: int = 4, tree_type: str = "max", pixdim=(1.0, 1.0, 1.0)):
         """loop_operations (disccofan.c:137-205): run the reference's operation tuples in order.
@@ -263,7 +269,8 @@ class Tree:
 
     def free(self):
         if getattr(self, "_h", None):
-            lib().dcf_tree_free(self._h)
+            if getattr(self.ctx, "_h", None):  # a closed context already freed its trees
+                lib().dcf_tree_free(self._h)
             self._h = None
 
     def __del__(self):

@@ -181,11 +181,18 @@ extern "C" int dcf_ctx_create(int device, unsigned flags, dcf_ctx **out)
     return DCF_OK;
 }
 
+static void tree_release(dcf_tree *t);
+
 extern "C" int dcf_ctx_destroy(dcf_ctx *ctx)
 {
     if (!ctx) return DCF_EINVAL;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    for (dcf_tree *t : ctx->trees) {
+        tree_release(t);
+        delete t;
+    }
+    ctx->trees.clear();
     prof_collect(ctx);
     for (auto &e : ctx->event_pool) (void)hipEventDestroy(e);
     for (auto &f : ctx->free_blocks) (void)hipFree(f.second);
@@ -274,6 +281,7 @@ extern "C" int dcf_tree_free(dcf_tree *t)
 {
     if (!t) return DCF_EINVAL;
     (void)hipSetDevice(t->ctx->device);
+    t->ctx->trees.erase(t);
     tree_release(t);
     delete t;
     return DCF_OK;
@@ -338,6 +346,7 @@ extern "C" int dcf_tree_build(dcf_ctx *ctx, const void *img, dcf_dtype dtype, co
     if (t->tree_min && (rc = dcf_ops_minmax(t)) != DCF_OK) goto fail;
     if ((rc = dcf_build_tree(t)) != DCF_OK) goto fail;
     if ((rc = dcf_accumulate_group(t, dcf_group_of(attr_idx))) != DCF_OK) goto fail;
+    ctx->trees.insert(t);
     *out = t;
     return DCF_OK;
 fail:

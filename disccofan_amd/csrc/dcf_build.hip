@@ -6,12 +6,12 @@
  * a hierarchical bit queue and fuses slabs pairwise; here the image is cut into tiles that each
  * map to one workgroup:
  *
- *   k_tile_build   one tile in LDS: every intra-tile edge is merged with the lock-free zip of
- *                  dcf_zip.h (ds_cmpst_b64 on LDS link words), then the tile is canonicalised
- *                  and its link words are written to HBM once.
- *   k_seam         every edge that crosses a tile border is zipped in HBM (64-bit CAS at agent
- *                  scope): the on-GPU counterpart of the reference's boundary-tree merge
- *                  (src/sources/boundary.c:427-583), with all seams of all tiles in flight at once.
+ *   k_tile_build   one tile in LDS: the intra-tile edges are merged with the lock-free zip of
+ *                  dcf_zip.h (ds_cmpst_b64 on LDS link words) in a hierarchical seam order, then
+ *                  the tile is canonicalised and its link words are written to HBM once.
+ *   k_seam         the edges that cross tile borders are zipped in HBM (64-bit CAS at agent
+ *                  scope), one thread per tile face walking its seam in order: the on-GPU
+ *                  counterpart of the reference's boundary-tree merge (src/sources/boundary.c:427-583).
  *   k_canon        levelroot of every pixel with path halving, canonical parent array and
  *                  levelroot flags (the compression loop of flood.c:127-141).
  *   scan           node numbering (hipcub exclusive sum of the flags)
@@ -140,9 +140,18 @@ static Fwd make_fwd(int conn, int is3d)
 struct TileGeom {
     int W, H, D;
     int TX, TY, TZ;
-    int ntx, nty;
+    int ntx, nty, ntz;
 };
 
+/*
+ * Merge order inside a tile.  Zipping all edges at once is correct but makes every edge of a
+ * seam walk the full chain of its two blocks (O(edges x depth) work on smooth 16-bit tiles, whose
+ * trees are single chains thousands of nodes deep).  Instead the tile is merged hierarchically,
+ * one thread per seam walking its edges in seam order: the first zip of a seam interleaves the
+ * two chains, every later zip starts next to the previous one and meets the merged chain within
+ * a few nodes.  Rows first (x edges), then row blocks of 1, 2, 4 ... rows (y seams), then plane
+ * blocks (z seams); each level ends with a barrier.
+ */
 __global__ __launch_bounds__(TILE_THREADS) void k_tile_build(ImgView iv, uint64_t *__restrict__ link, TileGeom g, Fwd fwd)
 {
     __shared__ uint64_t slink[TILE_MAX];
@@ -154,40 +163,79 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_build(ImgView iv, uint64_
     const int x0 = tx * g.TX, y0 = ty * g.TY, z0 = tz * g.TZ;
     const int ex = min(g.TX, g.W - x0), ey = min(g.TY, g.H - y0), ez = min(g.TZ, g.D - z0);
     const int np = ex * ey * ez;
+    const int exy = ex * ey;
     const int64_t WH = (int64_t)g.W * g.H;
     const int64_t base = (int64_t)x0 + (int64_t)y0 * g.W + (int64_t)z0 * WH;
 
     for (int l = threadIdx.x; l < np; l += TILE_THREADS) {
-        int lx = l % ex, ly = (l / ex) % ey, lz = l / (ex * ey);
+        int lx = l % ex, ly = (l / ex) % ey, lz = l / exy;
         int64_t p = base + lx + (int64_t)ly * g.W + (int64_t)lz * WH;
         skey[l] = px_key(iv, p);
         slink[l] = dcf_make_link(0, DCF_NONE);
     }
     __syncthreads();
-
     LdsMem m{ slink };
-    for (int l = threadIdx.x; l < np; l += TILE_THREADS) {
-        int lx = l % ex, ly = (l / ex) % ey, lz = l / (ex * ey);
-        uint32_t kl = skey[l];
-        for (int k = 0; k < fwd.n; k++) {
-            int nx = lx + fwd.dx[k], ny = ly + fwd.dy[k], nz = lz + fwd.dz[k];
-            if (nx < 0 || nx >= ex || ny < 0 || ny >= ey || nz < 0 || nz >= ez) continue;
-            int nl = nx + ny * ex + nz * ex * ey;
-            dcf_zip(m, (uint32_t)l, kl, (uint32_t)nl, skey[nl]);
-        }
+
+    /* rows: one thread per row, sequential x edges */
+    for (int r = threadIdx.x; r < ey * ez; r += TILE_THREADS) {
+        int l0 = r * ex;
+        for (int x = 1; x < ex; x++) dcf_zip(m, (uint32_t)(l0 + x - 1), skey[l0 + x - 1], (uint32_t)(l0 + x), skey[l0 + x]);
     }
     __syncthreads();
+
+    /* y seams inside every plane: level lv joins row blocks [a, a+2^lv) and [a+2^lv, a+2^(lv+1)) */
+    for (int lv = 0; (1 << lv) < ey; lv++) {
+        const int step = 2 << lv, half = 1 << lv;
+        const int per_plane = (ey - half + step - 1) / step;
+        for (int s = threadIdx.x; s < per_plane * ez; s += TILE_THREADS) {
+            int lz = s / per_plane, ys = (s % per_plane) * step + half; /* first row of the upper block */
+            int lrow = lz * exy + (ys - 1) * ex;
+            for (int x = 0; x < ex; x++) {
+                int l = lrow + x;
+                for (int k = 0; k < fwd.n; k++) {
+                    if (fwd.dz[k] != 0 || fwd.dy[k] != 1) continue;
+                    int nx = x + fwd.dx[k];
+                    if (nx < 0 || nx >= ex) continue;
+                    int nl = l + ex + fwd.dx[k];
+                    dcf_zip(m, (uint32_t)l, skey[l], (uint32_t)nl, skey[nl]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    /* z seams: level lv joins plane blocks; one thread walks the whole face */
+    for (int lv = 0; (1 << lv) < ez; lv++) {
+        const int step = 2 << lv, half = 1 << lv;
+        const int nseam = (ez - half + step - 1) / step;
+        for (int s = threadIdx.x; s < nseam; s += TILE_THREADS) {
+            int zs = s * step + half;
+            int lpl = (zs - 1) * exy;
+            for (int y = 0; y < ey; y++)
+                for (int x = 0; x < ex; x++) {
+                    int l = lpl + y * ex + x;
+                    for (int k = 0; k < fwd.n; k++) {
+                        if (fwd.dz[k] != 1) continue;
+                        int nx = x + fwd.dx[k], ny = y + fwd.dy[k];
+                        if (nx < 0 || nx >= ex || ny < 0 || ny >= ey) continue;
+                        int nl = l + exy + fwd.dy[k] * ex + fwd.dx[k];
+                        dcf_zip(m, (uint32_t)l, skey[l], (uint32_t)nl, skey[nl]);
+                    }
+                }
+        }
+        __syncthreads();
+    }
 
     for (int l = threadIdx.x; l < np; l += TILE_THREADS) slr[l] = (uint16_t)find_levelroot(m, (uint32_t)l, skey[l]);
     __syncthreads();
 
     for (int l = threadIdx.x; l < np; l += TILE_THREADS) {
-        int lx = l % ex, ly = (l / ex) % ey, lz = l / (ex * ey);
+        int lx = l % ex, ly = (l / ex) % ey, lz = l / exy;
         int64_t p = base + lx + (int64_t)ly * g.W + (int64_t)lz * WH;
         uint64_t out;
         int r = slr[l];
         if (r != l) {
-            int rx = r % ex, ry = (r / ex) % ey, rz = r / (ex * ey);
+            int rx = r % ex, ry = (r / ex) % ey, rz = r / exy;
             out = dcf_make_link(skey[l], (uint32_t)(base + rx + (int64_t)ry * g.W + (int64_t)rz * WH));
         } else {
             uint64_t w = slink[l];
@@ -196,7 +244,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_build(ImgView iv, uint64_
                 out = dcf_make_link(0, DCF_NONE);
             } else {
                 int q = slr[z];
-                int qx = q % ex, qy = (q / ex) % ey, qz = q / (ex * ey);
+                int qx = q % ex, qy = (q / ex) % ey, qz = q / exy;
                 out = dcf_make_link(dcf_link_key(w), (uint32_t)(base + qx + (int64_t)qy * g.W + (int64_t)qz * WH));
             }
         }
@@ -205,28 +253,62 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile_build(ImgView iv, uint64_
 }
 
 /* ------------------------------------------------------------------------------------------------ */
-/* seam merge: zip every edge whose endpoints lie in different tiles                                */
+/* seam merge: the edges that cross tile borders                                                   */
 /* ------------------------------------------------------------------------------------------------ */
-__global__ __launch_bounds__(256) void k_seam(ImgView iv, uint64_t *__restrict__ link, TileGeom g, Fwd fwd, int64_t n)
+/*
+ * One thread per (tile, face) walks the face in order and zips every edge leaving the tile through
+ * it, so the edges of one seam are zipped one after the other (same argument as inside a tile).
+ * An edge belongs to the face of the first axis on which it changes tile (x, then y, then z), so
+ * every cross-tile edge is zipped exactly once.
+ */
+enum { FACE_X_LAST = 0, FACE_X_FIRST, FACE_Y_LAST, FACE_Y_FIRST, FACE_Z_LAST, NFACES };
+
+__device__ __forceinline__ int edge_face(int dx, int dy, int dz, bool cx, bool cy, bool cz)
 {
-    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
+    if (cx) return dx > 0 ? FACE_X_LAST : FACE_X_FIRST;
+    if (cy) return dy > 0 ? FACE_Y_LAST : FACE_Y_FIRST;
+    if (cz) return FACE_Z_LAST;
+    return -1;
+}
+
+__global__ __launch_bounds__(64) void k_seam(ImgView iv, uint64_t *__restrict__ link, TileGeom g, Fwd fwd, int64_t ntiles)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles * NFACES) return;
+    const int face = (int)(t % NFACES);
+    const int64_t tile = t / NFACES;
+    const int tx = (int)(tile % g.ntx), ty = (int)((tile / g.ntx) % g.nty), tz = (int)(tile / ((int64_t)g.ntx * g.nty));
+    const int x0 = tx * g.TX, y0 = ty * g.TY, z0 = tz * g.TZ;
+    const int ex = min(g.TX, g.W - x0), ey = min(g.TY, g.H - y0), ez = min(g.TZ, g.D - z0);
     const int64_t WH = (int64_t)g.W * g.H;
-    int x = (int)(p % g.W), y = (int)((p / g.W) % g.H), z = (int)(p / WH);
-    int lx = x % g.TX, ly = y % g.TY, lz = z % g.TZ;
-    bool interior = lx > 0 && lx < g.TX - 1 && ly > 0 && ly < g.TY - 1 && (g.D == 1 || (lz > 0 && lz < g.TZ - 1));
-    if (interior) return;
+    /* the face as a 2D range (u, v) -> (lx, ly, lz) */
+    int nu, nv;
+    if (face == FACE_X_LAST || face == FACE_X_FIRST) { nu = ey; nv = ez; }
+    else if (face == FACE_Y_LAST || face == FACE_Y_FIRST) { nu = ex; nv = ez; }
+    else { nu = ex; nv = ey; }
     GlobalMem m{ link };
-    uint32_t kp = 0;
-    bool have = false;
-    for (int k = 0; k < fwd.n; k++) {
-        int nx = x + fwd.dx[k], ny = y + fwd.dy[k], nz = z + fwd.dz[k];
-        if (nx < 0 || nx >= g.W || ny < 0 || ny >= g.H || nz < 0 || nz >= g.D) continue;
-        if (nx / g.TX == x / g.TX && ny / g.TY == y / g.TY && nz / g.TZ == z / g.TZ) continue;
-        int64_t q = (int64_t)nx + (int64_t)ny * g.W + (int64_t)nz * WH;
-        if (!have) { kp = px_key(iv, p); have = true; }
-        dcf_zip(m, (uint32_t)p, kp, (uint32_t)q, px_key(iv, q));
-    }
+    for (int v = 0; v < nv; v++)
+        for (int u = 0; u < nu; u++) {
+            int lx, ly, lz;
+            if (face == FACE_X_LAST) { lx = ex - 1; ly = u; lz = v; }
+            else if (face == FACE_X_FIRST) { lx = 0; ly = u; lz = v; }
+            else if (face == FACE_Y_LAST) { lx = u; ly = ey - 1; lz = v; }
+            else if (face == FACE_Y_FIRST) { lx = u; ly = 0; lz = v; }
+            else { lx = u; ly = v; lz = ez - 1; }
+            int x = x0 + lx, y = y0 + ly, z = z0 + lz;
+            int64_t p = (int64_t)x + (int64_t)y * g.W + (int64_t)z * WH;
+            uint32_t kp = 0;
+            bool have = false;
+            for (int k = 0; k < fwd.n; k++) {
+                int nx = x + fwd.dx[k], ny = y + fwd.dy[k], nz = z + fwd.dz[k];
+                if (nx < 0 || nx >= g.W || ny < 0 || ny >= g.H || nz < 0 || nz >= g.D) continue;
+                bool cx = nx / g.TX != tx, cy = ny / g.TY != ty, cz = nz / g.TZ != tz;
+                if (edge_face(fwd.dx[k], fwd.dy[k], fwd.dz[k], cx, cy, cz) != face) continue;
+                int64_t q = (int64_t)nx + (int64_t)ny * g.W + (int64_t)nz * WH;
+                if (!have) { kp = px_key(iv, p); have = true; }
+                dcf_zip(m, (uint32_t)p, kp, (uint32_t)q, px_key(iv, q));
+            }
+        }
 }
 
 /* ------------------------------------------------------------------------------------------------ */
@@ -402,13 +484,13 @@ int dcf_build_tree(dcf_tree *t)
         return dcf_set_error(ctx, DCF_EINVAL, "tile %dx%dx%d exceeds %d pixels", g.TX, g.TY, g.TZ, TILE_MAX);
     g.ntx = (g.W + g.TX - 1) / g.TX;
     g.nty = (g.H + g.TY - 1) / g.TY;
-    int ntz = (g.D + g.TZ - 1) / g.TZ;
-    int64_t ntiles = (int64_t)g.ntx * g.nty * ntz;
+    g.ntz = (g.D + g.TZ - 1) / g.TZ;
+    int64_t ntiles = (int64_t)g.ntx * g.nty * g.ntz;
     Fwd fwd = make_fwd(t->conn, is3d);
     ImgView iv = view_of(t);
 
     DCF_LAUNCH(ctx, "k_tile_build", k_tile_build, dim3((unsigned)ntiles), dim3(TILE_THREADS), 0, iv, t->link, g, fwd);
-    DCF_LAUNCH(ctx, "k_seam", k_seam, dim3(nblk(n, 256)), dim3(256), 0, iv, t->link, g, fwd, n);
+    DCF_LAUNCH(ctx, "k_seam", k_seam, dim3(nblk(ntiles * NFACES, 64)), dim3(64), 0, iv, t->link, g, fwd, ntiles);
 
     uint32_t *flag = (uint32_t *)dcf_alloc(ctx, (size_t)n * 4);
     if (!flag) return dcf_set_error(ctx, DCF_ENOMEM, "levelroot flags");

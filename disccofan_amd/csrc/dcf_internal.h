@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <map>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -63,8 +64,9 @@ struct dcf_ctx {
     std::vector<hipEvent_t> event_pool;
     std::map<std::string, KernelStat> stats;
     std::vector<std::string> names;
-    int tile_x = 64, tile_y = 64, tile_z = 1;      /* 2D tile */
-    int tile3_x = 16, tile3_y = 16, tile3_z = 16;  /* 3D tile */
+    std::set<dcf_tree *> trees; /* live trees, freed by dcf_ctx_destroy */
+    int tile_x = 32, tile_y = 32, tile_z = 1;      /* 2D tile */
+    int tile3_x = 8, tile3_y = 8, tile3_z = 16;    /* 3D tile */
 };
 
 struct dcf_tree {

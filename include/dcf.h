@@ -62,6 +62,7 @@ enum {
 
 /* ---- context ---------------------------------------------------------------------------------- */
 int dcf_ctx_create(int device, unsigned flags, dcf_ctx **out);
+/* frees the context's remaining trees too; their handles are invalid afterwards */
 int dcf_ctx_destroy(dcf_ctx *ctx);
 /* Use an existing HIP stream (hipStream_t passed as void*) for every launch of this context;
  * NULL restores the context's own stream. */
