@@ -30,7 +30,7 @@ HIP_FLAGS = [
     "-Wno-unused-command-line-argument",
 ]
 
-SOURCES = ["dcf_build.hip", "dcf_ops.hip", "dcf_api.cpp"]
+SOURCES = ["dcf_build.hip", "dcf_flood.hip", "dcf_ops.hip", "dcf_api.cpp"]
 
 
 def _run(cmd):

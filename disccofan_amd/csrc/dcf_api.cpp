@@ -275,6 +275,7 @@ static void tree_release(dcf_tree *t)
     dcf_release(ctx, t->ngrey);
     dcf_release(ctx, t->counters);
     dcf_free_attr(ctx, t->attr);
+    dcf_free_attr(ctx, t->attr_pix);
 }
 
 extern "C" int dcf_tree_free(dcf_tree *t)
@@ -338,14 +339,12 @@ extern "C" int dcf_tree_build(dcf_ctx *ctx, const void *img, dcf_dtype dtype, co
     }
     t->gmax = (float *)dcf_alloc(ctx, 16);
     t->counters = (uint32_t *)dcf_alloc(ctx, 64 * 4);
-    t->link = (uint64_t *)dcf_alloc(ctx, (size_t)n * 8);
     t->parent = (int32_t *)dcf_alloc(ctx, (size_t)n * 4);
     t->p2n = (uint32_t *)dcf_alloc(ctx, (size_t)n * 4);
-    if (!t->gmax || !t->counters || !t->link || !t->parent || !t->p2n) { rc = dcf_set_error(ctx, DCF_ENOMEM, "tree arrays"); goto fail; }
+    if (!t->gmax || !t->counters || !t->parent || !t->p2n) { rc = dcf_set_error(ctx, DCF_ENOMEM, "tree arrays"); goto fail; }
     /* find_extremas (preprocessing.c:7-30): only a min-tree reads gmax */
     if (t->tree_min && (rc = dcf_ops_minmax(t)) != DCF_OK) goto fail;
-    if ((rc = dcf_build_tree(t)) != DCF_OK) goto fail;
-    if ((rc = dcf_accumulate_group(t, dcf_group_of(attr_idx))) != DCF_OK) goto fail;
+    if ((rc = dcf_build_tree(t, dcf_group_of(attr_idx))) != DCF_OK) goto fail;
     ctx->trees.insert(t);
     *out = t;
     return DCF_OK;

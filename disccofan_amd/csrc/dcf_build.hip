@@ -350,6 +350,28 @@ __global__ __launch_bounds__(256) void k_nodes(ImgView iv, const int32_t *__rest
     ngrey[k] = px_grey(iv, p);
 }
 
+/* levelroot flags from a canonical parent array */
+__global__ __launch_bounds__(256) void k_flags(ImgView iv, const int32_t *__restrict__ parent, uint32_t *__restrict__ flag, int64_t n)
+{
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    int32_t r = parent[p];
+    flag[p] = (r < 0 || px_key(iv, r) != px_key(iv, p)) ? 1u : 0u;
+}
+
+/* node-indexed attribute records from the pixel-indexed ones of the levelroots */
+__global__ __launch_bounds__(256) void k_gather_attr(const uint32_t *__restrict__ flag, const uint32_t *__restrict__ p2n,
+                                                     DevAttr P, DevAttr A, int64_t n)
+{
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n || !flag[p]) return;
+    uint32_t k = p2n[p];
+    A.area[k] = P.area[p];
+    for (int i = 0; i < A.nsum; i++) A.s[i][k] = P.s[i][p];
+    if (A.has_ext)
+        for (int i = 0; i < 3; i++) { A.mn[i][k] = P.mn[i][p]; A.mx[i][k] = P.mx[i][p]; }
+}
+
 /* ------------------------------------------------------------------------------------------------ */
 /* attribute accumulation                                                                           */
 /* ------------------------------------------------------------------------------------------------ */
@@ -471,11 +493,84 @@ static ImgView view_of(dcf_tree *t)
     return iv;
 }
 
-int dcf_build_tree(dcf_tree *t)
+/* node numbering from levelroot flags: p2n (scan), K, node parent / grey; consumes flag */
+static int compact_nodes(dcf_tree *t, uint32_t *flag)
 {
     dcf_ctx *ctx = t->ctx;
     const int64_t n = t->n;
+    ImgView iv = view_of(t);
+    size_t need = 0;
+    DCF_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, need, flag, t->p2n, (int)n, ctx->stream));
+    void *scan_tmp = dcf_alloc(ctx, need > 0 ? need : 16);
+    if (!scan_tmp) return dcf_set_error(ctx, DCF_ENOMEM, "scan scratch");
+    {
+        hipEvent_t ea = nullptr, eb = nullptr;
+        if (ctx->profiling) dcf_prof_begin(ctx, "scan_nodes", &ea, &eb);
+        hipError_t se = hipcub::DeviceScan::ExclusiveSum(scan_tmp, need, flag, t->p2n, (int)n, ctx->stream);
+        if (ctx->profiling) dcf_prof_end(ctx, "scan_nodes", ea, eb);
+        dcf_release(ctx, scan_tmp);
+        if (se != hipSuccess) return dcf_set_error(ctx, DCF_EDEVICE, "scan: %s", hipGetErrorString(se));
+    }
+    uint32_t last[2];
+    DCF_HIP(ctx, hipMemcpyAsync(&last[0], t->p2n + (n - 1), 4, hipMemcpyDeviceToHost, ctx->stream));
+    DCF_HIP(ctx, hipMemcpyAsync(&last[1], flag + (n - 1), 4, hipMemcpyDeviceToHost, ctx->stream));
+    DCF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    t->K = (int64_t)last[0] + last[1];
+    dcf_release(ctx, t->npar);
+    dcf_release(ctx, t->ngrey);
+    t->npar = (int32_t *)dcf_alloc(ctx, (size_t)(t->K > 0 ? t->K : 1) * 4);
+    t->ngrey = (float *)dcf_alloc(ctx, (size_t)(t->K > 0 ? t->K : 1) * 4);
+    if (!t->npar || !t->ngrey) return dcf_set_error(ctx, DCF_ENOMEM, "node arrays");
+    DCF_LAUNCH(ctx, "k_nodes", k_nodes, dim3(nblk(n, 256)), dim3(256), 0, iv, t->parent, flag, t->p2n, t->npar, t->ngrey, n);
+    /* depth bound for the doubling loops: the number of distinct keys */
+    t->max_rounds = t->dtype == DCF_U8 ? 9 : (t->dtype == DCF_U16 ? 17 : 33);
+    return DCF_OK;
+}
+
+static int alloc_attr(dcf_ctx *ctx, DevAttr &a, int grp, int64_t K);
+static int leaffix_group(dcf_tree *t, int grp);
+
+/* u8/u16: flooding build (parent + pixel-indexed attributes), then node arrays and node attributes */
+static int flood_and_gather(dcf_tree *t, int grp, bool renumber)
+{
+    dcf_ctx *ctx = t->ctx;
+    const int64_t n = t->n;
+    int rc = dcf_flood_build(t, grp);
+    if (rc) { dcf_free_attr(ctx, t->attr_pix); return rc; }
+    uint32_t *flag = (uint32_t *)dcf_alloc(ctx, (size_t)n * 4);
+    if (!flag) { dcf_free_attr(ctx, t->attr_pix); return dcf_set_error(ctx, DCF_ENOMEM, "levelroot flags"); }
+    ImgView iv = view_of(t);
+    DCF_LAUNCH(ctx, "k_flags", k_flags, dim3(nblk(n, 256)), dim3(256), 0, iv, t->parent, flag, n);
+    if (renumber && (rc = compact_nodes(t, flag)) != DCF_OK) {
+        dcf_release(ctx, flag);
+        dcf_free_attr(ctx, t->attr_pix);
+        return rc;
+    }
+    dcf_free_attr(ctx, t->attr);
+    if ((rc = alloc_attr(ctx, t->attr, grp, t->K)) != DCF_OK) {
+        dcf_release(ctx, flag);
+        dcf_free_attr(ctx, t->attr_pix);
+        return rc;
+    }
+    DCF_LAUNCH(ctx, "k_gather_attr", k_gather_attr, dim3(nblk(n, 256)), dim3(256), 0, flag, t->p2n, t->attr_pix, t->attr, n);
+    dcf_release(ctx, flag);
+    dcf_free_attr(ctx, t->attr_pix);
+    t->grp = grp;
+    return DCF_OK;
+}
+
+int dcf_build_tree(dcf_tree *t, int grp)
+{
+    if (t->dtype != DCF_F32) return flood_and_gather(t, grp, true);
+
+    /* float images: tile zips + seam zips + canonicalisation, then the doubling leaffix */
+    dcf_ctx *ctx = t->ctx;
+    const int64_t n = t->n;
     const int is3d = t->dims[2] > 1;
+    if (!t->link) {
+        t->link = (uint64_t *)dcf_alloc(ctx, (size_t)n * 8);
+        if (!t->link) return dcf_set_error(ctx, DCF_ENOMEM, "link words");
+    }
     TileGeom g;
     g.W = (int)t->dims[0]; g.H = (int)t->dims[1]; g.D = (int)t->dims[2];
     if (is3d) { g.TX = ctx->tile3_x; g.TY = ctx->tile3_y; g.TZ = ctx->tile3_z; }
@@ -495,33 +590,16 @@ int dcf_build_tree(dcf_tree *t)
     uint32_t *flag = (uint32_t *)dcf_alloc(ctx, (size_t)n * 4);
     if (!flag) return dcf_set_error(ctx, DCF_ENOMEM, "levelroot flags");
     DCF_LAUNCH(ctx, "k_canon", k_canon, dim3(nblk(n, 256)), dim3(256), 0, iv, t->link, t->parent, flag, n);
-
-    size_t need = 0;
-    DCF_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, need, flag, t->p2n, (int)n, ctx->stream));
-    void *scan_tmp = dcf_alloc(ctx, need > 0 ? need : 16);
-    if (!scan_tmp) { dcf_release(ctx, flag); return dcf_set_error(ctx, DCF_ENOMEM, "scan scratch"); }
-    {
-        hipEvent_t ea = nullptr, eb = nullptr;
-        if (ctx->profiling) dcf_prof_begin(ctx, "scan_nodes", &ea, &eb);
-        hipError_t se = hipcub::DeviceScan::ExclusiveSum(scan_tmp, need, flag, t->p2n, (int)n, ctx->stream);
-        if (ctx->profiling) dcf_prof_end(ctx, "scan_nodes", ea, eb);
-        dcf_release(ctx, scan_tmp);
-        if (se != hipSuccess) { dcf_release(ctx, flag); return dcf_set_error(ctx, DCF_EDEVICE, "scan: %s", hipGetErrorString(se)); }
-    }
-    uint32_t last[2];
-    DCF_HIP(ctx, hipMemcpyAsync(&last[0], t->p2n + (n - 1), 4, hipMemcpyDeviceToHost, ctx->stream));
-    DCF_HIP(ctx, hipMemcpyAsync(&last[1], flag + (n - 1), 4, hipMemcpyDeviceToHost, ctx->stream));
-    DCF_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    t->K = (int64_t)last[0] + last[1];
-
-    t->npar = (int32_t *)dcf_alloc(ctx, (size_t)(t->K > 0 ? t->K : 1) * 4);
-    t->ngrey = (float *)dcf_alloc(ctx, (size_t)(t->K > 0 ? t->K : 1) * 4);
-    if (!t->npar || !t->ngrey) { dcf_release(ctx, flag); return dcf_set_error(ctx, DCF_ENOMEM, "node arrays"); }
-    DCF_LAUNCH(ctx, "k_nodes", k_nodes, dim3(nblk(n, 256)), dim3(256), 0, iv, t->parent, flag, t->p2n, t->npar, t->ngrey, n);
+    int rc = compact_nodes(t, flag);
     dcf_release(ctx, flag);
-    /* depth bound for the doubling loops: the number of distinct keys */
-    t->max_rounds = t->dtype == DCF_U8 ? 9 : (t->dtype == DCF_U16 ? 17 : 33);
-    return DCF_OK;
+    if (rc) return rc;
+    return leaffix_group(t, grp);
+}
+
+int dcf_accumulate_group(dcf_tree *t, int grp)
+{
+    if (t->dtype != DCF_F32) return flood_and_gather(t, grp, false);
+    return leaffix_group(t, grp);
 }
 
 void dcf_free_attr(dcf_ctx *ctx, DevAttr &a)
@@ -554,7 +632,7 @@ static int alloc_attr(dcf_ctx *ctx, DevAttr &a, int grp, int64_t K)
     return DCF_OK;
 }
 
-int dcf_accumulate_group(dcf_tree *t, int grp)
+static int leaffix_group(dcf_tree *t, int grp)
 {
     dcf_ctx *ctx = t->ctx;
     const int64_t K = t->K, n = t->n;

@@ -1,0 +1,711 @@
+/*
+ * dcf_flood.hip -- level-synchronous union-find construction of the canonical max-tree with its
+ * attributes, for u8 and u16 images.
+ *
+ * The reference floods pixels one at a time in decreasing (grey, index) order and merges the
+ * attribute of every finished pixel into its parent (tree_flood, src/sources/flood.c:222-247;
+ * the same order as flood_attributes, flood.c:184-219).  Here all pixels of one grey level are
+ * processed together:
+ *
+ *   k_union   every pixel of level k is united with its already-processed neighbours (level > k,
+ *             or level k with a larger index) in a lock-free union-find that always links the
+ *             root of higher priority (level, index) under the lower one.  A set's root is thus
+ *             its minimum-index pixel of the lowest level: the canonical levelroot.  Old roots
+ *             (from levels > k) that get linked are pushed on the `absorbed` list.
+ *   k_final   every pixel of level k takes find(p) as its parent when it is not the root, and
+ *             adds its own attribute contribution to its root; every absorbed old root takes the
+ *             new root as parent and merges its (complete) attributes into it.  Attributes are
+ *             therefore accumulated level by level, exactly like the reference's merges during
+ *             flooding, with no separate leaffix.
+ *
+ * The critical path is the number of levels, not the depth of the tree (u16 trees of smooth
+ * images are ~50k nodes deep).  16-bit keys use two passes of 256 levels:
+ *   pass 1  levels = key >> 8: the coarse tree (coarse levelroot cl(p), coarse parent, and the
+ *           exact attributes of every coarse node region);
+ *   pass 2  levels = key & 255 inside every coarse node N: its level region LR(N) = {p : cl(p) = N}
+ *           is refined, and every coarse child C of N enters N's union-find as ONE super-vertex
+ *           S_C (priority above all of LR(N), attributes = the totals of C's region).  A pixel of
+ *           LR(N) adjacent to a pixel q of a child region is united with S_C, C being the
+ *           ancestor of cl(q) whose coarse parent is N (binary lifting on the coarse tree).
+ *           When S_C is absorbed at fine level d, the fine node it joins is the parent of C's
+ *           lowest fine node (the final root of LR(C)).
+ * Pixels with equal keys are merged by the union-find itself (no chains), so flat zones cost
+ * nothing extra.
+ */
+#include <hipcub/hipcub.hpp>
+
+#include "dcf_internal.h"
+
+#define NLEV 256
+#define LIFT 8
+
+__device__ __forceinline__ uint32_t fl_key(const ImgView &iv, int64_t p)
+{
+    if (iv.dtype == DCF_U8) {
+        uint32_t v = ((const uint8_t *)iv.img)[p];
+        return iv.invert ? (uint32_t)(*iv.gmax) - v : v;
+    }
+    uint32_t v = ((const uint16_t *)iv.img)[p];
+    return iv.invert ? (uint32_t)(*iv.gmax) - v : v;
+}
+
+__device__ __forceinline__ float fl_grey(const ImgView &iv, int64_t p) { return (float)fl_key(iv, p); }
+
+struct Nbr {
+    int n;
+    int8_t dx[26], dy[26], dz[26];
+};
+
+struct Geo {
+    int W, H, D;
+    int64_t n;
+};
+
+/* ------------------------------------------------------------------------------------------------ */
+/* union-find                                                                                       */
+/* ------------------------------------------------------------------------------------------------ */
+__device__ __forceinline__ uint32_t ld(const uint32_t *a, uint32_t i)
+{
+    return __hip_atomic_load(&a[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint32_t uf_find(uint32_t *uf, uint32_t x)
+{
+    uint32_t p = ld(uf, x);
+    while (p != x) {
+        uint32_t gp = ld(uf, p);
+        if (gp != p) __hip_atomic_store(&uf[x], gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); /* halving */
+        x = p;
+        p = gp;
+    }
+    return x;
+}
+
+/* priority of a union-find element in the current pass: (level, id); pixels of the current pass
+ * have level = their digit, super-vertices have level NLEV (above every pixel) */
+struct Prio {
+    const ImgView *iv;
+    int pass;  /* 1: digit = key >> shift, 2: digit = key & 255 */
+    int shift;
+    uint32_t n;
+    __device__ uint32_t level(uint32_t x) const
+    {
+        if (x >= n) return NLEV;
+        uint32_t k = fl_key(*iv, x);
+        return pass == 1 ? (k >> shift) : (k & 255u);
+    }
+};
+
+/* unite the sets of a and b; when the root of higher priority gets linked and it belongs to a
+ * level above `lev`, it is an old root completed earlier: record it */
+__device__ void uf_union(uint32_t *uf, const Prio &pr, uint32_t a, uint32_t b, uint32_t lev, uint32_t *absorbed,
+                         uint32_t *nabs)
+{
+    for (;;) {
+        a = uf_find(uf, a);
+        b = uf_find(uf, b);
+        if (a == b) return;
+        uint32_t la = pr.level(a), lb = pr.level(b);
+        /* hi = the root that goes under the other */
+        bool a_hi = la > lb || (la == lb && a > b);
+        uint32_t hi = a_hi ? a : b, lo = a_hi ? b : a;
+        uint32_t lh = a_hi ? la : lb;
+        uint32_t exp = hi;
+        if (__hip_atomic_compare_exchange_strong(&uf[hi], &exp, lo, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT)) {
+            if (lh > lev) {
+                uint32_t slot = atomicAdd(nabs, 1u);
+                absorbed[slot] = hi;
+            }
+            return;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------------------------------ */
+/* counting sort of the pixels by level digit                                                       */
+/* ------------------------------------------------------------------------------------------------ */
+__global__ __launch_bounds__(256) void k_lvl_hist(ImgView iv, int pass, int shift, int64_t n, uint32_t *__restrict__ hist)
+{
+    __shared__ uint32_t sh[NLEV];
+    sh[threadIdx.x] = 0;
+    __syncthreads();
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t k = fl_key(iv, p);
+        uint32_t d = pass == 1 ? (k >> shift) : (k & 255u);
+        atomicAdd(&sh[d], 1u);
+    }
+    __syncthreads();
+    if (sh[threadIdx.x]) atomicAdd(&hist[threadIdx.x], sh[threadIdx.x]);
+}
+
+/* each block owns a contiguous pixel range; one global atomic per (block, level) reserves the
+ * block's slots, so every level's list is ordered by block (near raster order) */
+__global__ __launch_bounds__(256) void k_lvl_scatter(ImgView iv, int pass, int shift, int64_t n, int64_t per_block,
+                                                     uint32_t *__restrict__ cursor, uint32_t *__restrict__ order)
+{
+    __shared__ uint32_t cnt[NLEV], base[NLEV];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    int64_t p0 = (int64_t)blockIdx.x * per_block, p1 = min(p0 + per_block, n);
+    for (int64_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+        uint32_t k = fl_key(iv, p);
+        atomicAdd(&cnt[pass == 1 ? (k >> shift) : (k & 255u)], 1u);
+    }
+    __syncthreads();
+    if (cnt[threadIdx.x]) base[threadIdx.x] = atomicAdd(&cursor[threadIdx.x], cnt[threadIdx.x]);
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    for (int64_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+        uint32_t k = fl_key(iv, p);
+        uint32_t d = pass == 1 ? (k >> shift) : (k & 255u);
+        uint32_t r = atomicAdd(&cnt[d], 1u);
+        order[base[d] + r] = (uint32_t)p;
+    }
+}
+
+/* ------------------------------------------------------------------------------------------------ */
+/* per-level kernels                                                                                */
+/* ------------------------------------------------------------------------------------------------ */
+struct LevelArgs {
+    ImgView iv;
+    Geo g;
+    Nbr nb;
+    int pass, shift;
+    uint32_t *uf;
+    uint32_t *absorbed;
+    uint32_t *nabs;        /* [0] running count */
+    /* pass 2 only */
+    const uint32_t *cl;    /* coarse levelroot pixel of every pixel */
+    const uint32_t *cid;   /* coarse node id of a coarse levelroot pixel */
+    const int32_t *lift;   /* LIFT x Kc: 2^j-th coarse ancestor (node id), -1 above the root */
+    const uint32_t *cnkey; /* coarse key of every coarse node */
+};
+
+__device__ __forceinline__ void coords(const Geo &g, int64_t p, int &x, int &y, int &z)
+{
+    int64_t WH = (int64_t)g.W * g.H;
+    x = (int)(p % g.W);
+    y = (int)((p / g.W) % g.H);
+    z = (int)(p / WH);
+}
+
+/* the ancestor of coarse node e whose coarse parent is the node of coarse key kn (e's coarse
+ * key is above kn): the highest ancestor whose key is still above kn */
+__device__ __forceinline__ uint32_t child_toward(const LevelArgs &a, uint32_t e, uint32_t kn, uint32_t kc_count)
+{
+    for (int j = LIFT - 1; j >= 0; j--) {
+        int32_t u = a.lift[(int64_t)j * kc_count + e];
+        if (u >= 0 && a.cnkey[u] > kn) e = (uint32_t)u;
+    }
+    return e;
+}
+
+__global__ __launch_bounds__(256) void k_union(LevelArgs a, const uint32_t *__restrict__ lvl, uint32_t cnt, uint32_t lev,
+                                               uint32_t kc_count)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cnt) return;
+    uint32_t p = lvl[i];
+    Prio pr{ &a.iv, a.pass, a.shift, (uint32_t)a.g.n };
+    int x, y, z;
+    coords(a.g, p, x, y, z);
+    const int64_t WH = (int64_t)a.g.W * a.g.H;
+    uint32_t kp = fl_key(a.iv, p);
+    uint32_t cp = kp >> 8; /* pass 2: coarse key */
+    uint32_t np_ = 0, kn = 0;
+    if (a.pass == 2) {
+        np_ = a.cl[p];
+        kn = a.cnkey[a.cid[np_]];
+    }
+    for (int k = 0; k < a.nb.n; k++) {
+        int nx = x + a.nb.dx[k], ny = y + a.nb.dy[k], nz = z + a.nb.dz[k];
+        if (nx < 0 || nx >= a.g.W || ny < 0 || ny >= a.g.H || nz < 0 || nz >= a.g.D) continue;
+        uint32_t q = (uint32_t)(nx + (int64_t)ny * a.g.W + nz * WH);
+        uint32_t kq = fl_key(a.iv, q);
+        if (a.pass == 1) {
+            uint32_t dq = kq >> a.shift;
+            if (dq > lev || (dq == lev && q > p)) uf_union(a.uf, pr, p, q, lev, a.absorbed, a.nabs);
+        } else {
+            uint32_t cq = kq >> 8;
+            if (cq == cp) {
+                uint32_t dq = kq & 255u;
+                if (dq > lev || (dq == lev && q > p)) uf_union(a.uf, pr, p, q, lev, a.absorbed, a.nabs);
+            } else if (cq > cp) {
+                /* q lies in the region of a coarse child of N = cl(p): unite with its super-vertex */
+                uint32_t e = a.cid[a.cl[q]];
+                uint32_t c = child_toward(a, e, kn, kc_count);
+                uf_union(a.uf, pr, p, (uint32_t)a.g.n + c, lev, a.absorbed, a.nabs);
+            }
+        }
+    }
+    (void)np_;
+}
+
+/* ------------------------------------------------------------------------------------------------ */
+/* wave-aggregated attribute updates                                                                */
+/* ------------------------------------------------------------------------------------------------ */
+/* All pixels of one level that fall in the same component add into the same root record; plain
+ * atomics would serialise on that address (a 512^3 level is one component of ~500k voxels).  A
+ * wavefront first groups its lanes by target root (ballot on the leader's root), reduces each
+ * group with cross-lane shuffles and issues one atomic per group and field.  All 64 lanes must
+ * call these functions (inactive lanes pass valid = false). */
+__device__ __forceinline__ uint64_t wsum64(uint64_t v)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        uint32_t lo = __shfl_xor((uint32_t)v, o, 64);
+        uint32_t hi = __shfl_xor((uint32_t)(v >> 32), o, 64);
+        v += ((uint64_t)hi << 32) | lo;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wmin32(uint32_t v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wmax32(uint32_t v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor(v, o, 64));
+    return v;
+}
+
+/* one lane's contribution: count, up to DCF_MAX_SUMS sums, 3 minima and 3 maxima */
+struct Contrib {
+    uint32_t area;
+    int64_t s[DCF_MAX_SUMS];
+    uint32_t mn[3], mx[3];
+};
+
+__device__ __forceinline__ void contrib_own(const LevelArgs &a, int grp, uint32_t p, Contrib &c)
+{
+    c.area = 1;
+    if (grp == 0) return;
+    int x, y, z;
+    coords(a.g, p, x, y, z);
+    if (grp == 1) {
+        c.mn[0] = (uint32_t)x; c.mn[1] = (uint32_t)y; c.mn[2] = (uint32_t)z;
+        c.mx[0] = (uint32_t)x + 1; c.mx[1] = (uint32_t)y + 1; c.mx[2] = (uint32_t)z + 1;
+    } else if (grp == 2) {
+        c.s[0] = x; c.s[1] = y; c.s[2] = z;
+    } else if (grp == 3) {
+        int64_t gv = (int64_t)fl_key(a.iv, p);
+        c.s[0] = gv; c.s[1] = gv * x; c.s[2] = gv * y; c.s[3] = gv * z;
+    } else {
+        c.s[0] = x; c.s[1] = y; c.s[2] = z;
+        c.s[3] = (int64_t)x * x; c.s[4] = (int64_t)y * y; c.s[5] = (int64_t)z * z;
+        c.s[6] = (int64_t)x * y; c.s[7] = (int64_t)y * z; c.s[8] = (int64_t)x * z;
+    }
+}
+
+__device__ __forceinline__ void contrib_rec(const DevAttr &B, uint32_t h, Contrib &c)
+{
+    c.area = B.area[h];
+    for (int i = 0; i < B.nsum; i++) c.s[i] = B.s[i][h];
+    if (B.has_ext)
+        for (int i = 0; i < 3; i++) { c.mn[i] = B.mn[i][h]; c.mx[i] = B.mx[i][h]; }
+}
+
+__device__ __forceinline__ void contrib_zero(Contrib &c)
+{
+    c.area = 0;
+    for (int i = 0; i < DCF_MAX_SUMS; i++) c.s[i] = 0;
+    for (int i = 0; i < 3; i++) { c.mn[i] = 0xFFFFFFFFu; c.mx[i] = 0; }
+}
+
+__device__ void wave_add(const DevAttr &A, bool valid, uint32_t r, const Contrib &c)
+{
+    const int lane = threadIdx.x & 63;
+    uint64_t pending = __ballot(valid);
+    while (pending) {
+        int leader = __ffsll((unsigned long long)pending) - 1;
+        uint32_t rl = __shfl(r, leader, 64);
+        bool mine = ((pending >> lane) & 1ull) && r == rl;
+        uint64_t m = __ballot(mine);
+        if (__popcll(m) == 1) {
+            /* the leader's root is unique in the wave: the remaining roots are likely distinct too,
+             * so every pending lane updates its own root directly */
+            if ((pending >> lane) & 1ull) {
+                atomicAdd(&A.area[r], c.area);
+                for (int i = 0; i < A.nsum; i++)
+                    if (c.s[i]) atomicAdd((unsigned long long *)&A.s[i][r], (unsigned long long)c.s[i]);
+                if (A.has_ext)
+                    for (int i = 0; i < 3; i++) { atomicMin(&A.mn[i][r], c.mn[i]); atomicMax(&A.mx[i][r], c.mx[i]); }
+            }
+            break;
+        }
+        uint64_t area = wsum64(mine ? c.area : 0);
+        if (lane == leader) atomicAdd(&A.area[rl], (uint32_t)area);
+        for (int i = 0; i < A.nsum; i++) {
+            uint64_t v = wsum64(mine ? (uint64_t)c.s[i] : 0);
+            if (lane == leader && v) atomicAdd((unsigned long long *)&A.s[i][rl], (unsigned long long)v);
+        }
+        if (A.has_ext)
+            for (int i = 0; i < 3; i++) {
+                uint32_t lo = wmin32(mine ? c.mn[i] : 0xFFFFFFFFu), hi = wmax32(mine ? c.mx[i] : 0u);
+                if (lane == leader) { atomicMin(&A.mn[i][rl], lo); atomicMax(&A.mx[i][rl], hi); }
+            }
+        pending &= ~m;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_final_pixels(LevelArgs a, const uint32_t *__restrict__ lvl, uint32_t cnt,
+                                                      DevAttr A, int grp, int32_t *__restrict__ par, uint32_t *__restrict__ mark)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    /* the level's unions are complete: its absorbed entries end at the current count */
+    if (i == 0) *mark = *a.nabs;
+    bool valid = i < cnt;
+    uint32_t r = 0;
+    Contrib c;
+    contrib_zero(c);
+    if (valid) {
+        uint32_t p = lvl[i];
+        r = uf_find(a.uf, p);
+        if (r != p) par[p] = (int32_t)r;
+        contrib_own(a, grp, p, c);
+    }
+    wave_add(A, valid, r, c);
+}
+
+/* absorbed old roots of this level: parent + attribute merge.  Pixel roots carry their complete
+ * attributes in A; super-vertices (id >= n) carry the coarse totals in S and record their
+ * joining node in spar. */
+__global__ __launch_bounds__(256) void k_final_absorbed(LevelArgs a, const uint32_t *__restrict__ marks, DevAttr A, DevAttr S,
+                                                        int32_t *__restrict__ par, int32_t *__restrict__ spar)
+{
+    const uint32_t start = marks[0], end = marks[1];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    /* every lane of a wave runs the same number of iterations so wave_add sees the full wave */
+    for (uint32_t base = start + blockIdx.x * blockDim.x; base < end; base += stride) {
+        uint32_t i = base + threadIdx.x;
+        bool valid = i < end;
+        uint32_t r = 0;
+        Contrib c;
+        contrib_zero(c);
+        if (valid) {
+            uint32_t h = a.absorbed[i];
+            r = uf_find(a.uf, h);
+            if (h < (uint32_t)a.g.n) {
+                par[h] = (int32_t)r;
+                contrib_rec(A, h, c);
+            } else {
+                uint32_t cc = h - (uint32_t)a.g.n;
+                spar[cc] = (int32_t)r;
+                contrib_rec(S, cc, c);
+            }
+        }
+        wave_add(A, valid, r, c);
+    }
+}
+
+/* ------------------------------------------------------------------------------------------------ */
+/* between the passes                                                                               */
+/* ------------------------------------------------------------------------------------------------ */
+__global__ __launch_bounds__(256) void k_iota(uint32_t *a, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = (uint32_t)i;
+}
+
+__global__ __launch_bounds__(256) void k_fill_i32(int32_t *a, int32_t v, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+__global__ __launch_bounds__(256) void k_fill_u32f(uint32_t *a, uint32_t v, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+__global__ __launch_bounds__(256) void k_fill_i64f(int64_t *a, int64_t v, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+/* coarse levelroot of every pixel: par1 holds the levelroot for non-roots, -1 for roots */
+__global__ __launch_bounds__(256) void k_coarse_cl(const int32_t *__restrict__ par1, uint32_t *__restrict__ cl,
+                                                   uint32_t *__restrict__ flag, int64_t n)
+{
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    int32_t r = par1[p];
+    /* par1 of a coarse levelroot is its coarse parent (a pixel of lower coarse key) or -1; of any
+     * other pixel it is its levelroot (same coarse key).  The caller passes the key test result
+     * through flag on input: 1 for levelroots. */
+    cl[p] = flag[p] ? (uint32_t)p : (uint32_t)r;
+}
+
+__global__ __launch_bounds__(256) void k_coarse_flag(ImgView iv, const int32_t *__restrict__ par1, uint32_t *__restrict__ flag,
+                                                     int64_t n)
+{
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    int32_t r = par1[p];
+    flag[p] = (r < 0 || (fl_key(iv, r) >> 8) != (fl_key(iv, p) >> 8)) ? 1u : 0u;
+}
+
+/* per coarse node: key, parent id (lift level 0) and the region totals used by its super-vertex */
+__global__ __launch_bounds__(256) void k_coarse_nodes(ImgView iv, const int32_t *__restrict__ par1,
+                                                      const uint32_t *__restrict__ flag, const uint32_t *__restrict__ cidx,
+                                                      uint32_t *__restrict__ cnkey, int32_t *__restrict__ lift0,
+                                                      uint32_t *__restrict__ cpix, DevAttr A, DevAttr S, int64_t n)
+{
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n || !flag[p]) return;
+    uint32_t c = cidx[p];
+    int32_t r = par1[p];
+    cnkey[c] = fl_key(iv, p) >> 8;
+    lift0[c] = r < 0 ? -1 : (int32_t)cidx[r];
+    cpix[c] = (uint32_t)p;
+    S.area[c] = A.area[p];
+    for (int i = 0; i < A.nsum; i++) S.s[i][c] = A.s[i][p];
+    if (A.has_ext)
+        for (int i = 0; i < 3; i++) { S.mn[i][c] = A.mn[i][p]; S.mx[i][c] = A.mx[i][p]; }
+}
+
+__global__ __launch_bounds__(256) void k_lift(int32_t *__restrict__ lift, int j, int64_t kc)
+{
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= kc) return;
+    int32_t u = lift[(j - 1) * kc + c];
+    lift[j * kc + c] = u < 0 ? -1 : lift[(j - 1) * kc + u];
+}
+
+/* after pass 2: the lowest fine node of every coarse node hangs below the fine node its
+ * super-vertex joined (or is the root) */
+__global__ __launch_bounds__(256) void k_fine_roots(uint32_t *uf, const uint32_t *__restrict__ cpix,
+                                                    const int32_t *__restrict__ spar, int32_t *__restrict__ par, int64_t kc)
+{
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= kc) return;
+    uint32_t fr = uf_find(uf, cpix[c]);
+    par[fr] = spar[c];
+}
+
+/* ------------------------------------------------------------------------------------------------ */
+/* host side                                                                                        */
+/* ------------------------------------------------------------------------------------------------ */
+static inline unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+static Nbr make_nbr(int conn, int is3d)
+{
+    Nbr f;
+    f.n = 0;
+    for (int z = -1; z <= 1; z++)
+        for (int y = -1; y <= 1; y++)
+            for (int x = -1; x <= 1; x++) {
+                if (!x && !y && !z) continue;
+                if (!is3d && z) continue;
+                int l1 = abs(x) + abs(y) + abs(z);
+                bool ok = (!is3d) ? (conn == 8 || l1 == 1) : (conn == 26 || l1 == 1);
+                if (!ok) continue;
+                f.dx[f.n] = (int8_t)x; f.dy[f.n] = (int8_t)y; f.dz[f.n] = (int8_t)z;
+                f.n++;
+            }
+    return f;
+}
+
+struct Scratch {
+    dcf_ctx *ctx;
+    std::vector<void *> ptrs;
+    void *get(size_t bytes)
+    {
+        void *p = dcf_alloc(ctx, bytes);
+        if (p) ptrs.push_back(p);
+        return p;
+    }
+    ~Scratch()
+    {
+        for (void *p : ptrs) dcf_release(ctx, p);
+    }
+};
+
+static int alloc_attr_n(dcf_ctx *ctx, Scratch *sc, DevAttr &a, int grp, int64_t n)
+{
+    a = DevAttr();
+    a.nsum = grp == 2 ? 3 : (grp == 3 ? 4 : (grp == 4 ? 9 : 0));
+    a.has_ext = grp == 1;
+    size_t k = (size_t)(n > 0 ? n : 1);
+    auto get = [&](size_t b) -> void * { return sc ? sc->get(b) : dcf_alloc(ctx, b); };
+    bool ok = (a.area = (uint32_t *)get(k * 4)) != nullptr;
+    for (int i = 0; ok && i < a.nsum; i++) ok = (a.s[i] = (int64_t *)get(k * 8)) != nullptr;
+    for (int i = 0; ok && a.has_ext && i < 3; i++) {
+        ok = (a.mn[i] = (uint32_t *)get(k * 4)) != nullptr;
+        ok = ok && (a.mx[i] = (uint32_t *)get(k * 4)) != nullptr;
+    }
+    if (!ok) return dcf_set_error(ctx, DCF_ENOMEM, "attribute arrays");
+    return DCF_OK;
+}
+
+static int clear_attr(dcf_ctx *ctx, DevAttr &a, int64_t n)
+{
+    DCF_LAUNCH(ctx, "k_fill", k_fill_u32f, dim3(nblk(n, 256)), dim3(256), 0, a.area, 0u, n);
+    for (int i = 0; i < a.nsum; i++) DCF_LAUNCH(ctx, "k_fill", k_fill_i64f, dim3(nblk(n, 256)), dim3(256), 0, a.s[i], (int64_t)0, n);
+    if (a.has_ext)
+        for (int i = 0; i < 3; i++) {
+            DCF_LAUNCH(ctx, "k_fill", k_fill_u32f, dim3(nblk(n, 256)), dim3(256), 0, a.mn[i], 0xFFFFFFFFu, n);
+            DCF_LAUNCH(ctx, "k_fill", k_fill_u32f, dim3(nblk(n, 256)), dim3(256), 0, a.mx[i], 0u, n);
+        }
+    return DCF_OK;
+}
+
+/* one flooding pass over all levels (high to low) */
+static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAttr &S, int grp, int32_t *par, int32_t *spar,
+                    int64_t kc)
+{
+    const int64_t n = la.g.n;
+    uint32_t *hist = (uint32_t *)sc.get(2 * NLEV * 4);
+    uint32_t *order = (uint32_t *)sc.get((size_t)n * 4);
+    if (!hist || !order) return dcf_set_error(ctx, DCF_ENOMEM, "level lists");
+    uint32_t *cursor = hist + NLEV;
+    DCF_LAUNCH(ctx, "k_fill", k_fill_u32f, dim3(2), dim3(256), 0, hist, 0u, (int64_t)2 * NLEV);
+    unsigned hb = nblk(n, 256);
+    if (hb > 2048) hb = 2048;
+    DCF_LAUNCH(ctx, "k_lvl_hist", k_lvl_hist, dim3(hb), dim3(256), 0, la.iv, la.pass, la.shift, n, hist);
+    uint32_t h[NLEV];
+    DCF_HIP(ctx, hipMemcpyAsync(h, hist, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+    DCF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    uint32_t off[NLEV + 1];
+    off[0] = 0;
+    for (int i = 0; i < NLEV; i++) off[i + 1] = off[i] + h[i];
+    DCF_HIP(ctx, hipMemcpyAsync(cursor, off, NLEV * 4, hipMemcpyHostToDevice, ctx->stream));
+    int64_t per_block = 4096;
+    DCF_LAUNCH(ctx, "k_lvl_scatter", k_lvl_scatter, dim3(nblk(n, (int)per_block)), dim3(256), 0, la.iv, la.pass, la.shift, n,
+               per_block, cursor, order);
+    /* nabs[0]: running count of absorbed roots; nabs[1 + i]: where the i-th processed level's
+     * entries start (written on the device, so the level loop never waits for the host) */
+    uint32_t *nabs = la.nabs;
+    DCF_LAUNCH(ctx, "k_fill", k_fill_u32f, dim3(2), dim3(256), 0, nabs, 0u, (int64_t)(NLEV + 2));
+    int ord = 0;
+    for (int lev = NLEV - 1; lev >= 0; lev--) {
+        uint32_t cnt = h[lev];
+        if (!cnt) continue;
+        const uint32_t *lvl = order + off[lev];
+        DCF_LAUNCH(ctx, "k_union", k_union, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, cnt, (uint32_t)lev, (uint32_t)kc);
+        DCF_LAUNCH(ctx, "k_final_pixels", k_final_pixels, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, cnt, A, grp, par,
+                   nabs + 2 + ord);
+        unsigned ab = nblk((int64_t)cnt * la.nb.n, 256);
+        if (ab > 4096) ab = 4096;
+        DCF_LAUNCH(ctx, "k_final_absorbed", k_final_absorbed, dim3(ab), dim3(256), 0, la, (const uint32_t *)(nabs + 1 + ord), A,
+                   S, par, spar);
+        ord++;
+    }
+    return DCF_OK;
+}
+
+int dcf_flood_build(dcf_tree *t, int grp)
+{
+    dcf_ctx *ctx = t->ctx;
+    const int64_t n = t->n;
+    const int is3d = t->dims[2] > 1;
+    Scratch sc{ ctx, {} };
+    LevelArgs la;
+    la.iv.img = t->img;
+    la.iv.dtype = t->dtype;
+    la.iv.invert = t->tree_min;
+    la.iv.gmax = t->gmax;
+    la.g.W = (int)t->dims[0];
+    la.g.H = (int)t->dims[1];
+    la.g.D = (int)t->dims[2];
+    la.g.n = n;
+    la.nb = make_nbr(t->conn, is3d);
+    const bool two = t->dtype == DCF_U16;
+    int64_t abs_cap = n + 16;
+
+    /* pixel-indexed attribute records (valid at roots) */
+    DevAttr A;
+    int rc = alloc_attr_n(ctx, &sc, A, grp, n);
+    if (rc) return rc;
+    la.uf = (uint32_t *)sc.get((size_t)n * 4);
+    la.absorbed = (uint32_t *)sc.get((size_t)abs_cap * 4);
+    la.nabs = (uint32_t *)sc.get((NLEV + 4) * 4);
+    int32_t *par1 = two ? (int32_t *)sc.get((size_t)n * 4) : t->parent;
+    if (!la.uf || !la.absorbed || !la.nabs || !par1) return dcf_set_error(ctx, DCF_ENOMEM, "flood scratch");
+    la.cl = nullptr; la.cid = nullptr; la.lift = nullptr; la.cnkey = nullptr;
+
+    /* ---- pass 1 ---- */
+    la.pass = 1;
+    la.shift = two ? 8 : 0;
+    if ((rc = clear_attr(ctx, A, n))) return rc;
+    DCF_LAUNCH(ctx, "k_iota", k_iota, dim3(nblk(n, 256)), dim3(256), 0, la.uf, n);
+    DCF_LAUNCH(ctx, "k_fill", k_fill_i32, dim3(nblk(n, 256)), dim3(256), 0, par1, -1, n);
+    DevAttr none;
+    if ((rc = run_pass(ctx, la, sc, A, none, grp, par1, nullptr, 0))) return rc;
+    /* roots of pass 1 kept par1 = -1 except non-roots (levelroot) and absorbed roots (parent) */
+
+    if (!two) {
+        t->attr_pix = A;
+        /* hand the arrays over to the tree: detach them from the scratch list */
+        for (auto it = sc.ptrs.begin(); it != sc.ptrs.end();) {
+            bool keep = *it == A.area;
+            for (int i = 0; i < A.nsum; i++) keep = keep || *it == A.s[i];
+            for (int i = 0; A.has_ext && i < 3; i++) keep = keep || *it == A.mn[i] || *it == A.mx[i];
+            if (keep) it = sc.ptrs.erase(it);
+            else ++it;
+        }
+        return DCF_OK;
+    }
+
+    /* ---- coarse nodes ---- */
+    uint32_t *flag = (uint32_t *)sc.get((size_t)n * 4);
+    uint32_t *cidx = (uint32_t *)sc.get((size_t)n * 4);
+    uint32_t *cl = (uint32_t *)sc.get((size_t)n * 4);
+    if (!flag || !cidx || !cl) return dcf_set_error(ctx, DCF_ENOMEM, "coarse arrays");
+    DCF_LAUNCH(ctx, "k_coarse_flag", k_coarse_flag, dim3(nblk(n, 256)), dim3(256), 0, la.iv, par1, flag, n);
+    DCF_LAUNCH(ctx, "k_coarse_cl", k_coarse_cl, dim3(nblk(n, 256)), dim3(256), 0, par1, cl, flag, n);
+    size_t need = 0;
+    DCF_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, need, flag, cidx, (int)n, ctx->stream));
+    void *tmp = sc.get(need > 0 ? need : 16);
+    if (!tmp) return dcf_set_error(ctx, DCF_ENOMEM, "scan scratch");
+    DCF_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(tmp, need, flag, cidx, (int)n, ctx->stream));
+    uint32_t last[2];
+    DCF_HIP(ctx, hipMemcpyAsync(&last[0], cidx + (n - 1), 4, hipMemcpyDeviceToHost, ctx->stream));
+    DCF_HIP(ctx, hipMemcpyAsync(&last[1], flag + (n - 1), 4, hipMemcpyDeviceToHost, ctx->stream));
+    DCF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const int64_t kc = (int64_t)last[0] + last[1];
+    uint32_t *cnkey = (uint32_t *)sc.get((size_t)kc * 4);
+    uint32_t *cpix = (uint32_t *)sc.get((size_t)kc * 4);
+    int32_t *lift = (int32_t *)sc.get((size_t)kc * 4 * LIFT);
+    int32_t *spar = (int32_t *)sc.get((size_t)kc * 4);
+    if (!cnkey || !cpix || !lift || !spar) return dcf_set_error(ctx, DCF_ENOMEM, "coarse node arrays");
+    DevAttr S;
+    if ((rc = alloc_attr_n(ctx, &sc, S, grp, kc))) return rc;
+    DCF_LAUNCH(ctx, "k_coarse_nodes", k_coarse_nodes, dim3(nblk(n, 256)), dim3(256), 0, la.iv, par1, flag, cidx, cnkey, lift,
+               cpix, A, S, n);
+    for (int j = 1; j < LIFT; j++) DCF_LAUNCH(ctx, "k_lift", k_lift, dim3(nblk(kc, 256)), dim3(256), 0, lift, j, kc);
+    DCF_LAUNCH(ctx, "k_fill", k_fill_i32, dim3(nblk(kc, 256)), dim3(256), 0, spar, -1, kc);
+
+    /* ---- pass 2 ---- */
+    uint32_t *uf2 = (uint32_t *)sc.get((size_t)(n + kc) * 4);
+    uint32_t *abs2 = (uint32_t *)sc.get((size_t)(n + kc + 16) * 4);
+    if (!uf2 || !abs2) return dcf_set_error(ctx, DCF_ENOMEM, "fine union-find");
+    la.uf = uf2;
+    la.absorbed = abs2;
+    la.pass = 2;
+    la.shift = 0;
+    la.cl = cl;
+    la.cid = cidx;
+    la.lift = lift;
+    la.cnkey = cnkey;
+    DCF_LAUNCH(ctx, "k_iota", k_iota, dim3(nblk(n + kc, 256)), dim3(256), 0, la.uf, n + kc);
+    DCF_LAUNCH(ctx, "k_fill", k_fill_i32, dim3(nblk(n, 256)), dim3(256), 0, t->parent, -1, n);
+    if ((rc = clear_attr(ctx, A, n))) return rc;
+    if ((rc = run_pass(ctx, la, sc, A, S, grp, t->parent, spar, kc))) return rc;
+    DCF_LAUNCH(ctx, "k_fine_roots", k_fine_roots, dim3(nblk(kc, 256)), dim3(256), 0, la.uf, cpix, spar, t->parent, kc);
+
+    t->attr_pix = A;
+    for (auto it = sc.ptrs.begin(); it != sc.ptrs.end();) {
+        bool keep = *it == A.area;
+        for (int i = 0; i < A.nsum; i++) keep = keep || *it == A.s[i];
+        for (int i = 0; A.has_ext && i < 3; i++) keep = keep || *it == A.mn[i] || *it == A.mx[i];
+        if (keep) it = sc.ptrs.erase(it);
+        else ++it;
+    }
+    return DCF_OK;
+}

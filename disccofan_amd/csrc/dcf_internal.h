@@ -88,7 +88,8 @@ struct dcf_tree {
     int64_t K = 0;
     int32_t *npar = nullptr;
     float *ngrey = nullptr;
-    DevAttr attr;
+    DevAttr attr;       /* node-indexed */
+    DevAttr attr_pix;   /* pixel-indexed records at levelroots, produced by the flooding build */
     uint32_t *counters = nullptr; /* 64 device words: per-round early-exit counters */
     int max_rounds = 32;
     float host_gmax = 0; /* filled lazily */
@@ -126,8 +127,11 @@ void dcf_release(dcf_ctx *ctx, void *p);
 int dcf_group_of(int attr_idx);
 
 /* ---- launchers (dcf_build.hip) ---------------------------------------------------------------- */
-int dcf_build_tree(dcf_tree *t);              /* link -> parent -> nodes (no attributes) */
-int dcf_accumulate_group(dcf_tree *t, int grp); /* own sums + leaffix of group grp */
+int dcf_build_tree(dcf_tree *t, int grp);      /* parent, nodes and the attributes of group grp */
+int dcf_accumulate_group(dcf_tree *t, int grp); /* attributes of group grp on the existing tree */
+
+/* ---- launchers (dcf_flood.hip) ---------------------------------------------------------------- */
+int dcf_flood_build(dcf_tree *t, int grp);      /* u8/u16: canonical parent + pixel-indexed attributes */
 
 /* ---- launchers (dcf_ops.hip) ------------------------------------------------------------------ */
 int dcf_ops_filter(dcf_tree *t, int attr_idx, int decision, double lambda, int extract, void *out_dev,

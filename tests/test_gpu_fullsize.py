@@ -1,0 +1,98 @@
+"""The benchmark workloads at (or near) full size on the GPU, against the CPU oracle (OpenMP slab
+build) bit for bit, plus size-independent properties where the oracle would take too long.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import disccofan_amd as dcf
+from disccofan_amd import synth
+from oracle.oracle import OracleTree
+
+pytestmark = pytest.mark.gpu
+THREADS = 16
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = dcf.Context(0)
+    yield c
+    c.close()
+
+
+def same(a, b, what):
+    a = np.asarray(a).reshape(-1)
+    b = np.asarray(b).reshape(-1)
+    bad = np.nonzero(a != b)[0]
+    assert len(bad) == 0, "%s differs at %d positions, first %s: gpu %r oracle %r" % (what, len(bad), bad[:5], a[bad[:5]],
+                                                                                    b[bad[:5]])
+
+
+@pytest.fixture(scope="module")
+def vnoise8192():
+    return synth.vnoise(8192, 8192)
+
+
+@pytest.mark.parametrize("tree_type", ["max", "min"])
+def test_vnoise_8192_u16(ctx, vnoise8192, tree_type):
+    """8192^2 u16, 8-conn, area opening lambda=1000 subtractive, 64-bin area spectrum."""
+    img = vnoise8192
+    say("vnoise 8192^2 u16", tree_type, "sha", synth.sha256(img)[:16])
+    t = ctx.build(img, 8, tree_type, 0)
+    o = OracleTree(img, 8, tree_type, 0, threads=THREADS)
+    same(t.parent(), o.parent().astype(np.int32), "parent")
+    f = t.filter(0, 1000, 3, out_dtype=np.uint16)
+    same(f, o.filter(0, 1000, 3).astype(np.uint16), "subtractive filter")
+    lv = synth.log_lambdas(64, 1, 2 ** 26)
+    same(t.spectrum1d(0, lv), o.spectrum1d(0, lv), "spectrum")
+
+
+def test_spheres_256_u8(ctx):
+    img = synth.spheres(256, 256, 256)
+    say("spheres 256^3 u8 6-conn")
+    t = ctx.build(img, 6, "max", 0)
+    o = OracleTree(img, 6, "max", 0, threads=THREADS)
+    same(t.parent(), o.parent().astype(np.int32), "parent")
+    same(t.filter(0, 500, 0, out_dtype=np.uint8), o.filter(0, 500, 0).astype(np.uint8), "direct filter")
+    lv = synth.log_lambdas(48, 1, 2 ** 27)
+    same(t.spectrum1d(0, lv), o.spectrum1d(0, lv), "spectrum")
+
+
+def test_spheres_512_u8_properties(ctx):
+    """512^3 u8: the full volume against closed-form properties (the oracle is exercised at 256^3)."""
+    img = synth.spheres(512, 512, 512)
+    say("spheres 512^3 u8 6-conn properties")
+    t = ctx.build(img, 6, "max", 0)
+    # lambda = 1 keeps every node: the filter is the identity
+    same(t.filter(0, 1, 0, out_dtype=np.uint8), img, "identity filter")
+    # the pattern spectrum of a max-tree with background sums to the total grey, up to the
+    # rounding of each node's float product dh * area (pattern.c:91 multiplies in float)
+    lv = synth.log_lambdas(48, 1, 2 ** 27)
+    spec = t.spectrum1d(0, lv)
+    total = float(img.astype(np.int64).sum())
+    assert abs(spec.sum() - total) <= 2.0 ** -23 * total, (spec.sum(), total)
+    # the root is the one node whose area is the whole volume
+    area = t.check(0)
+    assert area.max() == np.float32(img.size)
+    par = t.parent()
+    assert (par < 0).sum() == 1
+
+
+def test_blobs_4096_precision_form(ctx):
+    """4096^2 f32 blobs quantised to u16, 8-conn, trace / ncomp and the 32x32 ncomp x area spectrum."""
+    img = synth.blobs(4096, 4096)
+    say("blobs 4096^2 u16 inertia")
+    t = ctx.build(img, 8, "max", 13)
+    o = OracleTree(img, 8, "max", 13, threads=THREADS)
+    same(t.parent(), o.parent().astype(np.int32), "parent")
+    same(t.check(13), o.check(13), "trace")
+    same(t.check(17), o.check(17), "ncomp")
+    l1 = synth.log_lambdas(32, 0.16, 100.0)
+    l2 = synth.log_lambdas(32, 1, 2 ** 24)
+    same(t.spectrum2d(17, l1, 0, l2), o.spectrum2d(17, l1, 0, l2), "pattern2D")
