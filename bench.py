@@ -93,6 +93,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--kernels", action="store_true", help="print the per-kernel event times to stderr")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB 5 times (the reference point for the "
+                         "FETCH_SIZE / WRITE_SIZE counters) and print its bandwidth")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -100,6 +103,23 @@ def main():
         sys.exit(2)
 
     import torch
+
+    if args.calibrate_copy:
+        nb = args.calibrate_copy << 20
+        src = torch.ones(nb // 4, dtype=torch.int32, device="cuda:0")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(5):
+            dst.copy_(src)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 5
+        print(json.dumps({"copy_bytes_read": nb, "copy_bytes_written": nb, "ms": round(ms, 4),
+                          "GBps_read_plus_write": round(2 * nb / ms / 1e6, 1)}), flush=True)
+        return
 
     import disccofan_amd as dcf
 

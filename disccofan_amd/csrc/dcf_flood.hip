@@ -64,60 +64,110 @@ struct Geo {
 /* ------------------------------------------------------------------------------------------------ */
 /* union-find                                                                                       */
 /* ------------------------------------------------------------------------------------------------ */
+/* uf[x] is the parent of x, or for a root UF_ROOT | level(x), so that a find returns the root's level
+ * with the load that detects the root (no image read after it).  Element ids stay below 2^31. */
+#define UF_ROOT 0x80000000u
+#define UF_LVL 0x1FFu
+
 __device__ __forceinline__ uint32_t ld(const uint32_t *a, uint32_t i)
 {
     return __hip_atomic_load(&a[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ __forceinline__ uint32_t uf_find(uint32_t *uf, uint32_t x)
+/* workgroup-scope load: may be served by the CU's L1 and be stale.  Every value an entry ever holds
+ * is an ancestor of the element (links only turn a root into a child, compression only moves a
+ * pointer up), so a stale read still leads towards the root; a stale root tag is caught by the
+ * link's CAS, which is performed in L2 and returns the fresh parent.  The root of a big component
+ * is read by every thread that touches it, and this keeps those reads in the CUs' L1s. */
+__device__ __forceinline__ uint32_t ldc(const uint32_t *a, uint32_t i)
 {
-    uint32_t p = ld(uf, x);
-    while (p != x) {
-        uint32_t gp = ld(uf, p);
-        if (gp != p) __hip_atomic_store(&uf[x], gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); /* halving */
+    return __hip_atomic_load(&a[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ void st(uint32_t *a, uint32_t i, uint32_t v)
+{
+    __hip_atomic_store(&a[i], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+/* root of x with path splitting; tag = the root's entry.  Only non-root entries are rewritten, and
+ * only ever to an ancestor, so these stores cannot race with a link (links CAS root tags). */
+__device__ __forceinline__ uint32_t uf_find(uint32_t *uf, uint32_t x, uint32_t &tag)
+{
+    uint32_t p = ldc(uf, x);
+    if (p & UF_ROOT) {
+        tag = p;
+        return x;
+    }
+    for (;;) {
+        const uint32_t gp = ldc(uf, p);
+        if (gp & UF_ROOT) {
+            tag = gp;
+            return p;
+        }
+        st(uf, x, gp); /* splitting */
         x = p;
         p = gp;
     }
-    return x;
 }
 
-/* priority of a union-find element in the current pass: (level, id); pixels of the current pass
- * have level = their digit, super-vertices have level NLEV (above every pixel) */
-struct Prio {
-    const ImgView *iv;
-    int pass;  /* 1: digit = key >> shift, 2: digit = key & 255 */
-    int shift;
-    uint32_t n;
-    __device__ uint32_t level(uint32_t x) const
-    {
-        if (x >= n) return NLEV;
-        uint32_t k = fl_key(*iv, x);
-        return pass == 1 ? (k >> shift) : (k & 255u);
-    }
-};
-
-/* unite the sets of a and b; when the root of higher priority gets linked and it belongs to a
- * level above `lev`, it is an old root completed earlier: record it */
-__device__ void uf_union(uint32_t *uf, const Prio &pr, uint32_t a, uint32_t b, uint32_t lev, uint32_t *absorbed,
-                         uint32_t *nabs)
+/* the root reached from x whose (fresh) parent is known to be p */
+__device__ __forceinline__ uint32_t uf_find_from(uint32_t *uf, uint32_t x, uint32_t p, uint32_t &tag)
 {
     for (;;) {
-        a = uf_find(uf, a);
-        b = uf_find(uf, b);
-        if (a == b) return;
-        uint32_t la = pr.level(a), lb = pr.level(b);
-        /* hi = the root that goes under the other */
-        bool a_hi = la > lb || (la == lb && a > b);
+        const uint32_t gp = ldc(uf, p);
+        if (gp & UF_ROOT) {
+            tag = gp;
+            return p;
+        }
+        st(uf, x, gp);
+        x = p;
+        p = gp;
+    }
+}
+
+/* block-local list of the old roots linked by this block's threads, flushed with one global
+ * atomic per block */
+struct AbsBuf {
+    uint32_t *buf;
+    uint32_t *n;
+};
+
+/* unite the sets of roots a (tag ta) and b (tag tb) during the level `lev`.  A set that reaches a
+ * completed (old) set is linked under that set's root, so the threads of a band of level-`lev`
+ * pixels along a big old component each CAS their own root's entry and never a shared one; old
+ * roots merge among themselves by (level, id), current-level roots by id.  A linked old root is
+ * recorded.  The canonical levelroot of a set that contains old roots (its minimum-index pixel of
+ * level `lev`) is installed as its root after the unions (k_canon_min, k_fix).  Returns the
+ * surviving root (possibly stale) and its tag. */
+__device__ __forceinline__ uint32_t uf_unite(uint32_t *uf, uint32_t a, uint32_t ta, uint32_t b, uint32_t tb, uint32_t lev,
+                                             const AbsBuf &ab, uint32_t &tout)
+{
+    for (;;) {
+        if (a == b) {
+            tout = ta;
+            return a;
+        }
+        uint32_t la = ta & UF_LVL, lb = tb & UF_LVL;
+        /* elements of the current level go under completed (older) sets; old sets merge by
+         * (level, id) and current-level sets by id */
+        const bool an = la == lev, bn = lb == lev;
+        bool a_hi = an != bn ? an : (la > lb || (la == lb && a > b));
         uint32_t hi = a_hi ? a : b, lo = a_hi ? b : a;
-        uint32_t lh = a_hi ? la : lb;
-        uint32_t exp = hi;
+        uint32_t th = a_hi ? ta : tb, tl = a_hi ? tb : ta;
+        uint32_t exp = th;
         if (__hip_atomic_compare_exchange_strong(&uf[hi], &exp, lo, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                                  __HIP_MEMORY_SCOPE_AGENT)) {
-            if (lh > lev) {
-                uint32_t slot = atomicAdd(nabs, 1u);
-                absorbed[slot] = hi;
-            }
-            return;
+            if ((th & UF_LVL) > lev) ab.buf[atomicAdd(ab.n, 1u)] = hi;
+            tout = tl;
+            return lo;
+        }
+        /* hi has been linked meanwhile: exp is its fresh parent; lo may be stale as well */
+        if (a_hi) {
+            a = uf_find_from(uf, a, exp, ta);
+            b = uf_find(uf, b, tb);
+        } else {
+            b = uf_find_from(uf, b, exp, tb);
+            a = uf_find(uf, a, ta);
         }
     }
 }
@@ -175,9 +225,11 @@ struct LevelArgs {
     uint32_t *uf;
     uint32_t *absorbed;
     uint32_t *nabs;        /* [0] running count */
+    uint32_t *canon;       /* per element: minimum pixel of the current level in its set (sets rooted at an old root) */
+    uint32_t *fix;         /* old set roots to re-root, all levels of the pass back to back */
+    uint32_t *fcount;      /* [0] running count of fix, [1 + i] where level i's entries start */
     /* pass 2 only */
-    const uint32_t *cl;    /* coarse levelroot pixel of every pixel */
-    const uint32_t *cid;   /* coarse node id of a coarse levelroot pixel */
+    const uint32_t *cnode; /* coarse node id of every pixel */
     const int32_t *lift;   /* LIFT x Kc: 2^j-th coarse ancestor (node id), -1 above the root */
     const uint32_t *cnkey; /* coarse key of every coarse node */
 };
@@ -190,10 +242,14 @@ __device__ __forceinline__ void coords(const Geo &g, int64_t p, int &x, int &y, 
     z = (int)(p / WH);
 }
 
-/* the ancestor of coarse node e whose coarse parent is the node of coarse key kn (e's coarse
- * key is above kn): the highest ancestor whose key is still above kn */
-__device__ __forceinline__ uint32_t child_toward(const LevelArgs &a, uint32_t e, uint32_t kn, uint32_t kc_count)
+/* the child of coarse node N (id nid) on the path up from its descendant e: e itself when its
+ * coarse parent is N, else the highest ancestor of e whose coarse key is above N's (binary lifting) */
+__device__ __forceinline__ uint32_t child_toward(const LevelArgs &a, uint32_t e, uint32_t nid, uint32_t kc_count)
 {
+    int32_t up = a.lift[e];
+    if (up == (int32_t)nid) return e;
+    uint32_t kn = a.cnkey[nid];
+    e = (uint32_t)up;
     for (int j = LIFT - 1; j >= 0; j--) {
         int32_t u = a.lift[(int64_t)j * kc_count + e];
         if (u >= 0 && a.cnkey[u] > kn) e = (uint32_t)u;
@@ -201,45 +257,90 @@ __device__ __forceinline__ uint32_t child_toward(const LevelArgs &a, uint32_t e,
     return e;
 }
 
+#define CH 8 /* neighbours gathered together */
+
+/* every pixel p of the level is united with each already-processed neighbour (level digit above
+ * the current one, or equal with a larger index); in pass 2 a neighbour in a coarse child region
+ * stands for that child's super-vertex.  The neighbours' roots are found together (independent
+ * loads in flight), duplicates dropped, then p's set is united with each distinct root. */
+template <int NB>
 __global__ __launch_bounds__(256) void k_union(LevelArgs a, const uint32_t *__restrict__ lvl, uint32_t cnt, uint32_t lev,
                                                uint32_t kc_count)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= cnt) return;
-    uint32_t p = lvl[i];
-    Prio pr{ &a.iv, a.pass, a.shift, (uint32_t)a.g.n };
-    int x, y, z;
+    __shared__ uint32_t s_abs[256 * NB];
+    __shared__ uint32_t s_n, s_base;
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    const AbsBuf ab{ s_abs, &s_n };
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t NONE = 0xFFFFFFFFu;
+    const bool valid = i < cnt;
+    const uint32_t p = valid ? lvl[i] : 0u;
+    int x = 0, y = 0, z = 0;
     coords(a.g, p, x, y, z);
     const int64_t WH = (int64_t)a.g.W * a.g.H;
-    uint32_t kp = fl_key(a.iv, p);
-    uint32_t cp = kp >> 8; /* pass 2: coarse key */
-    uint32_t np_ = 0, kn = 0;
-    if (a.pass == 2) {
-        np_ = a.cl[p];
-        kn = a.cnkey[a.cid[np_]];
-    }
-    for (int k = 0; k < a.nb.n; k++) {
-        int nx = x + a.nb.dx[k], ny = y + a.nb.dy[k], nz = z + a.nb.dz[k];
-        if (nx < 0 || nx >= a.g.W || ny < 0 || ny >= a.g.H || nz < 0 || nz >= a.g.D) continue;
-        uint32_t q = (uint32_t)(nx + (int64_t)ny * a.g.W + nz * WH);
-        uint32_t kq = fl_key(a.iv, q);
-        if (a.pass == 1) {
-            uint32_t dq = kq >> a.shift;
-            if (dq > lev || (dq == lev && q > p)) uf_union(a.uf, pr, p, q, lev, a.absorbed, a.nabs);
-        } else {
-            uint32_t cq = kq >> 8;
-            if (cq == cp) {
-                uint32_t dq = kq & 255u;
-                if (dq > lev || (dq == lev && q > p)) uf_union(a.uf, pr, p, q, lev, a.absorbed, a.nabs);
-            } else if (cq > cp) {
-                /* q lies in the region of a coarse child of N = cl(p): unite with its super-vertex */
-                uint32_t e = a.cid[a.cl[q]];
-                uint32_t c = child_toward(a, e, kn, kc_count);
-                uf_union(a.uf, pr, p, (uint32_t)a.g.n + c, lev, a.absorbed, a.nabs);
+    const uint32_t kp = fl_key(a.iv, p);
+    const uint32_t cp = kp >> 8;
+    const uint32_t nid = (valid && a.pass == 2) ? a.cnode[p] : 0u;
+    uint32_t tp = 0, rp = 0;
+    if (valid) rp = uf_find(a.uf, p, tp);
+#pragma unroll
+    for (int k0 = 0; k0 < NB; k0 += CH) {
+        uint32_t cur[CH], val[CH], node[CH];
+#pragma unroll
+        for (int j = 0; j < CH; j++) {
+            cur[j] = NONE;
+            const int k = k0 + j;
+            if (k >= NB || !valid) continue;
+            int nx = x + a.nb.dx[k], ny = y + a.nb.dy[k], nz = z + a.nb.dz[k];
+            if (nx < 0 || nx >= a.g.W || ny < 0 || ny >= a.g.H || nz < 0 || nz >= a.g.D) continue;
+            uint32_t q = (uint32_t)(nx + (int64_t)ny * a.g.W + nz * WH);
+            uint32_t kq = fl_key(a.iv, q);
+            if (a.pass == 1) {
+                uint32_t dq = kq >> a.shift;
+                if (dq > lev || (dq == lev && q > p)) cur[j] = q;
+            } else {
+                uint32_t cq = kq >> 8;
+                if (cq == cp) {
+                    uint32_t dq = kq & 255u;
+                    if (dq > lev || (dq == lev && q > p)) cur[j] = q;
+                } else if (cq > cp) {
+                    cur[j] = (uint32_t)a.g.n + child_toward(a, a.cnode[q], nid, kc_count);
+                }
             }
         }
+        /* lockstep finds with path splitting (each visited node is pointed at its grandparent) */
+#pragma unroll
+        for (int j = 0; j < CH; j++) {
+            node[j] = cur[j];
+            val[j] = cur[j] != NONE ? ldc(a.uf, cur[j]) : UF_ROOT;
+        }
+        bool more = true;
+        while (more) {
+            more = false;
+#pragma unroll
+            for (int j = 0; j < CH; j++)
+                if (!(val[j] & UF_ROOT)) {
+                    const uint32_t pj = val[j];
+                    const uint32_t gp = ldc(a.uf, pj);
+                    if (!(gp & UF_ROOT)) st(a.uf, node[j], gp);
+                    node[j] = pj;
+                    val[j] = gp;
+                    more |= !(gp & UF_ROOT);
+                }
+        }
+#pragma unroll
+        for (int j = 0; j < CH; j++) {
+            bool live = cur[j] != NONE;
+#pragma unroll
+            for (int m = 0; m < j; m++) live = live && !(cur[m] != NONE && node[m] == node[j]);
+            if (live) rp = uf_unite(a.uf, rp, tp, node[j], val[j], lev, ab, tp);
+        }
     }
-    (void)np_;
+    __syncthreads();
+    if (threadIdx.x == 0 && s_n) s_base = atomicAdd(a.nabs, s_n);
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < s_n; j += blockDim.x) a.absorbed[s_base + j] = s_abs[j];
 }
 
 /* ------------------------------------------------------------------------------------------------ */
@@ -363,11 +464,61 @@ __global__ __launch_bounds__(256) void k_final_pixels(LevelArgs a, const uint32_
     contrib_zero(c);
     if (valid) {
         uint32_t p = lvl[i];
-        r = uf_find(a.uf, p);
+        uint32_t tag;
+        r = uf_find(a.uf, p, tag);
         if (r != p) par[p] = (int32_t)r;
         contrib_own(a, grp, p, c);
     }
     wave_add(A, valid, r, c);
+}
+
+/* sets of the level that reached an old root are still rooted there: their canonical levelroot is
+ * their minimum pixel of this level.  One atomicMin per wave and set; the first one to touch a set
+ * queues its root for k_fix. */
+__global__ __launch_bounds__(256) void k_canon_min(LevelArgs a, const uint32_t *__restrict__ lvl, uint32_t cnt, uint32_t lev)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    bool want = false;
+    uint32_t p = 0, r = 0;
+    if (i < cnt) {
+        p = lvl[i];
+        uint32_t tag;
+        r = uf_find(a.uf, p, tag);
+        want = (tag & UF_LVL) != lev;
+    }
+    uint64_t pend = __ballot(want);
+    while (pend) {
+        const int leader = __ffsll((unsigned long long)pend) - 1;
+        const uint32_t rl = __shfl(r, leader, 64);
+        const bool mine = want && r == rl;
+        const uint64_t m = __ballot(mine);
+        if (m == (1ull << leader)) {
+            if ((pend >> lane) & 1ull) {
+                if (atomicMin(&a.canon[r], p) == 0xFFFFFFFFu) a.fix[atomicAdd(a.fcount, 1u)] = r;
+            }
+            break;
+        }
+        const uint32_t pm = wmin32(mine ? p : 0xFFFFFFFFu);
+        if (lane == leader && atomicMin(&a.canon[rl], pm) == 0xFFFFFFFFu) a.fix[atomicAdd(a.fcount, 1u)] = rl;
+        pend &= ~m;
+    }
+}
+
+/* re-root every queued set at its canonical levelroot c: c becomes the root (tagged with this level)
+ * and the old root r hangs below it, queued as absorbed so that its node gets parent c and merges
+ * its attributes into c */
+__global__ __launch_bounds__(256) void k_fix(LevelArgs a, uint32_t ord, uint32_t lev)
+{
+    const uint32_t start = a.fcount[1 + ord], end = a.fcount[0];
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.fcount[2 + ord] = end;
+    for (uint32_t j = start + blockIdx.x * blockDim.x + threadIdx.x; j < end; j += gridDim.x * blockDim.x) {
+        const uint32_t r = a.fix[j];
+        const uint32_t c = a.canon[r];
+        st(a.uf, c, UF_ROOT | lev);
+        st(a.uf, r, c);
+        a.absorbed[atomicAdd(a.nabs, 1u)] = r;
+    }
 }
 
 /* absorbed old roots of this level: parent + attribute merge.  Pixel roots carry their complete
@@ -387,7 +538,8 @@ __global__ __launch_bounds__(256) void k_final_absorbed(LevelArgs a, const uint3
         contrib_zero(c);
         if (valid) {
             uint32_t h = a.absorbed[i];
-            r = uf_find(a.uf, h);
+            uint32_t tag;
+            r = uf_find(a.uf, h, tag);
             if (h < (uint32_t)a.g.n) {
                 par[h] = (int32_t)r;
                 contrib_rec(A, h, c);
@@ -404,10 +556,18 @@ __global__ __launch_bounds__(256) void k_final_absorbed(LevelArgs a, const uint3
 /* ------------------------------------------------------------------------------------------------ */
 /* between the passes                                                                               */
 /* ------------------------------------------------------------------------------------------------ */
-__global__ __launch_bounds__(256) void k_iota(uint32_t *a, int64_t n)
+/* every element a singleton root: pixels tagged with their level digit, super-vertices (ids n and
+ * above) with NLEV */
+__global__ __launch_bounds__(256) void k_uf_init(ImgView iv, int pass, int shift, int64_t n, int64_t total, uint32_t *uf)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) a[i] = (uint32_t)i;
+    if (i >= total) return;
+    uint32_t lv = NLEV;
+    if (i < n) {
+        uint32_t k = fl_key(iv, i);
+        lv = pass == 1 ? (k >> shift) : (k & 255u);
+    }
+    uf[i] = UF_ROOT | lv;
 }
 
 __global__ __launch_bounds__(256) void k_fill_i32(int32_t *a, int32_t v, int64_t n)
@@ -428,17 +588,15 @@ __global__ __launch_bounds__(256) void k_fill_i64f(int64_t *a, int64_t v, int64_
     if (i < n) a[i] = v;
 }
 
-/* coarse levelroot of every pixel: par1 holds the levelroot for non-roots, -1 for roots */
-__global__ __launch_bounds__(256) void k_coarse_cl(const int32_t *__restrict__ par1, uint32_t *__restrict__ cl,
-                                                   uint32_t *__restrict__ flag, int64_t n)
+/* coarse node id of every pixel.  par1 of a coarse levelroot is its coarse parent (a pixel of
+ * lower coarse key) or -1; of any other pixel it is its levelroot (same coarse key); flag marks the
+ * levelroots and cidx numbers them. */
+__global__ __launch_bounds__(256) void k_coarse_node_of(const int32_t *__restrict__ par1, const uint32_t *__restrict__ flag,
+                                                        const uint32_t *__restrict__ cidx, uint32_t *__restrict__ cnode, int64_t n)
 {
     int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
-    int32_t r = par1[p];
-    /* par1 of a coarse levelroot is its coarse parent (a pixel of lower coarse key) or -1; of any
-     * other pixel it is its levelroot (same coarse key).  The caller passes the key test result
-     * through flag on input: 1 for levelroots. */
-    cl[p] = flag[p] ? (uint32_t)p : (uint32_t)r;
+    cnode[p] = cidx[flag[p] ? p : (int64_t)par1[p]];
 }
 
 __global__ __launch_bounds__(256) void k_coarse_flag(ImgView iv, const int32_t *__restrict__ par1, uint32_t *__restrict__ flag,
@@ -484,7 +642,8 @@ __global__ __launch_bounds__(256) void k_fine_roots(uint32_t *uf, const uint32_t
 {
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= kc) return;
-    uint32_t fr = uf_find(uf, cpix[c]);
+    uint32_t tag;
+    uint32_t fr = uf_find(uf, cpix[c], tag);
     par[fr] = spar[c];
 }
 
@@ -582,12 +741,29 @@ static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAtt
      * entries start (written on the device, so the level loop never waits for the host) */
     uint32_t *nabs = la.nabs;
     DCF_LAUNCH(ctx, "k_fill", k_fill_u32f, dim3(2), dim3(256), 0, nabs, 0u, (int64_t)(NLEV + 2));
+    /* canonical-root bookkeeping for sets rooted at old roots (every element is re-rooted at most once) */
+    const int64_t nel = n + kc;
+    la.canon = (uint32_t *)sc.get((size_t)nel * 4);
+    la.fix = (uint32_t *)sc.get((size_t)(nel + 16) * 4);
+    la.fcount = (uint32_t *)sc.get((NLEV + 4) * 4);
+    if (!la.canon || !la.fix || !la.fcount) return dcf_set_error(ctx, DCF_ENOMEM, "canonical root lists");
+    DCF_LAUNCH(ctx, "k_fill", k_fill_u32f, dim3(nblk(nel, 256)), dim3(256), 0, la.canon, 0xFFFFFFFFu, nel);
+    DCF_LAUNCH(ctx, "k_fill", k_fill_u32f, dim3(2), dim3(256), 0, la.fcount, 0u, (int64_t)(NLEV + 4));
     int ord = 0;
     for (int lev = NLEV - 1; lev >= 0; lev--) {
         uint32_t cnt = h[lev];
         if (!cnt) continue;
         const uint32_t *lvl = order + off[lev];
-        DCF_LAUNCH(ctx, "k_union", k_union, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, cnt, (uint32_t)lev, (uint32_t)kc);
+        switch (la.nb.n) {
+        case 4: DCF_LAUNCH(ctx, "k_union", k_union<4>, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, cnt, (uint32_t)lev, (uint32_t)kc); break;
+        case 6: DCF_LAUNCH(ctx, "k_union", k_union<6>, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, cnt, (uint32_t)lev, (uint32_t)kc); break;
+        case 8: DCF_LAUNCH(ctx, "k_union", k_union<8>, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, cnt, (uint32_t)lev, (uint32_t)kc); break;
+        default: DCF_LAUNCH(ctx, "k_union", k_union<26>, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, cnt, (uint32_t)lev, (uint32_t)kc); break;
+        }
+        DCF_LAUNCH(ctx, "k_canon_min", k_canon_min, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, cnt, (uint32_t)lev);
+        unsigned fb = nblk(cnt, 256);
+        if (fb > 1024) fb = 1024;
+        DCF_LAUNCH(ctx, "k_fix", k_fix, dim3(fb), dim3(256), 0, la, (uint32_t)ord, (uint32_t)lev);
         DCF_LAUNCH(ctx, "k_final_pixels", k_final_pixels, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, cnt, A, grp, par,
                    nabs + 2 + ord);
         unsigned ab = nblk((int64_t)cnt * la.nb.n, 256);
@@ -604,6 +780,8 @@ int dcf_flood_build(dcf_tree *t, int grp)
     dcf_ctx *ctx = t->ctx;
     const int64_t n = t->n;
     const int is3d = t->dims[2] > 1;
+    /* union-find ids (pixels + coarse super-vertices) must stay below the root tag bit */
+    if (n >= ((int64_t)1 << 30)) return dcf_set_error(ctx, DCF_EUNSUPPORTED, "integer images are limited to 2^30 pixels");
     Scratch sc{ ctx, {} };
     LevelArgs la;
     la.iv.img = t->img;
@@ -627,13 +805,13 @@ int dcf_flood_build(dcf_tree *t, int grp)
     la.nabs = (uint32_t *)sc.get((NLEV + 4) * 4);
     int32_t *par1 = two ? (int32_t *)sc.get((size_t)n * 4) : t->parent;
     if (!la.uf || !la.absorbed || !la.nabs || !par1) return dcf_set_error(ctx, DCF_ENOMEM, "flood scratch");
-    la.cl = nullptr; la.cid = nullptr; la.lift = nullptr; la.cnkey = nullptr;
+    la.cnode = nullptr; la.lift = nullptr; la.cnkey = nullptr;
 
     /* ---- pass 1 ---- */
     la.pass = 1;
     la.shift = two ? 8 : 0;
     if ((rc = clear_attr(ctx, A, n))) return rc;
-    DCF_LAUNCH(ctx, "k_iota", k_iota, dim3(nblk(n, 256)), dim3(256), 0, la.uf, n);
+    DCF_LAUNCH(ctx, "k_uf_init", k_uf_init, dim3(nblk(n, 256)), dim3(256), 0, la.iv, 1, la.shift, n, n, la.uf);
     DCF_LAUNCH(ctx, "k_fill", k_fill_i32, dim3(nblk(n, 256)), dim3(256), 0, par1, -1, n);
     DevAttr none;
     if ((rc = run_pass(ctx, la, sc, A, none, grp, par1, nullptr, 0))) return rc;
@@ -655,10 +833,9 @@ int dcf_flood_build(dcf_tree *t, int grp)
     /* ---- coarse nodes ---- */
     uint32_t *flag = (uint32_t *)sc.get((size_t)n * 4);
     uint32_t *cidx = (uint32_t *)sc.get((size_t)n * 4);
-    uint32_t *cl = (uint32_t *)sc.get((size_t)n * 4);
-    if (!flag || !cidx || !cl) return dcf_set_error(ctx, DCF_ENOMEM, "coarse arrays");
+    uint32_t *cnode = (uint32_t *)sc.get((size_t)n * 4);
+    if (!flag || !cidx || !cnode) return dcf_set_error(ctx, DCF_ENOMEM, "coarse arrays");
     DCF_LAUNCH(ctx, "k_coarse_flag", k_coarse_flag, dim3(nblk(n, 256)), dim3(256), 0, la.iv, par1, flag, n);
-    DCF_LAUNCH(ctx, "k_coarse_cl", k_coarse_cl, dim3(nblk(n, 256)), dim3(256), 0, par1, cl, flag, n);
     size_t need = 0;
     DCF_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, need, flag, cidx, (int)n, ctx->stream));
     void *tmp = sc.get(need > 0 ? need : 16);
@@ -679,6 +856,7 @@ int dcf_flood_build(dcf_tree *t, int grp)
     DCF_LAUNCH(ctx, "k_coarse_nodes", k_coarse_nodes, dim3(nblk(n, 256)), dim3(256), 0, la.iv, par1, flag, cidx, cnkey, lift,
                cpix, A, S, n);
     for (int j = 1; j < LIFT; j++) DCF_LAUNCH(ctx, "k_lift", k_lift, dim3(nblk(kc, 256)), dim3(256), 0, lift, j, kc);
+    DCF_LAUNCH(ctx, "k_coarse_node_of", k_coarse_node_of, dim3(nblk(n, 256)), dim3(256), 0, par1, flag, cidx, cnode, n);
     DCF_LAUNCH(ctx, "k_fill", k_fill_i32, dim3(nblk(kc, 256)), dim3(256), 0, spar, -1, kc);
 
     /* ---- pass 2 ---- */
@@ -689,11 +867,10 @@ int dcf_flood_build(dcf_tree *t, int grp)
     la.absorbed = abs2;
     la.pass = 2;
     la.shift = 0;
-    la.cl = cl;
-    la.cid = cidx;
+    la.cnode = cnode;
     la.lift = lift;
     la.cnkey = cnkey;
-    DCF_LAUNCH(ctx, "k_iota", k_iota, dim3(nblk(n + kc, 256)), dim3(256), 0, la.uf, n + kc);
+    DCF_LAUNCH(ctx, "k_uf_init", k_uf_init, dim3(nblk(n + kc, 256)), dim3(256), 0, la.iv, 2, 0, n, n + kc, la.uf);
     DCF_LAUNCH(ctx, "k_fill", k_fill_i32, dim3(nblk(n, 256)), dim3(256), 0, t->parent, -1, n);
     if ((rc = clear_attr(ctx, A, n))) return rc;
     if ((rc = run_pass(ctx, la, sc, A, S, grp, t->parent, spar, kc))) return rc;
