@@ -35,25 +35,20 @@ WORKLOADS = {
 }
 
 
-def algorithmic_bytes(kernel: str, n: int, k_nodes: int, B: int) -> float:
-    """Bytes one launch of `kernel` must move at minimum (each logical array touched once).
-    n pixels, k_nodes nodes, B bytes per pixel of the input."""
+def algorithmic_bytes(kernel: str, n: int, k_nodes: int, B: int, passes: int):
+    """(units one tree makes this kernel process, algorithmic bytes per unit): every logical array is
+    counted once per stage (build: read image + write parent = B + 4 per voxel; attribute: read
+    parent + write area = 8 per voxel; filter: read parent, image and area, write output = 2B + 8
+    per voxel).  The level kernels of the build together see every voxel once per pass, so their
+    per-launch figure is (passes * n / launches) voxels."""
     table = {
-        # build: read the image, write one 4-byte parent per pixel
-        "k_tile_build": n * (B + 4),
-        "k_canon": n * (B + 4),
-        # attribute doubling round over the nodes: read ancestor + area, write area + ancestor
-        "k_leaf_round": k_nodes * 16,
-        "k_leaf_copy": k_nodes * 8,
-        "k_own": n * 4 + k_nodes * 4,
-        "k_nodes": n * (4 + 4 + 4),
-        # filter: read the pixel's node and write the output pixel
-        "k_filter_out": n * (4 + B) + k_nodes * 4,
-        "k_direct_jump": k_nodes * 9,
-        "k_direct_init": k_nodes * 16,
-        "k_seam": 0.0,
+        "k_union": (passes * n, B + 4),
+        "k_final_pixels": (passes * n, 8),
+        "k_final_held": (passes * n, 8),
+        "k_filter_out": (n, 2 * B + 8),
+        "k_direct_jump": (k_nodes, 9),
     }
-    return float(table.get(kernel, 0.0))
+    return table.get(kernel, (0, 0.0))
 
 
 def make_image(name: str) -> np.ndarray:
@@ -65,23 +60,21 @@ def make_image(name: str) -> np.ndarray:
 
 
 def cpu_baseline(img: np.ndarray, w: dict, threads: int):
-    """The CPU oracle (the reference algorithm restated in C, OpenMP slabs) on a crop of the image."""
+    """The CPU oracle (the reference algorithm restated in C: radix ranks, slab flooding, slab fuse,
+    the same filter walks; OpenMP) on the whole workload image, build + filter of every tree of
+    the step, timed on this host."""
     from oracle.oracle import OracleTree
-    if img.ndim == 2:
-        crop = np.ascontiguousarray(img[:2048, :2048])
-    else:
-        crop = np.ascontiguousarray(img[:128])
     t0 = time.perf_counter()
     vox = 0
     for tt in w["trees"]:
-        o = OracleTree(crop, w["conn"], tt, w["attr"], threads=threads)
+        o = OracleTree(img, w["conn"], tt, w["attr"], threads=threads)
         o.filter(w["attr"], w["lam"], w["decision"])
-        vox += crop.size
+        vox += img.size
         del o
     dt = time.perf_counter() - t0
     return {"value": vox / dt / 1e6, "unit": "Mvoxels/s", "cores": threads, "kind": "port",
-            "sample": "%s crop of the same image, %s, build+filter, %d OpenMP threads, %.2f s" % (
-                "x".join(map(str, crop.shape)), "+".join(w["trees"]) + "-tree", threads, dt)}
+            "sample": "the full %s image, %s, build+filter, %d OpenMP threads, %.2f s" % (
+                "x".join(map(str, img.shape)), "+".join(w["trees"]) + "-tree", threads, dt)}
 
 
 def main():
@@ -93,6 +86,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--kernels", action="store_true", help="print the per-kernel event times to stderr")
+    ap.add_argument("--roofline-kernel", default="k_union",
+                    help="the kernel bracketed by HIP events in the timed region (the dominant one); "
+                         "--kernels brackets every kernel instead")
+    ap.add_argument("--no-profile", action="store_true",
+                    help="time the steps without per-launch HIP events (no roofline object)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB 5 times (the reference point for the "
                          "FETCH_SIZE / WRITE_SIZE counters) and print its bandwidth")
@@ -156,7 +154,8 @@ def main():
         print("warmup %d: %.1f ms" % (i, (time.perf_counter() - t1) * 1e3), file=sys.stderr, flush=True)
 
     ctx.reset_stats()
-    ctx.set_profiling(True)
+    ctx.profile_only(None if args.kernels else args.roofline_kernel)
+    ctx.set_profiling(not args.no_profile)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -169,7 +168,7 @@ def main():
     vox_per_step = n * len(w["trees"])
     value = vox_per_step / (ms_per_step / 1e3) / 1e6
 
-    # dominant kernel from the event timings of the timed region
+    # dominant kernel from the event timings of the timed region (none with --no-profile)
     names = ctx.kernel_names()
     stats = {k: ctx.kernel_stats(k) for k in names}
     exact = {}
@@ -182,27 +181,39 @@ def main():
                 cnt -= c2
                 ms -= m2
         exact[k] = (cnt, ms)
-    dom = max(exact, key=lambda k: exact[k][1])
-    launches, tot_ms = exact[dom]
-    avg_ms = tot_ms / max(launches, 1)
     K = state.get("K", 0)
-    alg = algorithmic_bytes(dom, n, K, B)
-    achieved = alg / (avg_ms / 1e3) / 1e9 if alg > 0 else 0.0
-    if args.kernels:
-        tot = sum(v[1] for v in exact.values())
-        for k in sorted(exact, key=lambda k: -exact[k][1]):
-            c, m = exact[k]
-            print("%-16s launches %5d  total %9.3f ms  avg %8.4f ms  %5.1f%%" % (k, c, m, m / max(c, 1), 100 * m / tot),
-                  file=sys.stderr)
-        print("nodes K = %d (%.1f%% of pixels)" % (K, 100.0 * K / n), file=sys.stderr)
-
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_%s_%s.json" % (dom, args.workload))
-    if os.path.exists(tpath):
-        try:
-            traffic = json.load(open(tpath)).get("bytes_per_launch")
-        except Exception:
-            traffic = None
+    passes = 2 if B == 2 else 1
+    step_bytes = float(vox_per_step) * (3 * B + 20)
+    roof = None
+    if exact:
+        dom = args.roofline_kernel if args.roofline_kernel in exact else max(exact, key=lambda k: exact[k][1])
+        launches, tot_ms = exact[dom]
+        avg_ms = tot_ms / max(launches, 1)
+        units, per_unit = algorithmic_bytes(dom, n, K, B, passes)
+        runs = args.steps * len(w["trees"])
+        alg = units * runs * per_unit / max(launches, 1)
+        achieved = alg / (avg_ms / 1e3) / 1e9 if alg > 0 else 0.0
+        traffic = None
+        tpath = os.path.join(ROOT, "profiles", "traffic_%s_%s.json" % (dom, args.workload))
+        if os.path.exists(tpath):
+            try:
+                traffic = json.load(open(tpath)).get("bytes_per_launch")
+            except Exception:
+                traffic = None
+        roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic, "avg_launch_ms": round(avg_ms, 4),
+                "launches": launches, "algorithmic_bytes_per_launch": round(alg),
+                "share_of_step": round(tot_ms / (ms_per_step * args.steps), 4),
+                "step_algorithmic_bytes": step_bytes,
+                "step_achieved": round(step_bytes / (ms_per_step / 1e3) / 1e9, 3),
+                "step_frac": round(step_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS, 6)}
+        if args.kernels:
+            tot = sum(v[1] for v in exact.values())
+            for k in sorted(exact, key=lambda k: -exact[k][1]):
+                c, m = exact[k]
+                print("%-16s launches %5d  total %9.3f ms  avg %8.4f ms  %5.1f%%" % (k, c, m, m / max(c, 1), 100 * m / tot),
+                      file=sys.stderr)
+    print("nodes K = %d (%.1f%% of pixels)" % (K, 100.0 * K / n), file=sys.stderr)
 
     cpu = None
     if not args.no_cpu:
@@ -225,10 +236,7 @@ def main():
         "config": {"workload": w["desc"], "shape": list(img.shape), "voxels_per_step": vox_per_step,
                    "connectivity": w["conn"], "trees": list(w["trees"]), "attribute": w["attr"],
                    "lambda": w["lam"], "decision": w["decision"], "bit_exact_vs_oracle": "see tests/test_gpu_fullsize.py"},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                     "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": alg,
-                     "share_of_step": round(tot_ms / (ms_per_step * args.steps), 4)},
+        "roofline": roof,
         "cpu_baseline": cpu,
         "gen_s": round(gen_s, 2),
     }

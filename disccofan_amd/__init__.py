@@ -40,7 +40,7 @@ DECISIONS = ["Direct", "Min", "Max", "Subtractive"]
 # exported symbols of libdcf.so (kept in sync with include/dcf.h)
 EXPORTS = [
     "dcf_ctx_create", "dcf_ctx_destroy", "dcf_ctx_set_stream", "dcf_ctx_stream", "dcf_last_error",
-    "dcf_ctx_synchronize", "dcf_ctx_set_profiling", "dcf_kernel_stats", "dcf_kernel_stats_reset",
+    "dcf_ctx_synchronize", "dcf_ctx_set_profiling", "dcf_ctx_profile_only", "dcf_kernel_stats", "dcf_kernel_stats_reset",
     "dcf_kernel_name", "dcf_tree_build", "dcf_tree_free", "dcf_tree_accumulate", "dcf_tree_group",
     "dcf_tree_parent", "dcf_tree_num_nodes", "dcf_filter", "dcf_check", "dcf_spectrum1d", "dcf_spectrum2d",
     "dcf_lvec_read", "dcf_lvec_free", "dcf_attr_name", "dcf_attr_short_name", "dcf_attr_group",
@@ -76,6 +76,7 @@ def lib():
     L.dcf_last_error.argtypes = [vp]
     L.dcf_ctx_synchronize.argtypes = [vp]
     L.dcf_ctx_set_profiling.argtypes = [vp, i]
+    L.dcf_ctx_profile_only.argtypes = [vp, C.c_char_p]
     L.dcf_kernel_stats.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
     L.dcf_kernel_stats_reset.argtypes = [vp]
     L.dcf_kernel_name.restype = C.c_char_p
@@ -177,6 +178,10 @@ class Context:
 
     def set_profiling(self, on: bool):
         self._check(lib().dcf_ctx_set_profiling(self._h, 1 if on else 0))
+
+    def profile_only(self, name: str | None):
+        """Record events only around kernels whose name contains `name` (None: every kernel)."""
+        self._check(lib().dcf_ctx_profile_only(self._h, name.encode() if name else None))
 
     def kernel_stats(self, name: str):
         n = C.c_int64()

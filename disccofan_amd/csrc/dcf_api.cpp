@@ -226,6 +226,13 @@ extern "C" int dcf_ctx_set_profiling(dcf_ctx *ctx, int on)
     return DCF_OK;
 }
 
+extern "C" int dcf_ctx_profile_only(dcf_ctx *ctx, const char *name)
+{
+    if (!ctx) return DCF_EINVAL;
+    ctx->prof_only = name ? name : "";
+    return DCF_OK;
+}
+
 extern "C" int dcf_kernel_stats(dcf_ctx *ctx, const char *name, int64_t *launches, double *total_ms)
 {
     if (!ctx || !name) return DCF_EINVAL;

@@ -505,9 +505,10 @@ static int compact_nodes(dcf_tree *t, uint32_t *flag)
     if (!scan_tmp) return dcf_set_error(ctx, DCF_ENOMEM, "scan scratch");
     {
         hipEvent_t ea = nullptr, eb = nullptr;
-        if (ctx->profiling) dcf_prof_begin(ctx, "scan_nodes", &ea, &eb);
+        if (ctx->profiling && (ctx->prof_only.empty() || strstr("scan_nodes", ctx->prof_only.c_str())))
+            dcf_prof_begin(ctx, "scan_nodes", &ea, &eb);
         hipError_t se = hipcub::DeviceScan::ExclusiveSum(scan_tmp, need, flag, t->p2n, (int)n, ctx->stream);
-        if (ctx->profiling) dcf_prof_end(ctx, "scan_nodes", ea, eb);
+        if (ea) dcf_prof_end(ctx, "scan_nodes", ea, eb);
         dcf_release(ctx, scan_tmp);
         if (se != hipSuccess) return dcf_set_error(ctx, DCF_EDEVICE, "scan: %s", hipGetErrorString(se));
     }

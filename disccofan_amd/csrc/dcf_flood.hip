@@ -137,7 +137,7 @@ struct AbsBuf {
  * pixels along a big old component each CAS their own root's entry and never a shared one; old
  * roots merge among themselves by (level, id), current-level roots by id.  A linked old root is
  * recorded.  The canonical levelroot of a set that contains old roots (its minimum-index pixel of
- * level `lev`) is installed as its root after the unions (k_canon_min, k_fix).  Returns the
+ * level `lev`) is installed as its root after the unions (k_final_pixels, k_fix).  Returns the
  * surviving root (possibly stale) and its tag. */
 __device__ __forceinline__ uint32_t uf_unite(uint32_t *uf, uint32_t a, uint32_t ta, uint32_t b, uint32_t tb, uint32_t lev,
                                              const AbsBuf &ab, uint32_t &tout)
@@ -452,46 +452,113 @@ __device__ void wave_add(const DevAttr &A, bool valid, uint32_t r, const Contrib
     }
 }
 
-__global__ __launch_bounds__(256) void k_final_pixels(LevelArgs a, const uint32_t *__restrict__ lvl, uint32_t cnt,
-                                                      DevAttr A, int grp, int32_t *__restrict__ par, uint32_t *__restrict__ mark)
+/* after the unions of a level, every pixel p of the level finds its set's root r.  A root of this
+ * level is the canonical levelroot: p takes it as parent and adds its own attribute contribution.
+ * A set still rooted at an old root has its canonical levelroot computed here (minimum pixel of
+ * the level, one atomicMin per wave and set; the first to touch a set queues its root for k_fix)
+ * and p keeps r in `held` for k_final_held. */
+/* Block-level variant for kernels run with FB threads per block: when every valid lane of the block
+ * targets the same root (the pixels of one big set), the block reduces its contributions through
+ * LDS and issues one atomic per field; otherwise each wave aggregates on its own.  Same-address
+ * atomics from thousands of waves were the cost of the final passes on levels that are one big
+ * component.  Every thread of the block must call these. */
+#define FB 1024
+#define FW (FB / 64)
+#define R_EMPTY 0xFFFFFFFFu
+#define R_MIXED 0xFFFFFFFEu
+
+struct BlockScratch {
+    uint32_t root[FW];
+    uint64_t red[FW][1 + DCF_MAX_SUMS + 6];
+};
+
+/* the root shared by all valid lanes of the block, or R_EMPTY / R_MIXED */
+__device__ __forceinline__ uint32_t block_common_root(BlockScratch &bs, bool valid, uint32_t r)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    /* the level's unions are complete: its absorbed entries end at the current count */
-    if (i == 0) *mark = *a.nabs;
-    bool valid = i < cnt;
-    uint32_t r = 0;
-    Contrib c;
-    contrib_zero(c);
-    if (valid) {
-        uint32_t p = lvl[i];
-        uint32_t tag;
-        r = uf_find(a.uf, p, tag);
-        if (r != p) par[p] = (int32_t)r;
-        contrib_own(a, grp, p, c);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t vm = __ballot(valid);
+    uint32_t code = R_EMPTY;
+    if (vm) {
+        const uint32_t rl = __shfl(r, __ffsll((unsigned long long)vm) - 1, 64);
+        code = __ballot(valid && r != rl) ? R_MIXED : rl;
     }
-    wave_add(A, valid, r, c);
+    if (lane == 0) bs.root[w] = code;
+    __syncthreads();
+    uint32_t R = R_EMPTY;
+    for (int k = 0; k < FW; k++) {
+        const uint32_t ck = bs.root[k];
+        if (ck == R_EMPTY) continue;
+        if (ck == R_MIXED || (R != R_EMPTY && ck != R)) return R_MIXED;
+        R = ck;
+    }
+    return R;
 }
 
-/* sets of the level that reached an old root are still rooted there: their canonical levelroot is
- * their minimum pixel of this level.  One atomicMin per wave and set; the first one to touch a set
- * queues its root for k_fix. */
-__global__ __launch_bounds__(256) void k_canon_min(LevelArgs a, const uint32_t *__restrict__ lvl, uint32_t cnt, uint32_t lev)
+__device__ void block_add(const DevAttr &A, BlockScratch &bs, bool valid, uint32_t r, const Contrib &c)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int lane = threadIdx.x & 63;
-    bool want = false;
-    uint32_t p = 0, r = 0;
-    if (i < cnt) {
-        p = lvl[i];
-        uint32_t tag;
-        r = uf_find(a.uf, p, tag);
-        want = (tag & UF_LVL) != lev;
+    const uint32_t R = block_common_root(bs, valid, r);
+    if (R == R_EMPTY) return;
+    if (R == R_MIXED) {
+        wave_add(A, valid, r, c);
+        return;
     }
-    uint64_t pend = __ballot(want);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t area = wsum64(valid ? c.area : 0);
+    if (lane == 0) bs.red[w][0] = area;
+    for (int i = 0; i < A.nsum; i++) {
+        const uint64_t v = wsum64(valid ? (uint64_t)c.s[i] : 0);
+        if (lane == 0) bs.red[w][1 + i] = v;
+    }
+    if (A.has_ext)
+        for (int i = 0; i < 3; i++) {
+            const uint32_t lo = wmin32(valid ? c.mn[i] : 0xFFFFFFFFu), hi = wmax32(valid ? c.mx[i] : 0u);
+            if (lane == 0) { bs.red[w][1 + DCF_MAX_SUMS + i] = lo; bs.red[w][1 + DCF_MAX_SUMS + 3 + i] = hi; }
+        }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t t = 0;
+        for (int k = 0; k < FW; k++) t += bs.red[k][0];
+        atomicAdd(&A.area[R], (uint32_t)t);
+        for (int i = 0; i < A.nsum; i++) {
+            uint64_t v = 0;
+            for (int k = 0; k < FW; k++) v += bs.red[k][1 + i];
+            if (v) atomicAdd((unsigned long long *)&A.s[i][R], (unsigned long long)v);
+        }
+        if (A.has_ext)
+            for (int i = 0; i < 3; i++) {
+                uint32_t lo = 0xFFFFFFFFu, hi = 0;
+                for (int k = 0; k < FW; k++) {
+                    lo = min(lo, (uint32_t)bs.red[k][1 + DCF_MAX_SUMS + i]);
+                    hi = max(hi, (uint32_t)bs.red[k][1 + DCF_MAX_SUMS + 3 + i]);
+                }
+                atomicMin(&A.mn[i][R], lo);
+                atomicMax(&A.mx[i][R], hi);
+            }
+    }
+}
+
+/* canonical levelroot candidates of sets still rooted at old roots (see k_final_pixels) */
+__device__ void block_canon_min(const LevelArgs &a, BlockScratch &bs, bool old, uint32_t r, uint32_t p)
+{
+    const uint32_t R = block_common_root(bs, old, r);
+    if (R == R_EMPTY) return;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (R != R_MIXED) {
+        const uint32_t pm = wmin32(old ? p : 0xFFFFFFFFu);
+        if (lane == 0) bs.red[w][0] = pm;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t m = 0xFFFFFFFFu;
+            for (int k = 0; k < FW; k++) m = min(m, (uint32_t)bs.red[k][0]);
+            if (atomicMin(&a.canon[R], m) == 0xFFFFFFFFu) a.fix[atomicAdd(a.fcount, 1u)] = R;
+        }
+        return;
+    }
+    uint64_t pend = __ballot(old);
     while (pend) {
         const int leader = __ffsll((unsigned long long)pend) - 1;
         const uint32_t rl = __shfl(r, leader, 64);
-        const bool mine = want && r == rl;
+        const bool mine = old && r == rl;
         const uint64_t m = __ballot(mine);
         if (m == (1ull << leader)) {
             if ((pend >> lane) & 1ull) {
@@ -503,6 +570,56 @@ __global__ __launch_bounds__(256) void k_canon_min(LevelArgs a, const uint32_t *
         if (lane == leader && atomicMin(&a.canon[rl], pm) == 0xFFFFFFFFu) a.fix[atomicAdd(a.fcount, 1u)] = rl;
         pend &= ~m;
     }
+}
+
+__global__ __launch_bounds__(FB) void k_final_pixels(LevelArgs a, const uint32_t *__restrict__ lvl, uint32_t cnt, uint32_t lev,
+                                                     DevAttr A, int grp, int32_t *__restrict__ par, uint32_t *__restrict__ held)
+{
+    __shared__ BlockScratch bs;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < cnt;
+    uint32_t p = 0, r = 0;
+    bool fresh = false, old = false;
+    Contrib c;
+    contrib_zero(c);
+    if (valid) {
+        p = lvl[i];
+        uint32_t tag;
+        r = uf_find(a.uf, p, tag);
+        fresh = (tag & UF_LVL) == lev;
+        old = !fresh;
+        held[i] = old ? r : 0xFFFFFFFFu;
+        if (fresh) {
+            if (r != p) par[p] = (int32_t)r;
+            contrib_own(a, grp, p, c);
+        }
+    }
+    block_add(A, bs, fresh, r, c);
+    __syncthreads();
+    block_canon_min(a, bs, old, r, p);
+}
+
+/* the pixels of re-rooted sets: parent = the canonical levelroot, contribution added there */
+__global__ __launch_bounds__(FB) void k_final_held(LevelArgs a, const uint32_t *__restrict__ lvl, uint32_t cnt,
+                                                   const uint32_t *__restrict__ held, DevAttr A, int grp,
+                                                   int32_t *__restrict__ par, uint32_t *__restrict__ mark)
+{
+    __shared__ BlockScratch bs;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    /* unions and re-rooting are complete: this level's absorbed entries end at the current count */
+    if (i == 0) *mark = *a.nabs;
+    const uint32_t r = i < cnt ? held[i] : 0xFFFFFFFFu;
+    const bool valid = r != 0xFFFFFFFFu;
+    uint32_t cr = 0;
+    Contrib c;
+    contrib_zero(c);
+    if (valid) {
+        const uint32_t p = lvl[i];
+        cr = a.canon[r];
+        if (cr != p) par[p] = (int32_t)cr;
+        contrib_own(a, grp, p, c);
+    }
+    block_add(A, bs, valid, cr, c);
 }
 
 /* re-root every queued set at its canonical levelroot c: c becomes the root (tagged with this level)
@@ -746,7 +863,8 @@ static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAtt
     la.canon = (uint32_t *)sc.get((size_t)nel * 4);
     la.fix = (uint32_t *)sc.get((size_t)(nel + 16) * 4);
     la.fcount = (uint32_t *)sc.get((NLEV + 4) * 4);
-    if (!la.canon || !la.fix || !la.fcount) return dcf_set_error(ctx, DCF_ENOMEM, "canonical root lists");
+    uint32_t *hold = (uint32_t *)sc.get((size_t)n * 4);
+    if (!la.canon || !la.fix || !la.fcount || !hold) return dcf_set_error(ctx, DCF_ENOMEM, "canonical root lists");
     DCF_LAUNCH(ctx, "k_fill", k_fill_u32f, dim3(nblk(nel, 256)), dim3(256), 0, la.canon, 0xFFFFFFFFu, nel);
     DCF_LAUNCH(ctx, "k_fill", k_fill_u32f, dim3(2), dim3(256), 0, la.fcount, 0u, (int64_t)(NLEV + 4));
     int ord = 0;
@@ -760,12 +878,14 @@ static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAtt
         case 8: DCF_LAUNCH(ctx, "k_union", k_union<8>, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, cnt, (uint32_t)lev, (uint32_t)kc); break;
         default: DCF_LAUNCH(ctx, "k_union", k_union<26>, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, cnt, (uint32_t)lev, (uint32_t)kc); break;
         }
-        DCF_LAUNCH(ctx, "k_canon_min", k_canon_min, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, cnt, (uint32_t)lev);
+        uint32_t *held = hold + off[lev];
+        DCF_LAUNCH(ctx, "k_final_pixels", k_final_pixels, dim3(nblk(cnt, FB)), dim3(FB), 0, la, lvl, cnt, (uint32_t)lev, A, grp,
+                   par, held);
         unsigned fb = nblk(cnt, 256);
         if (fb > 1024) fb = 1024;
         DCF_LAUNCH(ctx, "k_fix", k_fix, dim3(fb), dim3(256), 0, la, (uint32_t)ord, (uint32_t)lev);
-        DCF_LAUNCH(ctx, "k_final_pixels", k_final_pixels, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, cnt, A, grp, par,
-                   nabs + 2 + ord);
+        DCF_LAUNCH(ctx, "k_final_held", k_final_held, dim3(nblk(cnt, FB)), dim3(FB), 0, la, lvl, cnt, (const uint32_t *)held,
+                   A, grp, par, nabs + 2 + ord);
         unsigned ab = nblk((int64_t)cnt * la.nb.n, 256);
         if (ab > 4096) ab = 4096;
         DCF_LAUNCH(ctx, "k_final_absorbed", k_final_absorbed, dim3(ab), dim3(256), 0, la, (const uint32_t *)(nabs + 1 + ord), A,

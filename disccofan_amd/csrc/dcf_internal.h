@@ -20,6 +20,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstring>
 #include <map>
 #include <set>
 #include <string>
@@ -57,6 +58,7 @@ struct dcf_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     int profiling = 0;
+    std::string prof_only; /* empty: every kernel */
     struct Pending { std::string name; hipEvent_t a, b; };
     std::multimap<size_t, void *> free_blocks;
     std::map<void *, size_t> live_blocks;
@@ -111,10 +113,11 @@ void dcf_prof_end(dcf_ctx *ctx, const char *name, hipEvent_t a, hipEvent_t b);
 #define DCF_LAUNCH(ctx, name, kern, grid, block, shmem, ...)                                       \
     do {                                                                                           \
         hipEvent_t ea_ = nullptr, eb_ = nullptr;                                                   \
-        if ((ctx)->profiling) dcf_prof_begin((ctx), (name), &ea_, &eb_);                           \
+        if ((ctx)->profiling && ((ctx)->prof_only.empty() || strstr((name), (ctx)->prof_only.c_str()))) \
+            dcf_prof_begin((ctx), (name), &ea_, &eb_);                                             \
         hipLaunchKernelGGL(kern, (grid), (block), (shmem), (ctx)->stream, __VA_ARGS__);           \
         hipError_t le_ = hipGetLastError();                                                        \
-        if ((ctx)->profiling) dcf_prof_end((ctx), (name), ea_, eb_);                               \
+        if (ea_) dcf_prof_end((ctx), (name), ea_, eb_);                                            \
         if (le_ != hipSuccess)                                                                     \
             return dcf_set_error((ctx), DCF_EDEVICE, "launch %s: %s", (name), hipGetErrorString(le_)); \
     } while (0)

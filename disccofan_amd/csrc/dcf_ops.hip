@@ -361,8 +361,14 @@ static int rule_values(dcf_tree *t, int attr_idx, int decision, double lambda, u
     dcf_ctx *ctx = t->ctx;
     const int64_t K = t->K;
     AttrCtx c = attr_ctx(t);
+    /* For an increasing attribute (area / volume, idx 0) a node passes only if its parent passes, so
+     * the min, max and subtractive rules (filtering.c:55-261) all give the direct rule's output; the
+     * subtractive kernels can still be forced with DCF_ALL_RULES.  The min and max rules of a
+     * non-increasing attribute depend on the reference's raster scan order (its max rule keeps stale
+     * values, filtering.c:156-208) and are not implemented on this path. */
     bool general_sub = decision == DCF_SUBTRACTIVE && !(attr_idx == 0 && !(flags & DCF_ALL_RULES));
-    if (decision == DCF_DIRECT || (decision == DCF_SUBTRACTIVE && !general_sub)) {
+    bool min_max_inc = (decision == DCF_MIN || decision == DCF_MAX) && attr_idx == 0;
+    if (decision == DCF_DIRECT || (decision == DCF_SUBTRACTIVE && !general_sub) || min_max_inc) {
         int32_t *J = (int32_t *)dcf_alloc(ctx, (size_t)K * 4);
         uint8_t *pass = (uint8_t *)dcf_alloc(ctx, (size_t)K);
         if (!J || !pass) { dcf_release(ctx, J); dcf_release(ctx, pass); return dcf_set_error(ctx, DCF_ENOMEM, "filter scratch"); }
@@ -401,7 +407,8 @@ static int rule_values(dcf_tree *t, int attr_idx, int decision, double lambda, u
         dcf_release(ctx, F0); dcf_release(ctx, F1); dcf_release(ctx, A0); dcf_release(ctx, A1);
         return DCF_OK;
     }
-    return dcf_set_error(ctx, DCF_EUNSUPPORTED, "decision %d (%s) is not implemented on the GPU path", decision,
+    return dcf_set_error(ctx, DCF_EUNSUPPORTED,
+                         "decision %d (%s) is implemented on the GPU path for the increasing area attribute only", decision,
                          dcf_decision_name(decision));
 }
 
@@ -415,11 +422,12 @@ int dcf_ops_filter(dcf_tree *t, int attr_idx, int decision, double lambda, int e
     if (rc == DCF_OK) {
         ImgView iv = view_of(t);
         hipEvent_t ea = nullptr, eb = nullptr;
-        if (ctx->profiling) dcf_prof_begin(ctx, "k_filter_out", &ea, &eb);
+        if (ctx->profiling && (ctx->prof_only.empty() || strstr("k_filter_out", ctx->prof_only.c_str())))
+            dcf_prof_begin(ctx, "k_filter_out", &ea, &eb);
         hipLaunchKernelGGL(k_filter_out, dim3(nblk(t->n, 256)), dim3(256), 0, ctx->stream, iv, t->p2n, val, extract, out_dev,
                            (int)out_dtype, t->n);
         hipError_t le = hipGetLastError();
-        if (ctx->profiling) dcf_prof_end(ctx, "k_filter_out", ea, eb);
+        if (ea) dcf_prof_end(ctx, "k_filter_out", ea, eb);
         if (le != hipSuccess) rc = dcf_set_error(ctx, DCF_EDEVICE, "launch k_filter_out: %s", hipGetErrorString(le));
     }
     dcf_release(ctx, val);

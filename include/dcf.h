@@ -75,6 +75,9 @@ int dcf_ctx_synchronize(dcf_ctx *ctx);
  * launch is bracketed by two events; dcf_kernel_stats returns the launch count and the summed
  * event time of kernels whose name contains `name`. */
 int dcf_ctx_set_profiling(dcf_ctx *ctx, int on);
+/* restrict the events to kernels whose name contains `name` (NULL or "" = every kernel), so that a
+ * timed run can measure one kernel without bracketing every other launch */
+int dcf_ctx_profile_only(dcf_ctx *ctx, const char *name);
 int dcf_kernel_stats(dcf_ctx *ctx, const char *name, int64_t *launches, double *total_ms);
 int dcf_kernel_stats_reset(dcf_ctx *ctx);
 /* name of the i-th distinct profiled kernel, or NULL past the end */

@@ -82,6 +82,8 @@ def _compare_all(ctx, img, conn, tree_type, lam_area=20, lam_nc=0.25, label=""):
     for dec in (0, 3):
         assert_same(t.filter(0, lam_area, dec), o.filter(0, lam_area, dec), "filter area d%d %s" % (dec, label))
         assert_same(t.filter(0, lam_area, dec, all_rules=True), o.filter(0, lam_area, dec), "general d%d %s" % (dec, label))
+    for dec in (1, 2):
+        assert_same(t.filter(0, lam_area, dec), o.filter(0, lam_area, dec), "filter area d%d %s" % (dec, label))
     assert_same(t.filter(0, lam_area, 0, extract=True), o.filter(0, lam_area, 0, extract=True), "extract " + label)
     lv = synth.log_lambdas(16, 1, 2 ** 14)
     assert_same(t.spectrum1d(0, lv), o.spectrum1d(0, lv), "spectrum " + label)
@@ -227,3 +229,9 @@ def test_errors(ctx, tmp_path):
     with pytest.raises(dcf.DcfError) as e:
         ctx.run_operations(img, [("dmp", 0, "x")])
     assert e.value.code == -4
+    # min / max rules of a non-increasing attribute depend on the reference's scan order: refused
+    t17 = ctx.build(img, 4, "max", 17)
+    for dec in (1, 2):
+        with pytest.raises(dcf.DcfError) as e:
+            t17.filter(17, 0.3, dec)
+        assert e.value.code == -4

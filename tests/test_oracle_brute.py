@@ -1,0 +1,111 @@
+"""Pins the CPU oracle where the reference's golden vectors leave it open, with checks that do not
+restate the oracle's own code:
+
+* the canonical parent array and node areas against connected components of every upper level set
+  (scipy.ndimage.label), in 2D and 3D, for every connectivity, for max-trees and (by inversion)
+  min-trees;
+* the filter rules: for an increasing attribute (area) the direct, min, max and subtractive rules
+  coincide (a node passes iff all its ancestors pass), so a wrong sign, index or ancestor step in
+  any one of them breaks the equality; the direct rule is also checked against its definition
+  walked on the brute-force tree;
+* the 2D pattern spectrum against the 1D spectra of its two axes (marginals), which fails for a
+  transposed bin index;
+* find_scale against numpy's searchsorted.
+"""
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleTree, brute_tree, find_scale
+
+
+def rand_img(rng, shape, levels):
+    return rng.integers(0, levels, size=shape).astype(np.uint8)
+
+
+CASES_2D = [((9, 13), 4), ((9, 13), 8), ((16, 5), 4), ((7, 7), 8)]
+CASES_3D = [((4, 5, 6), 6), ((4, 5, 6), 26), ((3, 3, 7), 6)]
+
+
+@pytest.mark.parametrize("shape,conn", CASES_2D + CASES_3D)
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_parent_and_area_match_brute_force(shape, conn, seed):
+    rng = np.random.default_rng(seed * 101 + conn)
+    img = rand_img(rng, shape, 6)
+    bp, barea = brute_tree(img, conn)
+    o = OracleTree(img, conn, "max", 0, threads=1)
+    assert np.array_equal(o.parent().reshape(-1).astype(np.int64), bp)
+    assert np.array_equal(o.check(0).reshape(-1), barea.astype(np.float32))
+
+
+@pytest.mark.parametrize("shape,conn", CASES_2D[:2] + CASES_3D[:2])
+def test_min_tree_is_max_tree_of_the_inverse(shape, conn):
+    rng = np.random.default_rng(7 + conn)
+    img = rand_img(rng, shape, 5)
+    inv = (img.max() - img).astype(np.uint8)
+    bp, barea = brute_tree(inv, conn)
+    o = OracleTree(img, conn, "min", 0, threads=1)
+    assert np.array_equal(o.parent().reshape(-1).astype(np.int64), bp)
+    assert np.array_equal(o.check(0).reshape(-1), barea.astype(np.float32))
+    # filtering a min-tree returns gmax - (the max-tree filter of the inverse)
+    om = OracleTree(inv, conn, "max", 0, threads=1)
+    for lam in (2, 5, 11):
+        assert np.array_equal(o.filter(0, lam, 0), img.max() - om.filter(0, lam, 0))
+
+
+def brute_direct(img, parent, area, lam):
+    """direct rule by definition: the grey of the nearest ancestor-or-self node whose area >= lam
+    (each pixel's node is its levelroot), 0 when there is none"""
+    flat = img.reshape(-1)
+    out = np.zeros(flat.size, dtype=np.float32)
+    for p in range(flat.size):
+        v = p if parent[p] < 0 or flat[parent[p]] != flat[p] else parent[p]  # levelroot of p
+        while v >= 0 and area[v] < lam:
+            v = parent[v]
+        out[p] = flat[v] if v >= 0 else 0
+    return out.reshape(img.shape)
+
+
+@pytest.mark.parametrize("shape,conn", CASES_2D + CASES_3D)
+def test_rules_against_definition(shape, conn):
+    rng = np.random.default_rng(99 + conn)
+    img = rand_img(rng, shape, 7)
+    bp, barea = brute_tree(img, conn)
+    o = OracleTree(img, conn, "max", 0, threads=1)
+    for lam in (1, 2, 3, 8, 25, img.size, img.size + 1):
+        d = o.filter(0, lam, 0)
+        assert np.array_equal(d, brute_direct(img, bp, barea, lam)), lam
+        for rule in (1, 2, 3):
+            assert np.array_equal(o.filter(0, lam, rule), d), (lam, rule)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_rules_coincide_for_area_on_larger_images(seed):
+    rng = np.random.default_rng(seed)
+    img = rng.integers(0, 40, size=(48, 64)).astype(np.uint8)
+    o = OracleTree(img, 8, "max", 0, threads=1)
+    for lam in (3, 17, 200):
+        d = o.filter(0, lam, 0)
+        for rule in (1, 2, 3):
+            assert np.array_equal(o.filter(0, lam, rule), d)
+
+
+def test_spectrum2d_marginals_are_the_1d_spectra():
+    rng = np.random.default_rng(5)
+    img = rng.integers(0, 30, size=(40, 56)).astype(np.uint8)
+    l1 = np.array([0.0, 0.16, 0.2, 0.3, 0.5, 1.0, 2.0], dtype=np.float32)  # ncomp axis
+    l2 = np.array([0.0, 1, 2, 4, 8, 16, 64, 256, 1024], dtype=np.float32)  # area axis
+    o4 = OracleTree(img, 4, "max", 17, threads=1)
+    s2 = o4.spectrum2d(17, l1, 0, l2).reshape(len(l2), len(l1))  # bin s1 + s2 * n1
+    assert np.array_equal(s2.sum(axis=0), o4.spectrum1d(17, l1))
+    o0 = OracleTree(img, 4, "max", 0, threads=1)
+    assert np.array_equal(s2.sum(axis=1), o0.spectrum1d(0, l2))
+    # every bin sums to the total grey of the image (background = 1)
+    assert s2.sum() == img.astype(np.int64).sum()
+
+
+def test_find_scale_is_searchsorted():
+    lv = np.array([0.0, 1.0, 1.5, 2.25, 3.375, 10.0, 100.0], dtype=np.float32)
+    vals = np.concatenate([lv.astype(np.float64), lv.astype(np.float64) - 1e-9, [0.0, 5.0, 1e9, 99.99]])
+    for a in vals:
+        want = max(int(np.searchsorted(lv.astype(np.float64), a, side="right")) - 1, 0)
+        assert find_scale(lv, a) == want, a
