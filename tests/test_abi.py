@@ -76,3 +76,27 @@ def test_context_without_gpu_fails_loudly():
     import disccofan_amd as d
     with pytest.raises(d.DcfError):
         d.Context(0)
+
+
+def _build_c_caller(tmp_path):
+    import subprocess
+    exe = str(tmp_path / "c_caller")
+    libdir = os.path.join(ROOT, "disccofan_amd", "lib")
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "c", "c_caller.c"), "-L", libdir, "-ldcf",
+                    "-Wl,-rpath," + libdir, "-o", exe], check=True)
+    return exe
+
+
+def test_c_caller_links_and_reads_tables(tmp_path):
+    """include/dcf.h compiles as C99 and a C program links libdcf.so and calls the host-only entry points"""
+    import subprocess
+    import disccofan_amd as d
+    exe = _build_c_caller(tmp_path)
+    r = subprocess.run([exe, "tables"], capture_output=True, text=True, check=True)
+    rows = [l.split() for l in r.stdout.splitlines() if not l.startswith("decision")]
+    assert [row[1] for row in rows] == d.ATTRIBUTES
+    lv = os.path.join(ROOT, "tests", "golden", "lvec.txt")
+    r = subprocess.run([exe, "lvec", lv], capture_output=True, text=True, check=True)
+    vals = [float(x) for x in r.stdout.split()[2:]]
+    assert vals == [float(x) for x in d.lvec_read(lv)]

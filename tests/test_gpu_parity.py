@@ -122,6 +122,30 @@ def test_random_3d(ctx, seed):
             _compare_all(ctx, img, conn, tt, label="rand3d s%d" % seed)
 
 
+@pytest.mark.parametrize("shape,conn", [((57, 83), 4), ((61, 47), 8), ((9, 23, 31), 6), ((7, 19, 17), 26)])
+def test_every_attribute(ctx, shape, conn):
+    """check() of all 18 attributes (every group of AttribsArray) against the oracle; the eigenvalue
+    attributes 14-16 come from closed-form 2x2 / 3x3 solvers on both sides and may differ in the last
+    float32 bit, every other attribute is bit-exact."""
+    rng = np.random.default_rng(300 + conn)
+    img = rng.integers(0, 40, size=shape).astype(np.uint8)
+    say("every attribute", shape, conn)
+    t = ctx.build(img, conn, "max", 0)
+    o = OracleTree(img, conn, "max", 0)
+    for a in range(18):
+        t.accumulate(a)
+        o.reaccumulate(a)
+        g, w = t.check(a), o.check(a)
+        if a in (14, 15, 16):
+            ulp = np.abs(g.view(np.int32).astype(np.int64) - w.view(np.int32).astype(np.int64))
+            assert ulp.max() <= 1, "attribute %d differs by %d ulp" % (a, ulp.max())
+        else:
+            assert_same(g, w, "check attr %d" % a)
+        lam = float(np.median(w))
+        if a not in (14, 15, 16):
+            assert_same(t.filter(a, lam, 0), o.filter(a, lam, 0), "direct attr %d" % a)
+
+
 EDGE = {
     "1x1": np.array([[7]], np.uint8),
     "1xN": (np.arange(300) * 7 % 251).astype(np.uint8).reshape(1, 300),
@@ -235,3 +259,29 @@ def test_errors(ctx, tmp_path):
         with pytest.raises(dcf.DcfError) as e:
             t17.filter(17, 0.3, dec)
         assert e.value.code == -4
+
+
+def test_c_caller_on_gpu(tmp_path):
+    """a plain C program (tests/c/c_caller.c) drives build + filter + spectrum through the C ABI;
+    its outputs agree with the oracle"""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    libdir = os.path.join(root, "disccofan_amd", "lib")
+    exe = str(tmp_path / "c_caller")
+    subprocess.run(["gcc", "-std=c99", "-O1", "-I", os.path.join(root, "include"), os.path.join(root, "tests", "c", "c_caller.c"),
+                    "-L", libdir, "-ldcf", "-Wl,-rpath," + libdir, "-o", exe], check=True)
+    say("c_caller run 300 200")
+    r = subprocess.run([exe, "run", "300", "200"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = dict(l.split() for l in r.stdout.splitlines())
+    # the same synthetic image as c_caller.c (64-bit LCG, top 5 bits)
+    s = 12345
+    vals = np.empty(300 * 200, np.uint8)
+    for i in range(vals.size):
+        s = (s * 6364136223846793005 + 1442695040888963407) % (1 << 64)
+        vals[i] = s >> 59
+    img = vals.reshape(200, 300)
+    o = OracleTree(img, 4, "max", 0)
+    assert int(got["nodes"]) == o.stats()["nodes"]
+    assert int(got["filter_sum"]) == int(o.filter(0, 25.0, 0).astype(np.int64).sum())
+    assert float(got["spectrum_sum"]) == float(got["grey_sum"]) == float(img.astype(np.int64).sum())
