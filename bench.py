@@ -32,6 +32,10 @@ WORKLOADS = {
                             "lambda=1000 subtractive"),
     "spheres512": dict(shape=(512, 512, 512), dtype="u8", conn=6, trees=("max",), attr=0, lam=500.0, decision=0,
                        desc="512^3 u8 blurred noise + 5000 spheres, 6-conn max-tree, volume opening lambda=500 direct"),
+    "batch256": dict(kind="batch", count=256, lo=64, hi=2048, conn=4, trees=("max",), attr=0, decision=0,
+                     desc="256 2D images, W and H uniform in [64,2048], half u8 half u16 (blurred noise + disks, "
+                          "per-image seed), 4-conn max-tree, area opening with per-image lambda in [10,5000] direct, "
+                          "one batched call"),
 }
 
 
@@ -51,12 +55,34 @@ def algorithmic_bytes(kernel: str, n: int, k_nodes: int, B: int, passes: int):
     return table.get(kernel, (0, 0.0))
 
 
+def make_batch(w: dict):
+    from disccofan_amd import synth
+    shapes = synth.batch_shapes(w["count"], w["lo"], w["hi"])
+    imgs = [synth.disks(H, W, is16, seed) for (W, H, is16, seed, lam) in shapes]
+    return imgs, [lam for (W, H, is16, seed, lam) in shapes]
+
+
 def make_image(name: str) -> np.ndarray:
     from disccofan_amd import synth
     w = WORKLOADS[name]
     if name == "vnoise8192":
         return synth.vnoise(*w["shape"])
     return synth.spheres(*w["shape"])
+
+
+def cpu_baseline_batch(imgs, lams, w: dict, threads: int):
+    """The CPU oracle on every image of the batch, build + filter with the image's own lambda."""
+    from oracle.oracle import OracleTree
+    t0 = time.perf_counter()
+    vox = 0
+    for a, lam in zip(imgs, lams):
+        o = OracleTree(a, w["conn"], "max", w["attr"], threads=threads)
+        o.filter(w["attr"], lam, w["decision"])
+        vox += a.size
+        del o
+    dt = time.perf_counter() - t0
+    return {"value": vox / dt / 1e6, "unit": "Mvoxels/s", "cores": threads, "kind": "port",
+            "sample": "all %d images of the batch, build+filter, %d OpenMP threads, %.2f s" % (len(imgs), threads, dt)}
 
 
 def cpu_baseline(img: np.ndarray, w: dict, threads: int):
@@ -122,30 +148,53 @@ def main():
     import disccofan_amd as dcf
 
     w = WORKLOADS[args.workload]
-    t0 = time.perf_counter()
-    img = make_image(args.workload)
-    gen_s = time.perf_counter() - t0
-    n = img.size
-    B = img.dtype.itemsize
-    tdt = torch.int16 if B == 2 else torch.uint8
+    batch = w.get("kind") == "batch"
     dev = torch.device("cuda:0")
-    d_img = torch.from_numpy(img.reshape(-1).view(np.int16 if B == 2 else np.uint8)).to(dev)
-    d_out = torch.empty(n, dtype=tdt, device=dev)
-    torch.cuda.synchronize()
-
     ctx = dcf.Context(0)
     stream = torch.cuda.current_stream()
     ctx.set_stream(stream.cuda_stream)
-    np_dtype = np.uint16 if B == 2 else np.uint8
-
     state = {}
+    t0 = time.perf_counter()
+    if batch:
+        imgs, lams = make_batch(w)
+        gen_s = time.perf_counter() - t0
+        n = int(sum(a.size for a in imgs))
+        B = sum(a.nbytes for a in imgs) / n
+        passes = 1 if all(a.dtype == np.uint8 for a in imgs) else 2
+        d_in = [torch.from_numpy(a.view(np.int16) if a.dtype == np.uint16 else a).to(dev) for a in imgs]
+        d_out = [torch.empty_like(t) for t in d_in]
+        torch.cuda.synchronize()
+        items = [(ti.data_ptr(), a.dtype, a.shape, lam, to.data_ptr()) for ti, a, lam, to in zip(d_in, imgs, lams, d_out)]
+        shape_desc = [len(imgs), n]
 
-    def step():
-        for tt in w["trees"]:
-            tr = ctx.build_device(d_img.data_ptr(), np_dtype, img.shape, w["conn"], tt, w["attr"])
-            tr.filter_device(w["attr"], w["lam"], w["decision"], d_out.data_ptr(), np_dtype)
-            state["K"] = tr.num_nodes
-            tr.free()
+        def step():
+            ctx.batch_filter_device(items, connectivity=w["conn"], tree_type="max", attr_idx=w["attr"],
+                                    decision=w["decision"])
+
+        def plain_step():
+            for ti, a, lam, to in zip(d_in, imgs, lams, d_out):
+                tr = ctx.build_device(ti.data_ptr(), a.dtype, a.shape, w["conn"], "max", w["attr"])
+                tr.filter_device(w["attr"], lam, w["decision"], to.data_ptr(), a.dtype)
+                tr.free()
+    else:
+        img = make_image(args.workload)
+        gen_s = time.perf_counter() - t0
+        n = img.size
+        B = img.dtype.itemsize
+        passes = 2 if B == 2 else 1
+        tdt = torch.int16 if B == 2 else torch.uint8
+        d_img = torch.from_numpy(img.reshape(-1).view(np.int16 if B == 2 else np.uint8)).to(dev)
+        d_out = torch.empty(n, dtype=tdt, device=dev)
+        torch.cuda.synchronize()
+        np_dtype = np.uint16 if B == 2 else np.uint8
+        shape_desc = list(img.shape)
+
+        def step():
+            for tt in w["trees"]:
+                tr = ctx.build_device(d_img.data_ptr(), np_dtype, img.shape, w["conn"], tt, w["attr"])
+                tr.filter_device(w["attr"], w["lam"], w["decision"], d_out.data_ptr(), np_dtype)
+                state["K"] = tr.num_nodes
+                tr.free()
 
     for i in range(args.warmup):
         t1 = time.perf_counter()
@@ -182,7 +231,6 @@ def main():
                 ms -= m2
         exact[k] = (cnt, ms)
     K = state.get("K", 0)
-    passes = 2 if B == 2 else 1
     step_bytes = float(vox_per_step) * (3 * B + 20)
     roof = None
     if exact:
@@ -213,12 +261,27 @@ def main():
                 c, m = exact[k]
                 print("%-16s launches %5d  total %9.3f ms  avg %8.4f ms  %5.1f%%" % (k, c, m, m / max(c, 1), 100 * m / tot),
                       file=sys.stderr)
-    print("nodes K = %d (%.1f%% of pixels)" % (K, 100.0 * K / n), file=sys.stderr)
+    if not batch:
+        print("nodes K = %d (%.1f%% of pixels)" % (K, 100.0 * K / n), file=sys.stderr)
+
+    plain = None
+    if batch:
+        # the plain form on the same device-resident images: one build + filter call per image
+        plain_step()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        plain_step()
+        torch.cuda.synchronize()
+        pms = (time.perf_counter() - t1) * 1e3
+        plain = {"value": round(n / (pms / 1e3) / 1e6, 3), "ms_per_step": round(pms, 3),
+                 "form": "one dcf_tree_build + dcf_filter per image, same images and lambdas, 1 timed step"}
+        print("plain per-image loop: %.1f ms (%.1f Mvox/s); batched %.1f ms" % (pms, plain["value"], ms_per_step),
+              file=sys.stderr)
 
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(img, w, threads)
+        cpu = cpu_baseline_batch(imgs, lams, w, threads) if batch else cpu_baseline(img, w, threads)
 
     line = {
         "metric": "Mvoxels/s (build+filter)",
@@ -231,13 +294,16 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u16" if B == 2 else "u8",
+        "dtype": "u8+u16" if batch else ("u16" if B == 2 else "u8"),
         "data": "synthetic (seeded generator, disccofan_amd/synth)",
-        "config": {"workload": w["desc"], "shape": list(img.shape), "voxels_per_step": vox_per_step,
+        "config": {"workload": w["desc"], "shape": shape_desc, "voxels_per_step": vox_per_step,
                    "connectivity": w["conn"], "trees": list(w["trees"]), "attribute": w["attr"],
-                   "lambda": w["lam"], "decision": w["decision"], "bit_exact_vs_oracle": "see tests/test_gpu_fullsize.py"},
+                   "lambda": "per image" if batch else w["lam"], "decision": w["decision"],
+                   "bit_exact_vs_oracle": "tests/test_gpu_batch.py" if batch else "tests/test_gpu_fullsize.py"},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "plain_form": plain,
+        "batched_vs_plain": round(plain["ms_per_step"] / ms_per_step, 3) if plain else None,
         "gen_s": round(gen_s, 2),
     }
     print(json.dumps(line), flush=True)

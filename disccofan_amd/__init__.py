@@ -18,6 +18,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import sys
 import weakref
 
 import numpy as np
@@ -40,7 +41,7 @@ DECISIONS = ["Direct", "Min", "Max", "Subtractive"]
 # exported symbols of libdcf.so (kept in sync with include/dcf.h)
 EXPORTS = [
     "dcf_ctx_create", "dcf_ctx_destroy", "dcf_ctx_set_stream", "dcf_ctx_stream", "dcf_last_error",
-    "dcf_ctx_synchronize", "dcf_ctx_set_profiling", "dcf_ctx_profile_only", "dcf_kernel_stats", "dcf_kernel_stats_reset",
+    "dcf_ctx_synchronize", "dcf_ctx_set_profiling", "dcf_ctx_profile_only", "dcf_batch_filter", "dcf_kernel_stats", "dcf_kernel_stats_reset",
     "dcf_kernel_name", "dcf_tree_build", "dcf_tree_free", "dcf_tree_accumulate", "dcf_tree_group",
     "dcf_tree_parent", "dcf_tree_num_nodes", "dcf_filter", "dcf_check", "dcf_spectrum1d", "dcf_spectrum2d",
     "dcf_lvec_read", "dcf_lvec_free", "dcf_attr_name", "dcf_attr_short_name", "dcf_attr_group",
@@ -54,6 +55,12 @@ class DcfError(RuntimeError):
         self.code = code
 
 
+class BatchImage(C.Structure):
+    """dcf_batch_image of include/dcf.h"""
+    _fields_ = [("img", C.c_void_p), ("dtype", C.c_int), ("dims", C.c_uint64 * 3), ("lam", C.c_double),
+                ("out", C.c_void_p)]
+
+
 _lib = None
 
 
@@ -64,6 +71,15 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError("libdcf.so not built: run `python -m disccofan_amd.build` (%s missing)" % LIB_PATH)
+    # PyTorch ships its own HIP runtime: when both live in one process, torch's must be initialised
+    # first (a runtime loaded before it hides the GPU from torch), so do that when torch is imported
+    torch = sys.modules.get("torch")
+    if torch is not None:
+        try:
+            if torch.cuda.is_available():
+                torch.cuda.init()
+        except Exception:
+            pass
     L = C.CDLL(LIB_PATH)
     vp, i, u, d = C.c_void_p, C.c_int, C.c_uint, C.c_double
     u64p = C.POINTER(C.c_uint64)
@@ -77,6 +93,7 @@ def lib():
     L.dcf_ctx_synchronize.argtypes = [vp]
     L.dcf_ctx_set_profiling.argtypes = [vp, i]
     L.dcf_ctx_profile_only.argtypes = [vp, C.c_char_p]
+    L.dcf_batch_filter.argtypes = [vp, C.POINTER(BatchImage), i, i, i, i, i, i, u]
     L.dcf_kernel_stats.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
     L.dcf_kernel_stats_reset.argtypes = [vp]
     L.dcf_kernel_name.restype = C.c_char_p
@@ -227,6 +244,41 @@ class Context:
         t = Tree(self, h, shape, code, keep)
         self._trees.add(t)
         return t
+
+    def batch_filter(self, images, lambdas, connectivity: int = 4, tree_type: str = "max", attr_idx: int = 0,
+                     decision: int = 0, extract: bool = False, all_rules: bool = False):
+        """The batched form: every host image (u8 or u16, all 2D or all 3D) gets its own tree and is
+        filtered with its own lambda, all in one build over the batch.  Returns the outputs."""
+        arrs = [np.ascontiguousarray(a) for a in images]
+        outs = [np.empty_like(a) for a in arrs]
+        items = (BatchImage * len(arrs))()
+        for k, (a, o, lam) in enumerate(zip(arrs, outs, lambdas)):
+            items[k].img = a.ctypes.data
+            items[k].dtype = _dtype_code(a.dtype)
+            items[k].dims = (C.c_uint64 * 3)(*_dims_of(a.shape))
+            items[k].lam = float(lam)
+            items[k].out = o.ctypes.data
+        self._batch(items, len(arrs), connectivity, tree_type, attr_idx, decision, extract, all_rules, 0)
+        return outs
+
+    def batch_filter_device(self, items, connectivity: int = 4, tree_type: str = "max", attr_idx: int = 0,
+                            decision: int = 0, extract: bool = False, all_rules: bool = False):
+        """Same on device buffers: items = [(img_ptr, dtype, shape, lam, out_ptr), ...]."""
+        arr = (BatchImage * len(items))()
+        for k, (ip, dt, shape, lam, op) in enumerate(items):
+            arr[k].img = ip
+            arr[k].dtype = _dtype_code(dt)
+            arr[k].dims = (C.c_uint64 * 3)(*_dims_of(shape))
+            arr[k].lam = float(lam)
+            arr[k].out = op
+        self._batch(arr, len(items), connectivity, tree_type, attr_idx, decision, extract, all_rules, DCF_DEVICE_PTR)
+
+    def _batch(self, arr, count, connectivity, tree_type, attr_idx, decision, extract, all_rules, flags):
+        if tree_type not in ("max", "min"):
+            raise ValueError("tree_type must be 'max' or 'min'")
+        fl = flags | (DCF_ALL_RULES if all_rules else 0)
+        self._check(lib().dcf_batch_filter(self._h, arr, int(count), int(connectivity), 1 if tree_type == "min" else 0,
+                                           int(attr_idx), int(decision), 1 if extract else 0, fl))
 
     def run_operations(self, img, operations, connectivity: int = 4, tree_type: str = "max", pixdim=(1.0, 1.0, 1.0)):
         """loop_operations (disccofan.c:137-205): run the reference's operation tuples in order.

@@ -361,6 +361,163 @@ fail:
     return rc;
 }
 
+/* ------------------------------------------------------------------------------------------------ */
+/* batched form                                                                                     */
+/* ------------------------------------------------------------------------------------------------ */
+extern "C" int dcf_batch_filter(dcf_ctx *ctx, const dcf_batch_image *images, int count, int connectivity, int tree_type,
+                                int attr_idx, int decision, int extract, unsigned flags)
+{
+    if (!ctx) return DCF_EINVAL;
+    if (!images || count <= 0) return dcf_set_error(ctx, DCF_EINVAL, "empty batch");
+    if (tree_type != DCF_MAX_TREE && tree_type != DCF_MIN_TREE) return dcf_set_error(ctx, DCF_EINVAL, "bad tree type %d", tree_type);
+    if (attr_idx < 0 || attr_idx >= DCF_NUM_ATTR) return dcf_set_error(ctx, DCF_EINVAL, "bad attribute index %d", attr_idx);
+    if (decision < 0 || decision > 3) return dcf_set_error(ctx, DCF_EINVAL, "bad decision %d", decision);
+    std::vector<BatchImg> tab((size_t)count);
+    std::vector<double> lam((size_t)count);
+    uint64_t N = 0;
+    int is3d = -1, all_u8 = 1;
+    size_t max_u8 = 0;
+    for (int i = 0; i < count; i++) {
+        const dcf_batch_image &b = images[i];
+        if (!b.img || !b.out) return dcf_set_error(ctx, DCF_EINVAL, "image %d: null pointer", i);
+        if (b.dtype != DCF_U8 && b.dtype != DCF_U16) return dcf_set_error(ctx, DCF_EINVAL, "image %d: dtype must be u8 or u16", i);
+        if (!b.dims[0] || !b.dims[1] || !b.dims[2]) return dcf_set_error(ctx, DCF_EINVAL, "image %d: empty", i);
+        if (b.dims[0] > 0x7FFFFFFFull || b.dims[1] > 0x7FFFFFFFull || b.dims[2] > 0x7FFFFFFFull)
+            return dcf_set_error(ctx, DCF_EINVAL, "image %d: dimension too large", i);
+        const int d3 = b.dims[2] > 1;
+        if (is3d < 0) is3d = d3;
+        else if (is3d != d3) return dcf_set_error(ctx, DCF_EINVAL, "image %d: a batch is all 2D or all 3D", i);
+        const uint64_t n = b.dims[0] * b.dims[1] * b.dims[2];
+        tab[i].off = N;
+        tab[i].W = (uint32_t)b.dims[0];
+        tab[i].H = (uint32_t)b.dims[1];
+        tab[i].D = (uint32_t)b.dims[2];
+        tab[i].pad = 0;
+        lam[i] = b.lambda;
+        N += n;
+        if (N >= (1ull << 30)) return dcf_set_error(ctx, DCF_EUNSUPPORTED, "a batch is limited to 2^30 pixels");
+        if (b.dtype != DCF_U8) all_u8 = 0;
+        else if (n > max_u8) max_u8 = (size_t)n;
+    }
+    /* check_connectivity (io.c:64-79) */
+    int conn = connectivity;
+    if (is3d && conn == 4) conn = 6;
+    else if (conn != 4 && conn != 6 && conn != 8 && conn != 26)
+        return dcf_set_error(ctx, DCF_EINVAL, "wrong connectivity %d (must be 4 or 8 for 2D, 6 or 26 for 3D)", connectivity);
+    else if (conn == 8 && is3d) conn = 26;
+    else if (conn == 26 && !is3d) conn = 8;
+    if (!is3d && conn == 6) conn = 4;
+    if (hipSetDevice(ctx->device) != hipSuccess) return dcf_set_error(ctx, DCF_EDEVICE, "hipSetDevice");
+
+    const dcf_dtype cdt = all_u8 ? DCF_U8 : DCF_U16;
+    const size_t csz = all_u8 ? 1 : 2;
+    const bool dev = (flags & DCF_DEVICE_PTR) != 0;
+    dcf_tree *t = new dcf_tree();
+    t->ctx = ctx;
+    t->dims[0] = N;
+    t->n = (int64_t)N;
+    t->conn = conn;
+    t->dtype = cdt;
+    t->nbatch = count;
+    t->batch3d = is3d;
+    BatchImg *dtab = nullptr;
+    double *dlam = nullptr;
+    BatchOut *douts = nullptr;
+    uint32_t *gm = nullptr;
+    uint8_t *staging = nullptr, *obuf = nullptr;
+    std::vector<BatchOut> outs((size_t)count);
+    std::vector<size_t> boff((size_t)count);
+    int rc = DCF_OK;
+    t->img = dcf_alloc(ctx, (size_t)N * csz);
+    t->own_img = 1;
+    dtab = (BatchImg *)dcf_alloc(ctx, (size_t)count * sizeof(BatchImg));
+    dlam = (double *)dcf_alloc(ctx, (size_t)count * 8);
+    douts = (BatchOut *)dcf_alloc(ctx, (size_t)count * sizeof(BatchOut));
+    t->gmax = (float *)dcf_alloc(ctx, 16);
+    t->counters = (uint32_t *)dcf_alloc(ctx, 64 * 4);
+    t->parent = (int32_t *)dcf_alloc(ctx, (size_t)N * 4);
+    t->p2n = (uint32_t *)dcf_alloc(ctx, (size_t)N * 4);
+    if (!t->img || !dtab || !dlam || !douts || !t->gmax || !t->counters || !t->parent || !t->p2n) {
+        rc = dcf_set_error(ctx, DCF_ENOMEM, "batch arrays");
+        goto done;
+    }
+    t->bimg = dtab;
+    if (hipMemcpyAsync(dtab, tab.data(), tab.size() * sizeof(BatchImg), hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(dlam, lam.data(), lam.size() * 8, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) {
+        rc = dcf_set_error(ctx, DCF_EDEVICE, "batch table upload");
+        goto done;
+    }
+    /* the images, back to back in the batch's key type */
+    if (!dev && !all_u8 && max_u8) {
+        staging = (uint8_t *)dcf_alloc(ctx, max_u8);
+        if (!staging) { rc = dcf_set_error(ctx, DCF_ENOMEM, "batch staging"); goto done; }
+    }
+    for (int i = 0; i < count && rc == DCF_OK; i++) {
+        const dcf_batch_image &b = images[i];
+        const int64_t n = (int64_t)(b.dims[0] * b.dims[1] * b.dims[2]);
+        uint8_t *dst = (uint8_t *)t->img + tab[i].off * csz;
+        if (dev) {
+            rc = dcf_batch_copy_in(ctx, b.img, b.dtype, dst, cdt, n);
+        } else if (b.dtype == cdt) {
+            if (hipMemcpyAsync(dst, b.img, (size_t)n * csz, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+                rc = dcf_set_error(ctx, DCF_EDEVICE, "image %d upload", i);
+        } else {
+            if (hipMemcpyAsync(staging, b.img, (size_t)n, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+                rc = dcf_set_error(ctx, DCF_EDEVICE, "image %d upload", i);
+            else
+                rc = dcf_batch_copy_in(ctx, staging, DCF_U8, dst, cdt, n);
+        }
+    }
+    if (rc != DCF_OK) goto done;
+    /* min-trees: every image inverted against its own grey maximum (preprocessing.c:7-30,84-91) */
+    if (tree_type == DCF_MIN_TREE) {
+        gm = (uint32_t *)dcf_alloc(ctx, (size_t)count * 4);
+        if (!gm) { rc = dcf_set_error(ctx, DCF_ENOMEM, "batch grey maxima"); goto done; }
+        if ((rc = dcf_batch_invert(ctx, t->img, cdt, dtab, count, gm, (int64_t)N)) != DCF_OK) goto done;
+    }
+    if ((rc = dcf_build_tree(t, dcf_group_of(attr_idx))) != DCF_OK) goto done;
+    /* outputs: straight into device buffers, or into one staging buffer copied back per image */
+    {
+        size_t ob = 0;
+        for (int i = 0; i < count; i++) {
+            boff[i] = ob;
+            ob += (size_t)(images[i].dims[0] * images[i].dims[1] * images[i].dims[2]) * (images[i].dtype == DCF_U8 ? 1 : 2);
+        }
+        if (!dev) {
+            obuf = (uint8_t *)dcf_alloc(ctx, ob);
+            if (!obuf) { rc = dcf_set_error(ctx, DCF_ENOMEM, "batch output staging"); goto done; }
+        }
+        for (int i = 0; i < count; i++) {
+            outs[i].ptr = dev ? images[i].out : (void *)(obuf + boff[i]);
+            outs[i].dtype = images[i].dtype;
+        }
+        if (hipMemcpyAsync(douts, outs.data(), outs.size() * sizeof(BatchOut), hipMemcpyHostToDevice, ctx->stream) != hipSuccess) {
+            rc = dcf_set_error(ctx, DCF_EDEVICE, "batch output table upload");
+            goto done;
+        }
+    }
+    if ((rc = dcf_ops_batch_filter(t, attr_idx, decision, dlam, extract, douts, gm, flags)) != DCF_OK) goto done;
+    if (!dev)
+        for (int i = 0; i < count && rc == DCF_OK; i++) {
+            const size_t bytes = (size_t)(images[i].dims[0] * images[i].dims[1] * images[i].dims[2]) * (images[i].dtype == DCF_U8 ? 1 : 2);
+            if (hipMemcpyAsync(images[i].out, obuf + boff[i], bytes, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
+                rc = dcf_set_error(ctx, DCF_EDEVICE, "image %d download", i);
+        }
+done:
+    /* the host tables above are read by asynchronous copies: wait before they go out of scope */
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess && rc == DCF_OK) rc = dcf_set_error(ctx, DCF_EDEVICE, "batch synchronize");
+    t->bimg = nullptr;
+    tree_release(t);
+    delete t;
+    dcf_release(ctx, dtab);
+    dcf_release(ctx, dlam);
+    dcf_release(ctx, douts);
+    dcf_release(ctx, gm);
+    dcf_release(ctx, staging);
+    dcf_release(ctx, obuf);
+    return rc;
+}
+
 extern "C" int dcf_tree_accumulate(dcf_tree *t, int attr_idx)
 {
     if (!t) return DCF_EINVAL;

@@ -232,14 +232,34 @@ struct LevelArgs {
     const uint32_t *cnode; /* coarse node id of every pixel */
     const int32_t *lift;   /* LIFT x Kc: 2^j-th coarse ancestor (node id), -1 above the root */
     const uint32_t *cnkey; /* coarse key of every coarse node */
+    /* batched form: image table (nullptr for one image) */
+    const BatchImg *bimg;
+    int nbatch;
 };
 
-__device__ __forceinline__ void coords(const Geo &g, int64_t p, int &x, int &y, int &z)
+/* local coordinates of pixel p and the dimensions of its image (pixel ids are below 2^30, so the
+ * local index fits 32-bit division) */
+struct PixGeo {
+    int x, y, z;
+    int W, H, D;
+};
+
+__device__ __forceinline__ PixGeo pix_geo(const LevelArgs &a, uint32_t p)
 {
-    int64_t WH = (int64_t)g.W * g.H;
-    x = (int)(p % g.W);
-    y = (int)((p / g.W) % g.H);
-    z = (int)(p / WH);
+    PixGeo g;
+    uint32_t local = p;
+    if (a.bimg) {
+        const BatchImg b = a.bimg[batch_locate(a.bimg, a.nbatch, p)];
+        local = p - (uint32_t)b.off;
+        g.W = (int)b.W; g.H = (int)b.H; g.D = (int)b.D;
+    } else {
+        g.W = a.g.W; g.H = a.g.H; g.D = a.g.D;
+    }
+    const uint32_t row = local / (uint32_t)g.W;
+    g.x = (int)(local - row * (uint32_t)g.W);
+    g.z = (int)(row / (uint32_t)g.H);
+    g.y = (int)(row - (uint32_t)g.z * (uint32_t)g.H);
+    return g;
 }
 
 /* the child of coarse node N (id nid) on the path up from its descendant e: e itself when its
@@ -276,9 +296,9 @@ __global__ __launch_bounds__(256) void k_union(LevelArgs a, const uint32_t *__re
     const uint32_t NONE = 0xFFFFFFFFu;
     const bool valid = i < cnt;
     const uint32_t p = valid ? lvl[i] : 0u;
-    int x = 0, y = 0, z = 0;
-    coords(a.g, p, x, y, z);
-    const int64_t WH = (int64_t)a.g.W * a.g.H;
+    const PixGeo pg = pix_geo(a, p);
+    const int x = pg.x, y = pg.y, z = pg.z;
+    const int WH = pg.W * pg.H;
     const uint32_t kp = fl_key(a.iv, p);
     const uint32_t cp = kp >> 8;
     const uint32_t nid = (valid && a.pass == 2) ? a.cnode[p] : 0u;
@@ -293,8 +313,8 @@ __global__ __launch_bounds__(256) void k_union(LevelArgs a, const uint32_t *__re
             const int k = k0 + j;
             if (k >= NB || !valid) continue;
             int nx = x + a.nb.dx[k], ny = y + a.nb.dy[k], nz = z + a.nb.dz[k];
-            if (nx < 0 || nx >= a.g.W || ny < 0 || ny >= a.g.H || nz < 0 || nz >= a.g.D) continue;
-            uint32_t q = (uint32_t)(nx + (int64_t)ny * a.g.W + nz * WH);
+            if (nx < 0 || nx >= pg.W || ny < 0 || ny >= pg.H || nz < 0 || nz >= pg.D) continue;
+            const uint32_t q = (uint32_t)((int)p + a.nb.dx[k] + a.nb.dy[k] * pg.W + a.nb.dz[k] * WH);
             uint32_t kq = fl_key(a.iv, q);
             if (a.pass == 1) {
                 uint32_t dq = kq >> a.shift;
@@ -384,8 +404,8 @@ __device__ __forceinline__ void contrib_own(const LevelArgs &a, int grp, uint32_
 {
     c.area = 1;
     if (grp == 0) return;
-    int x, y, z;
-    coords(a.g, p, x, y, z);
+    const PixGeo pg = pix_geo(a, p);
+    const int x = pg.x, y = pg.y, z = pg.z;
     if (grp == 1) {
         c.mn[0] = (uint32_t)x; c.mn[1] = (uint32_t)y; c.mn[2] = (uint32_t)z;
         c.mx[0] = (uint32_t)x + 1; c.mx[1] = (uint32_t)y + 1; c.mx[2] = (uint32_t)z + 1;
@@ -899,7 +919,7 @@ int dcf_flood_build(dcf_tree *t, int grp)
 {
     dcf_ctx *ctx = t->ctx;
     const int64_t n = t->n;
-    const int is3d = t->dims[2] > 1;
+    const int is3d = t->bimg ? t->batch3d : t->dims[2] > 1;
     /* union-find ids (pixels + coarse super-vertices) must stay below the root tag bit */
     if (n >= ((int64_t)1 << 30)) return dcf_set_error(ctx, DCF_EUNSUPPORTED, "integer images are limited to 2^30 pixels");
     Scratch sc{ ctx, {} };
@@ -926,6 +946,8 @@ int dcf_flood_build(dcf_tree *t, int grp)
     int32_t *par1 = two ? (int32_t *)sc.get((size_t)n * 4) : t->parent;
     if (!la.uf || !la.absorbed || !la.nabs || !par1) return dcf_set_error(ctx, DCF_ENOMEM, "flood scratch");
     la.cnode = nullptr; la.lift = nullptr; la.cnkey = nullptr;
+    la.bimg = t->bimg;
+    la.nbatch = t->nbatch;
 
     /* ---- pass 1 ---- */
     la.pass = 1;

@@ -71,6 +71,31 @@ struct dcf_ctx {
     int tile3_x = 8, tile3_y = 8, tile3_z = 16;    /* 3D tile */
 };
 
+/* batched form: one entry per image of a batch laid out back to back in one pixel space */
+struct BatchImg {
+    uint64_t off;     /* first pixel of the image */
+    uint32_t W, H, D; /* its dimensions */
+    uint32_t pad;
+};
+
+/* output of one batch image: device pointer and element type */
+struct BatchOut {
+    void *ptr;
+    int dtype;
+};
+
+/* the image of batch pixel p (binary search over the image offsets) */
+__device__ __forceinline__ int batch_locate(const BatchImg *b, int nb, uint64_t p)
+{
+    int lo = 0, hi = nb - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (b[mid].off <= p) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 struct dcf_tree {
     dcf_ctx *ctx = nullptr;
     uint64_t dims[3] = { 1, 1, 1 };
@@ -96,6 +121,11 @@ struct dcf_tree {
     int max_rounds = 32;
     float host_gmax = 0; /* filled lazily */
     int host_gmax_valid = 0;
+    /* batched form (dcf_batch_filter): device image table, image count, 3D flag, per-node lambda */
+    const BatchImg *bimg = nullptr;
+    int nbatch = 0;
+    int batch3d = 0;
+    const double *nlam = nullptr;
 };
 
 /* ---- error / launch helpers ------------------------------------------------------------------ */
@@ -135,6 +165,12 @@ int dcf_accumulate_group(dcf_tree *t, int grp); /* attributes of group grp on th
 
 /* ---- launchers (dcf_flood.hip) ---------------------------------------------------------------- */
 int dcf_flood_build(dcf_tree *t, int grp);      /* u8/u16: canonical parent + pixel-indexed attributes */
+/* batched form: per-image lambda (device, one per image), outputs (device table) */
+int dcf_ops_batch_filter(dcf_tree *t, int attr_idx, int decision, const double *lam_img, int extract, const BatchOut *outs,
+                         const uint32_t *gmax_img, unsigned flags);
+/* batched form: widen / copy one image into the batch buffer; per-image grey max and inversion */
+int dcf_batch_copy_in(dcf_ctx *ctx, const void *src, int src_dtype, void *dst, int dst_dtype, int64_t n);
+int dcf_batch_invert(dcf_ctx *ctx, void *cat, int dtype, const BatchImg *bimg, int nb, uint32_t *gmax_img, int64_t n);
 
 /* ---- launchers (dcf_ops.hip) ------------------------------------------------------------------ */
 int dcf_ops_filter(dcf_tree *t, int attr_idx, int decision, double lambda, int extract, void *out_dev,

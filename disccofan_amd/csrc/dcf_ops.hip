@@ -170,12 +170,14 @@ __global__ __launch_bounds__(256) void k_attr_eval(AttrCtx c, int idx, double *_
 }
 
 /* direct rule: J(k) = k if attr(k) >= lambda else parent; pass flags */
-__global__ __launch_bounds__(256) void k_direct_init(AttrCtx c, int idx, double lambda, const int32_t *__restrict__ npar,
-                                                     int32_t *__restrict__ J, uint8_t *__restrict__ pass, int64_t K)
+/* nlam: per-node lambda of a batch (each node takes its image's lambda), or nullptr */
+__global__ __launch_bounds__(256) void k_direct_init(AttrCtx c, int idx, double lambda, const double *__restrict__ nlam,
+                                                     const int32_t *__restrict__ npar, int32_t *__restrict__ J,
+                                                     uint8_t *__restrict__ pass, int64_t K)
 {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= K) return;
-    bool ok = attr_value(c, idx, k) >= lambda;
+    bool ok = attr_value(c, idx, k) >= (nlam ? nlam[k] : lambda);
     pass[k] = ok;
     J[k] = ok ? (int32_t)k : npar[k];
 }
@@ -210,13 +212,13 @@ __global__ __launch_bounds__(256) void k_direct_value(const int32_t *__restrict_
 }
 
 /* subtractive rule: F(k) = pass ? g(k) - g(par) : 0 (root: pass ? g : 0); out = rootfix sum */
-__global__ __launch_bounds__(256) void k_sub_init(AttrCtx c, int idx, double lambda, const int32_t *__restrict__ npar,
-                                                  const float *__restrict__ ngrey, double *__restrict__ F,
-                                                  int32_t *__restrict__ A, int64_t K)
+__global__ __launch_bounds__(256) void k_sub_init(AttrCtx c, int idx, double lambda, const double *__restrict__ nlam,
+                                                  const int32_t *__restrict__ npar, const float *__restrict__ ngrey,
+                                                  double *__restrict__ F, int32_t *__restrict__ A, int64_t K)
 {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= K) return;
-    bool ok = attr_value(c, idx, k) >= lambda;
+    bool ok = attr_value(c, idx, k) >= (nlam ? nlam[k] : lambda);
     int32_t p = npar[k];
     F[k] = ok ? (p >= 0 ? (double)ngrey[k] - (double)ngrey[p] : (double)ngrey[k]) : 0.0;
     A[k] = p;
@@ -273,6 +275,36 @@ __global__ __launch_bounds__(256) void k_filter_out(ImgView iv, const uint32_t *
     if (out_dtype == DCF_F32) ((float *)out)[p] = f;
     else if (out_dtype == DCF_U16) ((uint16_t *)out)[p] = (uint16_t)f;
     else ((uint8_t *)out)[p] = (uint8_t)f;
+}
+
+/* batched form: every node takes the lambda of its image */
+__global__ __launch_bounds__(256) void k_node_lambda(const BatchImg *__restrict__ bimg, int nb, const double *__restrict__ lam,
+                                                     const uint32_t *__restrict__ p2n, double *__restrict__ nlam, int64_t n)
+{
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    /* every pixel of a node lies in the same image: concurrent writers store the same value */
+    nlam[p2n[p]] = lam[batch_locate(bimg, nb, (uint64_t)p)];
+}
+
+/* batched form: filtered value of every pixel into its image's output (extract on the possibly
+ * inverted greys, then min-tree re-inversion with the image's own grey maximum, as k_filter_out) */
+__global__ __launch_bounds__(256) void k_batch_out(ImgView iv, const BatchImg *__restrict__ bimg, int nb,
+                                                   const BatchOut *__restrict__ outs, const uint32_t *__restrict__ gmax_img,
+                                                   const uint32_t *__restrict__ p2n, const float *__restrict__ val, int extract,
+                                                   int64_t n)
+{
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int i = batch_locate(bimg, nb, (uint64_t)p);
+    const BatchOut o = outs[i];
+    const uint64_t local = (uint64_t)p - bimg[i].off;
+    float f = val[p2n[p]];
+    if (extract) f = px_grey_ops(iv, p) - f;
+    if (gmax_img) f = (float)gmax_img[i] - f;
+    if (o.dtype == DCF_F32) ((float *)o.ptr)[local] = f;
+    else if (o.dtype == DCF_U16) ((uint16_t *)o.ptr)[local] = (uint16_t)f;
+    else ((uint8_t *)o.ptr)[local] = (uint8_t)f;
 }
 
 __global__ __launch_bounds__(256) void k_check_out(const uint32_t *__restrict__ p2n, const double *__restrict__ av,
@@ -372,7 +404,7 @@ static int rule_values(dcf_tree *t, int attr_idx, int decision, double lambda, u
         int32_t *J = (int32_t *)dcf_alloc(ctx, (size_t)K * 4);
         uint8_t *pass = (uint8_t *)dcf_alloc(ctx, (size_t)K);
         if (!J || !pass) { dcf_release(ctx, J); dcf_release(ctx, pass); return dcf_set_error(ctx, DCF_ENOMEM, "filter scratch"); }
-        DCF_LAUNCH(ctx, "k_direct_init", k_direct_init, dim3(nblk(K, 256)), dim3(256), 0, c, attr_idx, lambda, t->npar, J, pass, K);
+        DCF_LAUNCH(ctx, "k_direct_init", k_direct_init, dim3(nblk(K, 256)), dim3(256), 0, c, attr_idx, lambda, t->nlam, t->npar, J, pass, K);
         DCF_LAUNCH(ctx, "k_fill", k_fill_u32, dim3(1), dim3(64), 0, t->counters, 0u, (int64_t)64);
         for (int r = 0; r < t->max_rounds; r++)
             DCF_LAUNCH(ctx, "k_direct_jump", k_direct_jump, dim3(nblk(K, 256)), dim3(256), 0, J, pass, t->counters, r, K);
@@ -388,7 +420,7 @@ static int rule_values(dcf_tree *t, int attr_idx, int decision, double lambda, u
             dcf_release(ctx, F0); dcf_release(ctx, F1); dcf_release(ctx, A0); dcf_release(ctx, A1);
             return dcf_set_error(ctx, DCF_ENOMEM, "filter scratch");
         }
-        DCF_LAUNCH(ctx, "k_sub_init", k_sub_init, dim3(nblk(K, 256)), dim3(256), 0, c, attr_idx, lambda, t->npar, t->ngrey, F0, A0, K);
+        DCF_LAUNCH(ctx, "k_sub_init", k_sub_init, dim3(nblk(K, 256)), dim3(256), 0, c, attr_idx, lambda, t->nlam, t->npar, t->ngrey, F0, A0, K);
         DCF_LAUNCH(ctx, "k_fill", k_fill_u32, dim3(1), dim3(64), 0, t->counters, 0u, (int64_t)64);
         double *fi = F0, *fo = F1;
         int32_t *ai = A0, *ao = A1;
@@ -430,6 +462,36 @@ int dcf_ops_filter(dcf_tree *t, int attr_idx, int decision, double lambda, int e
         if (ea) dcf_prof_end(ctx, "k_filter_out", ea, eb);
         if (le != hipSuccess) rc = dcf_set_error(ctx, DCF_EDEVICE, "launch k_filter_out: %s", hipGetErrorString(le));
     }
+    dcf_release(ctx, val);
+    return rc;
+}
+
+int dcf_ops_batch_filter(dcf_tree *t, int attr_idx, int decision, const double *lam_img, int extract, const BatchOut *outs,
+                         const uint32_t *gmax_img, unsigned flags)
+{
+    dcf_ctx *ctx = t->ctx;
+    double *nlam = (double *)dcf_alloc(ctx, (size_t)t->K * 8);
+    float *val = (float *)dcf_alloc(ctx, (size_t)t->K * 4);
+    if (!nlam || !val) {
+        dcf_release(ctx, nlam);
+        dcf_release(ctx, val);
+        return dcf_set_error(ctx, DCF_ENOMEM, "batch filter values");
+    }
+    int rc = DCF_OK;
+    hipLaunchKernelGGL(k_node_lambda, dim3(nblk(t->n, 256)), dim3(256), 0, ctx->stream, t->bimg, t->nbatch, lam_img, t->p2n, nlam,
+                       t->n);
+    if (hipGetLastError() != hipSuccess) rc = dcf_set_error(ctx, DCF_EDEVICE, "launch k_node_lambda");
+    if (rc == DCF_OK) {
+        t->nlam = nlam;
+        rc = rule_values(t, attr_idx, decision, 0.0, flags, val);
+        t->nlam = nullptr;
+    }
+    if (rc == DCF_OK) {
+        hipLaunchKernelGGL(k_batch_out, dim3(nblk(t->n, 256)), dim3(256), 0, ctx->stream, view_of(t), t->bimg, t->nbatch, outs, gmax_img,
+                           t->p2n, val, extract, t->n);
+        if (hipGetLastError() != hipSuccess) rc = dcf_set_error(ctx, DCF_EDEVICE, "launch k_batch_out");
+    }
+    dcf_release(ctx, nlam);
     dcf_release(ctx, val);
     return rc;
 }
@@ -546,5 +608,66 @@ int dcf_ops_minmax(dcf_tree *t)
     unsigned blocks = nblk(t->n, 256);
     if (blocks > 2048) blocks = 2048;
     DCF_LAUNCH(ctx, "k_minmax", k_minmax, dim3(blocks), dim3(256), 0, (const void *)t->img, (int)t->dtype, t->gmax, t->n);
+    return DCF_OK;
+}
+
+/* ------------------------------------------------------------------------------------------------ */
+/* batched form: input staging                                                                      */
+/* ------------------------------------------------------------------------------------------------ */
+__global__ __launch_bounds__(256) void k_batch_in(const void *__restrict__ src, int src_dtype, void *__restrict__ dst, int dst_dtype,
+                                                  int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t v = src_dtype == DCF_U8 ? ((const uint8_t *)src)[i] : ((const uint16_t *)src)[i];
+    if (dst_dtype == DCF_U8) ((uint8_t *)dst)[i] = (uint8_t)v;
+    else ((uint16_t *)dst)[i] = (uint16_t)v;
+}
+
+__global__ __launch_bounds__(256) void k_batch_gmax(const void *__restrict__ cat, int dtype, const BatchImg *__restrict__ bimg, int nb,
+                                                    uint32_t *__restrict__ gmax_img, int64_t n)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = p < n;
+    uint32_t v = 0;
+    int i = -1;
+    if (valid) {
+        v = dtype == DCF_U8 ? ((const uint8_t *)cat)[p] : ((const uint16_t *)cat)[p];
+        i = batch_locate(bimg, nb, (uint64_t)p);
+    }
+    /* one atomic per wave when the wave lies in one image (lane 0 is valid whenever any lane is) */
+    const int i0 = __shfl(i, 0, 64);
+    if (__ballot(valid && i != i0) == 0) {
+        uint32_t m = v;
+        for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor(m, o, 64));
+        if ((threadIdx.x & 63) == 0 && valid) atomicMax(&gmax_img[i0], m);
+    } else if (valid) {
+        atomicMax(&gmax_img[i], v);
+    }
+}
+
+/* min-tree of every image: its keys become gmax_i - g (preprocessing.c:84-91) */
+__global__ __launch_bounds__(256) void k_batch_invert(void *__restrict__ cat, int dtype, const BatchImg *__restrict__ bimg, int nb,
+                                                      const uint32_t *__restrict__ gmax_img, int64_t n)
+{
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const uint32_t g = gmax_img[batch_locate(bimg, nb, (uint64_t)p)];
+    if (dtype == DCF_U8) ((uint8_t *)cat)[p] = (uint8_t)(g - ((uint8_t *)cat)[p]);
+    else ((uint16_t *)cat)[p] = (uint16_t)(g - ((uint16_t *)cat)[p]);
+}
+
+int dcf_batch_copy_in(dcf_ctx *ctx, const void *src, int src_dtype, void *dst, int dst_dtype, int64_t n)
+{
+    DCF_LAUNCH(ctx, "k_batch_in", k_batch_in, dim3(nblk(n, 256)), dim3(256), 0, src, src_dtype, dst, dst_dtype, n);
+    return DCF_OK;
+}
+
+int dcf_batch_invert(dcf_ctx *ctx, void *cat, int dtype, const BatchImg *bimg, int nb, uint32_t *gmax_img, int64_t n)
+{
+    DCF_LAUNCH(ctx, "k_fill", k_fill_u32, dim3(nblk(nb, 256)), dim3(256), 0, gmax_img, 0u, (int64_t)nb);
+    DCF_LAUNCH(ctx, "k_batch_gmax", k_batch_gmax, dim3(nblk(n, 256)), dim3(256), 0, (const void *)cat, dtype, bimg, nb, gmax_img, n);
+    DCF_LAUNCH(ctx, "k_batch_invert", k_batch_invert, dim3(nblk(n, 256)), dim3(256), 0, cat, dtype, bimg, nb,
+               (const uint32_t *)gmax_img, n);
     return DCF_OK;
 }

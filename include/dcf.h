@@ -133,6 +133,24 @@ const char *dcf_attr_short_name(int attr_idx);
 int dcf_attr_group(int attr_idx);
 const char *dcf_decision_name(int decision);
 
+/* ---- batched form ------------------------------------------------------------------------------ *
+ * Many images of uneven sizes, each with its own lambda: the batch analogue of a loop of
+ * dcf_tree_build + dcf_filter per image (build_tree_with_attributes flood.c:175 + tree_filtering
+ * filtering.c:18 per image, as loop_operations runs them for one image).  Every image gets its own
+ * max-tree or min-tree (no pixel of one image touches another) and all of them are built by one
+ * level loop over the whole batch, then filtered with their own lambda.  Images must all be 2D or
+ * all be 3D; u8 and u16 may be mixed; out receives the image's own dtype.  The batch holds at most
+ * 2^30 pixels.  flags: DCF_DEVICE_PTR (img and out are device pointers), DCF_ALL_RULES. */
+typedef struct {
+    const void *img;  /* dims[0]*dims[1]*dims[2] values of dtype, x fastest */
+    dcf_dtype dtype;  /* DCF_U8 or DCF_U16 */
+    uint64_t dims[3];
+    double lambda;    /* this image's lambda */
+    void *out;        /* filtered (or extracted) image, same dtype and size */
+} dcf_batch_image;
+int dcf_batch_filter(dcf_ctx *ctx, const dcf_batch_image *images, int count, int connectivity, int tree_type, int attr_idx,
+                     int decision, int extract, unsigned flags);
+
 /* ---- operator driver (loop_operations, disccofan.c:137-205) ------------------------------------- *
  * ops: the reference's operation tuples, e.g. "(\"filter\",0,100),(\"pattern\",0,\"lvec.txt\",1)".
  * Runs them in order on one image with attribute-group reuse; outputs are written by the callback

@@ -1,0 +1,130 @@
+"""The batched form (dcf_batch_filter): every image of an uneven batch is filtered with its own
+tree and lambda; each output must equal the oracle's output for that image alone, bit for bit."""
+import sys
+
+import numpy as np
+import pytest
+
+import disccofan_amd as dcf
+from disccofan_amd import synth
+from oracle.oracle import OracleTree
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    # test_device_pointers uses torch: import it before the library loads (its HIP runtime first)
+    import torch  # noqa: F401
+    c = dcf.Context(0)
+    yield c
+    c.close()
+
+
+def random_batch(rng, count, is3d=False, u16_share=0.5, maxside=120):
+    imgs, lams = [], []
+    for _ in range(count):
+        if is3d:
+            shape = tuple(int(v) for v in rng.integers(2, 18, size=3))
+        else:
+            shape = (int(rng.integers(1, maxside)), int(rng.integers(1, maxside)))
+        if rng.random() < u16_share:
+            img = rng.integers(0, int(rng.choice([3, 50, 4000, 65536])), size=shape).astype(np.uint16)
+        else:
+            img = rng.integers(0, int(rng.choice([2, 9, 256])), size=shape).astype(np.uint8)
+        imgs.append(img)
+        lams.append(float(rng.choice([1.0, 3.0, 7.5, 40.0, 1e9])))
+    return imgs, lams
+
+
+def check_batch(ctx, imgs, lams, conn, tree_type, decision=0, extract=False, attr=0, label=""):
+    say("batch", label, len(imgs), "images conn", conn, tree_type, "decision", decision, "extract", extract)
+    outs = ctx.batch_filter(imgs, lams, connectivity=conn, tree_type=tree_type, attr_idx=attr, decision=decision,
+                            extract=extract)
+    for k, (img, lam, out) in enumerate(zip(imgs, lams, outs)):
+        o = OracleTree(img, conn, tree_type, attr)
+        want = o.filter(attr, lam, decision, extract=extract).astype(img.dtype)
+        bad = np.nonzero(out.reshape(-1) != want.reshape(-1))[0]
+        assert len(bad) == 0, "%s image %d %s %s lam %g: %d pixels differ, first %s" % (
+            label, k, img.shape, img.dtype, lam, len(bad), bad[:5])
+
+
+@pytest.mark.parametrize("seed", range(3))
+@pytest.mark.parametrize("conn", [4, 8])
+@pytest.mark.parametrize("tree_type", ["max", "min"])
+def test_mixed_batch(ctx, seed, conn, tree_type):
+    rng = np.random.default_rng(500 + seed)
+    imgs, lams = random_batch(rng, 14)
+    check_batch(ctx, imgs, lams, conn, tree_type, decision=0, label="mixed s%d" % seed)
+
+
+def test_rules_and_extract(ctx):
+    rng = np.random.default_rng(77)
+    imgs, lams = random_batch(rng, 9)
+    for dec in (1, 2, 3):
+        check_batch(ctx, imgs, lams, 4, "max", decision=dec, label="rule")
+    check_batch(ctx, imgs, lams, 8, "min", decision=0, extract=True, label="extract")
+
+
+def test_all_u8_batch(ctx):
+    rng = np.random.default_rng(91)
+    imgs, lams = random_batch(rng, 20, u16_share=0.0)
+    check_batch(ctx, imgs, lams, 4, "max", label="u8 only")
+
+
+def test_3d_batch(ctx):
+    rng = np.random.default_rng(93)
+    imgs, lams = random_batch(rng, 8, is3d=True)
+    check_batch(ctx, imgs, lams, 6, "max", label="3d")
+    check_batch(ctx, imgs, lams, 26, "min", label="3d")
+
+
+def test_non_area_attribute(ctx):
+    """the per-node lambda goes through the general attribute evaluation (here non-compactness)"""
+    rng = np.random.default_rng(95)
+    imgs, _ = random_batch(rng, 6, maxside=60)
+    lams = [0.2, 0.3, 0.5, 0.25, 1.0, 0.18]
+    check_batch(ctx, imgs, lams, 4, "max", decision=0, attr=17, label="ncomp")
+
+
+def test_uneven_workload_batch(ctx):
+    """the uneven-size batch generator of the batched workload, first 24 images, sides 64..512"""
+    shapes = synth.batch_shapes(24, 64, 512)
+    imgs = [synth.disks(H, W, is16, seed) for (W, H, is16, seed, lam) in shapes]
+    lams = [lam for (W, H, is16, seed, lam) in shapes]
+    check_batch(ctx, imgs, lams, 4, "max", label="workload")
+
+
+def test_device_pointers(ctx):
+    import torch
+    rng = np.random.default_rng(97)
+    imgs, lams = random_batch(rng, 6)
+    dev = torch.device("cuda:0")
+    tin = [torch.from_numpy(a.view(np.int16) if a.dtype == np.uint16 else a).to(dev) for a in imgs]
+    tout = [torch.empty_like(t) for t in tin]
+    torch.cuda.synchronize()
+    ctx.batch_filter_device([(ti.data_ptr(), a.dtype, a.shape, lam, to.data_ptr())
+                             for ti, a, lam, to in zip(tin, imgs, lams, tout)], connectivity=8)
+    host = ctx.batch_filter(imgs, lams, connectivity=8)
+    for to, h, a in zip(tout, host, imgs):
+        got = to.cpu().numpy().view(a.dtype)
+        assert np.array_equal(got, h)
+
+
+def test_batch_errors(ctx):
+    a2 = np.zeros((5, 6), np.uint8)
+    a3 = np.zeros((3, 4, 5), np.uint8)
+    with pytest.raises(dcf.DcfError) as e:
+        ctx.batch_filter([a2, a3], [1, 1])
+    assert e.value.code == -1
+    with pytest.raises(dcf.DcfError) as e:
+        ctx.batch_filter([a2.astype(np.float32)], [1])
+    assert e.value.code == -1
+    with pytest.raises(dcf.DcfError) as e:
+        ctx.batch_filter([], [])
+    assert e.value.code == -1
