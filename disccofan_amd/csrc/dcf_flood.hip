@@ -189,16 +189,44 @@ __global__ __launch_bounds__(256) void k_lvl_hist(ImgView iv, int pass, int shif
     if (sh[threadIdx.x]) atomicAdd(&hist[threadIdx.x], sh[threadIdx.x]);
 }
 
-/* each block owns a contiguous pixel range; one global atomic per (block, level) reserves the
- * block's slots, so every level's list is ordered by block (near raster order) */
-__global__ __launch_bounds__(256) void k_lvl_scatter(ImgView iv, int pass, int shift, int64_t n, int64_t per_block,
+/* each block owns one tile of the image (64x64 in 2D, 16x16x16 in 3D; a raster range of 4096 pixels
+ * for a batch); one global atomic per (block, level) reserves the block's slots, so every level's
+ * list is ordered by tile and the pixels a block of a level kernel handles lie close together */
+struct ScatterGeo {
+    int tiled;          /* 0: raster ranges of 4096 pixels */
+    int W, H, D;
+    int TW, TH, TD;     /* tile shape (TW*TH*TD = 4096) */
+    int ntx, nty;
+};
+
+__global__ __launch_bounds__(256) void k_lvl_scatter(ImgView iv, int pass, int shift, int64_t n, ScatterGeo sg,
                                                      uint32_t *__restrict__ cursor, uint32_t *__restrict__ order)
 {
     __shared__ uint32_t cnt[NLEV], base[NLEV];
     cnt[threadIdx.x] = 0;
     __syncthreads();
-    int64_t p0 = (int64_t)blockIdx.x * per_block, p1 = min(p0 + per_block, n);
-    for (int64_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+    const int tsz = 4096;
+    int64_t p0 = 0;
+    int x0 = 0, y0 = 0, z0 = 0;
+    if (sg.tiled) {
+        const int tx = blockIdx.x % sg.ntx, ty = (blockIdx.x / sg.ntx) % sg.nty, tz = blockIdx.x / (sg.ntx * sg.nty);
+        x0 = tx * sg.TW; y0 = ty * sg.TH; z0 = tz * sg.TD;
+    } else {
+        p0 = (int64_t)blockIdx.x * tsz;
+    }
+    /* pixel of local slot l of this block, or -1 outside the image */
+    auto pix = [&](int l) -> int64_t {
+        if (!sg.tiled) {
+            const int64_t p = p0 + l;
+            return p < n ? p : -1;
+        }
+        const int x = x0 + l % sg.TW, y = y0 + (l / sg.TW) % sg.TH, z = z0 + l / (sg.TW * sg.TH);
+        if (x >= sg.W || y >= sg.H || z >= sg.D) return -1;
+        return (int64_t)x + (int64_t)y * sg.W + (int64_t)z * sg.W * sg.H;
+    };
+    for (int l = threadIdx.x; l < tsz; l += blockDim.x) {
+        const int64_t p = pix(l);
+        if (p < 0) continue;
         uint32_t k = fl_key(iv, p);
         atomicAdd(&cnt[pass == 1 ? (k >> shift) : (k & 255u)], 1u);
     }
@@ -206,12 +234,24 @@ __global__ __launch_bounds__(256) void k_lvl_scatter(ImgView iv, int pass, int s
     if (cnt[threadIdx.x]) base[threadIdx.x] = atomicAdd(&cursor[threadIdx.x], cnt[threadIdx.x]);
     cnt[threadIdx.x] = 0;
     __syncthreads();
-    for (int64_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+    for (int l = threadIdx.x; l < tsz; l += blockDim.x) {
+        const int64_t p = pix(l);
+        if (p < 0) continue;
         uint32_t k = fl_key(iv, p);
         uint32_t d = pass == 1 ? (k >> shift) : (k & 255u);
         uint32_t r = atomicAdd(&cnt[d], 1u);
         order[base[d] + r] = (uint32_t)p;
     }
+}
+
+/* Blocks are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8), and each XCD has its own
+ * L2.  This renumbering gives every XCD one contiguous eighth of the level list, i.e. one compact
+ * part of the image, so neighbouring pixels share that XCD's L2. */
+__device__ __forceinline__ uint32_t xcd_block()
+{
+    const uint32_t nb = gridDim.x, b = blockIdx.x;
+    const uint32_t q = nb >> 3, r = nb & 7u, x = b & 7u, k = b >> 3;
+    return x * q + min(x, r) + k;
 }
 
 /* ------------------------------------------------------------------------------------------------ */
@@ -292,7 +332,7 @@ __global__ __launch_bounds__(256) void k_union(LevelArgs a, const uint32_t *__re
     if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
     const AbsBuf ab{ s_abs, &s_n };
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
     const uint32_t NONE = 0xFFFFFFFFu;
     const bool valid = i < cnt;
     const uint32_t p = valid ? lvl[i] : 0u;
@@ -596,7 +636,7 @@ __global__ __launch_bounds__(FB) void k_final_pixels(LevelArgs a, const uint32_t
                                                      DevAttr A, int grp, int32_t *__restrict__ par, uint32_t *__restrict__ held)
 {
     __shared__ BlockScratch bs;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
     const bool valid = i < cnt;
     uint32_t p = 0, r = 0;
     bool fresh = false, old = false;
@@ -625,7 +665,7 @@ __global__ __launch_bounds__(FB) void k_final_held(LevelArgs a, const uint32_t *
                                                    int32_t *__restrict__ par, uint32_t *__restrict__ mark)
 {
     __shared__ BlockScratch bs;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
     /* unions and re-rooting are complete: this level's absorbed entries end at the current count */
     if (i == 0) *mark = *a.nabs;
     const uint32_t r = i < cnt ? held[i] : 0xFFFFFFFFu;
@@ -871,9 +911,17 @@ static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAtt
     off[0] = 0;
     for (int i = 0; i < NLEV; i++) off[i + 1] = off[i] + h[i];
     DCF_HIP(ctx, hipMemcpyAsync(cursor, off, NLEV * 4, hipMemcpyHostToDevice, ctx->stream));
-    int64_t per_block = 4096;
-    DCF_LAUNCH(ctx, "k_lvl_scatter", k_lvl_scatter, dim3(nblk(n, (int)per_block)), dim3(256), 0, la.iv, la.pass, la.shift, n,
-               per_block, cursor, order);
+    ScatterGeo sg;
+    sg.tiled = la.bimg == nullptr;
+    sg.W = la.g.W; sg.H = la.g.H; sg.D = la.g.D;
+    if (sg.D > 1) { sg.TW = 16; sg.TH = 16; sg.TD = 16; }
+    else { sg.TW = 64; sg.TH = 64; sg.TD = 1; }
+    sg.ntx = (sg.W + sg.TW - 1) / sg.TW;
+    sg.nty = (sg.H + sg.TH - 1) / sg.TH;
+    const int64_t ntz = (sg.D + sg.TD - 1) / sg.TD;
+    const int64_t sblocks = sg.tiled ? (int64_t)sg.ntx * sg.nty * ntz : (int64_t)nblk(n, 4096);
+    DCF_LAUNCH(ctx, "k_lvl_scatter", k_lvl_scatter, dim3((unsigned)sblocks), dim3(256), 0, la.iv, la.pass, la.shift, n, sg, cursor,
+               order);
     /* nabs[0]: running count of absorbed roots; nabs[1 + i]: where the i-th processed level's
      * entries start (written on the device, so the level loop never waits for the host) */
     uint32_t *nabs = la.nabs;

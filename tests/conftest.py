@@ -20,3 +20,18 @@ GOLDEN = os.path.join(ROOT, "tests", "golden")
 def lena():
     import numpy as np
     return dict(np.load(os.path.join(GOLDEN, "lena512.npz")))
+
+
+@pytest.fixture(scope="session", autouse=True)
+def _torch_runtime_first(request):
+    """PyTorch brings its own HIP runtime and must initialise it before libdcf.so loads one (a
+    runtime loaded first hides the GPU from torch); the GPU tests that hand torch tensors to the
+    library rely on this, whatever order the test files run in."""
+    if any(item.get_closest_marker("gpu") for item in request.session.items):
+        try:
+            import torch
+            if torch.cuda.is_available():
+                torch.cuda.init()
+        except ImportError:
+            pass
+    yield
