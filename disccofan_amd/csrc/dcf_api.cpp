@@ -177,6 +177,17 @@ extern "C" int dcf_ctx_create(int device, unsigned flags, dcf_ctx **out)
             c->tile3_x = a; c->tile3_y = b; c->tile3_z = d;
         }
     }
+    auto shape3 = [](const char *env, int v[3]) {
+        const char *e = getenv(env);
+        int a = 0, b = 0, d = 0;
+        if (e && sscanf(e, "%dx%dx%d", &a, &b, &d) == 3 && a > 0 && b > 0 && d > 0 && (int64_t)a * b * d <= (1 << 20)) {
+            v[0] = a; v[1] = b; v[2] = d;
+        }
+    };
+    shape3("DCF_LEVEL_TILE2D", c->lt2);
+    shape3("DCF_LEVEL_GROUP2D", c->lg2);
+    shape3("DCF_LEVEL_TILE3D", c->lt3);
+    shape3("DCF_LEVEL_GROUP3D", c->lg3);
     *out = c;
     return DCF_OK;
 }

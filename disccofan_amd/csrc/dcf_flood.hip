@@ -195,8 +195,10 @@ __global__ __launch_bounds__(256) void k_lvl_hist(ImgView iv, int pass, int shif
 struct ScatterGeo {
     int tiled;          /* 0: raster ranges of 4096 pixels */
     int W, H, D;
-    int TW, TH, TD;     /* tile shape (TW*TH*TD = 4096) */
-    int ntx, nty;
+    int TW, TH, TD;     /* tile shape */
+    int GX, GY, GZ;     /* tiles per group: a group's tiles are consecutive in the list */
+    int ngx, ngy;       /* groups per row / column */
+    int ntx, nty, ntz;  /* tiles per row / column / slab */
 };
 
 __global__ __launch_bounds__(256) void k_lvl_scatter(ImgView iv, int pass, int shift, int64_t n, ScatterGeo sg,
@@ -205,12 +207,17 @@ __global__ __launch_bounds__(256) void k_lvl_scatter(ImgView iv, int pass, int s
     __shared__ uint32_t cnt[NLEV], base[NLEV];
     cnt[threadIdx.x] = 0;
     __syncthreads();
-    const int tsz = 4096;
+    const int tsz = sg.tiled ? sg.TW * sg.TH * sg.TD : 4096;
     int64_t p0 = 0;
     int x0 = 0, y0 = 0, z0 = 0;
     if (sg.tiled) {
-        const int tx = blockIdx.x % sg.ntx, ty = (blockIdx.x / sg.ntx) % sg.nty, tz = blockIdx.x / (sg.ntx * sg.nty);
+        const int gsz = sg.GX * sg.GY * sg.GZ;
+        const int g = blockIdx.x / gsz, w = blockIdx.x % gsz;
+        const int gx = g % sg.ngx, gy = (g / sg.ngx) % sg.ngy, gz = g / (sg.ngx * sg.ngy);
+        const int tx = gx * sg.GX + w % sg.GX, ty = gy * sg.GY + (w / sg.GX) % sg.GY, tz = gz * sg.GZ + w / (sg.GX * sg.GY);
         x0 = tx * sg.TW; y0 = ty * sg.TH; z0 = tz * sg.TD;
+        /* tiles of a partial group beyond the image: the block has no pixel */
+        if (tx >= sg.ntx || ty >= sg.nty || tz >= sg.ntz) x0 = sg.W;
     } else {
         p0 = (int64_t)blockIdx.x * tsz;
     }
@@ -914,12 +921,16 @@ static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAtt
     ScatterGeo sg;
     sg.tiled = la.bimg == nullptr;
     sg.W = la.g.W; sg.H = la.g.H; sg.D = la.g.D;
-    if (sg.D > 1) { sg.TW = 16; sg.TH = 16; sg.TD = 16; }
-    else { sg.TW = 64; sg.TH = 64; sg.TD = 1; }
+    const int *ts = sg.D > 1 ? ctx->lt3 : ctx->lt2, *gs = sg.D > 1 ? ctx->lg3 : ctx->lg2;
+    sg.TW = ts[0]; sg.TH = ts[1]; sg.TD = ts[2];
+    sg.GX = gs[0]; sg.GY = gs[1]; sg.GZ = gs[2];
     sg.ntx = (sg.W + sg.TW - 1) / sg.TW;
     sg.nty = (sg.H + sg.TH - 1) / sg.TH;
-    const int64_t ntz = (sg.D + sg.TD - 1) / sg.TD;
-    const int64_t sblocks = sg.tiled ? (int64_t)sg.ntx * sg.nty * ntz : (int64_t)nblk(n, 4096);
+    sg.ntz = (sg.D + sg.TD - 1) / sg.TD;
+    sg.ngx = (sg.ntx + sg.GX - 1) / sg.GX;
+    sg.ngy = (sg.nty + sg.GY - 1) / sg.GY;
+    const int64_t ngz = (sg.ntz + sg.GZ - 1) / sg.GZ;
+    const int64_t sblocks = sg.tiled ? (int64_t)sg.ngx * sg.ngy * ngz * sg.GX * sg.GY * sg.GZ : (int64_t)nblk(n, 4096);
     DCF_LAUNCH(ctx, "k_lvl_scatter", k_lvl_scatter, dim3((unsigned)sblocks), dim3(256), 0, la.iv, la.pass, la.shift, n, sg, cursor,
                order);
     /* nabs[0]: running count of absorbed roots; nabs[1 + i]: where the i-th processed level's

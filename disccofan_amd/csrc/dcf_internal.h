@@ -69,6 +69,10 @@ struct dcf_ctx {
     std::set<dcf_tree *> trees; /* live trees, freed by dcf_ctx_destroy */
     int tile_x = 32, tile_y = 32, tile_z = 1;      /* 2D tile */
     int tile3_x = 8, tile3_y = 8, tile3_z = 16;    /* 3D tile */
+    /* level-list tiles of the integer build (DCF_LEVEL_TILE2D / DCF_LEVEL_TILE3D, "WxHxD") and the
+     * groups of tiles kept together in the list (DCF_LEVEL_GROUP2D / DCF_LEVEL_GROUP3D) */
+    int lt2[3] = { 64, 64, 1 }, lg2[3] = { 1, 1, 1 };
+    int lt3[3] = { 16, 16, 16 }, lg3[3] = { 1, 1, 1 };
 };
 
 /* batched form: one entry per image of a batch laid out back to back in one pixel space */
