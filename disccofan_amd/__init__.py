@@ -41,7 +41,7 @@ DECISIONS = ["Direct", "Min", "Max", "Subtractive"]
 # exported symbols of libdcf.so (kept in sync with include/dcf.h)
 EXPORTS = [
     "dcf_ctx_create", "dcf_ctx_destroy", "dcf_ctx_set_stream", "dcf_ctx_stream", "dcf_last_error",
-    "dcf_ctx_synchronize", "dcf_ctx_set_profiling", "dcf_ctx_profile_only", "dcf_batch_filter", "dcf_kernel_stats", "dcf_kernel_stats_reset",
+    "dcf_ctx_synchronize", "dcf_ctx_set_profiling", "dcf_ctx_profile_only", "dcf_batch_filter", "dcf_dmp", "dcf_tree_write_csv", "dcf_kernel_stats", "dcf_kernel_stats_reset",
     "dcf_kernel_name", "dcf_tree_build", "dcf_tree_free", "dcf_tree_accumulate", "dcf_tree_group",
     "dcf_tree_parent", "dcf_tree_num_nodes", "dcf_filter", "dcf_check", "dcf_spectrum1d", "dcf_spectrum2d",
     "dcf_lvec_read", "dcf_lvec_free", "dcf_attr_name", "dcf_attr_short_name", "dcf_attr_group",
@@ -106,6 +106,8 @@ def lib():
     L.dcf_tree_num_nodes.argtypes = [vp, C.POINTER(C.c_int64)]
     L.dcf_filter.argtypes = [vp, i, i, d, i, vp, i, u]
     L.dcf_check.argtypes = [vp, i, vp, u]
+    L.dcf_dmp.argtypes = [vp, i, i, vp, i, vp, i, u]
+    L.dcf_tree_write_csv.argtypes = [vp, i, C.c_char_p]
     L.dcf_spectrum1d.argtypes = [vp, i, vp, i, vp]
     L.dcf_spectrum2d.argtypes = [vp, i, vp, i, i, vp, i, vp]
     L.dcf_lvec_read.argtypes = [C.c_char_p, d, C.POINTER(C.POINTER(C.c_float)), C.POINTER(i)]
@@ -282,7 +284,8 @@ class Context:
 
     def run_operations(self, img, operations, connectivity: int = 4, tree_type: str = "max", pixdim=(1.0, 1.0, 1.0)):
         """loop_operations (disccofan.c:137-205): run the reference's operation tuples in order.
-        Returns one output per operation (image, check image, or spectrum)."""
+        Returns one output per operation (image, check image, spectrum, or a differential profile
+        stacked along a leading axis)."""
         a = np.ascontiguousarray(img)
         if not isinstance(operations, str):
             operations = ",".join(_tuple_text(t) for t in operations)
@@ -294,6 +297,11 @@ class Context:
             if kind in (0, 1):
                 arr = np.ctypeslib.as_array(C.cast(data, C.POINTER(C.c_float)), shape=(count,)).copy()
                 outs[idx] = arr.reshape(shape)
+            elif kind == 5:
+                outs[idx] = C.string_at(data, count).decode()
+            elif kind == 4:
+                arr = np.ctypeslib.as_array(C.cast(data, C.POINTER(C.c_float)), shape=(count,)).copy()
+                outs[idx] = arr.reshape((-1,) + tuple(shape))
             else:
                 outs[idx] = np.ctypeslib.as_array(C.cast(data, C.POINTER(C.c_double)), shape=(count,)).copy()
 
@@ -368,6 +376,19 @@ class Tree:
         flags = DCF_DEVICE_PTR | (DCF_ALL_RULES if all_rules else 0)
         self.ctx._check(lib().dcf_filter(self._h, int(attr_idx), int(decision), float(lam), 1 if extract else 0,
                                          C.c_void_p(out_ptr), _dtype_code(out_dtype), flags))
+
+    def dmp(self, attr_idx: int, lambdas, decision: int = 0, out_dtype=np.float32) -> np.ndarray:
+        """perform_dmp_operation (operation.c:63-77): g - filter(lambda_i) for every lambda after
+        lambda_0, stacked along a leading axis (min-trees stay in the inverted grey space)."""
+        lv = np.ascontiguousarray(lambdas, dtype=np.float32)
+        out = np.empty((max(len(lv) - 1, 0),) + tuple(self.shape), dtype=out_dtype)
+        self.ctx._check(lib().dcf_dmp(self._h, int(attr_idx), int(decision), _ptr(lv), len(lv), _ptr(out),
+                                      _dtype_code(out_dtype), 0))
+        return out
+
+    def write_csv(self, path: str, attr_idx: int):
+        """write_tree (io.c:1431-1573): the tree as the reference's CSV"""
+        self.ctx._check(lib().dcf_tree_write_csv(self._h, int(attr_idx), os.fsencode(path)))
 
     def check(self, attr_idx: int) -> np.ndarray:
         out = np.empty(self.shape, dtype=np.float32)

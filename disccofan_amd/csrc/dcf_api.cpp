@@ -596,6 +596,97 @@ extern "C" int dcf_filter(dcf_tree *t, int attr_idx, int decision, double lambda
     return rc;
 }
 
+extern "C" int dcf_dmp(dcf_tree *t, int attr_idx, int decision, const float *lambdas, int n, void *out, dcf_dtype out_dtype,
+                       unsigned flags)
+{
+    if (!t || !out || !lambdas) return DCF_EINVAL;
+    dcf_ctx *ctx = t->ctx;
+    if (n < 2) return dcf_set_error(ctx, DCF_EINVAL, "a differential profile needs at least 2 lambdas (lambda_0 = 0 first)");
+    int rc = need_group(t, attr_idx);
+    if (rc) return rc;
+    if (decision < 0 || decision > 3) return dcf_set_error(ctx, DCF_EINVAL, "bad decision index %d", decision);
+    if (out_dtype != DCF_U8 && out_dtype != DCF_U16 && out_dtype != DCF_F32) return dcf_set_error(ctx, DCF_EINVAL, "bad output dtype");
+    (void)hipSetDevice(ctx->device);
+    const size_t one = (size_t)t->n * dtype_size(out_dtype);
+    const bool dev = (flags & DCF_DEVICE_PTR) != 0;
+    void *dst = dev ? nullptr : dcf_alloc(ctx, one);
+    if (!dev && !dst) return dcf_set_error(ctx, DCF_ENOMEM, "output staging");
+    for (int i = 1; i < n && rc == DCF_OK; i++) {
+        void *o = dev ? (void *)((uint8_t *)out + (size_t)(i - 1) * one) : dst;
+        /* lambda_cur is the float threshold passed on as a double (operation.c:67-68) */
+        rc = dcf_ops_filter(t, attr_idx, decision, (double)lambdas[i], 1, o, out_dtype, (flags & ~DCF_DEVICE_PTR) | DCF_KEEP_INVERTED);
+        if (rc == DCF_OK && !dev) {
+            if (hipMemcpyAsync((uint8_t *)out + (size_t)(i - 1) * one, dst, one, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+                hipStreamSynchronize(ctx->stream) != hipSuccess)
+                rc = dcf_set_error(ctx, DCF_EDEVICE, "output download");
+        }
+    }
+    if (!dev) dcf_release(ctx, dst);
+    return rc;
+}
+
+/* write_tree (io.c:1431-1573): one row per levelroot in raster order, the reference's printf formats */
+static int tree_csv_text(dcf_tree *t, int attr_idx, std::string &text)
+{
+    dcf_ctx *ctx = t->ctx;
+    int rc = need_group(t, attr_idx);
+    if (rc) return rc;
+    (void)hipSetDevice(ctx->device);
+    const int grp = dcf_group_of(attr_idx);
+    std::vector<uint32_t> n2p;
+    std::vector<int32_t> npar;
+    std::vector<float> grey;
+    std::vector<double> size, cols;
+    int ncol = 0;
+    if ((rc = dcf_ops_tree_table(t, grp, n2p, npar, grey, size, cols, ncol)) != DCF_OK) return rc;
+    text.clear();
+    text += "index,x,y,z,gval,parent,flux,area,";
+    if (grp == 1) text += "area_enc_rect,diag_enc_rect,x_extent,y_extent,z_extent";
+    else if (grp == 2) text += "x_mean,y_mean,z_mean";
+    else if (grp == 3) text += "x_wmean,y_wmean,z_wmean";
+    else if (grp == 4) text += "x_mean,y_mean,z_mean,x_wmean,y_wmean,z_wmean,elong,flat,spars,ncomp";
+    text += "\n";
+    const uint64_t W = t->dims[0], WH = t->dims[0] * t->dims[1];
+    char buf[512];
+    for (int64_t k = 0; k < t->K; k++) {
+        const uint64_t i = n2p[(size_t)k];
+        const long x = (long)((i % WH) % W), y = (long)((i % WH) / W), z = (long)(i / WH);
+        const int32_t pk = npar[(size_t)k];
+        const long par = pk < 0 ? -1L : (long)n2p[(size_t)pk];
+        const float g = grey[(size_t)k];
+        const float dh = pk < 0 ? g : g - grey[(size_t)pk];
+        const long sz = (long)size[(size_t)k];
+        int len = snprintf(buf, sizeof buf, "%ld,%ld,%ld,%ld,%f,%ld,%f,%ld,", (long)i, x, y, z, (double)g, par,
+                           (double)(dh * (float)sz), sz);
+        const double *c = cols.data() + (size_t)k * ncol;
+        auto f = [&](int j) { return (double)(float)c[j]; };
+        if (grp == 1)
+            len += snprintf(buf + len, sizeof buf - len, "%.2f,%.2f,%.2f,%.2f,%.2f", f(0), f(1), f(2), f(3), f(4));
+        else if (grp == 2 || grp == 3)
+            len += snprintf(buf + len, sizeof buf - len, "%.2f,%.2f,%.2f", f(0), f(1), f(2));
+        else if (grp == 4)
+            len += snprintf(buf + len, sizeof buf - len, "%.2f,%.2f,%.2f,%.2f,%.2f,%.2f,%.3f,%.3f,%.3f,%.4f ", f(0), f(1), f(2), f(3),
+                            f(4), f(5), f(6), f(7), f(8), f(9));
+        text.append(buf, (size_t)len);
+        text += "\n";
+    }
+    return DCF_OK;
+}
+
+extern "C" int dcf_tree_write_csv(dcf_tree *t, int attr_idx, const char *path)
+{
+    if (!t || !path) return DCF_EINVAL;
+    std::string text;
+    int rc = tree_csv_text(t, attr_idx, text);
+    if (rc) return rc;
+    FILE *f = fopen(path, "wb");
+    if (!f) return dcf_set_error(t->ctx, DCF_EIO, "cannot open %s", path);
+    const size_t w = fwrite(text.data(), 1, text.size(), f);
+    const int cl = fclose(f);
+    if (w != text.size() || cl != 0) return dcf_set_error(t->ctx, DCF_EIO, "cannot write %s", path);
+    return DCF_OK;
+}
+
 extern "C" int dcf_check(dcf_tree *t, int attr_idx, float *out, unsigned flags)
 {
     if (!t || !out) return DCF_EINVAL;
@@ -774,7 +865,14 @@ static int to_ops(dcf_ctx *ctx, const std::vector<OpTuple> &tp, std::vector<Op> 
             if (o.attr2 < 0 || o.attr2 >= DCF_NUM_ATTR) return dcf_set_error(ctx, DCF_EINVAL, "bad attribute index %d", o.attr2);
         } else if (o.name == "check") {
             if (t.items.size() > 2) return dcf_set_error(ctx, DCF_EINVAL, "check takes 2 parameters");
-        } else if (o.name == "dmp" || o.name == "csl" || o.name == "tree") {
+        } else if (o.name == "dmp") {
+            /* ("dmp", attr, lvec[, imscale]); the decision stays 0 (config.c:355-379) */
+            if (t.items.size() < 3 || t.items.size() > 4) return dcf_set_error(ctx, DCF_EINVAL, "dmp takes 3 or 4 parameters");
+            o.lvec = t.items[2];
+            o.imscale = t.items.size() > 3 ? atof(t.items[3].c_str()) : 1.0;
+        } else if (o.name == "tree") {
+            if (t.items.size() != 2) return dcf_set_error(ctx, DCF_EINVAL, "tree takes 2 parameters");
+        } else if (o.name == "csl") {
             return dcf_set_error(ctx, DCF_EUNSUPPORTED, "operation \"%s\" is outside the GPU path", o.name.c_str());
         } else {
             return dcf_set_error(ctx, DCF_EINVAL, "operation \"%s\" not recognized", o.name.c_str());
@@ -824,6 +922,20 @@ extern "C" int dcf_run_operations(dcf_ctx *ctx, const void *img, dcf_dtype dtype
             rc = dcf_spectrum1d(t, o.attr, lv, nl, sp.data());
             dcf_lvec_free(lv);
             if (rc == DCF_OK && cb) cb(user, (int)i, 2, sp.data(), nl);
+        } else if (o.name == "tree") {
+            std::string text;
+            rc = tree_csv_text(t, o.attr, text);
+            if (rc == DCF_OK && cb) cb(user, (int)i, 5, text.data(), (int64_t)text.size());
+        } else if (o.name == "dmp") {
+            float *lv = nullptr;
+            int nl = 0;
+            rc = dcf_lvec_read(o.lvec.c_str(), o.imscale, &lv, &nl);
+            if (rc) { rc = dcf_set_error(ctx, rc, "couldn't read the lvec file: %s", o.lvec.c_str()); break; }
+            std::vector<float> prof((size_t)n * (size_t)(nl - 1));
+            rc = nl < 2 ? dcf_set_error(ctx, DCF_EINVAL, "empty lvec file: %s", o.lvec.c_str())
+                        : dcf_dmp(t, o.attr, o.decision, lv, nl, prof.data(), DCF_F32, 0);
+            dcf_lvec_free(lv);
+            if (rc == DCF_OK && cb) cb(user, (int)i, 4, prof.data(), (int64_t)prof.size());
         } else if (o.name == "pattern2D") {
             float *l1 = nullptr, *l2 = nullptr;
             int n1 = 0, n2 = 0;

@@ -132,6 +132,10 @@ struct dcf_tree {
     const double *nlam = nullptr;
 };
 
+/* internal filter flag (never set by callers): a min-tree's output stays in the inverted grey
+ * space, as perform_dmp_operation writes it (operation.c:63-77) */
+#define DCF_KEEP_INVERTED 0x80000000u
+
 /* ---- error / launch helpers ------------------------------------------------------------------ */
 int dcf_set_error(dcf_ctx *ctx, int code, const char *fmt, ...);
 #define DCF_HIP(ctx, call)                                                                         \
@@ -172,6 +176,9 @@ int dcf_flood_build(dcf_tree *t, int grp);      /* u8/u16: canonical parent + pi
 /* batched form: per-image lambda (device, one per image), outputs (device table) */
 int dcf_ops_batch_filter(dcf_tree *t, int attr_idx, int decision, const double *lam_img, int extract, const BatchOut *outs,
                          const uint32_t *gmax_img, unsigned flags);
+/* write_tree: per node its levelroot pixel, parent node, grey, area function and printed columns */
+int dcf_ops_tree_table(dcf_tree *t, int grp, std::vector<uint32_t> &n2p, std::vector<int32_t> &npar, std::vector<float> &grey,
+                       std::vector<double> &size, std::vector<double> &cols, int &ncol);
 /* batched form: widen / copy one image into the batch buffer; per-image grey max and inversion */
 int dcf_batch_copy_in(dcf_ctx *ctx, const void *src, int src_dtype, void *dst, int dst_dtype, int64_t n);
 int dcf_batch_invert(dcf_ctx *ctx, void *cat, int dtype, const BatchImg *bimg, int nb, uint32_t *gmax_img, int64_t n);

@@ -18,6 +18,8 @@
  * operation order, including its float evaluation of area_attribute (attributes.c:105: the
  * ulong area is converted to float before the product), so bins and thresholds match bit for bit.
  */
+#include <vector>
+
 #include "dcf_internal.h"
 
 struct AttrCtx {
@@ -265,13 +267,13 @@ __device__ __forceinline__ float px_grey_ops(const ImgView &iv, int64_t p)
 
 /* out[p] = post(val[p2n[p]]), post = extract (g - f) then min-tree re-inversion (gmax - f) */
 __global__ __launch_bounds__(256) void k_filter_out(ImgView iv, const uint32_t *__restrict__ p2n, const float *__restrict__ val,
-                                                    int extract, void *__restrict__ out, int out_dtype, int64_t n)
+                                                    int extract, int reinvert, void *__restrict__ out, int out_dtype, int64_t n)
 {
     int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     float f = val[p2n[p]];
     if (extract) f = px_grey_ops(iv, p) - f;
-    if (iv.invert) f = *iv.gmax - f;
+    if (reinvert && iv.invert) f = *iv.gmax - f;
     if (out_dtype == DCF_F32) ((float *)out)[p] = f;
     else if (out_dtype == DCF_U16) ((uint16_t *)out)[p] = (uint16_t)f;
     else ((uint8_t *)out)[p] = (uint8_t)f;
@@ -456,8 +458,9 @@ int dcf_ops_filter(dcf_tree *t, int attr_idx, int decision, double lambda, int e
         hipEvent_t ea = nullptr, eb = nullptr;
         if (ctx->profiling && (ctx->prof_only.empty() || strstr("k_filter_out", ctx->prof_only.c_str())))
             dcf_prof_begin(ctx, "k_filter_out", &ea, &eb);
-        hipLaunchKernelGGL(k_filter_out, dim3(nblk(t->n, 256)), dim3(256), 0, ctx->stream, iv, t->p2n, val, extract, out_dev,
-                           (int)out_dtype, t->n);
+        const int reinvert = (flags & DCF_KEEP_INVERTED) ? 0 : 1;
+        hipLaunchKernelGGL(k_filter_out, dim3(nblk(t->n, 256)), dim3(256), 0, ctx->stream, iv, t->p2n, val, extract, reinvert,
+                           out_dev, (int)out_dtype, t->n);
         hipError_t le = hipGetLastError();
         if (ea) dcf_prof_end(ctx, "k_filter_out", ea, eb);
         if (le != hipSuccess) rc = dcf_set_error(ctx, DCF_EDEVICE, "launch k_filter_out: %s", hipGetErrorString(le));
@@ -670,4 +673,126 @@ int dcf_batch_invert(dcf_ctx *ctx, void *cat, int dtype, const BatchImg *bimg, i
     DCF_LAUNCH(ctx, "k_batch_invert", k_batch_invert, dim3(nblk(n, 256)), dim3(256), 0, cat, dtype, bimg, nb,
                (const uint32_t *)gmax_img, n);
     return DCF_OK;
+}
+
+/* ------------------------------------------------------------------------------------------------ */
+/* tree export (write_tree, io.c:1431-1573)                                                         */
+/* ------------------------------------------------------------------------------------------------ */
+/* node -> its levelroot pixel (nodes are numbered in raster order of their levelroots) */
+__global__ __launch_bounds__(256) void k_node_pixel(ImgView iv, const int32_t *__restrict__ parent, const uint32_t *__restrict__ p2n,
+                                                    uint32_t *__restrict__ n2p, int64_t n)
+{
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int32_t r = parent[p];
+    if (r < 0 || px_grey_ops(iv, r) != px_grey_ops(iv, p)) n2p[p2n[p]] = (uint32_t)p;
+}
+
+/* the group's area function and printed columns of every node: group 1 attributes 1-5, group 2
+ * attributes 6-8, group 3 attributes 9-11, group 4 the entries 3-12 of inertia_attribute_arr
+ * (attributes.c:628-688; its 2D / 3D choice follows the image, not the node; entries 6-8, the
+ * grey-weighted means, are filled from the group-3 records by the caller) */
+__global__ __launch_bounds__(256) void k_csv_cols(AttrCtx c, int grp, int is3d, double *__restrict__ size,
+                                                  double *__restrict__ cols, int ncol, int64_t K)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    size[k] = attr_area(c, grp, k);
+    double *o = cols + k * ncol;
+    if (grp == 1) { for (int j = 0; j < 5; j++) o[j] = attr_value(c, 1 + j, k); return; }
+    if (grp == 2) { for (int j = 0; j < 3; j++) o[j] = attr_value(c, 6 + j, k); return; }
+    if (grp == 3) { for (int j = 0; j < 3; j++) o[j] = attr_value(c, 9 + j, k); return; }
+    if (grp != 4) return;
+    const DevAttr &A = c.A;
+    const float p0 = c.p0, p1 = c.p1, p2 = c.p2;
+    const uint32_t area = A.area[k];
+    const double in0 = (double)p0 * p1 * p2 * area;
+    const double in1 = p0 * (double)A.s[0][k], in2 = p1 * (double)A.s[1][k], in3 = p2 * (double)A.s[2][k];
+    const double in4 = (double)(p0 * p0) * (double)A.s[3][k];
+    const double in5 = (double)(p1 * p1) * (double)A.s[4][k];
+    const double in6 = (double)(p2 * p2) * (double)A.s[5][k];
+    const double in7 = (double)(p0 * p1) * (double)A.s[6][k];
+    const double in8 = (double)(p1 * p2) * (double)A.s[7][k];
+    const double in9 = (double)(p0 * p2) * (double)A.s[8][k];
+    o[0] = in1 / in0;
+    o[1] = in2 / in0;
+    o[2] = in3 / in0;
+    const double corr = in0 / 12;
+    if (!is3d) {
+        const double m0 = in4 - in1 * in1 / in0 + p0 * corr, m1 = in5 - in2 * in2 / in0 + p1 * corr, m2 = in7 - in1 * in2 / in0;
+        double ev[2];
+        eig2(m0, m2, m1, ev);
+        o[6] = ev[1] / ev[0];
+        o[7] = -1;
+        o[8] = M_PI * sqrt(4 * ev[1] / in0) * sqrt(4 * ev[0] / in0) / in0;
+        o[9] = (m0 + m1) / (in0 * in0); /* pow(A, 2.0) is the correctly rounded square */
+    } else {
+        const double m6[6] = { in4 - in1 * in1 / in0 + p0 * corr, in5 - in2 * in2 / in0 + p1 * corr,
+                               in6 - in3 * in3 / in0 + p2 * corr, in7 - in1 * in2 / in0,
+                               in8 - in2 * in3 / in0, in9 - in1 * in3 / in0 };
+        double ev[3];
+        eig3(m6, ev);
+        o[6] = ev[2] / ev[1];
+        o[7] = ev[1] / ev[0];
+        o[8] = M_PI * sqrt(20 * ev[2] / in0) * sqrt(20 * ev[1] / in0) * sqrt(20 * ev[0] / in0) / (6 * in0);
+        o[9] = (m6[0] + m6[1] + m6[2]) / pow(in0, 5.0 / 3.0);
+    }
+}
+
+/* grey-weighted means (attributes 9-11) of the group-3 records into columns 3-5 of group 4 */
+__global__ __launch_bounds__(256) void k_csv_wmeans(AttrCtx c3, double *__restrict__ cols, int ncol, int64_t K)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    for (int j = 0; j < 3; j++) cols[k * ncol + 3 + j] = attr_value(c3, 9 + j, k);
+}
+
+int dcf_ops_tree_table(dcf_tree *t, int grp, std::vector<uint32_t> &n2p, std::vector<int32_t> &npar, std::vector<float> &grey,
+                       std::vector<double> &size, std::vector<double> &cols, int &ncol)
+{
+    dcf_ctx *ctx = t->ctx;
+    const int64_t K = t->K;
+    ncol = grp == 1 ? 5 : (grp == 2 || grp == 3 ? 3 : (grp == 4 ? 10 : 0));
+    uint32_t *d_n2p = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4);
+    double *d_size = (double *)dcf_alloc(ctx, (size_t)K * 8);
+    double *d_cols = (double *)dcf_alloc(ctx, (size_t)K * (ncol ? ncol : 1) * 8);
+    int rc = DCF_OK;
+    if (!d_n2p || !d_size || !d_cols) rc = dcf_set_error(ctx, DCF_ENOMEM, "tree table");
+    if (rc == DCF_OK) {
+        hipLaunchKernelGGL(k_node_pixel, dim3(nblk(t->n, 256)), dim3(256), 0, ctx->stream, view_of(t), t->parent, t->p2n, d_n2p, t->n);
+        hipLaunchKernelGGL(k_csv_cols, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, attr_ctx(t), grp, t->dims[2] > 1 ? 1 : 0, d_size,
+                           d_cols, ncol, K);
+        if (hipGetLastError() != hipSuccess) rc = dcf_set_error(ctx, DCF_EDEVICE, "launch tree table kernels");
+    }
+    if (rc == DCF_OK && grp == 4) {
+        /* the weighted means need the grey sums of group 3: accumulate them beside the group-4 records */
+        DevAttr g4 = t->attr;
+        t->attr = DevAttr();
+        rc = dcf_accumulate_group(t, 3);
+        if (rc == DCF_OK) {
+            hipLaunchKernelGGL(k_csv_wmeans, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, attr_ctx(t), d_cols, ncol, K);
+            if (hipGetLastError() != hipSuccess) rc = dcf_set_error(ctx, DCF_EDEVICE, "launch k_csv_wmeans");
+        }
+        dcf_free_attr(ctx, t->attr);
+        t->attr = g4;
+        t->grp = 4;
+    }
+    if (rc == DCF_OK) {
+        n2p.resize((size_t)K);
+        npar.resize((size_t)K);
+        grey.resize((size_t)K);
+        size.resize((size_t)K);
+        cols.resize((size_t)K * ncol);
+        if (hipMemcpyAsync(n2p.data(), d_n2p, (size_t)K * 4, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipMemcpyAsync(npar.data(), t->npar, (size_t)K * 4, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipMemcpyAsync(grey.data(), t->ngrey, (size_t)K * 4, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipMemcpyAsync(size.data(), d_size, (size_t)K * 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            (ncol && hipMemcpyAsync(cols.data(), d_cols, (size_t)K * ncol * 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess) ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess)
+            rc = dcf_set_error(ctx, DCF_EDEVICE, "tree table download");
+    }
+    dcf_release(ctx, d_n2p);
+    dcf_release(ctx, d_size);
+    dcf_release(ctx, d_cols);
+    return rc;
 }

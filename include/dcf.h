@@ -111,6 +111,13 @@ int dcf_tree_num_nodes(dcf_tree *tree, int64_t *num_nodes);
 int dcf_filter(dcf_tree *tree, int attr_idx, int decision, double lambda, int extract, void *out,
                dcf_dtype out_dtype, unsigned flags);
 
+/* ---- differential profile (perform_dmp_operation, operation.c:63-77) ------------------------------ *
+ * lambdas as returned by dcf_lvec_read (lambda_0 = 0 first, n values): out receives n-1 images, image
+ * i = g - filter(lambda_{i+1}) with the given decision (the dmp operation uses the direct rule); for
+ * a min-tree g and the filter stay in the inverted grey space, as the reference writes them. */
+int dcf_dmp(dcf_tree *tree, int attr_idx, int decision, const float *lambdas, int n, void *out, dcf_dtype out_dtype,
+            unsigned flags);
+
 /* ---- check (tree_attribute_check, filtering.c:47-52): out[p] = (float) attr(levelroot(p)) ---- */
 int dcf_check(dcf_tree *tree, int attr_idx, float *out, unsigned flags);
 
@@ -121,6 +128,12 @@ int dcf_check(dcf_tree *tree, int attr_idx, float *out, unsigned flags);
 int dcf_spectrum1d(dcf_tree *tree, int attr_idx, const float *lambdas, int n, double *out);
 int dcf_spectrum2d(dcf_tree *tree, int attr1, const float *l1, int n1, int attr2, const float *l2, int n2,
                    double *out);
+
+/* ---- tree export (write_tree, io.c:1431-1573) --------------------------------------------------- *
+ * The "tree" operation's CSV: header, then one row per node (levelroot) in raster order --
+ * index,x,y,z,gval,parent,flux,area and the columns of attr_idx's group, with the reference's printf
+ * formats.  The group must be the accumulated one (DCF_EGROUP otherwise). */
+int dcf_tree_write_csv(dcf_tree *tree, int attr_idx, const char *path);
 
 /* ---- lambda-vector files (lambda_vector_read, lambdavec.c:35-84) -------------------------------- *
  * On success *lambdas (malloc'd, free with dcf_lvec_free) holds count+1 values, lambda_0 = 0. */
@@ -155,7 +168,8 @@ int dcf_batch_filter(dcf_ctx *ctx, const dcf_batch_image *images, int count, int
  * ops: the reference's operation tuples, e.g. "(\"filter\",0,100),(\"pattern\",0,\"lvec.txt\",1)".
  * Runs them in order on one image with attribute-group reuse; outputs are written by the callback
  * (kind: 0 filtered/extracted image (float, n values), 1 check image (float), 2 1D spectrum
- * (double, count values), 3 2D spectrum (double)).  Unsupported operations return
+ * (double, count values), 3 2D spectrum (double), 4 differential profile (float, (lambdas-1) images),
+ * 5 tree CSV text (char, count bytes)).  Unsupported operations return
  * DCF_EUNSUPPORTED before anything runs. */
 typedef void (*dcf_output_fn)(void *user, int op_index, int kind, const void *data, int64_t count);
 int dcf_run_operations(dcf_ctx *ctx, const void *img, dcf_dtype dtype, const uint64_t dims[3], int connectivity,

@@ -226,6 +226,94 @@ def test_workload_blobs_512(ctx):
     assert_same(t.spectrum2d(17, l1, 0, l2), o.spectrum2d(17, l1, 0, l2), "pattern2D")
 
 
+def test_tree_csv_matches_gt_tree(ctx, lena, tmp_path):
+    """("tree",15) on lena, 4-connectivity: the CSV of write_tree (io.c:1431-1573) against the
+    reference's GT_tree.txt, rebuilt line by line from its parsed columns with the reference's printf
+    formats.  Every column is compared as text; elong / flat / spars come from closed-form
+    eigenvalues (not EISPACK rs) and may differ by one unit of their last printed digit."""
+    t = ctx.build(lena["img"], 4, "max", 15)
+    path = tmp_path / "lena_tree.txt"
+    t.write_csv(str(path), 15)
+    lines = path.read_text().splitlines()
+    assert lines[0] == ("index,x,y,z,gval,parent,flux,area,x_mean,y_mean,z_mean,x_wmean,y_wmean,z_wmean,"
+                        "elong,flat,spars,ncomp")
+    order = np.argsort(lena["tree_index"])
+    cols = {k[5:]: lena[k][order] for k in lena if k.startswith("tree_")}
+    n = len(order)
+    assert len(lines) == n + 1
+    fmt_int = ("index", "x", "y", "z")
+    eig = {"elong": 14, "flat": 15, "spars": 16}
+    bad = []
+    for r in range(n):
+        got = lines[r + 1].split(",")
+        assert lines[r + 1].endswith(" "), "rows end with a space after ncomp"
+        want = ["%d" % cols[c][r] for c in fmt_int]
+        want += ["%f" % cols["gval"][r], "%d" % cols["parent"][r], "%f" % cols["flux"][r], "%d" % cols["area"][r]]
+        want += ["%.2f" % cols[c][r] for c in ("x_mean", "y_mean", "z_mean", "x_wmean", "y_wmean", "z_wmean")]
+        for j, w in enumerate(want):
+            if got[j] != w:
+                bad.append((r, j, got[j], w))
+        for j, c in enumerate(("elong", "flat", "spars")):
+            if abs(float(got[14 + j]) - cols[c][r]) > 0.0011:
+                bad.append((r, 14 + j, got[14 + j], cols[c][r]))
+        if got[17] != "%.4f " % cols["ncomp"][r]:
+            bad.append((r, 17, got[17], "%.4f " % cols["ncomp"][r]))
+    assert not bad, "%d cells differ, first %s" % (len(bad), bad[:5])
+
+
+def test_tree_csv_area_group_3d(ctx, tmp_path):
+    """group 0 (area) CSV of a 3D min-tree against the oracle's tree: one row per levelroot in raster
+    order with index, coordinates, grey (inverted space), parent levelroot, flux and area"""
+    rng = np.random.default_rng(47)
+    img = rng.integers(0, 12, size=(6, 9, 11)).astype(np.uint8)
+    t = ctx.build(img, 6, "min", 0)
+    o = OracleTree(img, 6, "min", 0)
+    path = tmp_path / "t.txt"
+    t.write_csv(str(path), 0)
+    lines = path.read_text().splitlines()
+    assert lines[0] == "index,x,y,z,gval,parent,flux,area,"
+    par = o.parent().reshape(-1)
+    g = o.grey().reshape(-1).astype(np.float32)
+    lr = np.nonzero((par < 0) | (g != g[np.maximum(par, 0)]))[0]
+    area = o.check(0).reshape(-1)
+    assert len(lines) == len(lr) + 1
+    W, H = img.shape[2], img.shape[1]
+    for r, i in enumerate(lr):
+        p = par[i]
+        dh = np.float32(g[i] - g[p]) if p >= 0 else g[i]
+        sz = int(area[i])
+        want = "%d,%d,%d,%d,%f,%d,%f,%d," % (i, i % (W * H) % W, i % (W * H) // W, i // (W * H), g[i], p,
+                                             np.float32(dh * np.float32(sz)), sz)
+        assert lines[r + 1] == want, (r, lines[r + 1], want)
+
+
+@pytest.mark.parametrize("tree_type", ["max", "min"])
+def test_dmp(ctx, tree_type):
+    """the differential profile: g - filter(lambda_i) per lambda after lambda_0 (operation.c:63-77);
+    a min-tree's profile stays in the inverted grey space"""
+    rng = np.random.default_rng(41)
+    img = rng.integers(0, 60, size=(70, 90)).astype(np.uint8)
+    lv = synth.log_lambdas(6, 2, 500)
+    t = ctx.build(img, 8, tree_type, 0)
+    o = OracleTree(img, 8, tree_type, 0)
+    prof = t.dmp(0, lv)
+    assert prof.shape == (len(lv) - 1,) + img.shape
+    for i in range(1, len(lv)):
+        ext = o.filter(0, float(lv[i]), 0, extract=True)
+        want = ext if tree_type == "max" else np.float32(o.gmax) - ext
+        assert_same(prof[i - 1], want, "dmp %s lambda %g" % (tree_type, lv[i]))
+
+
+def test_dmp_operation(ctx, tmp_path):
+    rng = np.random.default_rng(43)
+    img = rng.integers(0, 30, size=(40, 50)).astype(np.uint8)
+    lvf = tmp_path / "lv.txt"
+    lvf.write_text("#LambdaVector file. Keep this exact line in all similar lambda vector files\n3 10 40\n")
+    outs = ctx.run_operations(img, [("dmp", 0, str(lvf))], connectivity=4)
+    t = ctx.build(img, 4, "max", 0)
+    assert_same(outs[0], t.dmp(0, lvec_read(str(lvf))), "dmp operation")
+
+
 # ------------------------------------------------------------------------------------------------
 # error behaviour (status codes, no device fault)
 # ------------------------------------------------------------------------------------------------
@@ -251,7 +339,7 @@ def test_errors(ctx, tmp_path):
         ctx.run_operations(img, [("pattern", 0, str(tmp_path / "none.txt"))])
     assert e.value.code == -5
     with pytest.raises(dcf.DcfError) as e:
-        ctx.run_operations(img, [("dmp", 0, "x")])
+        ctx.run_operations(img, [("csl", 0, "x")])
     assert e.value.code == -4
     # min / max rules of a non-increasing attribute depend on the reference's scan order: refused
     t17 = ctx.build(img, 4, "max", 17)
