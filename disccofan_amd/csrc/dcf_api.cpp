@@ -184,6 +184,7 @@ extern "C" int dcf_ctx_create(int device, unsigned flags, dcf_ctx **out)
             v[0] = a; v[1] = b; v[2] = d;
         }
     };
+    if (const char *z = getenv("DCF_ZONES")) c->zones = atoi(z);
     shape3("DCF_LEVEL_TILE2D", c->lt2);
     shape3("DCF_LEVEL_GROUP2D", c->lg2);
     shape3("DCF_LEVEL_TILE3D", c->lt3);

@@ -411,6 +411,205 @@ __global__ __launch_bounds__(256) void k_union(LevelArgs a, const uint32_t *__re
 }
 
 /* ------------------------------------------------------------------------------------------------ */
+/* zone graph of a pass (coarse pass of u16 images, u8 planes)                                      */
+/* ------------------------------------------------------------------------------------------------ */
+/* A tile zone is a connected set of pixels of one digit inside one tile.  Its pixels are linked to
+ * its representative (its minimum pixel, the minimum raster index of the zone) before the pass, so
+ * the union-find starts with every zone as one set; the unions a level needs are then the distinct
+ * (zone, neighbouring zone) pairs, de-duplicated per tile in LDS, instead of one attempt per pixel
+ * and neighbour.  The canonical levelroot of a set is its minimum pixel of the level, which is its
+ * minimum zone representative of the level, so the level kernels work unchanged. */
+#define ZT_MAX 4096
+#define ZH_CAP 4096
+#define ZH_EMPTY 0xFFFFFFFFFFFFFFFFull
+
+struct TileGeo {
+    int W, H, D;
+    int TW, TH, TD;
+    int ntx, nty;
+};
+
+__device__ __forceinline__ uint32_t lds_ld(const uint32_t *a, uint32_t i)
+{
+    return __hip_atomic_load(&a[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ uint32_t lds_find(uint32_t *par, uint32_t x)
+{
+    uint32_t p = lds_ld(par, x);
+    while (p != x) {
+        const uint32_t gp = lds_ld(par, p);
+        if (gp != p) __hip_atomic_store(&par[x], gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        x = p;
+        p = gp;
+    }
+    return x;
+}
+
+__device__ __forceinline__ void lds_union(uint32_t *par, uint32_t a, uint32_t b)
+{
+    for (;;) {
+        a = lds_find(par, a);
+        b = lds_find(par, b);
+        if (a == b) return;
+        const uint32_t hi = a > b ? a : b, lo = a > b ? b : a;
+        uint32_t exp = hi;
+        if (__hip_atomic_compare_exchange_strong(&par[hi], &exp, lo, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_WORKGROUP))
+            return;
+    }
+}
+
+__device__ __forceinline__ void tile_origin(const TileGeo &tg, int b, int &x0, int &y0, int &z0)
+{
+    const int tx = b % tg.ntx, ty = (b / tg.ntx) % tg.nty, tz = b / (tg.ntx * tg.nty);
+    x0 = tx * tg.TW;
+    y0 = ty * tg.TH;
+    z0 = tz * tg.TD;
+}
+
+/* zones of every tile: zrep[p] = the zone's minimum pixel; uf[p] -> zrep for the other pixels */
+__global__ __launch_bounds__(256) void k_tile_zones(ImgView iv, int shift, TileGeo tg, Nbr nb, uint32_t *__restrict__ zrep,
+                                                    uint32_t *__restrict__ uf)
+{
+    __shared__ uint32_t par[ZT_MAX];
+    __shared__ uint16_t dg[ZT_MAX];
+    int x0, y0, z0;
+    tile_origin(tg, blockIdx.x, x0, y0, z0);
+    const int tsz = tg.TW * tg.TH * tg.TD;
+    for (int l = threadIdx.x; l < tsz; l += blockDim.x) {
+        const int x = x0 + l % tg.TW, y = y0 + (l / tg.TW) % tg.TH, z = z0 + l / (tg.TW * tg.TH);
+        const bool in = x < tg.W && y < tg.H && z < tg.D;
+        dg[l] = in ? (uint16_t)(fl_key(iv, (int64_t)x + (int64_t)y * tg.W + (int64_t)z * tg.W * tg.H) >> shift) : 0xFFFFu;
+        par[l] = (uint32_t)l;
+    }
+    __syncthreads();
+    for (int l = threadIdx.x; l < tsz; l += blockDim.x) {
+        if (dg[l] == 0xFFFFu) continue;
+        const int lx = l % tg.TW, ly = (l / tg.TW) % tg.TH, lz = l / (tg.TW * tg.TH);
+        for (int k = 0; k < nb.n; k++) {
+            /* the forward half of the neighbourhood: every pair once */
+            const int dx = nb.dx[k], dy = nb.dy[k], dz = nb.dz[k];
+            if (!(dz > 0 || (dz == 0 && (dy > 0 || (dy == 0 && dx > 0))))) continue;
+            const int nx = lx + dx, ny = ly + dy, nz = lz + dz;
+            if (nx < 0 || nx >= tg.TW || ny < 0 || ny >= tg.TH || nz < 0 || nz >= tg.TD) continue;
+            const int l2 = nx + ny * tg.TW + nz * tg.TW * tg.TH;
+            if (dg[l2] == dg[l]) lds_union(par, (uint32_t)l, (uint32_t)l2);
+        }
+    }
+    __syncthreads();
+    for (int l = threadIdx.x; l < tsz; l += blockDim.x) {
+        if (dg[l] == 0xFFFFu) continue;
+        const int r = (int)lds_find(par, (uint32_t)l);
+        const int x = x0 + l % tg.TW, y = y0 + (l / tg.TW) % tg.TH, z = z0 + l / (tg.TW * tg.TH);
+        const int rx = x0 + r % tg.TW, ry = y0 + (r / tg.TW) % tg.TH, rz = z0 + r / (tg.TW * tg.TH);
+        const uint32_t p = (uint32_t)((int64_t)x + (int64_t)y * tg.W + (int64_t)z * tg.W * tg.H);
+        const uint32_t rp = (uint32_t)((int64_t)rx + (int64_t)ry * tg.W + (int64_t)rz * tg.W * tg.H);
+        zrep[p] = rp;
+        if (p != rp) uf[p] = rp;
+    }
+}
+
+__device__ __forceinline__ uint32_t zh_hash(unsigned long long k)
+{
+    k ^= k >> 31;
+    k *= 0x9E3779B97F4A7C15ull;
+    return (uint32_t)(k >> 40);
+}
+
+/* distinct (zone, processed neighbour zone) pairs of every tile, appended to the bucket of the
+ * zone's level: a neighbour q of p (zone rp, digit d) is processed when its digit is higher, or
+ * equal in another zone (each such pair once, from the zone of smaller representative).  A level's
+ * bucket holds at most NB edges per pixel of the level, which bounds every append. */
+__global__ __launch_bounds__(256) void k_zone_edges(ImgView iv, int shift, TileGeo tg, Nbr nb, const uint32_t *__restrict__ zrep,
+                                                    const uint64_t *__restrict__ bucket, uint32_t *__restrict__ cursor,
+                                                    uint2 *__restrict__ edges)
+{
+    __shared__ unsigned long long tab[ZH_CAP];
+    __shared__ uint32_t cnt[NLEV], base[NLEV];
+    for (int i = threadIdx.x; i < ZH_CAP; i += blockDim.x) tab[i] = ZH_EMPTY;
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    int x0, y0, z0;
+    tile_origin(tg, blockIdx.x, x0, y0, z0);
+    const int tsz = tg.TW * tg.TH * tg.TD;
+    const int64_t WH = (int64_t)tg.W * tg.H;
+    for (int l = threadIdx.x; l < tsz; l += blockDim.x) {
+        const int x = x0 + l % tg.TW, y = y0 + (l / tg.TW) % tg.TH, z = z0 + l / (tg.TW * tg.TH);
+        if (x >= tg.W || y >= tg.H || z >= tg.D) continue;
+        const int64_t p = (int64_t)x + (int64_t)y * tg.W + (int64_t)z * WH;
+        const uint32_t dp = fl_key(iv, p) >> shift;
+        const uint32_t rp = zrep[p];
+        for (int k = 0; k < nb.n; k++) {
+            const int nx = x + nb.dx[k], ny = y + nb.dy[k], nz = z + nb.dz[k];
+            if (nx < 0 || nx >= tg.W || ny < 0 || ny >= tg.H || nz < 0 || nz >= tg.D) continue;
+            const int64_t q = (int64_t)nx + (int64_t)ny * tg.W + (int64_t)nz * WH;
+            const uint32_t dq = fl_key(iv, q) >> shift;
+            if (dq < dp) continue;
+            const uint32_t rq = zrep[q];
+            if (dq == dp && rq <= rp) continue;
+            const unsigned long long key = ((unsigned long long)rp << 32) | rq;
+            uint32_t h = zh_hash(key) & (ZH_CAP - 1);
+            bool placed = false;
+            for (int probe = 0; probe < 64; probe++) {
+                const unsigned long long prev = atomicCAS(&tab[h], ZH_EMPTY, key);
+                if (prev == ZH_EMPTY) {
+                    atomicAdd(&cnt[dp], 1u);
+                    placed = true;
+                    break;
+                }
+                if (prev == key) {
+                    placed = true;
+                    break;
+                }
+                h = (h + 1) & (ZH_CAP - 1);
+            }
+            if (!placed) {
+                /* table crowded: append this pair directly */
+                const uint32_t pos = atomicAdd(&cursor[dp], 1u);
+                edges[bucket[dp] + pos] = make_uint2(rp, rq);
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t c = cnt[threadIdx.x];
+    base[threadIdx.x] = c ? atomicAdd(&cursor[threadIdx.x], c) : 0u;
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < ZH_CAP; i += blockDim.x) {
+        const unsigned long long key = tab[i];
+        if (key == ZH_EMPTY) continue;
+        const uint32_t rp = (uint32_t)(key >> 32), rq = (uint32_t)key;
+        const uint32_t d = fl_key(iv, rp) >> shift;
+        const uint32_t j = atomicAdd(&cnt[d], 1u);
+        edges[bucket[d] + base[d] + j] = make_uint2(rp, rq);
+    }
+}
+
+/* the unions of one level from its zone edges (k_union's linking rules) */
+__global__ __launch_bounds__(256) void k_union_edges(LevelArgs a, const uint2 *__restrict__ edges, uint32_t cnt, uint32_t lev)
+{
+    __shared__ uint32_t s_abs[256 * 2];
+    __shared__ uint32_t s_n, s_base;
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    const AbsBuf ab{ s_abs, &s_n };
+    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i < cnt) {
+        const uint2 e = edges[i];
+        uint32_t ta, tb;
+        const uint32_t ra = uf_find(a.uf, e.x, ta);
+        const uint32_t rb = uf_find(a.uf, e.y, tb);
+        uint32_t tout;
+        (void)uf_unite(a.uf, ra, ta, rb, tb, lev, ab, tout);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && s_n) s_base = atomicAdd(a.nabs, s_n);
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < s_n; j += blockDim.x) a.absorbed[s_base + j] = s_abs[j];
+}
+
+/* ------------------------------------------------------------------------------------------------ */
 /* wave-aggregated attribute updates                                                                */
 /* ------------------------------------------------------------------------------------------------ */
 /* All pixels of one level that fall in the same component add into the same root record; plain
@@ -900,7 +1099,7 @@ static int clear_attr(dcf_ctx *ctx, DevAttr &a, int64_t n)
 
 /* one flooding pass over all levels (high to low) */
 static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAttr &S, int grp, int32_t *par, int32_t *spar,
-                    int64_t kc)
+                    int64_t kc, bool zones)
 {
     const int64_t n = la.g.n;
     uint32_t *hist = (uint32_t *)sc.get(2 * NLEV * 4);
@@ -933,6 +1132,32 @@ static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAtt
     const int64_t sblocks = sg.tiled ? (int64_t)sg.ngx * sg.ngy * ngz * sg.GX * sg.GY * sg.GZ : (int64_t)nblk(n, 4096);
     DCF_LAUNCH(ctx, "k_lvl_scatter", k_lvl_scatter, dim3((unsigned)sblocks), dim3(256), 0, la.iv, la.pass, la.shift, n, sg, cursor,
                order);
+    /* zone graph: zones pre-linked in the union-find, distinct zone pairs bucketed by level */
+    uint2 *edges = nullptr;
+    uint64_t bucket_h[NLEV];
+    uint32_t ecnt[NLEV];
+    if (zones) {
+        TileGeo tg;
+        tg.W = la.g.W; tg.H = la.g.H; tg.D = la.g.D;
+        if (tg.D > 1) { tg.TW = 16; tg.TH = 16; tg.TD = 16; }
+        else { tg.TW = 64; tg.TH = 64; tg.TD = 1; }
+        tg.ntx = (tg.W + tg.TW - 1) / tg.TW;
+        tg.nty = (tg.H + tg.TH - 1) / tg.TH;
+        const int64_t ntiles = (int64_t)tg.ntx * tg.nty * ((tg.D + tg.TD - 1) / tg.TD);
+        uint32_t *zrep = (uint32_t *)sc.get((size_t)n * 4);
+        uint64_t *bucket = (uint64_t *)sc.get(NLEV * 8);
+        uint32_t *ecur = (uint32_t *)sc.get(NLEV * 4);
+        edges = (uint2 *)sc.get((size_t)n * la.nb.n * sizeof(uint2));
+        if (!zrep || !bucket || !ecur || !edges) return dcf_set_error(ctx, DCF_ENOMEM, "zone graph");
+        for (int i = 0; i < NLEV; i++) bucket_h[i] = (uint64_t)la.nb.n * off[i];
+        DCF_HIP(ctx, hipMemcpyAsync(bucket, bucket_h, sizeof bucket_h, hipMemcpyHostToDevice, ctx->stream));
+        DCF_LAUNCH(ctx, "k_fill", k_fill_u32f, dim3(1), dim3(256), 0, ecur, 0u, (int64_t)NLEV);
+        DCF_LAUNCH(ctx, "k_tile_zones", k_tile_zones, dim3((unsigned)ntiles), dim3(256), 0, la.iv, la.shift, tg, la.nb, zrep, la.uf);
+        DCF_LAUNCH(ctx, "k_zone_edges", k_zone_edges, dim3((unsigned)ntiles), dim3(256), 0, la.iv, la.shift, tg, la.nb,
+                   (const uint32_t *)zrep, (const uint64_t *)bucket, ecur, edges);
+        DCF_HIP(ctx, hipMemcpyAsync(ecnt, ecur, sizeof ecnt, hipMemcpyDeviceToHost, ctx->stream));
+        DCF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
     /* nabs[0]: running count of absorbed roots; nabs[1 + i]: where the i-th processed level's
      * entries start (written on the device, so the level loop never waits for the host) */
     uint32_t *nabs = la.nabs;
@@ -951,7 +1176,11 @@ static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAtt
         uint32_t cnt = h[lev];
         if (!cnt) continue;
         const uint32_t *lvl = order + off[lev];
-        switch (la.nb.n) {
+        if (zones) {
+            if (ecnt[lev])
+                DCF_LAUNCH(ctx, "k_union", k_union_edges, dim3(nblk(ecnt[lev], 256)), dim3(256), 0, la,
+                           (const uint2 *)(edges + bucket_h[lev]), ecnt[lev], (uint32_t)lev);
+        } else switch (la.nb.n) {
         case 4: DCF_LAUNCH(ctx, "k_union", k_union<4>, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, cnt, (uint32_t)lev, (uint32_t)kc); break;
         case 6: DCF_LAUNCH(ctx, "k_union", k_union<6>, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, cnt, (uint32_t)lev, (uint32_t)kc); break;
         case 8: DCF_LAUNCH(ctx, "k_union", k_union<8>, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, cnt, (uint32_t)lev, (uint32_t)kc); break;
@@ -1015,7 +1244,11 @@ int dcf_flood_build(dcf_tree *t, int grp)
     DCF_LAUNCH(ctx, "k_uf_init", k_uf_init, dim3(nblk(n, 256)), dim3(256), 0, la.iv, 1, la.shift, n, n, la.uf);
     DCF_LAUNCH(ctx, "k_fill", k_fill_i32, dim3(nblk(n, 256)), dim3(256), 0, par1, -1, n);
     DevAttr none;
-    if ((rc = run_pass(ctx, la, sc, A, none, grp, par1, nullptr, 0))) return rc;
+    /* zone graph for the first pass (single images: the batch's tiles would straddle images) */
+    const int64_t edge_bytes = n * la.nb.n * (int64_t)sizeof(uint2);
+    const bool zones = !t->bimg && edge_bytes <= ((int64_t)8 << 30) &&
+                       (ctx->zones == 1 || (ctx->zones == 2 && (two || !is3d)));
+    if ((rc = run_pass(ctx, la, sc, A, none, grp, par1, nullptr, 0, zones))) return rc;
     /* roots of pass 1 kept par1 = -1 except non-roots (levelroot) and absorbed roots (parent) */
 
     if (!two) {
@@ -1074,7 +1307,7 @@ int dcf_flood_build(dcf_tree *t, int grp)
     DCF_LAUNCH(ctx, "k_uf_init", k_uf_init, dim3(nblk(n + kc, 256)), dim3(256), 0, la.iv, 2, 0, n, n + kc, la.uf);
     DCF_LAUNCH(ctx, "k_fill", k_fill_i32, dim3(nblk(n, 256)), dim3(256), 0, t->parent, -1, n);
     if ((rc = clear_attr(ctx, A, n))) return rc;
-    if ((rc = run_pass(ctx, la, sc, A, S, grp, t->parent, spar, kc))) return rc;
+    if ((rc = run_pass(ctx, la, sc, A, S, grp, t->parent, spar, kc, false))) return rc;
     DCF_LAUNCH(ctx, "k_fine_roots", k_fine_roots, dim3(nblk(kc, 256)), dim3(256), 0, la.uf, cpix, spar, t->parent, kc);
 
     t->attr_pix = A;
