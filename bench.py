@@ -48,7 +48,7 @@ def algorithmic_bytes(kernel: str, n: int, k_nodes: int, B: int, passes: int):
     table = {
         "k_union": (passes * n, B + 4),
         "k_final_pixels": (passes * n, 8),
-        "k_final_held": (passes * n, 8),
+        "k_final_rest": (passes * n, 8),
         "k_filter_out": (n, 2 * B + 8),
         "k_direct_jump": (k_nodes, 9),
     }

@@ -475,7 +475,11 @@ __global__ __launch_bounds__(256) void k_leaf_round(DevAttr src, DevAttr dst, co
     }
     /* count nodes that still have an ancestor at the next distance */
     unsigned long long b = __ballot(k < K && a2 >= 0);
-    if ((threadIdx.x & (warpSize - 1)) == 0 && b) atomicAdd(&active[round], (uint32_t)__popcll(b));
+    /* "another round is needed": a flag, set once (an atomic add per wave on one word serialised
+     * hundreds of thousands of updates per round) */
+    if ((threadIdx.x & (warpSize - 1)) == 0 && b &&
+        __hip_atomic_load(&active[round], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0u)
+        __hip_atomic_store(&active[round], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 /* ------------------------------------------------------------------------------------------------ */

@@ -722,7 +722,7 @@ __device__ void wave_add(const DevAttr &A, bool valid, uint32_t r, const Contrib
  * level is the canonical levelroot: p takes it as parent and adds its own attribute contribution.
  * A set still rooted at an old root has its canonical levelroot computed here (minimum pixel of
  * the level, one atomicMin per wave and set; the first to touch a set queues its root for k_fix)
- * and p keeps r in `held` for k_final_held. */
+ * and p keeps r in `held` for k_final_rest. */
 /* Block-level variant for kernels run with FB threads per block: when every valid lane of the block
  * targets the same root (the pixels of one big set), the block reduces its contributions through
  * LDS and issues one atomic per field; otherwise each wave aggregates on its own.  Same-address
@@ -865,27 +865,57 @@ __global__ __launch_bounds__(FB) void k_final_pixels(LevelArgs a, const uint32_t
     block_canon_min(a, bs, old, r, p);
 }
 
-/* the pixels of re-rooted sets: parent = the canonical levelroot, contribution added there */
-__global__ __launch_bounds__(FB) void k_final_held(LevelArgs a, const uint32_t *__restrict__ lvl, uint32_t cnt,
-                                                   const uint32_t *__restrict__ held, DevAttr A, int grp,
-                                                   int32_t *__restrict__ par, uint32_t *__restrict__ mark)
+/* the rest of a level, after k_fix: (1) the pixels of re-rooted sets take the canonical levelroot
+ * as parent and add their contribution there; (2) every old root absorbed at this level (linked in
+ * k_union or re-rooted in k_fix) takes the new root as parent and merges its complete record into
+ * it -- pixel roots from A, coarse super-vertices (ids n and above) from S, recording their joining
+ * node in spar.  marks[0] is where this level's absorbed entries start; the end is the current count,
+ * which the kernel hands on as the next level's start in marks[1]. */
+__global__ __launch_bounds__(FB) void k_final_rest(LevelArgs a, const uint32_t *__restrict__ lvl, uint32_t cnt,
+                                                   const uint32_t *__restrict__ held, DevAttr A, DevAttr S, int grp,
+                                                   int32_t *__restrict__ par, int32_t *__restrict__ spar, uint32_t *__restrict__ marks)
 {
     __shared__ BlockScratch bs;
     const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
-    /* unions and re-rooting are complete: this level's absorbed entries end at the current count */
-    if (i == 0) *mark = *a.nabs;
-    const uint32_t r = i < cnt ? held[i] : 0xFFFFFFFFu;
-    const bool valid = r != 0xFFFFFFFFu;
-    uint32_t cr = 0;
-    Contrib c;
-    contrib_zero(c);
-    if (valid) {
-        const uint32_t p = lvl[i];
-        cr = a.canon[r];
-        if (cr != p) par[p] = (int32_t)cr;
-        contrib_own(a, grp, p, c);
+    {
+        const uint32_t r = i < cnt ? held[i] : 0xFFFFFFFFu;
+        const bool valid = r != 0xFFFFFFFFu;
+        uint32_t cr = 0;
+        Contrib c;
+        contrib_zero(c);
+        if (valid) {
+            const uint32_t p = lvl[i];
+            cr = a.canon[r];
+            if (cr != p) par[p] = (int32_t)cr;
+            contrib_own(a, grp, p, c);
+        }
+        block_add(A, bs, valid, cr, c);
     }
-    block_add(A, bs, valid, cr, c);
+    const uint32_t start = marks[0], end = *a.nabs;
+    if (blockIdx.x == 0 && threadIdx.x == 0) marks[1] = end;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    /* every lane of a wave runs the same number of iterations so wave_add sees the full wave */
+    for (uint32_t base = start + blockIdx.x * blockDim.x; base < end; base += stride) {
+        const uint32_t j = base + threadIdx.x;
+        const bool valid = j < end;
+        uint32_t r = 0;
+        Contrib c;
+        contrib_zero(c);
+        if (valid) {
+            const uint32_t h = a.absorbed[j];
+            uint32_t tag;
+            r = uf_find(a.uf, h, tag);
+            if (h < (uint32_t)a.g.n) {
+                par[h] = (int32_t)r;
+                contrib_rec(A, h, c);
+            } else {
+                const uint32_t cc = h - (uint32_t)a.g.n;
+                spar[cc] = (int32_t)r;
+                contrib_rec(S, cc, c);
+            }
+        }
+        wave_add(A, valid, r, c);
+    }
 }
 
 /* re-root every queued set at its canonical levelroot c: c becomes the root (tagged with this level)
@@ -901,38 +931,6 @@ __global__ __launch_bounds__(256) void k_fix(LevelArgs a, uint32_t ord, uint32_t
         st(a.uf, c, UF_ROOT | lev);
         st(a.uf, r, c);
         a.absorbed[atomicAdd(a.nabs, 1u)] = r;
-    }
-}
-
-/* absorbed old roots of this level: parent + attribute merge.  Pixel roots carry their complete
- * attributes in A; super-vertices (id >= n) carry the coarse totals in S and record their
- * joining node in spar. */
-__global__ __launch_bounds__(256) void k_final_absorbed(LevelArgs a, const uint32_t *__restrict__ marks, DevAttr A, DevAttr S,
-                                                        int32_t *__restrict__ par, int32_t *__restrict__ spar)
-{
-    const uint32_t start = marks[0], end = marks[1];
-    const uint32_t stride = gridDim.x * blockDim.x;
-    /* every lane of a wave runs the same number of iterations so wave_add sees the full wave */
-    for (uint32_t base = start + blockIdx.x * blockDim.x; base < end; base += stride) {
-        uint32_t i = base + threadIdx.x;
-        bool valid = i < end;
-        uint32_t r = 0;
-        Contrib c;
-        contrib_zero(c);
-        if (valid) {
-            uint32_t h = a.absorbed[i];
-            uint32_t tag;
-            r = uf_find(a.uf, h, tag);
-            if (h < (uint32_t)a.g.n) {
-                par[h] = (int32_t)r;
-                contrib_rec(A, h, c);
-            } else {
-                uint32_t cc = h - (uint32_t)a.g.n;
-                spar[cc] = (int32_t)r;
-                contrib_rec(S, cc, c);
-            }
-        }
-        wave_add(A, valid, r, c);
     }
 }
 
@@ -1192,12 +1190,11 @@ static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAtt
         unsigned fb = nblk(cnt, 256);
         if (fb > 1024) fb = 1024;
         DCF_LAUNCH(ctx, "k_fix", k_fix, dim3(fb), dim3(256), 0, la, (uint32_t)ord, (uint32_t)lev);
-        DCF_LAUNCH(ctx, "k_final_held", k_final_held, dim3(nblk(cnt, FB)), dim3(FB), 0, la, lvl, cnt, (const uint32_t *)held,
-                   A, grp, par, nabs + 2 + ord);
-        unsigned ab = nblk((int64_t)cnt * la.nb.n, 256);
-        if (ab > 4096) ab = 4096;
-        DCF_LAUNCH(ctx, "k_final_absorbed", k_final_absorbed, dim3(ab), dim3(256), 0, la, (const uint32_t *)(nabs + 1 + ord), A,
-                   S, par, spar);
+        unsigned rb = nblk(cnt, FB), ab = nblk((int64_t)cnt * la.nb.n / 4, FB);
+        if (ab > 1024) ab = 1024;
+        if (ab > rb) rb = ab;
+        DCF_LAUNCH(ctx, "k_final_rest", k_final_rest, dim3(rb), dim3(FB), 0, la, lvl, cnt, (const uint32_t *)held, A, S, grp, par,
+                   spar, nabs + 1 + ord);
         ord++;
     }
     return DCF_OK;

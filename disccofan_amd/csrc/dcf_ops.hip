@@ -201,7 +201,11 @@ __global__ __launch_bounds__(256) void k_direct_jump(int32_t *__restrict__ J, co
         }
     }
     unsigned long long b = __ballot(more);
-    if ((threadIdx.x & (warpSize - 1)) == 0 && b) atomicAdd(&active[round], (uint32_t)__popcll(b));
+    /* "another round is needed": a flag, set once (an atomic add per wave on one word serialised
+     * hundreds of thousands of updates per round) */
+    if ((threadIdx.x & (warpSize - 1)) == 0 && b &&
+        __hip_atomic_load(&active[round], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0u)
+        __hip_atomic_store(&active[round], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __global__ __launch_bounds__(256) void k_direct_value(const int32_t *__restrict__ J, const float *__restrict__ ngrey,
@@ -244,7 +248,11 @@ __global__ __launch_bounds__(256) void k_sub_round(const double *__restrict__ F,
         A2[k] = a2;
     }
     unsigned long long b = __ballot(k < K && a2 >= 0);
-    if ((threadIdx.x & (warpSize - 1)) == 0 && b) atomicAdd(&active[round], (uint32_t)__popcll(b));
+    /* "another round is needed": a flag, set once (an atomic add per wave on one word serialised
+     * hundreds of thousands of updates per round) */
+    if ((threadIdx.x & (warpSize - 1)) == 0 && b &&
+        __hip_atomic_load(&active[round], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0u)
+        __hip_atomic_store(&active[round], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __global__ __launch_bounds__(256) void k_d2f(const double *__restrict__ F, float *__restrict__ val, int64_t K)
