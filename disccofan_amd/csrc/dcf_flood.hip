@@ -175,12 +175,15 @@ __device__ __forceinline__ uint32_t uf_unite(uint32_t *uf, uint32_t a, uint32_t 
 /* ------------------------------------------------------------------------------------------------ */
 /* counting sort of the pixels by level digit                                                       */
 /* ------------------------------------------------------------------------------------------------ */
-__global__ __launch_bounds__(256) void k_lvl_hist(ImgView iv, int pass, int shift, int64_t n, uint32_t *__restrict__ hist)
+/* only != nullptr: count only the pixels with only[p] == p (zone representatives) */
+__global__ __launch_bounds__(256) void k_lvl_hist(ImgView iv, int pass, int shift, int64_t n, const uint32_t *__restrict__ only,
+                                                  uint32_t *__restrict__ hist)
 {
     __shared__ uint32_t sh[NLEV];
     sh[threadIdx.x] = 0;
     __syncthreads();
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+        if (only && only[p] != (uint32_t)p) continue;
         uint32_t k = fl_key(iv, p);
         uint32_t d = pass == 1 ? (k >> shift) : (k & 255u);
         atomicAdd(&sh[d], 1u);
@@ -202,7 +205,8 @@ struct ScatterGeo {
 };
 
 __global__ __launch_bounds__(256) void k_lvl_scatter(ImgView iv, int pass, int shift, int64_t n, ScatterGeo sg,
-                                                     uint32_t *__restrict__ cursor, uint32_t *__restrict__ order)
+                                                     const uint32_t *__restrict__ only, uint32_t *__restrict__ cursor,
+                                                     uint32_t *__restrict__ order)
 {
     __shared__ uint32_t cnt[NLEV], base[NLEV];
     cnt[threadIdx.x] = 0;
@@ -223,13 +227,16 @@ __global__ __launch_bounds__(256) void k_lvl_scatter(ImgView iv, int pass, int s
     }
     /* pixel of local slot l of this block, or -1 outside the image */
     auto pix = [&](int l) -> int64_t {
+        int64_t p;
         if (!sg.tiled) {
-            const int64_t p = p0 + l;
-            return p < n ? p : -1;
+            p = p0 + l;
+            if (p >= n) return -1;
+        } else {
+            const int x = x0 + l % sg.TW, y = y0 + (l / sg.TW) % sg.TH, z = z0 + l / (sg.TW * sg.TH);
+            if (x >= sg.W || y >= sg.H || z >= sg.D) return -1;
+            p = (int64_t)x + (int64_t)y * sg.W + (int64_t)z * sg.W * sg.H;
         }
-        const int x = x0 + l % sg.TW, y = y0 + (l / sg.TW) % sg.TH, z = z0 + l / (sg.TW * sg.TH);
-        if (x >= sg.W || y >= sg.H || z >= sg.D) return -1;
-        return (int64_t)x + (int64_t)y * sg.W + (int64_t)z * sg.W * sg.H;
+        return (only && only[p] != (uint32_t)p) ? -1 : p;
     };
     for (int l = threadIdx.x; l < tsz; l += blockDim.x) {
         const int64_t p = pix(l);
@@ -282,6 +289,8 @@ struct LevelArgs {
     /* batched form: image table (nullptr for one image) */
     const BatchImg *bimg;
     int nbatch;
+    /* zone pass: the level lists hold zone representatives, whose records carry their zone's totals */
+    int zone;
 };
 
 /* local coordinates of pixel p and the dimensions of its image (pixel ids are below 2^30, so the
@@ -470,9 +479,10 @@ __device__ __forceinline__ void tile_origin(const TileGeo &tg, int b, int &x0, i
 
 /* zones of every tile: zrep[p] = the zone's minimum pixel; uf[p] -> zrep for the other pixels */
 __global__ __launch_bounds__(256) void k_tile_zones(ImgView iv, int shift, TileGeo tg, Nbr nb, uint32_t *__restrict__ zrep,
-                                                    uint32_t *__restrict__ uf)
+                                                    uint32_t *__restrict__ uf, uint32_t *__restrict__ zarea)
 {
     __shared__ uint32_t par[ZT_MAX];
+    __shared__ uint32_t zc[ZT_MAX];
     __shared__ uint16_t dg[ZT_MAX];
     int x0, y0, z0;
     tile_origin(tg, blockIdx.x, x0, y0, z0);
@@ -482,6 +492,7 @@ __global__ __launch_bounds__(256) void k_tile_zones(ImgView iv, int shift, TileG
         const bool in = x < tg.W && y < tg.H && z < tg.D;
         dg[l] = in ? (uint16_t)(fl_key(iv, (int64_t)x + (int64_t)y * tg.W + (int64_t)z * tg.W * tg.H) >> shift) : 0xFFFFu;
         par[l] = (uint32_t)l;
+        zc[l] = 0;
     }
     __syncthreads();
     for (int l = threadIdx.x; l < tsz; l += blockDim.x) {
@@ -507,6 +518,14 @@ __global__ __launch_bounds__(256) void k_tile_zones(ImgView iv, int shift, TileG
         const uint32_t rp = (uint32_t)((int64_t)rx + (int64_t)ry * tg.W + (int64_t)rz * tg.W * tg.H);
         zrep[p] = rp;
         if (p != rp) uf[p] = rp;
+        atomicAdd(&zc[r], 1u);
+    }
+    __syncthreads();
+    /* every zone lies in this tile: its pixel count goes to its representative's record */
+    for (int l = threadIdx.x; l < tsz; l += blockDim.x) {
+        if (dg[l] == 0xFFFFu || par[l] != (uint32_t)l) continue;
+        const int x = x0 + l % tg.TW, y = y0 + (l / tg.TW) % tg.TH, z = z0 + l / (tg.TW * tg.TH);
+        zarea[(int64_t)x + (int64_t)y * tg.W + (int64_t)z * tg.W * tg.H] = zc[l];
     }
 }
 
@@ -723,6 +742,37 @@ __device__ void wave_add(const DevAttr &A, bool valid, uint32_t r, const Contrib
  * A set still rooted at an old root has its canonical levelroot computed here (minimum pixel of
  * the level, one atomicMin per wave and set; the first to touch a set queues its root for k_fix)
  * and p keeps r in `held` for k_final_rest. */
+/* zone pass: the coordinate / grey sums and extents of every zone into its representative's record
+ * (the pixel counts come from k_tile_zones) */
+__global__ __launch_bounds__(256) void k_zone_totals(LevelArgs a, int grp, const uint32_t *__restrict__ zrep, DevAttr A, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    uint32_t r = 0;
+    Contrib c;
+    contrib_zero(c);
+    if (valid) {
+        r = zrep[i];
+        contrib_own(a, grp, (uint32_t)i, c);
+        c.area = 0;
+    }
+    wave_add(A, valid, r, c);
+}
+
+/* zone pass, after the levels: a pixel that is not its zone's representative r shares r's node: its
+ * levelroot is r when r is one (r's parent lies at a lower digit, or r is the root), else r's parent */
+__global__ __launch_bounds__(256) void k_zone_parents(ImgView iv, int pass, int shift, const uint32_t *__restrict__ zrep,
+                                                      int32_t *__restrict__ par, int64_t n)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const uint32_t r = zrep[p];
+    if (r == (uint32_t)p) return;
+    const int32_t q = par[r];
+    auto digit = [&](int64_t x) { const uint32_t k = fl_key(iv, x); return pass == 1 ? (k >> shift) : (k & 255u); };
+    par[p] = (q >= 0 && digit(q) == digit(r)) ? q : (int32_t)r;
+}
+
 /* Block-level variant for kernels run with FB threads per block: when every valid lane of the block
  * targets the same root (the pixels of one big set), the block reduces its contributions through
  * LDS and issues one atomic per field; otherwise each wave aggregates on its own.  Same-address
@@ -857,7 +907,8 @@ __global__ __launch_bounds__(FB) void k_final_pixels(LevelArgs a, const uint32_t
         held[i] = old ? r : 0xFFFFFFFFu;
         if (fresh) {
             if (r != p) par[p] = (int32_t)r;
-            contrib_own(a, grp, p, c);
+            if (!a.zone) contrib_own(a, grp, p, c);
+            else if (r != p) contrib_rec(A, p, c); /* a root already holds its own zone's totals */
         }
     }
     block_add(A, bs, fresh, r, c);
@@ -887,7 +938,8 @@ __global__ __launch_bounds__(FB) void k_final_rest(LevelArgs a, const uint32_t *
             const uint32_t p = lvl[i];
             cr = a.canon[r];
             if (cr != p) par[p] = (int32_t)cr;
-            contrib_own(a, grp, p, c);
+            if (!a.zone) contrib_own(a, grp, p, c);
+            else if (cr != p) contrib_rec(A, p, c);
         }
         block_add(A, bs, valid, cr, c);
     }
@@ -1100,16 +1152,39 @@ static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAtt
                     int64_t kc, bool zones)
 {
     const int64_t n = la.g.n;
-    uint32_t *hist = (uint32_t *)sc.get(2 * NLEV * 4);
+    uint32_t *hist = (uint32_t *)sc.get(3 * NLEV * 4);
     uint32_t *order = (uint32_t *)sc.get((size_t)n * 4);
     if (!hist || !order) return dcf_set_error(ctx, DCF_ENOMEM, "level lists");
-    uint32_t *cursor = hist + NLEV;
-    DCF_LAUNCH(ctx, "k_fill", k_fill_u32f, dim3(2), dim3(256), 0, hist, 0u, (int64_t)2 * NLEV);
+    uint32_t *cursor = hist + NLEV, *hist_all = hist + 2 * NLEV;
+    DCF_LAUNCH(ctx, "k_fill", k_fill_u32f, dim3(3), dim3(256), 0, hist, 0u, (int64_t)3 * NLEV);
     unsigned hb = nblk(n, 256);
     if (hb > 2048) hb = 2048;
-    DCF_LAUNCH(ctx, "k_lvl_hist", k_lvl_hist, dim3(hb), dim3(256), 0, la.iv, la.pass, la.shift, n, hist);
-    uint32_t h[NLEV];
+    /* zone pass: label the tile zones first (pixels pre-linked to their representative, zone pixel
+     * counts in the representatives' records); the level lists then hold representatives only */
+    uint32_t *zrep = nullptr;
+    TileGeo tg;
+    int64_t ntiles = 0;
+    la.zone = zones ? 1 : 0;
+    if (zones) {
+        tg.W = la.g.W; tg.H = la.g.H; tg.D = la.g.D;
+        if (tg.D > 1) { tg.TW = 16; tg.TH = 16; tg.TD = 16; }
+        else { tg.TW = 64; tg.TH = 64; tg.TD = 1; }
+        tg.ntx = (tg.W + tg.TW - 1) / tg.TW;
+        tg.nty = (tg.H + tg.TH - 1) / tg.TH;
+        ntiles = (int64_t)tg.ntx * tg.nty * ((tg.D + tg.TD - 1) / tg.TD);
+        zrep = (uint32_t *)sc.get((size_t)n * 4);
+        if (!zrep) return dcf_set_error(ctx, DCF_ENOMEM, "zone representatives");
+        DCF_LAUNCH(ctx, "k_tile_zones", k_tile_zones, dim3((unsigned)ntiles), dim3(256), 0, la.iv, la.shift, tg, la.nb, zrep, la.uf,
+                   A.area);
+        if (grp != 0)
+            DCF_LAUNCH(ctx, "k_zone_totals", k_zone_totals, dim3(nblk(n, 256)), dim3(256), 0, la, grp, (const uint32_t *)zrep, A, n);
+        DCF_LAUNCH(ctx, "k_lvl_hist", k_lvl_hist, dim3(hb), dim3(256), 0, la.iv, la.pass, la.shift, n, (const uint32_t *)nullptr,
+                   hist_all);
+    }
+    DCF_LAUNCH(ctx, "k_lvl_hist", k_lvl_hist, dim3(hb), dim3(256), 0, la.iv, la.pass, la.shift, n, (const uint32_t *)zrep, hist);
+    uint32_t h[NLEV], hall[NLEV];
     DCF_HIP(ctx, hipMemcpyAsync(h, hist, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+    if (zones) DCF_HIP(ctx, hipMemcpyAsync(hall, hist_all, sizeof hall, hipMemcpyDeviceToHost, ctx->stream));
     DCF_HIP(ctx, hipStreamSynchronize(ctx->stream));
     uint32_t off[NLEV + 1];
     off[0] = 0;
@@ -1128,29 +1203,25 @@ static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAtt
     sg.ngy = (sg.nty + sg.GY - 1) / sg.GY;
     const int64_t ngz = (sg.ntz + sg.GZ - 1) / sg.GZ;
     const int64_t sblocks = sg.tiled ? (int64_t)sg.ngx * sg.ngy * ngz * sg.GX * sg.GY * sg.GZ : (int64_t)nblk(n, 4096);
-    DCF_LAUNCH(ctx, "k_lvl_scatter", k_lvl_scatter, dim3((unsigned)sblocks), dim3(256), 0, la.iv, la.pass, la.shift, n, sg, cursor,
-               order);
-    /* zone graph: zones pre-linked in the union-find, distinct zone pairs bucketed by level */
+    DCF_LAUNCH(ctx, "k_lvl_scatter", k_lvl_scatter, dim3((unsigned)sblocks), dim3(256), 0, la.iv, la.pass, la.shift, n, sg,
+               (const uint32_t *)zrep, cursor, order);
+    /* zone graph: distinct zone pairs bucketed by level; a level's bucket holds at most NB edges per
+     * pixel of the level */
     uint2 *edges = nullptr;
     uint64_t bucket_h[NLEV];
     uint32_t ecnt[NLEV];
     if (zones) {
-        TileGeo tg;
-        tg.W = la.g.W; tg.H = la.g.H; tg.D = la.g.D;
-        if (tg.D > 1) { tg.TW = 16; tg.TH = 16; tg.TD = 16; }
-        else { tg.TW = 64; tg.TH = 64; tg.TD = 1; }
-        tg.ntx = (tg.W + tg.TW - 1) / tg.TW;
-        tg.nty = (tg.H + tg.TH - 1) / tg.TH;
-        const int64_t ntiles = (int64_t)tg.ntx * tg.nty * ((tg.D + tg.TD - 1) / tg.TD);
-        uint32_t *zrep = (uint32_t *)sc.get((size_t)n * 4);
         uint64_t *bucket = (uint64_t *)sc.get(NLEV * 8);
         uint32_t *ecur = (uint32_t *)sc.get(NLEV * 4);
         edges = (uint2 *)sc.get((size_t)n * la.nb.n * sizeof(uint2));
-        if (!zrep || !bucket || !ecur || !edges) return dcf_set_error(ctx, DCF_ENOMEM, "zone graph");
-        for (int i = 0; i < NLEV; i++) bucket_h[i] = (uint64_t)la.nb.n * off[i];
+        if (!bucket || !ecur || !edges) return dcf_set_error(ctx, DCF_ENOMEM, "zone graph");
+        uint64_t acc = 0;
+        for (int i = 0; i < NLEV; i++) {
+            bucket_h[i] = acc;
+            acc += (uint64_t)la.nb.n * hall[i];
+        }
         DCF_HIP(ctx, hipMemcpyAsync(bucket, bucket_h, sizeof bucket_h, hipMemcpyHostToDevice, ctx->stream));
         DCF_LAUNCH(ctx, "k_fill", k_fill_u32f, dim3(1), dim3(256), 0, ecur, 0u, (int64_t)NLEV);
-        DCF_LAUNCH(ctx, "k_tile_zones", k_tile_zones, dim3((unsigned)ntiles), dim3(256), 0, la.iv, la.shift, tg, la.nb, zrep, la.uf);
         DCF_LAUNCH(ctx, "k_zone_edges", k_zone_edges, dim3((unsigned)ntiles), dim3(256), 0, la.iv, la.shift, tg, la.nb,
                    (const uint32_t *)zrep, (const uint64_t *)bucket, ecur, edges);
         DCF_HIP(ctx, hipMemcpyAsync(ecnt, ecur, sizeof ecnt, hipMemcpyDeviceToHost, ctx->stream));
@@ -1197,6 +1268,9 @@ static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAtt
                    spar, nabs + 1 + ord);
         ord++;
     }
+    if (zones)
+        DCF_LAUNCH(ctx, "k_zone_parents", k_zone_parents, dim3(nblk(n, 256)), dim3(256), 0, la.iv, la.pass, la.shift,
+                   (const uint32_t *)zrep, par, n);
     return DCF_OK;
 }
 
