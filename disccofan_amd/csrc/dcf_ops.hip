@@ -18,6 +18,7 @@
  * operation order, including its float evaluation of area_attribute (attributes.c:105: the
  * ulong area is converted to float before the product), so bins and thresholds match bit for bit.
  */
+#include <hipcub/hipcub.hpp>
 #include <vector>
 
 #include "dcf_internal.h"
@@ -287,6 +288,59 @@ __global__ __launch_bounds__(256) void k_filter_out(ImgView iv, const uint32_t *
     else ((uint8_t *)out)[p] = (uint8_t)f;
 }
 
+/* ---- subtractive rule on float greys ---------------------------------------------------------- *
+ * tree_filter_subtractive (filtering.c:211-261) evaluates, top-down, f(u) = g(u) + f(p) - g(p) in float
+ * for a passing node u with parent p (f(p) for a failing one; the root: g or 0).  Float sums do not
+ * reassociate, so the nodes are processed in order of depth: depths by doubling, a radix sort by
+ * depth, then one workgroup walks the depths (a barrier between consecutive depths). */
+__global__ __launch_bounds__(256) void k_depth_init(const int32_t *__restrict__ npar, double *__restrict__ F, int32_t *__restrict__ A,
+                                                    int64_t K)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    F[k] = npar[k] >= 0 ? 1.0 : 0.0;
+    A[k] = npar[k];
+}
+
+__global__ __launch_bounds__(256) void k_depth_keys(const double *__restrict__ F, uint32_t *__restrict__ key, uint32_t *__restrict__ id,
+                                                    int64_t K)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    key[k] = (uint32_t)F[k];
+    id[k] = (uint32_t)k;
+}
+
+/* start[d] = first position of depth d in the depth-sorted order (depths 0..max all occur) */
+__global__ __launch_bounds__(256) void k_depth_starts(const uint32_t *__restrict__ key, uint32_t *__restrict__ start, int64_t K)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= K) return;
+    if (i == 0 || key[i - 1] != key[i]) start[key[i]] = (uint32_t)i;
+    if (i == K - 1) start[key[i] + 1] = (uint32_t)K;
+}
+
+__global__ __launch_bounds__(1024) void k_sub_sweep(const uint32_t *__restrict__ order, const uint32_t *__restrict__ start,
+                                                    uint32_t maxd, const int32_t *__restrict__ npar, const float *__restrict__ ngrey,
+                                                    const uint8_t *__restrict__ pass, float *__restrict__ val)
+{
+    for (uint32_t d = 0; d <= maxd; d++) {
+        const uint32_t a = start[d], b = start[d + 1];
+        for (uint32_t i = a + threadIdx.x; i < b; i += blockDim.x) {
+            const uint32_t u = order[i];
+            const int32_t p = npar[u];
+            float f;
+            if (p < 0) f = pass[u] ? ngrey[u] : 0.0f;
+            else if (pass[u]) {
+                const float t = ngrey[u] + val[p];
+                f = t - ngrey[p];
+            } else f = val[p];
+            val[u] = f;
+        }
+        __syncthreads();
+    }
+}
+
 /* batched form: every node takes the lambda of its image */
 __global__ __launch_bounds__(256) void k_node_lambda(const BatchImg *__restrict__ bimg, int nb, const double *__restrict__ lam,
                                                      const uint32_t *__restrict__ p2n, double *__restrict__ nlam, int64_t n)
@@ -398,6 +452,72 @@ static AttrCtx attr_ctx(dcf_tree *t)
 }
 
 /* node values of the rule (float per node) into val */
+/* the subtractive rule on float greys, exactly as the reference rounds it */
+static int sub_float(dcf_tree *t, const AttrCtx &c, int attr_idx, double lambda, float *val)
+{
+    dcf_ctx *ctx = t->ctx;
+    const int64_t K = t->K;
+    double *F0 = (double *)dcf_alloc(ctx, (size_t)K * 8), *F1 = (double *)dcf_alloc(ctx, (size_t)K * 8);
+    int32_t *A0 = (int32_t *)dcf_alloc(ctx, (size_t)K * 4), *A1 = (int32_t *)dcf_alloc(ctx, (size_t)K * 4);
+    uint32_t *k0 = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4), *k1 = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4);
+    uint32_t *v0 = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4), *v1 = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4);
+    uint8_t *pass = (uint8_t *)dcf_alloc(ctx, (size_t)K);
+    int32_t *J = (int32_t *)A1;
+    void *tmp = nullptr;
+    uint32_t *start = nullptr;
+    int rc = DCF_OK;
+    std::vector<uint32_t> act(64);
+    if (!F0 || !F1 || !A0 || !A1 || !k0 || !k1 || !v0 || !v1 || !pass) rc = dcf_set_error(ctx, DCF_ENOMEM, "subtractive scratch");
+    if (rc == DCF_OK) {
+        /* depth of every node: sums of ones up the tree by doubling (exact in double) */
+        hipLaunchKernelGGL(k_depth_init, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, t->npar, F0, A0, K);
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, t->counters, 0u, (int64_t)64);
+        double *fi = F0, *fo = F1;
+        int32_t *ai = A0, *ao = A1;
+        for (int r = 0; r < t->max_rounds; r++) {
+            hipLaunchKernelGGL(k_sub_round, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, fi, fo, ai, ao, t->counters, r, K);
+            double *tf = fi; fi = fo; fo = tf;
+            int32_t *ta = ai; ai = ao; ao = ta;
+        }
+        if (hipMemcpyAsync(act.data(), t->counters, 64 * 4, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess)
+            rc = dcf_set_error(ctx, DCF_EDEVICE, "depth rounds");
+        int ran = 0;
+        for (int r = 0; r < t->max_rounds; r++) { ran = r + 1; if (act[r] == 0) break; }
+        if (rc == DCF_OK && act[t->max_rounds - 1] != 0) rc = dcf_set_error(ctx, DCF_EDEVICE, "tree deeper than the doubling rounds");
+        const double *depth = (ran & 1) ? F1 : F0;
+        /* pass flags (the J pointers of k_direct_init are not used here) */
+        if (rc == DCF_OK) {
+            hipLaunchKernelGGL(k_depth_keys, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, depth, k0, v0, K);
+            hipLaunchKernelGGL(k_direct_init, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, c, attr_idx, lambda, t->nlam, t->npar, J,
+                               pass, K);
+            size_t need = 0;
+            if (hipcub::DeviceRadixSort::SortPairs(nullptr, need, k0, k1, v0, v1, (int)K, 0, 32, ctx->stream) != hipSuccess)
+                rc = dcf_set_error(ctx, DCF_EDEVICE, "depth sort size");
+            else if (!(tmp = dcf_alloc(ctx, need ? need : 16)))
+                rc = dcf_set_error(ctx, DCF_ENOMEM, "depth sort scratch");
+            else if (hipcub::DeviceRadixSort::SortPairs(tmp, need, k0, k1, v0, v1, (int)K, 0, 32, ctx->stream) != hipSuccess)
+                rc = dcf_set_error(ctx, DCF_EDEVICE, "depth sort");
+        }
+        uint32_t maxd = 0;
+        if (rc == DCF_OK && (hipMemcpyAsync(&maxd, k1 + (K - 1), 4, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+                             hipStreamSynchronize(ctx->stream) != hipSuccess))
+            rc = dcf_set_error(ctx, DCF_EDEVICE, "maximum depth");
+        if (rc == DCF_OK && !(start = (uint32_t *)dcf_alloc(ctx, ((size_t)maxd + 2) * 4)))
+            rc = dcf_set_error(ctx, DCF_ENOMEM, "depth starts");
+        if (rc == DCF_OK) {
+            hipLaunchKernelGGL(k_depth_starts, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, (const uint32_t *)k1, start, K);
+            hipLaunchKernelGGL(k_sub_sweep, dim3(1), dim3(1024), 0, ctx->stream, (const uint32_t *)v1, (const uint32_t *)start, maxd,
+                               t->npar, t->ngrey, (const uint8_t *)pass, val);
+            if (hipGetLastError() != hipSuccess) rc = dcf_set_error(ctx, DCF_EDEVICE, "launch k_sub_sweep");
+        }
+    }
+    dcf_release(ctx, F0); dcf_release(ctx, F1); dcf_release(ctx, A0); dcf_release(ctx, A1);
+    dcf_release(ctx, k0); dcf_release(ctx, k1); dcf_release(ctx, v0); dcf_release(ctx, v1);
+    dcf_release(ctx, pass); dcf_release(ctx, tmp); dcf_release(ctx, start);
+    return rc;
+}
+
 static int rule_values(dcf_tree *t, int attr_idx, int decision, double lambda, unsigned flags, float *val)
 {
     dcf_ctx *ctx = t->ctx;
@@ -408,7 +528,9 @@ static int rule_values(dcf_tree *t, int attr_idx, int decision, double lambda, u
      * subtractive kernels can still be forced with DCF_ALL_RULES.  The min and max rules of a
      * non-increasing attribute depend on the reference's raster scan order (its max rule keeps stale
      * values, filtering.c:156-208) and are not implemented on this path. */
-    bool general_sub = decision == DCF_SUBTRACTIVE && !(attr_idx == 0 && !(flags & DCF_ALL_RULES));
+    /* float greys: the subtractive rule's float rounding differs from the direct rule's copies */
+    const bool float_grey = t->dtype == DCF_F32;
+    bool general_sub = decision == DCF_SUBTRACTIVE && (float_grey || !(attr_idx == 0 && !(flags & DCF_ALL_RULES)));
     bool min_max_inc = (decision == DCF_MIN || decision == DCF_MAX) && attr_idx == 0;
     if (decision == DCF_DIRECT || (decision == DCF_SUBTRACTIVE && !general_sub) || min_max_inc) {
         int32_t *J = (int32_t *)dcf_alloc(ctx, (size_t)K * 4);
@@ -423,6 +545,7 @@ static int rule_values(dcf_tree *t, int attr_idx, int decision, double lambda, u
         dcf_release(ctx, pass);
         return DCF_OK;
     }
+    if (decision == DCF_SUBTRACTIVE && float_grey) return sub_float(t, c, attr_idx, lambda, val);
     if (decision == DCF_SUBTRACTIVE) {
         double *F0 = (double *)dcf_alloc(ctx, (size_t)K * 8), *F1 = (double *)dcf_alloc(ctx, (size_t)K * 8);
         int32_t *A0 = (int32_t *)dcf_alloc(ctx, (size_t)K * 4), *A1 = (int32_t *)dcf_alloc(ctx, (size_t)K * 4);

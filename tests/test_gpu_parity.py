@@ -146,6 +146,37 @@ def test_every_attribute(ctx, shape, conn):
             assert_same(t.filter(a, lam, 0), o.filter(a, lam, 0), "direct attr %d" % a)
 
 
+@pytest.mark.parametrize("case", ["few_values", "all_distinct", "3d", "negatives"])
+def test_float_images(ctx, case):
+    """f32 images (the reference's own grey type, transform_float keys flood.c:362-371): tree, areas and
+    every filter rule bit-exact, the subtractive rule with the reference's float rounding order"""
+    rng = np.random.default_rng(600 + len(case))
+    if case == "few_values":
+        img = (rng.integers(0, 9, size=(61, 77)) * 0.25).astype(np.float32)
+    elif case == "all_distinct":
+        img = rng.random((53, 67)).astype(np.float32) * 1000
+    elif case == "3d":
+        img = rng.normal(size=(9, 21, 23)).astype(np.float32)
+    else:
+        img = (rng.normal(size=(40, 50)) * 100).astype(np.float32)
+    for conn in ((6, 26) if img.ndim == 3 else (4, 8)):
+        for tt in ("max", "min"):
+            say("float", case, conn, tt)
+            o = OracleTree(img, conn, tt, 0)
+            t = ctx.build(img, conn, tt, 0)
+            assert_same(t.parent(), o.parent().astype(np.int32), "parent f32 %s" % case)
+            assert_same(t.check(0), o.check(0), "area f32 %s" % case)
+            for dec in (0, 3):
+                assert_same(t.filter(0, 12, dec), o.filter(0, 12, dec), "filter f32 %s d%d" % (case, dec))
+            for dec in (1, 2):
+                assert_same(t.filter(0, 12, dec), o.filter(0, 12, dec), "filter f32 %s d%d" % (case, dec))
+            # float spectra are double sums of float products; their order of summation differs (the
+            # reference's own result depends on its thread count): relative 1e-12
+            lv = synth.log_lambdas(12, 1, 4096)
+            gs, os_ = t.spectrum1d(0, lv), o.spectrum1d(0, lv)
+            assert np.allclose(gs, os_, rtol=1e-12, atol=0), ("spectrum f32", case, gs, os_)
+
+
 EDGE = {
     "1x1": np.array([[7]], np.uint8),
     "1xN": (np.arange(300) * 7 % 251).astype(np.uint8).reshape(1, 300),
