@@ -165,18 +165,6 @@ extern "C" int dcf_ctx_create(int device, unsigned flags, dcf_ctx **out)
         return DCF_EDEVICE;
     }
     c->stream = c->own;
-    const char *tx = getenv("DCF_TILE2D");
-    if (tx) {
-        int a = 0, b = 0;
-        if (sscanf(tx, "%dx%d", &a, &b) == 2 && a > 0 && b > 0 && a * b <= 4096) { c->tile_x = a; c->tile_y = b; }
-    }
-    const char *t3 = getenv("DCF_TILE3D");
-    if (t3) {
-        int a = 0, b = 0, d = 0;
-        if (sscanf(t3, "%dx%dx%d", &a, &b, &d) == 3 && a > 0 && b > 0 && d > 0 && a * b * d <= 4096) {
-            c->tile3_x = a; c->tile3_y = b; c->tile3_z = d;
-        }
-    }
     auto shape3 = [](const char *env, int v[3]) {
         const char *e = getenv(env);
         int a = 0, b = 0, d = 0;
@@ -287,7 +275,6 @@ static void tree_release(dcf_tree *t)
     dcf_ctx *ctx = t->ctx;
     if (t->own_img) dcf_release(ctx, t->img);
     dcf_release(ctx, t->gmax);
-    dcf_release(ctx, t->link);
     dcf_release(ctx, t->parent);
     dcf_release(ctx, t->p2n);
     dcf_release(ctx, t->npar);

@@ -1,6 +1,6 @@
 /*
  * dcf_flood.hip -- level-synchronous union-find construction of the canonical max-tree with its
- * attributes, for u8 and u16 images.
+ * attributes, for u8, u16 and (through dense ranks of their float keys) f32 images.
  *
  * The reference floods pixels one at a time in decreasing (grey, index) order and merges the
  * attribute of every finished pixel into its parent (tree_flood, src/sources/flood.c:222-247;
@@ -19,7 +19,8 @@
  *             flooding, with no separate leaffix.
  *
  * The critical path is the number of levels, not the depth of the tree (u16 trees of smooth
- * images are ~50k nodes deep).  16-bit keys use two passes of 256 levels:
+ * images are ~50k nodes deep).  Keys are taken in 8-bit digits, one pass of 256 levels per digit
+ * (u8 one pass, u16 two, float ranks up to four).  For 16-bit keys:
  *   pass 1  levels = key >> 8: the coarse tree (coarse levelroot cl(p), coarse parent, and the
  *           exact attributes of every coarse node region);
  *   pass 2  levels = key & 255 inside every coarse node N: its level region LR(N) = {p : cl(p) = N}
@@ -29,18 +30,19 @@
  *           ancestor of cl(q) whose coarse parent is N (binary lifting on the coarse tree).
  *           When S_C is absorbed at fine level d, the fine node it joins is the parent of C's
  *           lowest fine node (the final root of LR(C)).
- * Pixels with equal keys are merged by the union-find itself (no chains), so flat zones cost
- * nothing extra.
+ * A further pass repeats pass 2 one digit lower, with the tree of the previous pass (levels =
+ * key prefixes) as the coarse tree.  Pixels with equal keys are merged by the union-find itself
+ * (no chains), so flat zones cost nothing extra.
  */
 #include <hipcub/hipcub.hpp>
 
 #include "dcf_internal.h"
 
 #define NLEV 256
-#define LIFT 8
 
 __device__ __forceinline__ uint32_t fl_key(const ImgView &iv, int64_t p)
 {
+    if (iv.dtype == DCF_KEY32) return ((const uint32_t *)iv.img)[p];
     if (iv.dtype == DCF_U8) {
         uint32_t v = ((const uint8_t *)iv.img)[p];
         return iv.invert ? (uint32_t)(*iv.gmax) - v : v;
@@ -185,7 +187,7 @@ __global__ __launch_bounds__(256) void k_lvl_hist(ImgView iv, int pass, int shif
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
         if (only && only[p] != (uint32_t)p) continue;
         uint32_t k = fl_key(iv, p);
-        uint32_t d = pass == 1 ? (k >> shift) : (k & 255u);
+        uint32_t d = (k >> shift) & 255u;
         atomicAdd(&sh[d], 1u);
     }
     __syncthreads();
@@ -242,7 +244,7 @@ __global__ __launch_bounds__(256) void k_lvl_scatter(ImgView iv, int pass, int s
         const int64_t p = pix(l);
         if (p < 0) continue;
         uint32_t k = fl_key(iv, p);
-        atomicAdd(&cnt[pass == 1 ? (k >> shift) : (k & 255u)], 1u);
+        atomicAdd(&cnt[(k >> shift) & 255u], 1u);
     }
     __syncthreads();
     if (cnt[threadIdx.x]) base[threadIdx.x] = atomicAdd(&cursor[threadIdx.x], cnt[threadIdx.x]);
@@ -252,7 +254,7 @@ __global__ __launch_bounds__(256) void k_lvl_scatter(ImgView iv, int pass, int s
         const int64_t p = pix(l);
         if (p < 0) continue;
         uint32_t k = fl_key(iv, p);
-        uint32_t d = pass == 1 ? (k >> shift) : (k & 255u);
+        uint32_t d = (k >> shift) & 255u;
         uint32_t r = atomicAdd(&cnt[d], 1u);
         order[base[d] + r] = (uint32_t)p;
     }
@@ -284,7 +286,8 @@ struct LevelArgs {
     uint32_t *fcount;      /* [0] running count of fix, [1 + i] where level i's entries start */
     /* pass 2 only */
     const uint32_t *cnode; /* coarse node id of every pixel */
-    const int32_t *lift;   /* LIFT x Kc: 2^j-th coarse ancestor (node id), -1 above the root */
+    const int32_t *lift;   /* nlift x Kc: 2^j-th coarse ancestor (node id), -1 above the root */
+    int nlift;             /* 2^nlift exceeds the depth of the coarse tree */
     const uint32_t *cnkey; /* coarse key of every coarse node */
     /* batched form: image table (nullptr for one image) */
     const BatchImg *bimg;
@@ -326,7 +329,7 @@ __device__ __forceinline__ uint32_t child_toward(const LevelArgs &a, uint32_t e,
     if (up == (int32_t)nid) return e;
     uint32_t kn = a.cnkey[nid];
     e = (uint32_t)up;
-    for (int j = LIFT - 1; j >= 0; j--) {
+    for (int j = a.nlift - 1; j >= 0; j--) {
         int32_t u = a.lift[(int64_t)j * kc_count + e];
         if (u >= 0 && a.cnkey[u] > kn) e = (uint32_t)u;
     }
@@ -356,7 +359,7 @@ __global__ __launch_bounds__(256) void k_union(LevelArgs a, const uint32_t *__re
     const int x = pg.x, y = pg.y, z = pg.z;
     const int WH = pg.W * pg.H;
     const uint32_t kp = fl_key(a.iv, p);
-    const uint32_t cp = kp >> 8;
+    const uint32_t cp = a.pass == 2 ? kp >> (a.shift + 8) : 0u;
     const uint32_t nid = (valid && a.pass == 2) ? a.cnode[p] : 0u;
     uint32_t tp = 0, rp = 0;
     if (valid) rp = uf_find(a.uf, p, tp);
@@ -373,12 +376,12 @@ __global__ __launch_bounds__(256) void k_union(LevelArgs a, const uint32_t *__re
             const uint32_t q = (uint32_t)((int)p + a.nb.dx[k] + a.nb.dy[k] * pg.W + a.nb.dz[k] * WH);
             uint32_t kq = fl_key(a.iv, q);
             if (a.pass == 1) {
-                uint32_t dq = kq >> a.shift;
+                uint32_t dq = (kq >> a.shift) & 255u;
                 if (dq > lev || (dq == lev && q > p)) cur[j] = q;
             } else {
-                uint32_t cq = kq >> 8;
+                uint32_t cq = kq >> (a.shift + 8);
                 if (cq == cp) {
-                    uint32_t dq = kq & 255u;
+                    uint32_t dq = (kq >> a.shift) & 255u;
                     if (dq > lev || (dq == lev && q > p)) cur[j] = q;
                 } else if (cq > cp) {
                     cur[j] = (uint32_t)a.g.n + child_toward(a, a.cnode[q], nid, kc_count);
@@ -769,7 +772,7 @@ __global__ __launch_bounds__(256) void k_zone_parents(ImgView iv, int pass, int 
     const uint32_t r = zrep[p];
     if (r == (uint32_t)p) return;
     const int32_t q = par[r];
-    auto digit = [&](int64_t x) { const uint32_t k = fl_key(iv, x); return pass == 1 ? (k >> shift) : (k & 255u); };
+    auto digit = [&](int64_t x) { return (fl_key(iv, x) >> shift) & 255u; };
     par[p] = (q >= 0 && digit(q) == digit(r)) ? q : (int32_t)r;
 }
 
@@ -998,7 +1001,7 @@ __global__ __launch_bounds__(256) void k_uf_init(ImgView iv, int pass, int shift
     uint32_t lv = NLEV;
     if (i < n) {
         uint32_t k = fl_key(iv, i);
-        lv = pass == 1 ? (k >> shift) : (k & 255u);
+        lv = (k >> shift) & 255u;
     }
     uf[i] = UF_ROOT | lv;
 }
@@ -1032,26 +1035,27 @@ __global__ __launch_bounds__(256) void k_coarse_node_of(const int32_t *__restric
     cnode[p] = cidx[flag[p] ? p : (int64_t)par1[p]];
 }
 
+/* levelroots of the previous pass's tree, whose levels are the key prefixes key >> pshift */
 __global__ __launch_bounds__(256) void k_coarse_flag(ImgView iv, const int32_t *__restrict__ par1, uint32_t *__restrict__ flag,
-                                                     int64_t n)
+                                                     int pshift, int64_t n)
 {
     int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     int32_t r = par1[p];
-    flag[p] = (r < 0 || (fl_key(iv, r) >> 8) != (fl_key(iv, p) >> 8)) ? 1u : 0u;
+    flag[p] = (r < 0 || (fl_key(iv, r) >> pshift) != (fl_key(iv, p) >> pshift)) ? 1u : 0u;
 }
 
 /* per coarse node: key, parent id (lift level 0) and the region totals used by its super-vertex */
 __global__ __launch_bounds__(256) void k_coarse_nodes(ImgView iv, const int32_t *__restrict__ par1,
                                                       const uint32_t *__restrict__ flag, const uint32_t *__restrict__ cidx,
                                                       uint32_t *__restrict__ cnkey, int32_t *__restrict__ lift0,
-                                                      uint32_t *__restrict__ cpix, DevAttr A, DevAttr S, int64_t n)
+                                                      uint32_t *__restrict__ cpix, DevAttr A, DevAttr S, int pshift, int64_t n)
 {
     int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n || !flag[p]) return;
     uint32_t c = cidx[p];
     int32_t r = par1[p];
-    cnkey[c] = fl_key(iv, p) >> 8;
+    cnkey[c] = fl_key(iv, p) >> pshift;
     lift0[c] = r < 0 ? -1 : (int32_t)cidx[r];
     cpix[c] = (uint32_t)p;
     S.area[c] = A.area[p];
@@ -1280,108 +1284,123 @@ int dcf_flood_build(dcf_tree *t, int grp)
     const int64_t n = t->n;
     const int is3d = t->bimg ? t->batch3d : t->dims[2] > 1;
     /* union-find ids (pixels + coarse super-vertices) must stay below the root tag bit */
-    if (n >= ((int64_t)1 << 30)) return dcf_set_error(ctx, DCF_EUNSUPPORTED, "integer images are limited to 2^30 pixels");
+    if (n >= ((int64_t)1 << 30)) return dcf_set_error(ctx, DCF_EUNSUPPORTED, "images are limited to 2^30 pixels");
     Scratch sc{ ctx, {} };
     LevelArgs la;
-    la.iv.img = t->img;
-    la.iv.dtype = t->dtype;
-    la.iv.invert = t->tree_min;
-    la.iv.gmax = t->gmax;
     la.g.W = (int)t->dims[0];
     la.g.H = (int)t->dims[1];
     la.g.D = (int)t->dims[2];
     la.g.n = n;
     la.nb = make_nbr(t->conn, is3d);
-    const bool two = t->dtype == DCF_U16;
-    int64_t abs_cap = n + 16;
+    la.bimg = t->bimg;
+    la.nbatch = t->nbatch;
+    la.zone = 0;
+    la.cnode = nullptr; la.lift = nullptr; la.cnkey = nullptr; la.nlift = 0;
+    int rc = DCF_OK;
+    /* the keys, in passes of 8-bit digits: u8 one pass, u16 two; a float image is built on the dense
+     * ranks of its float keys (the same order, the same equalities), 1 to 4 passes */
+    int P;
+    uint32_t maxkey;
+    if (t->dtype == DCF_F32) {
+        uint32_t *rank = (uint32_t *)sc.get((size_t)n * 4);
+        if (!rank) return dcf_set_error(ctx, DCF_ENOMEM, "grey ranks");
+        uint32_t maxr = 0;
+        if ((rc = dcf_rank_keys(t, rank, &maxr)) != DCF_OK) return rc;
+        la.iv.img = rank;
+        la.iv.dtype = DCF_KEY32;
+        la.iv.invert = 0;
+        la.iv.gmax = nullptr;
+        P = maxr < 256u ? 1 : (maxr < 65536u ? 2 : (maxr < (1u << 24) ? 3 : 4));
+        maxkey = maxr;
+    } else {
+        la.iv.img = t->img;
+        la.iv.dtype = t->dtype;
+        la.iv.invert = t->tree_min;
+        la.iv.gmax = t->gmax;
+        P = t->dtype == DCF_U16 ? 2 : 1;
+        maxkey = t->dtype == DCF_U16 ? 65535u : 255u;
+    }
 
     /* pixel-indexed attribute records (valid at roots) */
     DevAttr A;
-    int rc = alloc_attr_n(ctx, &sc, A, grp, n);
-    if (rc) return rc;
+    if ((rc = alloc_attr_n(ctx, &sc, A, grp, n))) return rc;
     la.uf = (uint32_t *)sc.get((size_t)n * 4);
-    la.absorbed = (uint32_t *)sc.get((size_t)abs_cap * 4);
+    la.absorbed = (uint32_t *)sc.get((size_t)(n + 16) * 4);
     la.nabs = (uint32_t *)sc.get((NLEV + 4) * 4);
-    int32_t *par1 = two ? (int32_t *)sc.get((size_t)n * 4) : t->parent;
-    if (!la.uf || !la.absorbed || !la.nabs || !par1) return dcf_set_error(ctx, DCF_ENOMEM, "flood scratch");
-    la.cnode = nullptr; la.lift = nullptr; la.cnkey = nullptr;
-    la.bimg = t->bimg;
-    la.nbatch = t->nbatch;
+    int32_t *par = P == 1 ? t->parent : (int32_t *)sc.get((size_t)n * 4);
+    if (!la.uf || !la.absorbed || !la.nabs || !par) return dcf_set_error(ctx, DCF_ENOMEM, "flood scratch");
 
-    /* ---- pass 1 ---- */
+    /* ---- pass 1: the top digit over all pixels ---- */
     la.pass = 1;
-    la.shift = two ? 8 : 0;
+    la.shift = 8 * (P - 1);
     if ((rc = clear_attr(ctx, A, n))) return rc;
     DCF_LAUNCH(ctx, "k_uf_init", k_uf_init, dim3(nblk(n, 256)), dim3(256), 0, la.iv, 1, la.shift, n, n, la.uf);
-    DCF_LAUNCH(ctx, "k_fill", k_fill_i32, dim3(nblk(n, 256)), dim3(256), 0, par1, -1, n);
+    DCF_LAUNCH(ctx, "k_fill", k_fill_i32, dim3(nblk(n, 256)), dim3(256), 0, par, -1, n);
     DevAttr none;
     /* zone graph for the first pass (single images: the batch's tiles would straddle images) */
     const int64_t edge_bytes = n * la.nb.n * (int64_t)sizeof(uint2);
     const bool zones = !t->bimg && edge_bytes <= ((int64_t)8 << 30) &&
-                       (ctx->zones == 1 || (ctx->zones == 2 && (two || !is3d)));
-    if ((rc = run_pass(ctx, la, sc, A, none, grp, par1, nullptr, 0, zones))) return rc;
-    /* roots of pass 1 kept par1 = -1 except non-roots (levelroot) and absorbed roots (parent) */
+                       (ctx->zones == 1 || (ctx->zones == 2 && (P > 1 || !is3d)));
+    if ((rc = run_pass(ctx, la, sc, A, none, grp, par, nullptr, 0, zones))) return rc;
 
-    if (!two) {
-        t->attr_pix = A;
-        /* hand the arrays over to the tree: detach them from the scratch list */
-        for (auto it = sc.ptrs.begin(); it != sc.ptrs.end();) {
-            bool keep = *it == A.area;
-            for (int i = 0; i < A.nsum; i++) keep = keep || *it == A.s[i];
-            for (int i = 0; A.has_ext && i < 3; i++) keep = keep || *it == A.mn[i] || *it == A.mx[i];
-            if (keep) it = sc.ptrs.erase(it);
-            else ++it;
-        }
-        return DCF_OK;
+    /* ---- passes 2..P: the next digit inside every node of the previous pass's tree ---- */
+    for (int pass = 2; pass <= P; pass++) {
+        const int pshift = 8 * (P - pass + 1); /* the previous pass's levels: key >> pshift */
+        uint32_t *flag = (uint32_t *)sc.get((size_t)n * 4);
+        uint32_t *cidx = (uint32_t *)sc.get((size_t)n * 4);
+        uint32_t *cnode = (uint32_t *)sc.get((size_t)n * 4);
+        if (!flag || !cidx || !cnode) return dcf_set_error(ctx, DCF_ENOMEM, "coarse arrays");
+        DCF_LAUNCH(ctx, "k_coarse_flag", k_coarse_flag, dim3(nblk(n, 256)), dim3(256), 0, la.iv, (const int32_t *)par, flag, pshift, n);
+        size_t need = 0;
+        DCF_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, need, flag, cidx, (int)n, ctx->stream));
+        void *tmp = sc.get(need > 0 ? need : 16);
+        if (!tmp) return dcf_set_error(ctx, DCF_ENOMEM, "scan scratch");
+        DCF_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(tmp, need, flag, cidx, (int)n, ctx->stream));
+        uint32_t last[2];
+        DCF_HIP(ctx, hipMemcpyAsync(&last[0], cidx + (n - 1), 4, hipMemcpyDeviceToHost, ctx->stream));
+        DCF_HIP(ctx, hipMemcpyAsync(&last[1], flag + (n - 1), 4, hipMemcpyDeviceToHost, ctx->stream));
+        DCF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        const int64_t kc = (int64_t)last[0] + last[1];
+        if (n + kc >= ((int64_t)1 << 31)) return dcf_set_error(ctx, DCF_EUNSUPPORTED, "too many coarse nodes");
+        uint32_t *cnkey = (uint32_t *)sc.get((size_t)kc * 4);
+        uint32_t *cpix = (uint32_t *)sc.get((size_t)kc * 4);
+        /* the coarse tree's depth is at most the number of distinct prefixes (maxkey >> pshift) + 1 */
+        int nlift = 1;
+        while (nlift < 32 && ((uint64_t)(maxkey >> pshift) >> nlift) != 0) nlift++;
+        int32_t *lift = (int32_t *)sc.get((size_t)kc * 4 * nlift);
+        int32_t *spar = (int32_t *)sc.get((size_t)kc * 4);
+        if (!cnkey || !cpix || !lift || !spar) return dcf_set_error(ctx, DCF_ENOMEM, "coarse node arrays");
+        DevAttr S;
+        if ((rc = alloc_attr_n(ctx, &sc, S, grp, kc))) return rc;
+        DCF_LAUNCH(ctx, "k_coarse_nodes", k_coarse_nodes, dim3(nblk(n, 256)), dim3(256), 0, la.iv, (const int32_t *)par, flag, cidx,
+                   cnkey, lift, cpix, A, S, pshift, n);
+        for (int j = 1; j < nlift; j++) DCF_LAUNCH(ctx, "k_lift", k_lift, dim3(nblk(kc, 256)), dim3(256), 0, lift, j, kc);
+        DCF_LAUNCH(ctx, "k_coarse_node_of", k_coarse_node_of, dim3(nblk(n, 256)), dim3(256), 0, (const int32_t *)par, flag, cidx,
+                   cnode, n);
+        DCF_LAUNCH(ctx, "k_fill", k_fill_i32, dim3(nblk(kc, 256)), dim3(256), 0, spar, -1, kc);
+
+        uint32_t *uf2 = (uint32_t *)sc.get((size_t)(n + kc) * 4);
+        uint32_t *abs2 = (uint32_t *)sc.get((size_t)(n + kc + 16) * 4);
+        int32_t *npar = pass == P ? t->parent : (int32_t *)sc.get((size_t)n * 4);
+        if (!uf2 || !abs2 || !npar) return dcf_set_error(ctx, DCF_ENOMEM, "fine union-find");
+        la.uf = uf2;
+        la.absorbed = abs2;
+        la.pass = 2;
+        la.shift = 8 * (P - pass);
+        la.cnode = cnode;
+        la.lift = lift;
+        la.cnkey = cnkey;
+        la.nlift = nlift;
+        DCF_LAUNCH(ctx, "k_uf_init", k_uf_init, dim3(nblk(n + kc, 256)), dim3(256), 0, la.iv, 2, la.shift, n, n + kc, la.uf);
+        DCF_LAUNCH(ctx, "k_fill", k_fill_i32, dim3(nblk(n, 256)), dim3(256), 0, npar, -1, n);
+        if ((rc = clear_attr(ctx, A, n))) return rc;
+        if ((rc = run_pass(ctx, la, sc, A, S, grp, npar, spar, kc, false))) return rc;
+        DCF_LAUNCH(ctx, "k_fine_roots", k_fine_roots, dim3(nblk(kc, 256)), dim3(256), 0, la.uf, cpix, spar, npar, kc);
+        par = npar;
     }
 
-    /* ---- coarse nodes ---- */
-    uint32_t *flag = (uint32_t *)sc.get((size_t)n * 4);
-    uint32_t *cidx = (uint32_t *)sc.get((size_t)n * 4);
-    uint32_t *cnode = (uint32_t *)sc.get((size_t)n * 4);
-    if (!flag || !cidx || !cnode) return dcf_set_error(ctx, DCF_ENOMEM, "coarse arrays");
-    DCF_LAUNCH(ctx, "k_coarse_flag", k_coarse_flag, dim3(nblk(n, 256)), dim3(256), 0, la.iv, par1, flag, n);
-    size_t need = 0;
-    DCF_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, need, flag, cidx, (int)n, ctx->stream));
-    void *tmp = sc.get(need > 0 ? need : 16);
-    if (!tmp) return dcf_set_error(ctx, DCF_ENOMEM, "scan scratch");
-    DCF_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(tmp, need, flag, cidx, (int)n, ctx->stream));
-    uint32_t last[2];
-    DCF_HIP(ctx, hipMemcpyAsync(&last[0], cidx + (n - 1), 4, hipMemcpyDeviceToHost, ctx->stream));
-    DCF_HIP(ctx, hipMemcpyAsync(&last[1], flag + (n - 1), 4, hipMemcpyDeviceToHost, ctx->stream));
-    DCF_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    const int64_t kc = (int64_t)last[0] + last[1];
-    uint32_t *cnkey = (uint32_t *)sc.get((size_t)kc * 4);
-    uint32_t *cpix = (uint32_t *)sc.get((size_t)kc * 4);
-    int32_t *lift = (int32_t *)sc.get((size_t)kc * 4 * LIFT);
-    int32_t *spar = (int32_t *)sc.get((size_t)kc * 4);
-    if (!cnkey || !cpix || !lift || !spar) return dcf_set_error(ctx, DCF_ENOMEM, "coarse node arrays");
-    DevAttr S;
-    if ((rc = alloc_attr_n(ctx, &sc, S, grp, kc))) return rc;
-    DCF_LAUNCH(ctx, "k_coarse_nodes", k_coarse_nodes, dim3(nblk(n, 256)), dim3(256), 0, la.iv, par1, flag, cidx, cnkey, lift,
-               cpix, A, S, n);
-    for (int j = 1; j < LIFT; j++) DCF_LAUNCH(ctx, "k_lift", k_lift, dim3(nblk(kc, 256)), dim3(256), 0, lift, j, kc);
-    DCF_LAUNCH(ctx, "k_coarse_node_of", k_coarse_node_of, dim3(nblk(n, 256)), dim3(256), 0, par1, flag, cidx, cnode, n);
-    DCF_LAUNCH(ctx, "k_fill", k_fill_i32, dim3(nblk(kc, 256)), dim3(256), 0, spar, -1, kc);
-
-    /* ---- pass 2 ---- */
-    uint32_t *uf2 = (uint32_t *)sc.get((size_t)(n + kc) * 4);
-    uint32_t *abs2 = (uint32_t *)sc.get((size_t)(n + kc + 16) * 4);
-    if (!uf2 || !abs2) return dcf_set_error(ctx, DCF_ENOMEM, "fine union-find");
-    la.uf = uf2;
-    la.absorbed = abs2;
-    la.pass = 2;
-    la.shift = 0;
-    la.cnode = cnode;
-    la.lift = lift;
-    la.cnkey = cnkey;
-    DCF_LAUNCH(ctx, "k_uf_init", k_uf_init, dim3(nblk(n + kc, 256)), dim3(256), 0, la.iv, 2, 0, n, n + kc, la.uf);
-    DCF_LAUNCH(ctx, "k_fill", k_fill_i32, dim3(nblk(n, 256)), dim3(256), 0, t->parent, -1, n);
-    if ((rc = clear_attr(ctx, A, n))) return rc;
-    if ((rc = run_pass(ctx, la, sc, A, S, grp, t->parent, spar, kc, false))) return rc;
-    DCF_LAUNCH(ctx, "k_fine_roots", k_fine_roots, dim3(nblk(kc, 256)), dim3(256), 0, la.uf, cpix, spar, t->parent, kc);
-
     t->attr_pix = A;
+    /* hand the attribute arrays over to the tree: detach them from the scratch list */
     for (auto it = sc.ptrs.begin(); it != sc.ptrs.end();) {
         bool keep = *it == A.area;
         for (int i = 0; i < A.nsum; i++) keep = keep || *it == A.s[i];

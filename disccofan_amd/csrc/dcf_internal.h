@@ -3,7 +3,7 @@
  *
  * HBM layout of one tree over N pixels (all arrays indexed by raster index p = x + y*W + z*W*H):
  *   img        caller's image (u8/u16/f32), read in place; a min-tree reads gmax - v on the fly
- *   link[N]    u64  (key(parent) << 32) | parent   build state of the zip merges
+ *   rank[N]    u32  (f32 images, during the build) dense rank of the pixel's float key
  *   parent[N]  i32  canonical parent (levelroot form, root = -1)
  *   p2n[N]     u32  node index of the pixel's levelroot
  * and of the K nodes (levelroots, numbered in raster order of their levelroot pixel):
@@ -40,9 +40,12 @@ struct DevAttr {
 };
 
 /* what the kernels need to read a pixel's grey key and value */
+/* internal ImgView type: 32-bit keys (dense grey ranks of a float image), never inverted */
+#define DCF_KEY32 7
+
 struct ImgView {
     const void *img;
-    int dtype;        /* dcf_dtype */
+    int dtype;        /* dcf_dtype, or DCF_KEY32 */
     int invert;       /* min-tree: value = gmax - v */
     const float *gmax; /* device scalar (only read when invert) */
 };
@@ -67,8 +70,6 @@ struct dcf_ctx {
     std::map<std::string, KernelStat> stats;
     std::vector<std::string> names;
     std::set<dcf_tree *> trees; /* live trees, freed by dcf_ctx_destroy */
-    int tile_x = 32, tile_y = 32, tile_z = 1;      /* 2D tile */
-    int tile3_x = 8, tile3_y = 8, tile3_z = 16;    /* 3D tile */
     /* level-list tiles of the integer build (DCF_LEVEL_TILE2D / DCF_LEVEL_TILE3D, "WxHxD") and the
      * groups of tiles kept together in the list (DCF_LEVEL_GROUP2D / DCF_LEVEL_GROUP3D) */
     int lt2[3] = { 64, 64, 1 }, lg2[3] = { 1, 1, 1 };
@@ -116,7 +117,6 @@ struct dcf_tree {
     void *img = nullptr;  /* owned copy when the caller gave host memory */
     int own_img = 0;
     float *gmax = nullptr; /* device scalars: [0] max, [1] min */
-    uint64_t *link = nullptr;
     int32_t *parent = nullptr;
     uint32_t *p2n = nullptr;
     int64_t K = 0;
@@ -175,7 +175,9 @@ int dcf_build_tree(dcf_tree *t, int grp);      /* parent, nodes and the attribut
 int dcf_accumulate_group(dcf_tree *t, int grp); /* attributes of group grp on the existing tree */
 
 /* ---- launchers (dcf_flood.hip) ---------------------------------------------------------------- */
-int dcf_flood_build(dcf_tree *t, int grp);      /* u8/u16: canonical parent + pixel-indexed attributes */
+int dcf_flood_build(dcf_tree *t, int grp);      /* canonical parent + pixel-indexed attributes */
+/* f32 images: dense ranks of the tree's float keys (rank[p], and the largest rank) */
+int dcf_rank_keys(dcf_tree *t, uint32_t *rank, uint32_t *maxrank);
 /* batched form: per-image lambda (device, one per image), outputs (device table) */
 int dcf_ops_batch_filter(dcf_tree *t, int attr_idx, int decision, const double *lam_img, int extract, const BatchOut *outs,
                          const uint32_t *gmax_img, unsigned flags);

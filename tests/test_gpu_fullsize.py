@@ -96,3 +96,29 @@ def test_blobs_4096_precision_form(ctx):
     l1 = synth.log_lambdas(32, 0.16, 100.0)
     l2 = synth.log_lambdas(32, 1, 2 ** 24)
     same(t.spectrum2d(17, l1, 0, l2), o.spectrum2d(17, l1, 0, l2), "pattern2D")
+
+
+def test_float_four_passes(ctx):
+    """4608x4096 f32 with ~17.8M distinct values (more than 2^24, so the build takes four 8-bit digit
+    passes over the dense ranks): the ranks of a smooth field with ties broken at random, stored as
+    float32 (exact below 2^24, every second integer above), so the tree is deep and nearly every
+    pixel is its own node.  Parent array, direct and subtractive area filters, area spectrum."""
+    H, W = 4608, 4096
+    base = synth.vnoise(H, W).astype(np.float64)
+    rng = np.random.default_rng(4)
+    order = np.argsort(base.reshape(-1) + rng.random(H * W) * 0.5, kind="stable")
+    img = np.empty(H * W, np.float32)
+    img[order] = np.arange(H * W, dtype=np.float64).astype(np.float32)
+    img = img.reshape(H, W)
+    say("ranked float 4608x4096, distinct", len(np.unique(img)))
+    for tt in ("max", "min"):
+        say("ranked float", tt)
+        t = ctx.build(img, 8, tt, 0)
+        o = OracleTree(img, 8, tt, 0, threads=THREADS)
+        same(t.parent(), o.parent().astype(np.int32), "parent " + tt)
+        same(t.filter(0, 1000, 0), o.filter(0, 1000, 0), "direct filter " + tt)
+        same(t.filter(0, 1000, 3), o.filter(0, 1000, 3), "subtractive filter " + tt)
+        lv = synth.log_lambdas(32, 1, 2 ** 25)
+        gs, os_ = t.spectrum1d(0, lv), o.spectrum1d(0, lv)
+        assert np.allclose(gs, os_, rtol=1e-12, atol=0), (gs, os_)
+        t.free()

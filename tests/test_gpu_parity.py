@@ -146,10 +146,13 @@ def test_every_attribute(ctx, shape, conn):
             assert_same(t.filter(a, lam, 0), o.filter(a, lam, 0), "direct attr %d" % a)
 
 
-@pytest.mark.parametrize("case", ["few_values", "all_distinct", "3d", "negatives"])
+@pytest.mark.parametrize("case", ["few_values", "all_distinct", "3d", "negatives", "ramp", "many_distinct"])
 def test_float_images(ctx, case):
     """f32 images (the reference's own grey type, transform_float keys flood.c:362-371): tree, areas and
-    every filter rule bit-exact, the subtractive rule with the reference's float rounding order"""
+    every filter rule bit-exact, the subtractive rule with the reference's float rounding order.
+    The build runs on dense ranks of the float keys in 8-bit digit passes: up to 256 distinct values
+    take one pass, up to 65536 two, "ramp" and "many_distinct" (~10^5 distinct values) three; the
+    ramp's tree is one chain, so the third pass's coarse tree is ~350 nodes deep."""
     rng = np.random.default_rng(600 + len(case))
     if case == "few_values":
         img = (rng.integers(0, 9, size=(61, 77)) * 0.25).astype(np.float32)
@@ -157,6 +160,10 @@ def test_float_images(ctx, case):
         img = rng.random((53, 67)).astype(np.float32) * 1000
     elif case == "3d":
         img = rng.normal(size=(9, 21, 23)).astype(np.float32)
+    elif case == "ramp":
+        img = (np.arange(300 * 301, dtype=np.float64).reshape(300, 301) * 0.37 - 9000).astype(np.float32)
+    elif case == "many_distinct":
+        img = rng.normal(size=(317, 333)).astype(np.float32)
     else:
         img = (rng.normal(size=(40, 50)) * 100).astype(np.float32)
     for conn in ((6, 26) if img.ndim == 3 else (4, 8)):
