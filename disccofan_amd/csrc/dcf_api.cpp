@@ -173,6 +173,7 @@ extern "C" int dcf_ctx_create(int device, unsigned flags, dcf_ctx **out)
         }
     };
     if (const char *z = getenv("DCF_ZONES")) c->zones = atoi(z);
+    if (const char *z = getenv("DCF_TARGETS")) c->targets = atoi(z);
     shape3("DCF_LEVEL_TILE2D", c->lt2);
     shape3("DCF_LEVEL_GROUP2D", c->lg2);
     shape3("DCF_LEVEL_TILE3D", c->lt3);
@@ -363,8 +364,10 @@ fail:
 /* ------------------------------------------------------------------------------------------------ */
 /* batched form                                                                                     */
 /* ------------------------------------------------------------------------------------------------ */
-extern "C" int dcf_batch_filter(dcf_ctx *ctx, const dcf_batch_image *images, int count, int connectivity, int tree_type,
-                                int attr_idx, int decision, int extract, unsigned flags)
+/* one batch tree over images that are all u8 or all u16 (keys of that width: one digit pass for u8,
+ * two for u16) */
+static int batch_filter_same(dcf_ctx *ctx, const dcf_batch_image *images, int count, int connectivity, int tree_type,
+                             int attr_idx, int decision, int extract, unsigned flags)
 {
     if (!ctx) return DCF_EINVAL;
     if (!images || count <= 0) return dcf_set_error(ctx, DCF_EINVAL, "empty batch");
@@ -514,6 +517,39 @@ done:
     dcf_release(ctx, gm);
     dcf_release(ctx, staging);
     dcf_release(ctx, obuf);
+    return rc;
+}
+
+extern "C" int dcf_batch_filter(dcf_ctx *ctx, const dcf_batch_image *images, int count, int connectivity, int tree_type,
+                                int attr_idx, int decision, int extract, unsigned flags)
+{
+    if (!ctx) return DCF_EINVAL;
+    if (!images || count <= 0) return dcf_set_error(ctx, DCF_EINVAL, "empty batch");
+    /* every image is checked here, so that errors name the caller's index */
+    int is3d = -1;
+    uint64_t N = 0;
+    for (int i = 0; i < count; i++) {
+        const dcf_batch_image &b = images[i];
+        if (!b.img || !b.out) return dcf_set_error(ctx, DCF_EINVAL, "image %d: null pointer", i);
+        if (b.dtype != DCF_U8 && b.dtype != DCF_U16) return dcf_set_error(ctx, DCF_EINVAL, "image %d: dtype must be u8 or u16", i);
+        if (!b.dims[0] || !b.dims[1] || !b.dims[2]) return dcf_set_error(ctx, DCF_EINVAL, "image %d: empty", i);
+        if (b.dims[0] > 0x7FFFFFFFull || b.dims[1] > 0x7FFFFFFFull || b.dims[2] > 0x7FFFFFFFull)
+            return dcf_set_error(ctx, DCF_EINVAL, "image %d: dimension too large", i);
+        const int d3 = b.dims[2] > 1;
+        if (is3d < 0) is3d = d3;
+        else if (is3d != d3) return dcf_set_error(ctx, DCF_EINVAL, "image %d: a batch is all 2D or all 3D", i);
+        N += b.dims[0] * b.dims[1] * b.dims[2];
+        if (N >= (1ull << 30)) return dcf_set_error(ctx, DCF_EUNSUPPORTED, "a batch is limited to 2^30 pixels");
+    }
+    /* u8 and u16 images in separate trees: a u8 image on 16-bit keys would spend a whole first pass
+     * on one coarse level */
+    std::vector<dcf_batch_image> g8, g16;
+    for (int i = 0; i < count; i++) (images[i].dtype == DCF_U8 ? g8 : g16).push_back(images[i]);
+    int rc = DCF_OK;
+    if (!g8.empty())
+        rc = batch_filter_same(ctx, g8.data(), (int)g8.size(), connectivity, tree_type, attr_idx, decision, extract, flags);
+    if (rc == DCF_OK && !g16.empty())
+        rc = batch_filter_same(ctx, g16.data(), (int)g16.size(), connectivity, tree_type, attr_idx, decision, extract, flags);
     return rc;
 }
 

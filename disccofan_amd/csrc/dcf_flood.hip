@@ -206,59 +206,6 @@ struct ScatterGeo {
     int ntx, nty, ntz;  /* tiles per row / column / slab */
 };
 
-__global__ __launch_bounds__(256) void k_lvl_scatter(ImgView iv, int pass, int shift, int64_t n, ScatterGeo sg,
-                                                     const uint32_t *__restrict__ only, uint32_t *__restrict__ cursor,
-                                                     uint32_t *__restrict__ order)
-{
-    __shared__ uint32_t cnt[NLEV], base[NLEV];
-    cnt[threadIdx.x] = 0;
-    __syncthreads();
-    const int tsz = sg.tiled ? sg.TW * sg.TH * sg.TD : 4096;
-    int64_t p0 = 0;
-    int x0 = 0, y0 = 0, z0 = 0;
-    if (sg.tiled) {
-        const int gsz = sg.GX * sg.GY * sg.GZ;
-        const int g = blockIdx.x / gsz, w = blockIdx.x % gsz;
-        const int gx = g % sg.ngx, gy = (g / sg.ngx) % sg.ngy, gz = g / (sg.ngx * sg.ngy);
-        const int tx = gx * sg.GX + w % sg.GX, ty = gy * sg.GY + (w / sg.GX) % sg.GY, tz = gz * sg.GZ + w / (sg.GX * sg.GY);
-        x0 = tx * sg.TW; y0 = ty * sg.TH; z0 = tz * sg.TD;
-        /* tiles of a partial group beyond the image: the block has no pixel */
-        if (tx >= sg.ntx || ty >= sg.nty || tz >= sg.ntz) x0 = sg.W;
-    } else {
-        p0 = (int64_t)blockIdx.x * tsz;
-    }
-    /* pixel of local slot l of this block, or -1 outside the image */
-    auto pix = [&](int l) -> int64_t {
-        int64_t p;
-        if (!sg.tiled) {
-            p = p0 + l;
-            if (p >= n) return -1;
-        } else {
-            const int x = x0 + l % sg.TW, y = y0 + (l / sg.TW) % sg.TH, z = z0 + l / (sg.TW * sg.TH);
-            if (x >= sg.W || y >= sg.H || z >= sg.D) return -1;
-            p = (int64_t)x + (int64_t)y * sg.W + (int64_t)z * sg.W * sg.H;
-        }
-        return (only && only[p] != (uint32_t)p) ? -1 : p;
-    };
-    for (int l = threadIdx.x; l < tsz; l += blockDim.x) {
-        const int64_t p = pix(l);
-        if (p < 0) continue;
-        uint32_t k = fl_key(iv, p);
-        atomicAdd(&cnt[(k >> shift) & 255u], 1u);
-    }
-    __syncthreads();
-    if (cnt[threadIdx.x]) base[threadIdx.x] = atomicAdd(&cursor[threadIdx.x], cnt[threadIdx.x]);
-    cnt[threadIdx.x] = 0;
-    __syncthreads();
-    for (int l = threadIdx.x; l < tsz; l += blockDim.x) {
-        const int64_t p = pix(l);
-        if (p < 0) continue;
-        uint32_t k = fl_key(iv, p);
-        uint32_t d = (k >> shift) & 255u;
-        uint32_t r = atomicAdd(&cnt[d], 1u);
-        order[base[d] + r] = (uint32_t)p;
-    }
-}
 
 /* Blocks are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8), and each XCD has its own
  * L2.  This renumbering gives every XCD one contiguous eighth of the level list, i.e. one compact
@@ -338,10 +285,90 @@ __device__ __forceinline__ uint32_t child_toward(const LevelArgs &a, uint32_t e,
 
 #define CH 8 /* neighbours gathered together */
 
-/* every pixel p of the level is united with each already-processed neighbour (level digit above
- * the current one, or equal with a larger index); in pass 2 a neighbour in a coarse child region
- * stands for that child's super-vertex.  The neighbours' roots are found together (independent
- * loads in flight), duplicates dropped, then p's set is united with each distinct root. */
+/* union targets of pixel p at its level lev: every already-processed neighbour (level digit above
+ * lev, or equal with a larger index); in a refining pass a neighbour in a coarse child region stands
+ * for that child's super-vertex (duplicates of one super-vertex are dropped).  NONE marks no target. */
+template <int NB>
+__device__ __forceinline__ void pixel_targets(const LevelArgs &a, uint32_t p, uint32_t lev, uint32_t kc_count, uint32_t (&t)[NB])
+{
+    const uint32_t NONE = 0xFFFFFFFFu;
+    const PixGeo pg = pix_geo(a, p);
+    const int WH = pg.W * pg.H;
+    const uint32_t kp = fl_key(a.iv, p);
+    const uint32_t cp = a.pass == 2 ? kp >> (a.shift + 8) : 0u;
+    const uint32_t nid = a.pass == 2 ? a.cnode[p] : 0u;
+#pragma unroll
+    for (int k = 0; k < NB; k++) {
+        t[k] = NONE;
+        const int nx = pg.x + a.nb.dx[k], ny = pg.y + a.nb.dy[k], nz = pg.z + a.nb.dz[k];
+        if (nx < 0 || nx >= pg.W || ny < 0 || ny >= pg.H || nz < 0 || nz >= pg.D) continue;
+        const uint32_t q = (uint32_t)((int)p + a.nb.dx[k] + a.nb.dy[k] * pg.W + a.nb.dz[k] * WH);
+        const uint32_t kq = fl_key(a.iv, q);
+        const uint32_t dq = (kq >> a.shift) & 255u;
+        if (a.pass == 1) {
+            if (dq > lev || (dq == lev && q > p)) t[k] = q;
+        } else {
+            const uint32_t cq = kq >> (a.shift + 8);
+            if (cq == cp) {
+                if (dq > lev || (dq == lev && q > p)) t[k] = q;
+            } else if (cq > cp) {
+                const uint32_t sv = (uint32_t)a.g.n + child_toward(a, a.cnode[q], nid, kc_count);
+                bool dup = false;
+#pragma unroll
+                for (int m = 0; m < k; m++) dup = dup || t[m] == sv;
+                if (!dup) t[k] = sv;
+            }
+        }
+    }
+}
+
+/* p's set (root rp, tag tp) is united with the set of every target in cur: the targets' roots are
+ * found together (independent loads in flight, path splitting), duplicates dropped */
+__device__ __forceinline__ uint32_t unite_chunk(const LevelArgs &a, const uint32_t (&cur)[CH], uint32_t rp, uint32_t &tp, uint32_t lev,
+                                                const AbsBuf &ab)
+{
+    const uint32_t NONE = 0xFFFFFFFFu;
+    uint32_t val[CH], node[CH];
+#pragma unroll
+    for (int j = 0; j < CH; j++) {
+        node[j] = cur[j];
+        val[j] = cur[j] != NONE ? ldc(a.uf, cur[j]) : UF_ROOT;
+    }
+    bool more = true;
+    while (more) {
+        more = false;
+#pragma unroll
+        for (int j = 0; j < CH; j++)
+            if (!(val[j] & UF_ROOT)) {
+                const uint32_t pj = val[j];
+                const uint32_t gp = ldc(a.uf, pj);
+                if (!(gp & UF_ROOT)) st(a.uf, node[j], gp);
+                node[j] = pj;
+                val[j] = gp;
+                more |= !(gp & UF_ROOT);
+            }
+    }
+#pragma unroll
+    for (int j = 0; j < CH; j++) {
+        bool live = cur[j] != NONE;
+#pragma unroll
+        for (int m = 0; m < j; m++) live = live && !(cur[m] != NONE && node[m] == node[j]);
+        if (live) rp = uf_unite(a.uf, rp, tp, node[j], val[j], lev, ab, tp);
+    }
+    return rp;
+}
+
+/* the block's absorbed old roots, appended to the level's list */
+__device__ __forceinline__ void flush_absorbed(const LevelArgs &a, const uint32_t *s_abs, uint32_t &s_n, uint32_t &s_base)
+{
+    __syncthreads();
+    if (threadIdx.x == 0 && s_n) s_base = atomicAdd(a.nabs, s_n);
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < s_n; j += blockDim.x) a.absorbed[s_base + j] = s_abs[j];
+}
+
+/* every pixel p of the level is united with each already-processed neighbour (pixel_targets,
+ * evaluated here from the keys) */
 template <int NB>
 __global__ __launch_bounds__(256) void k_union(LevelArgs a, const uint32_t *__restrict__ lvl, uint32_t cnt, uint32_t lev,
                                                uint32_t kc_count)
@@ -353,73 +380,131 @@ __global__ __launch_bounds__(256) void k_union(LevelArgs a, const uint32_t *__re
     const AbsBuf ab{ s_abs, &s_n };
     const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
     const uint32_t NONE = 0xFFFFFFFFu;
-    const bool valid = i < cnt;
-    const uint32_t p = valid ? lvl[i] : 0u;
-    const PixGeo pg = pix_geo(a, p);
-    const int x = pg.x, y = pg.y, z = pg.z;
-    const int WH = pg.W * pg.H;
-    const uint32_t kp = fl_key(a.iv, p);
-    const uint32_t cp = a.pass == 2 ? kp >> (a.shift + 8) : 0u;
-    const uint32_t nid = (valid && a.pass == 2) ? a.cnode[p] : 0u;
-    uint32_t tp = 0, rp = 0;
-    if (valid) rp = uf_find(a.uf, p, tp);
+    if (i < cnt) {
+        const uint32_t p = lvl[i];
+        uint32_t t[NB];
+        pixel_targets<NB>(a, p, lev, kc_count, t);
+        uint32_t tp = 0, rp = uf_find(a.uf, p, tp);
 #pragma unroll
-    for (int k0 = 0; k0 < NB; k0 += CH) {
-        uint32_t cur[CH], val[CH], node[CH];
+        for (int k0 = 0; k0 < NB; k0 += CH) {
+            uint32_t cur[CH];
 #pragma unroll
-        for (int j = 0; j < CH; j++) {
-            cur[j] = NONE;
-            const int k = k0 + j;
-            if (k >= NB || !valid) continue;
-            int nx = x + a.nb.dx[k], ny = y + a.nb.dy[k], nz = z + a.nb.dz[k];
-            if (nx < 0 || nx >= pg.W || ny < 0 || ny >= pg.H || nz < 0 || nz >= pg.D) continue;
-            const uint32_t q = (uint32_t)((int)p + a.nb.dx[k] + a.nb.dy[k] * pg.W + a.nb.dz[k] * WH);
-            uint32_t kq = fl_key(a.iv, q);
-            if (a.pass == 1) {
-                uint32_t dq = (kq >> a.shift) & 255u;
-                if (dq > lev || (dq == lev && q > p)) cur[j] = q;
-            } else {
-                uint32_t cq = kq >> (a.shift + 8);
-                if (cq == cp) {
-                    uint32_t dq = (kq >> a.shift) & 255u;
-                    if (dq > lev || (dq == lev && q > p)) cur[j] = q;
-                } else if (cq > cp) {
-                    cur[j] = (uint32_t)a.g.n + child_toward(a, a.cnode[q], nid, kc_count);
-                }
-            }
-        }
-        /* lockstep finds with path splitting (each visited node is pointed at its grandparent) */
-#pragma unroll
-        for (int j = 0; j < CH; j++) {
-            node[j] = cur[j];
-            val[j] = cur[j] != NONE ? ldc(a.uf, cur[j]) : UF_ROOT;
-        }
-        bool more = true;
-        while (more) {
-            more = false;
-#pragma unroll
-            for (int j = 0; j < CH; j++)
-                if (!(val[j] & UF_ROOT)) {
-                    const uint32_t pj = val[j];
-                    const uint32_t gp = ldc(a.uf, pj);
-                    if (!(gp & UF_ROOT)) st(a.uf, node[j], gp);
-                    node[j] = pj;
-                    val[j] = gp;
-                    more |= !(gp & UF_ROOT);
-                }
-        }
-#pragma unroll
-        for (int j = 0; j < CH; j++) {
-            bool live = cur[j] != NONE;
-#pragma unroll
-            for (int m = 0; m < j; m++) live = live && !(cur[m] != NONE && node[m] == node[j]);
-            if (live) rp = uf_unite(a.uf, rp, tp, node[j], val[j], lev, ab, tp);
+            for (int j = 0; j < CH; j++) cur[j] = k0 + j < NB ? t[k0 + j] : NONE;
+            rp = unite_chunk(a, cur, rp, tp, lev, ab);
         }
     }
+    flush_absorbed(a, s_abs, s_n, s_base);
+}
+
+/* the same with the targets read from the level's target slots (NB per list position, written by
+ * k_lvl_scatter<NB>): no key, coarse-node or lifting loads at the level */
+template <int NB>
+__global__ __launch_bounds__(256) void k_union_tgt(LevelArgs a, const uint32_t *__restrict__ lvl, const uint32_t *__restrict__ tgt,
+                                                   uint32_t cnt, uint32_t lev)
+{
+    __shared__ uint32_t s_abs[256 * NB];
+    __shared__ uint32_t s_n, s_base;
+    if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
-    if (threadIdx.x == 0 && s_n) s_base = atomicAdd(a.nabs, s_n);
+    const AbsBuf ab{ s_abs, &s_n };
+    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    const uint32_t NONE = 0xFFFFFFFFu;
+    if (i < cnt) {
+        const uint32_t p = lvl[i];
+        uint32_t t[NB];
+        const uint32_t *src = tgt + (size_t)i * NB;
+        if (NB % 4 == 0) {
+#pragma unroll
+            for (int k = 0; k < NB; k += 4) {
+                const uint4 v = *(const uint4 *)(src + k);
+                t[k] = v.x; t[k + 1] = v.y; t[k + 2] = v.z; t[k + 3] = v.w;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < NB; k += 2) {
+                const uint2 v = *(const uint2 *)(src + k);
+                t[k] = v.x; t[k + 1] = v.y;
+            }
+        }
+        uint32_t tp = 0, rp = uf_find(a.uf, p, tp);
+#pragma unroll
+        for (int k0 = 0; k0 < NB; k0 += CH) {
+            uint32_t cur[CH];
+#pragma unroll
+            for (int j = 0; j < CH; j++) cur[j] = k0 + j < NB ? t[k0 + j] : NONE;
+            rp = unite_chunk(a, cur, rp, tp, lev, ab);
+        }
+    }
+    flush_absorbed(a, s_abs, s_n, s_base);
+}
+
+/* NB > 0: also the union targets of every listed pixel, NB slots at its list position (tgt) */
+template <int NB>
+__global__ __launch_bounds__(256) void k_lvl_scatter(ImgView iv, int pass, int shift, int64_t n, ScatterGeo sg,
+                                                     const uint32_t *__restrict__ only, uint32_t *__restrict__ cursor,
+                                                     uint32_t *__restrict__ order, LevelArgs a, uint32_t kc_count,
+                                                     uint32_t *__restrict__ tgt)
+{
+    __shared__ uint32_t cnt[NLEV], base[NLEV];
+    cnt[threadIdx.x] = 0;
     __syncthreads();
-    for (uint32_t j = threadIdx.x; j < s_n; j += blockDim.x) a.absorbed[s_base + j] = s_abs[j];
+    const int tsz = sg.tiled ? sg.TW * sg.TH * sg.TD : 4096;
+    int64_t p0 = 0;
+    int x0 = 0, y0 = 0, z0 = 0;
+    if (sg.tiled) {
+        const int gsz = sg.GX * sg.GY * sg.GZ;
+        const int g = blockIdx.x / gsz, w = blockIdx.x % gsz;
+        const int gx = g % sg.ngx, gy = (g / sg.ngx) % sg.ngy, gz = g / (sg.ngx * sg.ngy);
+        const int tx = gx * sg.GX + w % sg.GX, ty = gy * sg.GY + (w / sg.GX) % sg.GY, tz = gz * sg.GZ + w / (sg.GX * sg.GY);
+        x0 = tx * sg.TW; y0 = ty * sg.TH; z0 = tz * sg.TD;
+        /* tiles of a partial group beyond the image: the block has no pixel */
+        if (tx >= sg.ntx || ty >= sg.nty || tz >= sg.ntz) x0 = sg.W;
+    } else {
+        p0 = (int64_t)blockIdx.x * tsz;
+    }
+    /* pixel of local slot l of this block, or -1 outside the image */
+    auto pix = [&](int l) -> int64_t {
+        int64_t p;
+        if (!sg.tiled) {
+            p = p0 + l;
+            if (p >= n) return -1;
+        } else {
+            const int x = x0 + l % sg.TW, y = y0 + (l / sg.TW) % sg.TH, z = z0 + l / (sg.TW * sg.TH);
+            if (x >= sg.W || y >= sg.H || z >= sg.D) return -1;
+            p = (int64_t)x + (int64_t)y * sg.W + (int64_t)z * sg.W * sg.H;
+        }
+        return (only && only[p] != (uint32_t)p) ? -1 : p;
+    };
+    for (int l = threadIdx.x; l < tsz; l += blockDim.x) {
+        const int64_t p = pix(l);
+        if (p < 0) continue;
+        uint32_t k = fl_key(iv, p);
+        atomicAdd(&cnt[(k >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    if (cnt[threadIdx.x]) base[threadIdx.x] = atomicAdd(&cursor[threadIdx.x], cnt[threadIdx.x]);
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    for (int l = threadIdx.x; l < tsz; l += blockDim.x) {
+        const int64_t p = pix(l);
+        if (p < 0) continue;
+        uint32_t k = fl_key(iv, p);
+        uint32_t d = (k >> shift) & 255u;
+        uint32_t r = atomicAdd(&cnt[d], 1u);
+        order[base[d] + r] = (uint32_t)p;
+        if constexpr (NB > 0) {
+            uint32_t t[NB];
+            pixel_targets<NB>(a, (uint32_t)p, d, kc_count, t);
+            uint32_t *dst = tgt + (size_t)(base[d] + r) * NB;
+            if (NB % 4 == 0) {
+#pragma unroll
+                for (int k = 0; k < NB; k += 4) *(uint4 *)(dst + k) = make_uint4(t[k], t[k + 1], t[k + 2], t[k + 3]);
+            } else {
+#pragma unroll
+                for (int k = 0; k < NB; k += 2) *(uint2 *)(dst + k) = make_uint2(t[k], t[k + 1]);
+            }
+        }
+    }
 }
 
 /* ------------------------------------------------------------------------------------------------ */
@@ -1152,8 +1237,9 @@ static int clear_attr(dcf_ctx *ctx, DevAttr &a, int64_t n)
 }
 
 /* one flooding pass over all levels (high to low) */
+/* tgt: NB target slots per pixel for a pixel pass (nullptr: k_union evaluates the targets at each level) */
 static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAttr &S, int grp, int32_t *par, int32_t *spar,
-                    int64_t kc, bool zones)
+                    int64_t kc, bool zones, uint32_t *tgt)
 {
     const int64_t n = la.g.n;
     uint32_t *hist = (uint32_t *)sc.get(3 * NLEV * 4);
@@ -1207,8 +1293,18 @@ static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAtt
     sg.ngy = (sg.nty + sg.GY - 1) / sg.GY;
     const int64_t ngz = (sg.ntz + sg.GZ - 1) / sg.GZ;
     const int64_t sblocks = sg.tiled ? (int64_t)sg.ngx * sg.ngy * ngz * sg.GX * sg.GY * sg.GZ : (int64_t)nblk(n, 4096);
-    DCF_LAUNCH(ctx, "k_lvl_scatter", k_lvl_scatter, dim3((unsigned)sblocks), dim3(256), 0, la.iv, la.pass, la.shift, n, sg,
-               (const uint32_t *)zrep, cursor, order);
+    if (zones) tgt = nullptr;
+#define DCF_SCATTER(NBV)                                                                                                    \
+    DCF_LAUNCH(ctx, "k_lvl_scatter", k_lvl_scatter<NBV>, dim3((unsigned)sblocks), dim3(256), 0, la.iv, la.pass, la.shift, n, sg, \
+               (const uint32_t *)zrep, cursor, order, la, (uint32_t)kc, tgt)
+    if (!tgt) DCF_SCATTER(0);
+    else switch (la.nb.n) {
+        case 4: DCF_SCATTER(4); break;
+        case 6: DCF_SCATTER(6); break;
+        case 8: DCF_SCATTER(8); break;
+        default: DCF_SCATTER(26); break;
+        }
+#undef DCF_SCATTER
     /* zone graph: distinct zone pairs bucketed by level; a level's bucket holds at most NB edges per
      * pixel of the level */
     uint2 *edges = nullptr;
@@ -1253,6 +1349,14 @@ static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAtt
             if (ecnt[lev])
                 DCF_LAUNCH(ctx, "k_union", k_union_edges, dim3(nblk(ecnt[lev], 256)), dim3(256), 0, la,
                            (const uint2 *)(edges + bucket_h[lev]), ecnt[lev], (uint32_t)lev);
+        } else if (tgt) {
+            const uint32_t *tl = tgt + (size_t)off[lev] * la.nb.n;
+            switch (la.nb.n) {
+            case 4: DCF_LAUNCH(ctx, "k_union", k_union_tgt<4>, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, tl, cnt, (uint32_t)lev); break;
+            case 6: DCF_LAUNCH(ctx, "k_union", k_union_tgt<6>, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, tl, cnt, (uint32_t)lev); break;
+            case 8: DCF_LAUNCH(ctx, "k_union", k_union_tgt<8>, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, tl, cnt, (uint32_t)lev); break;
+            default: DCF_LAUNCH(ctx, "k_union", k_union_tgt<26>, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, tl, cnt, (uint32_t)lev); break;
+            }
         } else switch (la.nb.n) {
         case 4: DCF_LAUNCH(ctx, "k_union", k_union<4>, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, cnt, (uint32_t)lev, (uint32_t)kc); break;
         case 6: DCF_LAUNCH(ctx, "k_union", k_union<6>, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, cnt, (uint32_t)lev, (uint32_t)kc); break;
@@ -1341,7 +1445,16 @@ int dcf_flood_build(dcf_tree *t, int grp)
     const int64_t edge_bytes = n * la.nb.n * (int64_t)sizeof(uint2);
     const bool zones = !t->bimg && edge_bytes <= ((int64_t)8 << 30) &&
                        (ctx->zones == 1 || (ctx->zones == 2 && (P > 1 || !is3d)));
-    if ((rc = run_pass(ctx, la, sc, A, none, grp, par, nullptr, 0, zones))) return rc;
+    /* union targets precomputed by the level scatter, when they fit the budget: for batches (each
+     * neighbour lookup of a batch locates its image), not for single images, whose neighbour keys
+     * mostly hit the caches and whose target stream costs more than the lookups it saves */
+    uint32_t *tgt = nullptr;
+    const bool use_tgt = ctx->targets == 1 || (ctx->targets == 2 && t->bimg);
+    if (use_tgt && (int64_t)n * la.nb.n * 4 <= ((int64_t)16 << 30) && (!zones || P > 1)) {
+        tgt = (uint32_t *)sc.get((size_t)n * la.nb.n * 4);
+        if (!tgt) return dcf_set_error(ctx, DCF_ENOMEM, "union targets");
+    }
+    if ((rc = run_pass(ctx, la, sc, A, none, grp, par, nullptr, 0, zones, tgt))) return rc;
 
     /* ---- passes 2..P: the next digit inside every node of the previous pass's tree ---- */
     for (int pass = 2; pass <= P; pass++) {
@@ -1394,7 +1507,7 @@ int dcf_flood_build(dcf_tree *t, int grp)
         DCF_LAUNCH(ctx, "k_uf_init", k_uf_init, dim3(nblk(n + kc, 256)), dim3(256), 0, la.iv, 2, la.shift, n, n + kc, la.uf);
         DCF_LAUNCH(ctx, "k_fill", k_fill_i32, dim3(nblk(n, 256)), dim3(256), 0, npar, -1, n);
         if ((rc = clear_attr(ctx, A, n))) return rc;
-        if ((rc = run_pass(ctx, la, sc, A, S, grp, npar, spar, kc, false))) return rc;
+        if ((rc = run_pass(ctx, la, sc, A, S, grp, npar, spar, kc, false, tgt))) return rc;
         DCF_LAUNCH(ctx, "k_fine_roots", k_fine_roots, dim3(nblk(kc, 256)), dim3(256), 0, la.uf, cpix, spar, npar, kc);
         par = npar;
     }
