@@ -376,7 +376,7 @@ static int batch_filter_same(dcf_ctx *ctx, const dcf_batch_image *images, int co
     if (decision < 0 || decision > 3) return dcf_set_error(ctx, DCF_EINVAL, "bad decision %d", decision);
     std::vector<BatchImg> tab((size_t)count);
     std::vector<double> lam((size_t)count);
-    uint64_t N = 0;
+    uint64_t N = 0, tiles = 0;
     int is3d = -1, all_u8 = 1;
     size_t max_u8 = 0;
     for (int i = 0; i < count; i++) {
@@ -394,7 +394,9 @@ static int batch_filter_same(dcf_ctx *ctx, const dcf_batch_image *images, int co
         tab[i].W = (uint32_t)b.dims[0];
         tab[i].H = (uint32_t)b.dims[1];
         tab[i].D = (uint32_t)b.dims[2];
-        tab[i].pad = 0;
+        tab[i].tile0 = (uint32_t)tiles;
+        tiles += d3 ? ((b.dims[0] + ZONE_T3 - 1) / ZONE_T3) * ((b.dims[1] + ZONE_T3 - 1) / ZONE_T3) * ((b.dims[2] + ZONE_T3 - 1) / ZONE_T3)
+                    : ((b.dims[0] + ZONE_T2 - 1) / ZONE_T2) * ((b.dims[1] + ZONE_T2 - 1) / ZONE_T2);
         lam[i] = b.lambda;
         N += n;
         if (N >= (1ull << 30)) return dcf_set_error(ctx, DCF_EUNSUPPORTED, "a batch is limited to 2^30 pixels");
@@ -422,6 +424,7 @@ static int batch_filter_same(dcf_ctx *ctx, const dcf_batch_image *images, int co
     t->dtype = cdt;
     t->nbatch = count;
     t->batch3d = is3d;
+    t->batch_tiles = (int64_t)tiles;
     BatchImg *dtab = nullptr;
     double *dlam = nullptr;
     BatchOut *douts = nullptr;

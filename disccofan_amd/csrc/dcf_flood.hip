@@ -524,7 +524,43 @@ struct TileGeo {
     int W, H, D;
     int TW, TH, TD;
     int ntx, nty;
+    const BatchImg *bimg; /* a batch: every image has its own tiles, from its tile0 on */
+    int nbatch;
 };
+
+/* tile b: its origin in its image, the image's dimensions and first pixel */
+struct TileAt {
+    int x0, y0, z0;
+    int W, H, D;
+    int64_t off;
+};
+
+__device__ __forceinline__ TileAt tile_at(const TileGeo &tg, int b)
+{
+    TileAt t;
+    int ntx = tg.ntx, nty = tg.nty;
+    if (tg.bimg) {
+        int lo = 0, hi = tg.nbatch - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if ((int)tg.bimg[mid].tile0 <= b) lo = mid;
+            else hi = mid - 1;
+        }
+        const BatchImg im = tg.bimg[lo];
+        t.W = (int)im.W; t.H = (int)im.H; t.D = (int)im.D;
+        t.off = (int64_t)im.off;
+        b -= (int)im.tile0;
+        ntx = (t.W + tg.TW - 1) / tg.TW;
+        nty = (t.H + tg.TH - 1) / tg.TH;
+    } else {
+        t.W = tg.W; t.H = tg.H; t.D = tg.D;
+        t.off = 0;
+    }
+    t.x0 = (b % ntx) * tg.TW;
+    t.y0 = ((b / ntx) % nty) * tg.TH;
+    t.z0 = (b / (ntx * nty)) * tg.TD;
+    return t;
+}
 
 __device__ __forceinline__ uint32_t lds_ld(const uint32_t *a, uint32_t i)
 {
@@ -557,14 +593,6 @@ __device__ __forceinline__ void lds_union(uint32_t *par, uint32_t a, uint32_t b)
     }
 }
 
-__device__ __forceinline__ void tile_origin(const TileGeo &tg, int b, int &x0, int &y0, int &z0)
-{
-    const int tx = b % tg.ntx, ty = (b / tg.ntx) % tg.nty, tz = b / (tg.ntx * tg.nty);
-    x0 = tx * tg.TW;
-    y0 = ty * tg.TH;
-    z0 = tz * tg.TD;
-}
-
 /* zones of every tile: zrep[p] = the zone's minimum pixel; uf[p] -> zrep for the other pixels */
 __global__ __launch_bounds__(256) void k_tile_zones(ImgView iv, int shift, TileGeo tg, Nbr nb, uint32_t *__restrict__ zrep,
                                                     uint32_t *__restrict__ uf, uint32_t *__restrict__ zarea)
@@ -572,13 +600,14 @@ __global__ __launch_bounds__(256) void k_tile_zones(ImgView iv, int shift, TileG
     __shared__ uint32_t par[ZT_MAX];
     __shared__ uint32_t zc[ZT_MAX];
     __shared__ uint16_t dg[ZT_MAX];
-    int x0, y0, z0;
-    tile_origin(tg, blockIdx.x, x0, y0, z0);
+    const TileAt ta = tile_at(tg, blockIdx.x);
+    const int x0 = ta.x0, y0 = ta.y0, z0 = ta.z0;
+    const int64_t W = ta.W, WH = (int64_t)ta.W * ta.H;
     const int tsz = tg.TW * tg.TH * tg.TD;
     for (int l = threadIdx.x; l < tsz; l += blockDim.x) {
         const int x = x0 + l % tg.TW, y = y0 + (l / tg.TW) % tg.TH, z = z0 + l / (tg.TW * tg.TH);
-        const bool in = x < tg.W && y < tg.H && z < tg.D;
-        dg[l] = in ? (uint16_t)(fl_key(iv, (int64_t)x + (int64_t)y * tg.W + (int64_t)z * tg.W * tg.H) >> shift) : 0xFFFFu;
+        const bool in = x < ta.W && y < ta.H && z < ta.D;
+        dg[l] = in ? (uint16_t)(fl_key(iv, ta.off + x + y * W + z * WH) >> shift) : 0xFFFFu;
         par[l] = (uint32_t)l;
         zc[l] = 0;
     }
@@ -602,8 +631,8 @@ __global__ __launch_bounds__(256) void k_tile_zones(ImgView iv, int shift, TileG
         const int r = (int)lds_find(par, (uint32_t)l);
         const int x = x0 + l % tg.TW, y = y0 + (l / tg.TW) % tg.TH, z = z0 + l / (tg.TW * tg.TH);
         const int rx = x0 + r % tg.TW, ry = y0 + (r / tg.TW) % tg.TH, rz = z0 + r / (tg.TW * tg.TH);
-        const uint32_t p = (uint32_t)((int64_t)x + (int64_t)y * tg.W + (int64_t)z * tg.W * tg.H);
-        const uint32_t rp = (uint32_t)((int64_t)rx + (int64_t)ry * tg.W + (int64_t)rz * tg.W * tg.H);
+        const uint32_t p = (uint32_t)(ta.off + x + y * W + z * WH);
+        const uint32_t rp = (uint32_t)(ta.off + rx + ry * W + rz * WH);
         zrep[p] = rp;
         if (p != rp) uf[p] = rp;
         atomicAdd(&zc[r], 1u);
@@ -613,7 +642,7 @@ __global__ __launch_bounds__(256) void k_tile_zones(ImgView iv, int shift, TileG
     for (int l = threadIdx.x; l < tsz; l += blockDim.x) {
         if (dg[l] == 0xFFFFu || par[l] != (uint32_t)l) continue;
         const int x = x0 + l % tg.TW, y = y0 + (l / tg.TW) % tg.TH, z = z0 + l / (tg.TW * tg.TH);
-        zarea[(int64_t)x + (int64_t)y * tg.W + (int64_t)z * tg.W * tg.H] = zc[l];
+        zarea[ta.off + x + y * W + z * WH] = zc[l];
     }
 }
 
@@ -637,20 +666,20 @@ __global__ __launch_bounds__(256) void k_zone_edges(ImgView iv, int shift, TileG
     for (int i = threadIdx.x; i < ZH_CAP; i += blockDim.x) tab[i] = ZH_EMPTY;
     cnt[threadIdx.x] = 0;
     __syncthreads();
-    int x0, y0, z0;
-    tile_origin(tg, blockIdx.x, x0, y0, z0);
+    const TileAt ta = tile_at(tg, blockIdx.x);
+    const int x0 = ta.x0, y0 = ta.y0, z0 = ta.z0;
     const int tsz = tg.TW * tg.TH * tg.TD;
-    const int64_t WH = (int64_t)tg.W * tg.H;
+    const int64_t WH = (int64_t)ta.W * ta.H;
     for (int l = threadIdx.x; l < tsz; l += blockDim.x) {
         const int x = x0 + l % tg.TW, y = y0 + (l / tg.TW) % tg.TH, z = z0 + l / (tg.TW * tg.TH);
-        if (x >= tg.W || y >= tg.H || z >= tg.D) continue;
-        const int64_t p = (int64_t)x + (int64_t)y * tg.W + (int64_t)z * WH;
+        if (x >= ta.W || y >= ta.H || z >= ta.D) continue;
+        const int64_t p = ta.off + x + (int64_t)y * ta.W + (int64_t)z * WH;
         const uint32_t dp = fl_key(iv, p) >> shift;
         const uint32_t rp = zrep[p];
         for (int k = 0; k < nb.n; k++) {
             const int nx = x + nb.dx[k], ny = y + nb.dy[k], nz = z + nb.dz[k];
-            if (nx < 0 || nx >= tg.W || ny < 0 || ny >= tg.H || nz < 0 || nz >= tg.D) continue;
-            const int64_t q = (int64_t)nx + (int64_t)ny * tg.W + (int64_t)nz * WH;
+            if (nx < 0 || nx >= ta.W || ny < 0 || ny >= ta.H || nz < 0 || nz >= ta.D) continue;
+            const int64_t q = ta.off + nx + (int64_t)ny * ta.W + (int64_t)nz * WH;
             const uint32_t dq = fl_key(iv, q) >> shift;
             if (dq < dp) continue;
             const uint32_t rq = zrep[q];
@@ -1239,7 +1268,7 @@ static int clear_attr(dcf_ctx *ctx, DevAttr &a, int64_t n)
 /* one flooding pass over all levels (high to low) */
 /* tgt: NB target slots per pixel for a pixel pass (nullptr: k_union evaluates the targets at each level) */
 static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAttr &S, int grp, int32_t *par, int32_t *spar,
-                    int64_t kc, bool zones, uint32_t *tgt)
+                    int64_t kc, bool zones, uint32_t *tgt, int is3d, int64_t batch_tiles)
 {
     const int64_t n = la.g.n;
     uint32_t *hist = (uint32_t *)sc.get(3 * NLEV * 4);
@@ -1257,11 +1286,13 @@ static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAtt
     la.zone = zones ? 1 : 0;
     if (zones) {
         tg.W = la.g.W; tg.H = la.g.H; tg.D = la.g.D;
-        if (tg.D > 1) { tg.TW = 16; tg.TH = 16; tg.TD = 16; }
-        else { tg.TW = 64; tg.TH = 64; tg.TD = 1; }
+        if (is3d) { tg.TW = ZONE_T3; tg.TH = ZONE_T3; tg.TD = ZONE_T3; }
+        else { tg.TW = ZONE_T2; tg.TH = ZONE_T2; tg.TD = 1; }
         tg.ntx = (tg.W + tg.TW - 1) / tg.TW;
         tg.nty = (tg.H + tg.TH - 1) / tg.TH;
-        ntiles = (int64_t)tg.ntx * tg.nty * ((tg.D + tg.TD - 1) / tg.TD);
+        tg.bimg = la.bimg;
+        tg.nbatch = la.nbatch;
+        ntiles = la.bimg ? batch_tiles : (int64_t)tg.ntx * tg.nty * ((tg.D + tg.TD - 1) / tg.TD);
         zrep = (uint32_t *)sc.get((size_t)n * 4);
         if (!zrep) return dcf_set_error(ctx, DCF_ENOMEM, "zone representatives");
         DCF_LAUNCH(ctx, "k_tile_zones", k_tile_zones, dim3((unsigned)ntiles), dim3(256), 0, la.iv, la.shift, tg, la.nb, zrep, la.uf,
@@ -1276,6 +1307,21 @@ static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAtt
     DCF_HIP(ctx, hipMemcpyAsync(h, hist, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
     if (zones) DCF_HIP(ctx, hipMemcpyAsync(hall, hist_all, sizeof hall, hipMemcpyDeviceToHost, ctx->stream));
     DCF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (zones) {
+        /* zones holding more than half the pixels save little in the level loop and crowd the edge
+         * de-duplication: undo the pre-links and zone totals and take the pixel path */
+        uint64_t nz = 0;
+        for (int i = 0; i < NLEV; i++) nz += h[i];
+        if (nz * 2 > (uint64_t)n) {
+            zones = false;
+            la.zone = 0;
+            zrep = nullptr;
+            memcpy(h, hall, sizeof h);
+            DCF_LAUNCH(ctx, "k_uf_init", k_uf_init, dim3(nblk(n, 256)), dim3(256), 0, la.iv, la.pass, la.shift, n, n, la.uf);
+            int rc = clear_attr(ctx, A, n);
+            if (rc) return rc;
+        }
+    }
     uint32_t off[NLEV + 1];
     off[0] = 0;
     for (int i = 0; i < NLEV; i++) off[i + 1] = off[i] + h[i];
@@ -1441,20 +1487,20 @@ int dcf_flood_build(dcf_tree *t, int grp)
     DCF_LAUNCH(ctx, "k_uf_init", k_uf_init, dim3(nblk(n, 256)), dim3(256), 0, la.iv, 1, la.shift, n, n, la.uf);
     DCF_LAUNCH(ctx, "k_fill", k_fill_i32, dim3(nblk(n, 256)), dim3(256), 0, par, -1, n);
     DevAttr none;
-    /* zone graph for the first pass (single images: the batch's tiles would straddle images) */
+    /* zone graph for the first pass (in a batch every image has tiles of its own) */
     const int64_t edge_bytes = n * la.nb.n * (int64_t)sizeof(uint2);
-    const bool zones = !t->bimg && edge_bytes <= ((int64_t)8 << 30) &&
+    const bool zones = edge_bytes <= ((int64_t)8 << 30) &&
                        (ctx->zones == 1 || (ctx->zones == 2 && (P > 1 || !is3d)));
     /* union targets precomputed by the level scatter, when they fit the budget: for batches (each
      * neighbour lookup of a batch locates its image), not for single images, whose neighbour keys
      * mostly hit the caches and whose target stream costs more than the lookups it saves */
     uint32_t *tgt = nullptr;
     const bool use_tgt = ctx->targets == 1 || (ctx->targets == 2 && t->bimg);
-    if (use_tgt && (int64_t)n * la.nb.n * 4 <= ((int64_t)16 << 30) && (!zones || P > 1)) {
+    if (use_tgt && (int64_t)n * la.nb.n * 4 <= ((int64_t)16 << 30)) {
         tgt = (uint32_t *)sc.get((size_t)n * la.nb.n * 4);
         if (!tgt) return dcf_set_error(ctx, DCF_ENOMEM, "union targets");
     }
-    if ((rc = run_pass(ctx, la, sc, A, none, grp, par, nullptr, 0, zones, tgt))) return rc;
+    if ((rc = run_pass(ctx, la, sc, A, none, grp, par, nullptr, 0, zones, tgt, is3d, t->batch_tiles))) return rc;
 
     /* ---- passes 2..P: the next digit inside every node of the previous pass's tree ---- */
     for (int pass = 2; pass <= P; pass++) {
@@ -1507,7 +1553,7 @@ int dcf_flood_build(dcf_tree *t, int grp)
         DCF_LAUNCH(ctx, "k_uf_init", k_uf_init, dim3(nblk(n + kc, 256)), dim3(256), 0, la.iv, 2, la.shift, n, n + kc, la.uf);
         DCF_LAUNCH(ctx, "k_fill", k_fill_i32, dim3(nblk(n, 256)), dim3(256), 0, npar, -1, n);
         if ((rc = clear_attr(ctx, A, n))) return rc;
-        if ((rc = run_pass(ctx, la, sc, A, S, grp, npar, spar, kc, false, tgt))) return rc;
+        if ((rc = run_pass(ctx, la, sc, A, S, grp, npar, spar, kc, false, tgt, is3d, t->batch_tiles))) return rc;
         DCF_LAUNCH(ctx, "k_fine_roots", k_fine_roots, dim3(nblk(kc, 256)), dim3(256), 0, la.uf, cpix, spar, npar, kc);
         par = npar;
     }

@@ -86,8 +86,12 @@ struct dcf_ctx {
 struct BatchImg {
     uint64_t off;     /* first pixel of the image */
     uint32_t W, H, D; /* its dimensions */
-    uint32_t pad;
+    uint32_t tile0;   /* its first tile in the batch's zone tiles (ZONE_T2^2 / ZONE_T3^3 tiles) */
 };
+
+/* zone tiles of the first pass: 64x64 in 2D, 16x16x16 in 3D */
+#define ZONE_T2 64
+#define ZONE_T3 16
 
 /* output of one batch image: device pointer and element type */
 struct BatchOut {
@@ -133,6 +137,7 @@ struct dcf_tree {
     int host_gmax_valid = 0;
     /* batched form (dcf_batch_filter): device image table, image count, 3D flag, per-node lambda */
     const BatchImg *bimg = nullptr;
+    int64_t batch_tiles = 0; /* zone tiles of all images */
     int nbatch = 0;
     int batch3d = 0;
     const double *nlam = nullptr;
