@@ -122,6 +122,41 @@ def test_random_3d(ctx, seed):
             _compare_all(ctx, img, conn, tt, label="rand3d s%d" % seed)
 
 
+@pytest.mark.parametrize("zones,targets", [("0", "1"), ("1", "0"), ("1", "1"), ("0", "0")])
+def test_build_paths(monkeypatch, zones, targets):
+    """every combination of the build's optional paths on single images: the zone graph of the first
+    pass (DCF_ZONES, 1 forces it also for 3D u8 and zone-rich images) and the union targets precomputed
+    by the level scatter (DCF_TARGETS); the library reads both when a context is created"""
+    monkeypatch.setenv("DCF_ZONES", zones)
+    monkeypatch.setenv("DCF_TARGETS", targets)
+    c = dcf.Context(0)
+    try:
+        rng = np.random.default_rng(700 + 2 * int(zones) + int(targets))
+        cases = [
+            (rng.integers(0, 65536, size=(67, 91)).astype(np.uint16), 8),
+            (synth.vnoise(96, 128), 4),
+            (rng.integers(0, 12, size=(73, 59)).astype(np.uint8), 8),
+            (rng.integers(0, 5, size=(11, 17, 19)).astype(np.uint8), 6),
+            (rng.integers(0, 3000, size=(9, 13, 15)).astype(np.uint16), 26),
+            (rng.normal(size=(41, 37)).astype(np.float32), 8),
+        ]
+        for img, conn in cases:
+            for tt in ("max", "min"):
+                label = "zones=%s targets=%s %s %s" % (zones, targets, img.dtype, img.shape)
+                say("build path", label, conn, tt)
+                o = OracleTree(img, conn, tt, 0)
+                t = c.build(img, conn, tt, 0)
+                assert_same(t.parent(), o.parent().astype(np.int32), "parent " + label)
+                assert_same(t.check(0), o.check(0), "area " + label)
+                assert_same(t.filter(0, 15, 3), o.filter(0, 15, 3), "subtractive " + label)
+                t.accumulate(17)
+                o.reaccumulate(17)
+                assert_same(t.check(17), o.check(17), "ncomp " + label)
+                t.free()
+    finally:
+        c.close()
+
+
 @pytest.mark.parametrize("shape,conn", [((57, 83), 4), ((61, 47), 8), ((9, 23, 31), 6), ((7, 19, 17), 26)])
 def test_every_attribute(ctx, shape, conn):
     """check() of all 18 attributes (every group of AttribsArray) against the oracle; the eigenvalue
