@@ -8,15 +8,20 @@
  * processed together:
  *
  *   k_union   every pixel of level k is united with its already-processed neighbours (level > k,
- *             or level k with a larger index) in a lock-free union-find that always links the
- *             root of higher priority (level, index) under the lower one.  A set's root is thus
- *             its minimum-index pixel of the lowest level: the canonical levelroot.  Old roots
- *             (from levels > k) that get linked are pushed on the `absorbed` list.
- *   k_final   every pixel of level k takes find(p) as its parent when it is not the root, and
- *             adds its own attribute contribution to its root; every absorbed old root takes the
- *             new root as parent and merges its (complete) attributes into it.  Attributes are
- *             therefore accumulated level by level, exactly like the reference's merges during
- *             flooding, with no separate leaffix.
+ *             or level k with a larger index) in a lock-free union-find.  A set of level-k pixels
+ *             that reaches an old set (a root from a level > k) is linked under the old root, so a
+ *             thread only ever CASes its own set's root entry; between new sets the smaller index
+ *             wins, so a set with no old root is rooted at its minimum pixel, the canonical
+ *             levelroot.  Old roots linked under another root are pushed on the `absorbed` list.
+ *   k_final_pixels  every pixel of level k finds its root.  A root of level k is canonical: the
+ *             pixel takes it as parent and adds its attribute contribution to it.  A set rooted at
+ *             an old root r gets its canonical levelroot c by atomicMin; the contributions of its
+ *             pixels and the complete records of the old roots absorbed into it are kept pending
+ *             at r, and their parents are left as "the levelroot of r's set".
+ *   k_fix     every such set is re-rooted at c, and c's record becomes r's record plus what is
+ *             pending at r.  After the pass, k_held_resolve replaces the pending parents by c.
+ * Attributes are therefore accumulated level by level, exactly like the reference's merges during
+ * flooding, with no separate leaffix.
  *
  * The critical path is the number of levels, not the depth of the tree (u16 trees of smooth
  * images are ~50k nodes deep).  Keys are taken in 8-bit digits, one pass of 256 levels per digit
@@ -854,11 +859,6 @@ __device__ void wave_add(const DevAttr &A, bool valid, uint32_t r, const Contrib
     }
 }
 
-/* after the unions of a level, every pixel p of the level finds its set's root r.  A root of this
- * level is the canonical levelroot: p takes it as parent and adds its own attribute contribution.
- * A set still rooted at an old root has its canonical levelroot computed here (minimum pixel of
- * the level, one atomicMin per wave and set; the first to touch a set queues its root for k_fix)
- * and p keeps r in `held` for k_final_rest. */
 /* zone pass: the coordinate / grey sums and extents of every zone into its representative's record
  * (the pixel counts come from k_tile_zones) */
 __global__ __launch_bounds__(256) void k_zone_totals(LevelArgs a, int grp, const uint32_t *__restrict__ zrep, DevAttr A, int64_t n)
@@ -896,6 +896,9 @@ __global__ __launch_bounds__(256) void k_zone_parents(ImgView iv, int pass, int 
  * atomics from thousands of waves were the cost of the final passes on levels that are one big
  * component.  Every thread of the block must call these. */
 #define FB 1024
+/* a parent left pending during a pass: the set rooted at r, whose canonical levelroot is not known yet */
+#define HELD(r) (-2 - (int32_t)(r))
+#define HELD_ROOT(v) ((uint32_t)(-2 - (v)))
 #define FW (FB / 64)
 #define R_EMPTY 0xFFFFFFFFu
 #define R_MIXED 0xFFFFFFFEu
@@ -1005,61 +1008,47 @@ __device__ void block_canon_min(const LevelArgs &a, BlockScratch &bs, bool old, 
     }
 }
 
+/* after the unions of a level, every pixel p of the level finds its set's root r.  A root of this
+ * level is the canonical levelroot: p takes it as parent and adds its own attribute contribution.
+ * A set still rooted at an old root r gets its canonical levelroot computed here (minimum pixel of
+ * the level, one atomicMin per wave and set; the first to touch a set queues r for k_fix); p's
+ * contribution is kept pending at r (P) and its parent is left as HELD(r). */
 __global__ __launch_bounds__(FB) void k_final_pixels(LevelArgs a, const uint32_t *__restrict__ lvl, uint32_t cnt, uint32_t lev,
-                                                     DevAttr A, int grp, int32_t *__restrict__ par, uint32_t *__restrict__ held)
-{
-    __shared__ BlockScratch bs;
-    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
-    const bool valid = i < cnt;
-    uint32_t p = 0, r = 0;
-    bool fresh = false, old = false;
-    Contrib c;
-    contrib_zero(c);
-    if (valid) {
-        p = lvl[i];
-        uint32_t tag;
-        r = uf_find(a.uf, p, tag);
-        fresh = (tag & UF_LVL) == lev;
-        old = !fresh;
-        held[i] = old ? r : 0xFFFFFFFFu;
-        if (fresh) {
-            if (r != p) par[p] = (int32_t)r;
-            if (!a.zone) contrib_own(a, grp, p, c);
-            else if (r != p) contrib_rec(A, p, c); /* a root already holds its own zone's totals */
-        }
-    }
-    block_add(A, bs, fresh, r, c);
-    __syncthreads();
-    block_canon_min(a, bs, old, r, p);
-}
-
-/* the rest of a level, after k_fix: (1) the pixels of re-rooted sets take the canonical levelroot
- * as parent and add their contribution there; (2) every old root absorbed at this level (linked in
- * k_union or re-rooted in k_fix) takes the new root as parent and merges its complete record into
- * it -- pixel roots from A, coarse super-vertices (ids n and above) from S, recording their joining
- * node in spar.  marks[0] is where this level's absorbed entries start; the end is the current count,
- * which the kernel hands on as the next level's start in marks[1]. */
-__global__ __launch_bounds__(FB) void k_final_rest(LevelArgs a, const uint32_t *__restrict__ lvl, uint32_t cnt,
-                                                   const uint32_t *__restrict__ held, DevAttr A, DevAttr S, int grp,
-                                                   int32_t *__restrict__ par, int32_t *__restrict__ spar, uint32_t *__restrict__ marks)
+                                                     DevAttr A, DevAttr S, DevAttr P, int grp, int32_t *__restrict__ par,
+                                                     int32_t *__restrict__ spar, uint32_t *__restrict__ marks)
 {
     __shared__ BlockScratch bs;
     const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
     {
-        const uint32_t r = i < cnt ? held[i] : 0xFFFFFFFFu;
-        const bool valid = r != 0xFFFFFFFFu;
-        uint32_t cr = 0;
+        const bool valid = i < cnt;
+        uint32_t p = 0, r = 0;
+        bool fresh = false, old = false;
         Contrib c;
         contrib_zero(c);
         if (valid) {
-            const uint32_t p = lvl[i];
-            cr = a.canon[r];
-            if (cr != p) par[p] = (int32_t)cr;
-            if (!a.zone) contrib_own(a, grp, p, c);
-            else if (cr != p) contrib_rec(A, p, c);
+            p = lvl[i];
+            uint32_t tag;
+            r = uf_find(a.uf, p, tag);
+            fresh = (tag & UF_LVL) == lev;
+            old = !fresh;
+            if (fresh) {
+                if (r != p) par[p] = (int32_t)r;
+                if (!a.zone) contrib_own(a, grp, p, c);
+                else if (r != p) contrib_rec(A, p, c); /* a root already holds its own zone's totals */
+            } else {
+                par[p] = HELD(r);
+                if (!a.zone) contrib_own(a, grp, p, c);
+                else contrib_rec(A, p, c);
+            }
         }
-        block_add(A, bs, valid, cr, c);
+        block_add(A, bs, fresh, r, c);
+        __syncthreads();
+        block_add(P, bs, old, r, c);
+        __syncthreads();
+        block_canon_min(a, bs, old, r, p);
     }
+    /* the old roots linked under another root at this level: parent = the set's canonical levelroot
+     * (resolved after the pass), complete record pending at the set's root */
     const uint32_t start = marks[0], end = *a.nabs;
     if (blockIdx.x == 0 && threadIdx.x == 0) marks[1] = end;
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -1075,22 +1064,24 @@ __global__ __launch_bounds__(FB) void k_final_rest(LevelArgs a, const uint32_t *
             uint32_t tag;
             r = uf_find(a.uf, h, tag);
             if (h < (uint32_t)a.g.n) {
-                par[h] = (int32_t)r;
+                par[h] = HELD(r);
                 contrib_rec(A, h, c);
             } else {
                 const uint32_t cc = h - (uint32_t)a.g.n;
-                spar[cc] = (int32_t)r;
+                spar[cc] = HELD(r);
                 contrib_rec(S, cc, c);
             }
         }
-        wave_add(A, valid, r, c);
+        wave_add(P, valid, r, c);
     }
 }
 
-/* re-root every queued set at its canonical levelroot c: c becomes the root (tagged with this level)
- * and the old root r hangs below it, queued as absorbed so that its node gets parent c and merges
- * its attributes into c */
-__global__ __launch_bounds__(256) void k_fix(LevelArgs a, uint32_t ord, uint32_t lev)
+/* re-root every queued set at its canonical levelroot c: c becomes the root (tagged with this level),
+ * the old root r hangs below it, and c's record becomes r's complete record plus everything pending at
+ * r (the level's pixels of the set, the other old roots absorbed into it).  No other thread touches c's
+ * record: c is no set's root before this, and the set's contributions were all sent to r. */
+__global__ __launch_bounds__(256) void k_fix(LevelArgs a, uint32_t ord, uint32_t lev, DevAttr A, DevAttr S, DevAttr P,
+                                             int32_t *__restrict__ par, int32_t *__restrict__ spar)
 {
     const uint32_t start = a.fcount[1 + ord], end = a.fcount[0];
     if (blockIdx.x == 0 && threadIdx.x == 0) a.fcount[2 + ord] = end;
@@ -1099,7 +1090,36 @@ __global__ __launch_bounds__(256) void k_fix(LevelArgs a, uint32_t ord, uint32_t
         const uint32_t c = a.canon[r];
         st(a.uf, c, UF_ROOT | lev);
         st(a.uf, r, c);
-        a.absorbed[atomicAdd(a.nabs, 1u)] = r;
+        const bool sv = r >= (uint32_t)a.g.n;
+        const DevAttr &R = sv ? S : A;
+        const uint32_t ri = sv ? r - (uint32_t)a.g.n : r;
+        if (sv) spar[ri] = (int32_t)c;
+        else par[r] = (int32_t)c;
+        A.area[c] = R.area[ri] + P.area[r];
+        for (int k = 0; k < A.nsum; k++) A.s[k][c] = R.s[k][ri] + P.s[k][r];
+        if (A.has_ext)
+            for (int k = 0; k < 3; k++) {
+                A.mn[k][c] = min(R.mn[k][ri], P.mn[k][r]);
+                A.mx[k][c] = max(R.mx[k][ri], P.mx[k][r]);
+            }
+    }
+}
+
+/* after a pass: parents (and super-vertex joins) left as HELD(r) become the canonical levelroot of the
+ * set rooted at r; that levelroot itself, still HELD, has no parent in this pass */
+__global__ __launch_bounds__(256) void k_held_resolve(const uint32_t *__restrict__ canon, int32_t *__restrict__ par, int64_t n,
+                                                      int32_t *__restrict__ spar, int64_t kc)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const int32_t v = par[i];
+        if (v <= -2) {
+            const uint32_t c = canon[HELD_ROOT(v)];
+            par[i] = c == (uint32_t)i ? -1 : (int32_t)c;
+        }
+    } else if (i < n + kc) {
+        const int32_t v = spar[i - n];
+        if (v <= -2) spar[i - n] = (int32_t)canon[HELD_ROOT(v)];
     }
 }
 
@@ -1382,9 +1402,13 @@ static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAtt
     la.canon = (uint32_t *)sc.get((size_t)nel * 4);
     la.fix = (uint32_t *)sc.get((size_t)(nel + 16) * 4);
     la.fcount = (uint32_t *)sc.get((NLEV + 4) * 4);
-    uint32_t *hold = (uint32_t *)sc.get((size_t)n * 4);
-    if (!la.canon || !la.fix || !la.fcount || !hold) return dcf_set_error(ctx, DCF_ENOMEM, "canonical root lists");
+    if (!la.canon || !la.fix || !la.fcount) return dcf_set_error(ctx, DCF_ENOMEM, "canonical root lists");
     DCF_LAUNCH(ctx, "k_fill", k_fill_u32f, dim3(nblk(nel, 256)), dim3(256), 0, la.canon, 0xFFFFFFFFu, nel);
+    /* records pending at the old roots of the sets of a level (each element is such a root at one
+     * level at most) */
+    DevAttr P;
+    int rc = alloc_attr_n(ctx, &sc, P, grp, nel);
+    if (rc || (rc = clear_attr(ctx, P, nel))) return rc;
     DCF_LAUNCH(ctx, "k_fill", k_fill_u32f, dim3(2), dim3(256), 0, la.fcount, 0u, (int64_t)(NLEV + 4));
     int ord = 0;
     for (int lev = NLEV - 1; lev >= 0; lev--) {
@@ -1409,19 +1433,19 @@ static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAtt
         case 8: DCF_LAUNCH(ctx, "k_union", k_union<8>, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, cnt, (uint32_t)lev, (uint32_t)kc); break;
         default: DCF_LAUNCH(ctx, "k_union", k_union<26>, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, cnt, (uint32_t)lev, (uint32_t)kc); break;
         }
-        uint32_t *held = hold + off[lev];
-        DCF_LAUNCH(ctx, "k_final_pixels", k_final_pixels, dim3(nblk(cnt, FB)), dim3(FB), 0, la, lvl, cnt, (uint32_t)lev, A, grp,
-                   par, held);
-        unsigned fb = nblk(cnt, 256);
-        if (fb > 1024) fb = 1024;
-        DCF_LAUNCH(ctx, "k_fix", k_fix, dim3(fb), dim3(256), 0, la, (uint32_t)ord, (uint32_t)lev);
+        /* the grid covers the level's pixels and, striding, the roots absorbed at the level */
         unsigned rb = nblk(cnt, FB), ab = nblk((int64_t)cnt * la.nb.n / 4, FB);
         if (ab > 1024) ab = 1024;
         if (ab > rb) rb = ab;
-        DCF_LAUNCH(ctx, "k_final_rest", k_final_rest, dim3(rb), dim3(FB), 0, la, lvl, cnt, (const uint32_t *)held, A, S, grp, par,
+        DCF_LAUNCH(ctx, "k_final_pixels", k_final_pixels, dim3(rb), dim3(FB), 0, la, lvl, cnt, (uint32_t)lev, A, S, P, grp, par,
                    spar, nabs + 1 + ord);
+        unsigned fb = nblk(cnt, 256);
+        if (fb > 1024) fb = 1024;
+        DCF_LAUNCH(ctx, "k_fix", k_fix, dim3(fb), dim3(256), 0, la, (uint32_t)ord, (uint32_t)lev, A, S, P, par, spar);
         ord++;
     }
+    DCF_LAUNCH(ctx, "k_held_resolve", k_held_resolve, dim3(nblk(n + kc, 256)), dim3(256), 0, (const uint32_t *)la.canon, par, n,
+               spar, kc);
     if (zones)
         DCF_LAUNCH(ctx, "k_zone_parents", k_zone_parents, dim3(nblk(n, 256)), dim3(256), 0, la.iv, la.pass, la.shift,
                    (const uint32_t *)zrep, par, n);
