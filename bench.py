@@ -8,8 +8,8 @@ max-tree, direct rule, lambda=500) instead.
 
 Printed: ONE JSON line with the metric, a roofline object for the dominant kernel (per-launch
 algorithmic bytes / HIP-event launch time, measured inside the timed region on the stream the
-kernels run on), and a cpu_baseline measured with the CPU oracle on a bounded crop of the same
-image on this host.
+kernels run on), and a cpu_baseline measured with the CPU oracle (OpenMP) on the same image on this
+host.
 """
 from __future__ import annotations
 
@@ -39,16 +39,16 @@ WORKLOADS = {
 }
 
 
-def algorithmic_bytes(kernel: str, n: int, k_nodes: int, B: int, passes: int):
+def algorithmic_bytes(kernel: str, n: int, k_nodes: int, B: float, passes: float):
     """(units one tree makes this kernel process, algorithmic bytes per unit): every logical array is
     counted once per stage (build: read image + write parent = B + 4 per voxel; attribute: read
     parent + write area = 8 per voxel; filter: read parent, image and area, write output = 2B + 8
     per voxel).  The level kernels of the build together see every voxel once per pass, so their
-    per-launch figure is (passes * n / launches) voxels."""
+    per-launch figure is (passes * n / launches) voxels; `passes` is the voxel-weighted mean number of
+    8-bit digit passes (1 for u8, 2 for u16)."""
     table = {
         "k_union": (passes * n, B + 4),
         "k_final_pixels": (passes * n, 8),
-        "k_final_rest": (passes * n, 8),
         "k_filter_out": (n, 2 * B + 8),
         "k_direct_jump": (k_nodes, 9),
     }
@@ -120,6 +120,8 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB 5 times (the reference point for the "
                          "FETCH_SIZE / WRITE_SIZE counters) and print its bandwidth")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="batched workload: skip the per-image comparison loop (plain_form null)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -160,7 +162,8 @@ def main():
         gen_s = time.perf_counter() - t0
         n = int(sum(a.size for a in imgs))
         B = sum(a.nbytes for a in imgs) / n
-        passes = 1 if all(a.dtype == np.uint8 for a in imgs) else 2
+        # u8 and u16 images are built in separate batch trees: one digit pass for u8, two for u16
+        passes = sum((2 if a.dtype == np.uint16 else 1) * a.size for a in imgs) / n
         d_in = [torch.from_numpy(a.view(np.int16) if a.dtype == np.uint16 else a).to(dev) for a in imgs]
         d_out = [torch.empty_like(t) for t in d_in]
         torch.cuda.synchronize()
@@ -265,7 +268,7 @@ def main():
         print("nodes K = %d (%.1f%% of pixels)" % (K, 100.0 * K / n), file=sys.stderr)
 
     plain = None
-    if batch:
+    if batch and not args.no_plain:
         # the plain form on the same device-resident images: one build + filter call per image
         plain_step()
         torch.cuda.synchronize()
