@@ -199,11 +199,55 @@ __global__ __launch_bounds__(256) void k_lvl_hist(ImgView iv, int pass, int shif
     if (sh[threadIdx.x]) atomicAdd(&hist[threadIdx.x], sh[threadIdx.x]);
 }
 
-/* each block owns one tile of the image (64x64 in 2D, 16x16x16 in 3D; a raster range of 4096 pixels
- * for a batch); one global atomic per (block, level) reserves the block's slots, so every level's
- * list is ordered by tile and the pixels a block of a level kernel handles lie close together */
+struct TileGeo {
+    int W, H, D;
+    int TW, TH, TD;
+    int ntx, nty;
+    const BatchImg *bimg; /* a batch: every image has its own tiles, from its tile0 on */
+    int nbatch;
+};
+
+/* tile b: its origin in its image, the image's dimensions and first pixel */
+struct TileAt {
+    int x0, y0, z0;
+    int W, H, D;
+    int64_t off;
+};
+
+__device__ __forceinline__ TileAt tile_at(const TileGeo &tg, int b)
+{
+    TileAt t;
+    int ntx = tg.ntx, nty = tg.nty;
+    if (tg.bimg) {
+        int lo = 0, hi = tg.nbatch - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if ((int)tg.bimg[mid].tile0 <= b) lo = mid;
+            else hi = mid - 1;
+        }
+        const BatchImg im = tg.bimg[lo];
+        t.W = (int)im.W; t.H = (int)im.H; t.D = (int)im.D;
+        t.off = (int64_t)im.off;
+        b -= (int)im.tile0;
+        ntx = (t.W + tg.TW - 1) / tg.TW;
+        nty = (t.H + tg.TH - 1) / tg.TH;
+    } else {
+        t.W = tg.W; t.H = tg.H; t.D = tg.D;
+        t.off = 0;
+    }
+    t.x0 = (b % ntx) * tg.TW;
+    t.y0 = ((b / ntx) % nty) * tg.TH;
+    t.z0 = (b / (ntx * nty)) * tg.TD;
+    return t;
+}
+
+/* each block owns one tile of the image (64x64 in 2D, 16x16x16 in 3D; in a batch, one of its image's
+ * zone tiles); one global atomic per (block, level) reserves the block's slots, so every level's list
+ * is ordered by tile and the pixels a block of a level kernel handles lie close together */
 struct ScatterGeo {
-    int tiled;          /* 0: raster ranges of 4096 pixels */
+    int batch;          /* 0: one image's tiles (in groups), 1: a batch's tiles */
+    const BatchImg *bimg;
+    int nbatch;
     int W, H, D;
     int TW, TH, TD;     /* tile shape */
     int GX, GY, GZ;     /* tiles per group: a group's tiles are consecutive in the list */
@@ -453,10 +497,20 @@ __global__ __launch_bounds__(256) void k_lvl_scatter(ImgView iv, int pass, int s
     __shared__ uint32_t cnt[NLEV], base[NLEV];
     cnt[threadIdx.x] = 0;
     __syncthreads();
-    const int tsz = sg.tiled ? sg.TW * sg.TH * sg.TD : 4096;
+    const int tsz = sg.TW * sg.TH * sg.TD;
     int64_t p0 = 0;
     int x0 = 0, y0 = 0, z0 = 0;
-    if (sg.tiled) {
+    int iW = sg.W, iH = sg.H, iD = sg.D;
+    if (sg.batch) {
+        TileGeo tg;
+        tg.TW = sg.TW; tg.TH = sg.TH; tg.TD = sg.TD;
+        tg.bimg = sg.bimg;
+        tg.nbatch = sg.nbatch;
+        const TileAt ta = tile_at(tg, blockIdx.x);
+        x0 = ta.x0; y0 = ta.y0; z0 = ta.z0;
+        iW = ta.W; iH = ta.H; iD = ta.D;
+        p0 = ta.off;
+    } else {
         const int gsz = sg.GX * sg.GY * sg.GZ;
         const int g = blockIdx.x / gsz, w = blockIdx.x % gsz;
         const int gx = g % sg.ngx, gy = (g / sg.ngx) % sg.ngy, gz = g / (sg.ngx * sg.ngy);
@@ -464,20 +518,12 @@ __global__ __launch_bounds__(256) void k_lvl_scatter(ImgView iv, int pass, int s
         x0 = tx * sg.TW; y0 = ty * sg.TH; z0 = tz * sg.TD;
         /* tiles of a partial group beyond the image: the block has no pixel */
         if (tx >= sg.ntx || ty >= sg.nty || tz >= sg.ntz) x0 = sg.W;
-    } else {
-        p0 = (int64_t)blockIdx.x * tsz;
     }
     /* pixel of local slot l of this block, or -1 outside the image */
     auto pix = [&](int l) -> int64_t {
-        int64_t p;
-        if (!sg.tiled) {
-            p = p0 + l;
-            if (p >= n) return -1;
-        } else {
-            const int x = x0 + l % sg.TW, y = y0 + (l / sg.TW) % sg.TH, z = z0 + l / (sg.TW * sg.TH);
-            if (x >= sg.W || y >= sg.H || z >= sg.D) return -1;
-            p = (int64_t)x + (int64_t)y * sg.W + (int64_t)z * sg.W * sg.H;
-        }
+        const int x = x0 + l % sg.TW, y = y0 + (l / sg.TW) % sg.TH, z = z0 + l / (sg.TW * sg.TH);
+        if (x >= iW || y >= iH || z >= iD) return -1;
+        const int64_t p = p0 + (int64_t)x + (int64_t)y * iW + (int64_t)z * iW * iH;
         return (only && only[p] != (uint32_t)p) ? -1 : p;
     };
     for (int l = threadIdx.x; l < tsz; l += blockDim.x) {
@@ -525,47 +571,6 @@ __global__ __launch_bounds__(256) void k_lvl_scatter(ImgView iv, int pass, int s
 #define ZH_CAP 4096
 #define ZH_EMPTY 0xFFFFFFFFFFFFFFFFull
 
-struct TileGeo {
-    int W, H, D;
-    int TW, TH, TD;
-    int ntx, nty;
-    const BatchImg *bimg; /* a batch: every image has its own tiles, from its tile0 on */
-    int nbatch;
-};
-
-/* tile b: its origin in its image, the image's dimensions and first pixel */
-struct TileAt {
-    int x0, y0, z0;
-    int W, H, D;
-    int64_t off;
-};
-
-__device__ __forceinline__ TileAt tile_at(const TileGeo &tg, int b)
-{
-    TileAt t;
-    int ntx = tg.ntx, nty = tg.nty;
-    if (tg.bimg) {
-        int lo = 0, hi = tg.nbatch - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if ((int)tg.bimg[mid].tile0 <= b) lo = mid;
-            else hi = mid - 1;
-        }
-        const BatchImg im = tg.bimg[lo];
-        t.W = (int)im.W; t.H = (int)im.H; t.D = (int)im.D;
-        t.off = (int64_t)im.off;
-        b -= (int)im.tile0;
-        ntx = (t.W + tg.TW - 1) / tg.TW;
-        nty = (t.H + tg.TH - 1) / tg.TH;
-    } else {
-        t.W = tg.W; t.H = tg.H; t.D = tg.D;
-        t.off = 0;
-    }
-    t.x0 = (b % ntx) * tg.TW;
-    t.y0 = ((b / ntx) % nty) * tg.TH;
-    t.z0 = (b / (ntx * nty)) * tg.TD;
-    return t;
-}
 
 __device__ __forceinline__ uint32_t lds_ld(const uint32_t *a, uint32_t i)
 {
@@ -823,7 +828,9 @@ __device__ __forceinline__ void contrib_zero(Contrib &c)
     for (int i = 0; i < 3; i++) { c.mn[i] = 0xFFFFFFFFu; c.mx[i] = 0; }
 }
 
-__device__ void wave_add(const DevAttr &A, bool valid, uint32_t r, const Contrib &c)
+/* adds every valid lane's contribution to record r of A, or of B for the lanes with toB (all lanes
+ * of one root agree on it), one atomic per distinct root and field */
+__device__ void wave_add2(const DevAttr &A, const DevAttr &B, bool valid, bool toB, uint32_t r, const Contrib &c)
 {
     const int lane = threadIdx.x & 63;
     uint64_t pending = __ballot(valid);
@@ -836,27 +843,34 @@ __device__ void wave_add(const DevAttr &A, bool valid, uint32_t r, const Contrib
             /* the leader's root is unique in the wave: the remaining roots are likely distinct too,
              * so every pending lane updates its own root directly */
             if ((pending >> lane) & 1ull) {
-                atomicAdd(&A.area[r], c.area);
-                for (int i = 0; i < A.nsum; i++)
-                    if (c.s[i]) atomicAdd((unsigned long long *)&A.s[i][r], (unsigned long long)c.s[i]);
-                if (A.has_ext)
-                    for (int i = 0; i < 3; i++) { atomicMin(&A.mn[i][r], c.mn[i]); atomicMax(&A.mx[i][r], c.mx[i]); }
+                const DevAttr &D = toB ? B : A;
+                atomicAdd(&D.area[r], c.area);
+                for (int i = 0; i < D.nsum; i++)
+                    if (c.s[i]) atomicAdd((unsigned long long *)&D.s[i][r], (unsigned long long)c.s[i]);
+                if (D.has_ext)
+                    for (int i = 0; i < 3; i++) { atomicMin(&D.mn[i][r], c.mn[i]); atomicMax(&D.mx[i][r], c.mx[i]); }
             }
             break;
         }
+        const DevAttr &D = __shfl((int)toB, leader, 64) ? B : A;
         uint64_t area = wsum64(mine ? c.area : 0);
-        if (lane == leader) atomicAdd(&A.area[rl], (uint32_t)area);
-        for (int i = 0; i < A.nsum; i++) {
+        if (lane == leader) atomicAdd(&D.area[rl], (uint32_t)area);
+        for (int i = 0; i < D.nsum; i++) {
             uint64_t v = wsum64(mine ? (uint64_t)c.s[i] : 0);
-            if (lane == leader && v) atomicAdd((unsigned long long *)&A.s[i][rl], (unsigned long long)v);
+            if (lane == leader && v) atomicAdd((unsigned long long *)&D.s[i][rl], (unsigned long long)v);
         }
-        if (A.has_ext)
+        if (D.has_ext)
             for (int i = 0; i < 3; i++) {
                 uint32_t lo = wmin32(mine ? c.mn[i] : 0xFFFFFFFFu), hi = wmax32(mine ? c.mx[i] : 0u);
-                if (lane == leader) { atomicMin(&A.mn[i][rl], lo); atomicMax(&A.mx[i][rl], hi); }
+                if (lane == leader) { atomicMin(&D.mn[i][rl], lo); atomicMax(&D.mx[i][rl], hi); }
             }
         pending &= ~m;
     }
+}
+
+__device__ __forceinline__ void wave_add(const DevAttr &A, bool valid, uint32_t r, const Contrib &c)
+{
+    wave_add2(A, A, valid, false, r, c);
 }
 
 /* zone pass: the coordinate / grey sums and extents of every zone into its representative's record
@@ -905,6 +919,7 @@ __global__ __launch_bounds__(256) void k_zone_parents(ImgView iv, int pass, int 
 
 struct BlockScratch {
     uint32_t root[FW];
+    uint32_t dst[FW];
     uint64_t red[FW][1 + DCF_MAX_SUMS + 6];
 };
 
@@ -930,28 +945,35 @@ __device__ __forceinline__ uint32_t block_common_root(BlockScratch &bs, bool val
     return R;
 }
 
-__device__ void block_add(const DevAttr &A, BlockScratch &bs, bool valid, uint32_t r, const Contrib &c)
+/* block-wide wave_add2: one atomic per field when every valid lane of the block shares one root */
+__device__ void block_add2(const DevAttr &A0, const DevAttr &B, BlockScratch &bs, bool valid, bool toB, uint32_t r,
+                           const Contrib &c)
 {
     const uint32_t R = block_common_root(bs, valid, r);
     if (R == R_EMPTY) return;
     if (R == R_MIXED) {
-        wave_add(A, valid, r, c);
+        wave_add2(A0, B, valid, toB, r, c);
         return;
     }
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t tb = __ballot(valid && toB);
+    if (lane == 0) bs.dst[w] = tb ? 1u : 0u;
     const uint64_t area = wsum64(valid ? c.area : 0);
     if (lane == 0) bs.red[w][0] = area;
-    for (int i = 0; i < A.nsum; i++) {
+    for (int i = 0; i < A0.nsum; i++) {
         const uint64_t v = wsum64(valid ? (uint64_t)c.s[i] : 0);
         if (lane == 0) bs.red[w][1 + i] = v;
     }
-    if (A.has_ext)
+    if (A0.has_ext)
         for (int i = 0; i < 3; i++) {
             const uint32_t lo = wmin32(valid ? c.mn[i] : 0xFFFFFFFFu), hi = wmax32(valid ? c.mx[i] : 0u);
             if (lane == 0) { bs.red[w][1 + DCF_MAX_SUMS + i] = lo; bs.red[w][1 + DCF_MAX_SUMS + 3 + i] = hi; }
         }
     __syncthreads();
     if (threadIdx.x == 0) {
+        uint32_t anyb = 0;
+        for (int k = 0; k < FW; k++) anyb |= bs.dst[k];
+        const DevAttr &A = anyb ? B : A0;
         uint64_t t = 0;
         for (int k = 0; k < FW; k++) t += bs.red[k][0];
         atomicAdd(&A.area[R], (uint32_t)t);
@@ -1041,9 +1063,9 @@ __global__ __launch_bounds__(FB) void k_final_pixels(LevelArgs a, const uint32_t
                 else contrib_rec(A, p, c);
             }
         }
-        block_add(A, bs, fresh, r, c);
-        __syncthreads();
-        block_add(P, bs, old, r, c);
+        /* a root of this level is fresh for all its pixels, an old one held for all: one pass, the
+         * fresh contributions to A, the held ones pending in P */
+        block_add2(A, P, bs, valid, old, r, c);
         __syncthreads();
         block_canon_min(a, bs, old, r, p);
     }
@@ -1347,18 +1369,26 @@ static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAtt
     for (int i = 0; i < NLEV; i++) off[i + 1] = off[i] + h[i];
     DCF_HIP(ctx, hipMemcpyAsync(cursor, off, NLEV * 4, hipMemcpyHostToDevice, ctx->stream));
     ScatterGeo sg;
-    sg.tiled = la.bimg == nullptr;
+    sg.batch = la.bimg != nullptr;
+    sg.bimg = la.bimg;
+    sg.nbatch = la.nbatch;
     sg.W = la.g.W; sg.H = la.g.H; sg.D = la.g.D;
     const int *ts = sg.D > 1 ? ctx->lt3 : ctx->lt2, *gs = sg.D > 1 ? ctx->lg3 : ctx->lg2;
     sg.TW = ts[0]; sg.TH = ts[1]; sg.TD = ts[2];
     sg.GX = gs[0]; sg.GY = gs[1]; sg.GZ = gs[2];
+    if (la.bimg) {
+        /* a batch: every image's own zone tiles (their first indices are in the image table) */
+        const int tz = is3d ? ZONE_T3 : ZONE_T2;
+        sg.TW = tz; sg.TH = tz; sg.TD = is3d ? tz : 1;
+        sg.GX = sg.GY = sg.GZ = 1;
+    }
     sg.ntx = (sg.W + sg.TW - 1) / sg.TW;
     sg.nty = (sg.H + sg.TH - 1) / sg.TH;
     sg.ntz = (sg.D + sg.TD - 1) / sg.TD;
     sg.ngx = (sg.ntx + sg.GX - 1) / sg.GX;
     sg.ngy = (sg.nty + sg.GY - 1) / sg.GY;
     const int64_t ngz = (sg.ntz + sg.GZ - 1) / sg.GZ;
-    const int64_t sblocks = sg.tiled ? (int64_t)sg.ngx * sg.ngy * ngz * sg.GX * sg.GY * sg.GZ : (int64_t)nblk(n, 4096);
+    const int64_t sblocks = sg.batch ? batch_tiles : (int64_t)sg.ngx * sg.ngy * ngz * sg.GX * sg.GY * sg.GZ;
     if (zones) tgt = nullptr;
 #define DCF_SCATTER(NBV)                                                                                                    \
     DCF_LAUNCH(ctx, "k_lvl_scatter", k_lvl_scatter<NBV>, dim3((unsigned)sblocks), dim3(256), 0, la.iv, la.pass, la.shift, n, sg, \

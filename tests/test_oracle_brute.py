@@ -3,7 +3,7 @@ restate the oracle's own code:
 
 * the canonical parent array and node areas against connected components of every upper level set
   (scipy.ndimage.label), in 2D and 3D, for every connectivity, for max-trees and (by inversion)
-  min-trees;
+  min-trees, of integer and of float images (negative and fractional greys);
 * the filter rules: for an increasing attribute (area) the direct, min, max and subtractive rules
   coincide (a node passes iff all its ancestors pass), so a wrong sign, index or ancestor step in
   any one of them breaks the equality; the direct rule is also checked against its definition
@@ -33,6 +33,26 @@ def test_parent_and_area_match_brute_force(shape, conn, seed):
     img = rand_img(rng, shape, 6)
     bp, barea = brute_tree(img, conn)
     o = OracleTree(img, conn, "max", 0, threads=1)
+    assert np.array_equal(o.parent().reshape(-1).astype(np.int64), bp)
+    assert np.array_equal(o.check(0).reshape(-1), barea.astype(np.float32))
+
+
+@pytest.mark.parametrize("shape,conn", CASES_2D + CASES_3D)
+def test_float_images_match_brute_force(shape, conn):
+    """f32 images with negative and fractional greys: the oracle orders them through the reference's
+    float keys (flood.c:362-371), the brute force through numpy's float comparison; they agree for
+    every value but -0.0 / +0.0, which the keys tell apart and which the image avoids.  The min-tree
+    is the max-tree of gmax - g computed in float32 (preprocessing.c:84-91)."""
+    rng = np.random.default_rng(40 + conn + len(shape))
+    img = (rng.integers(-4, 5, size=shape) * 0.75 + rng.integers(0, 2, size=shape) * 0.125).astype(np.float32)
+    img[img == 0] = 0.0625
+    bp, barea = brute_tree(img, conn)
+    o = OracleTree(img, conn, "max", 0, threads=1)
+    assert np.array_equal(o.parent().reshape(-1).astype(np.int64), bp)
+    assert np.array_equal(o.check(0).reshape(-1), barea.astype(np.float32))
+    inv = (np.float32(img.max()) - img).astype(np.float32)
+    bp, barea = brute_tree(inv, conn)
+    o = OracleTree(img, conn, "min", 0, threads=1)
     assert np.array_equal(o.parent().reshape(-1).astype(np.int64), bp)
     assert np.array_equal(o.check(0).reshape(-1), barea.astype(np.float32))
 
