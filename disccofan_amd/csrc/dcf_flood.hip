@@ -785,53 +785,68 @@ __device__ __forceinline__ uint32_t wmax32(uint32_t v)
     return v;
 }
 
-/* one lane's contribution: count, up to DCF_MAX_SUMS sums, 3 minima and 3 maxima */
-struct Contrib {
+/* the sums and extents an attribute group accumulates (its record layout, as alloc_attr_n) */
+template <int G> struct GroupShape {
+    static constexpr int NS = G == 2 ? 3 : (G == 3 ? 4 : (G == 4 ? 9 : 0));
+    static constexpr bool EXT = G == 1;
+};
+
+/* one lane's contribution to a record of group G: count, the group's sums, its minima and maxima
+ * (sized at compile time, so it lives in registers) */
+template <int G> struct Contrib {
+    static constexpr int NS = GroupShape<G>::NS;
+    static constexpr bool EXT = GroupShape<G>::EXT;
     uint32_t area;
-    int64_t s[DCF_MAX_SUMS];
+    int64_t s[NS > 0 ? NS : 1];
     uint32_t mn[3], mx[3];
 };
 
-__device__ __forceinline__ void contrib_own(const LevelArgs &a, int grp, uint32_t p, Contrib &c)
+template <int G> __device__ __forceinline__ void contrib_own(const LevelArgs &a, uint32_t p, Contrib<G> &c)
 {
     c.area = 1;
-    if (grp == 0) return;
-    const PixGeo pg = pix_geo(a, p);
-    const int x = pg.x, y = pg.y, z = pg.z;
-    if (grp == 1) {
-        c.mn[0] = (uint32_t)x; c.mn[1] = (uint32_t)y; c.mn[2] = (uint32_t)z;
-        c.mx[0] = (uint32_t)x + 1; c.mx[1] = (uint32_t)y + 1; c.mx[2] = (uint32_t)z + 1;
-    } else if (grp == 2) {
-        c.s[0] = x; c.s[1] = y; c.s[2] = z;
-    } else if (grp == 3) {
-        int64_t gv = (int64_t)fl_key(a.iv, p);
-        c.s[0] = gv; c.s[1] = gv * x; c.s[2] = gv * y; c.s[3] = gv * z;
+    if constexpr (G == 0) {
+        return;
     } else {
-        c.s[0] = x; c.s[1] = y; c.s[2] = z;
-        c.s[3] = (int64_t)x * x; c.s[4] = (int64_t)y * y; c.s[5] = (int64_t)z * z;
-        c.s[6] = (int64_t)x * y; c.s[7] = (int64_t)y * z; c.s[8] = (int64_t)x * z;
+        const PixGeo pg = pix_geo(a, p);
+        const int x = pg.x, y = pg.y, z = pg.z;
+        if constexpr (G == 1) {
+            c.mn[0] = (uint32_t)x; c.mn[1] = (uint32_t)y; c.mn[2] = (uint32_t)z;
+            c.mx[0] = (uint32_t)x + 1; c.mx[1] = (uint32_t)y + 1; c.mx[2] = (uint32_t)z + 1;
+        } else if constexpr (G == 2) {
+            c.s[0] = x; c.s[1] = y; c.s[2] = z;
+        } else if constexpr (G == 3) {
+            int64_t gv = (int64_t)fl_key(a.iv, p);
+            c.s[0] = gv; c.s[1] = gv * x; c.s[2] = gv * y; c.s[3] = gv * z;
+        } else {
+            c.s[0] = x; c.s[1] = y; c.s[2] = z;
+            c.s[3] = (int64_t)x * x; c.s[4] = (int64_t)y * y; c.s[5] = (int64_t)z * z;
+            c.s[6] = (int64_t)x * y; c.s[7] = (int64_t)y * z; c.s[8] = (int64_t)x * z;
+        }
     }
 }
 
-__device__ __forceinline__ void contrib_rec(const DevAttr &B, uint32_t h, Contrib &c)
+template <int G> __device__ __forceinline__ void contrib_rec(const DevAttr &B, uint32_t h, Contrib<G> &c)
 {
     c.area = B.area[h];
-    for (int i = 0; i < B.nsum; i++) c.s[i] = B.s[i][h];
-    if (B.has_ext)
+#pragma unroll
+    for (int i = 0; i < Contrib<G>::NS; i++) c.s[i] = B.s[i][h];
+    if constexpr (Contrib<G>::EXT)
         for (int i = 0; i < 3; i++) { c.mn[i] = B.mn[i][h]; c.mx[i] = B.mx[i][h]; }
 }
 
-__device__ __forceinline__ void contrib_zero(Contrib &c)
+template <int G> __device__ __forceinline__ void contrib_zero(Contrib<G> &c)
 {
     c.area = 0;
-    for (int i = 0; i < DCF_MAX_SUMS; i++) c.s[i] = 0;
-    for (int i = 0; i < 3; i++) { c.mn[i] = 0xFFFFFFFFu; c.mx[i] = 0; }
+#pragma unroll
+    for (int i = 0; i < Contrib<G>::NS; i++) c.s[i] = 0;
+    if constexpr (Contrib<G>::EXT)
+        for (int i = 0; i < 3; i++) { c.mn[i] = 0xFFFFFFFFu; c.mx[i] = 0; }
 }
 
-/* adds every valid lane's contribution to record r of A, or of B for the lanes with toB (all lanes
- * of one root agree on it), one atomic per distinct root and field */
-__device__ void wave_add2(const DevAttr &A, const DevAttr &B, bool valid, bool toB, uint32_t r, const Contrib &c)
+/* adds every valid lane's contribution to record r of A, one atomic per distinct root and field */
+template <int G> __device__ void wave_add(const DevAttr &A, bool valid, uint32_t r, const Contrib<G> &c)
 {
+    constexpr int NS = Contrib<G>::NS;
     const int lane = threadIdx.x & 63;
     uint64_t pending = __ballot(valid);
     while (pending) {
@@ -843,48 +858,44 @@ __device__ void wave_add2(const DevAttr &A, const DevAttr &B, bool valid, bool t
             /* the leader's root is unique in the wave: the remaining roots are likely distinct too,
              * so every pending lane updates its own root directly */
             if ((pending >> lane) & 1ull) {
-                const DevAttr &D = toB ? B : A;
-                atomicAdd(&D.area[r], c.area);
-                for (int i = 0; i < D.nsum; i++)
-                    if (c.s[i]) atomicAdd((unsigned long long *)&D.s[i][r], (unsigned long long)c.s[i]);
-                if (D.has_ext)
-                    for (int i = 0; i < 3; i++) { atomicMin(&D.mn[i][r], c.mn[i]); atomicMax(&D.mx[i][r], c.mx[i]); }
+                atomicAdd(&A.area[r], c.area);
+#pragma unroll
+                for (int i = 0; i < NS; i++)
+                    if (c.s[i]) atomicAdd((unsigned long long *)&A.s[i][r], (unsigned long long)c.s[i]);
+                if constexpr (Contrib<G>::EXT)
+                    for (int i = 0; i < 3; i++) { atomicMin(&A.mn[i][r], c.mn[i]); atomicMax(&A.mx[i][r], c.mx[i]); }
             }
             break;
         }
-        const DevAttr &D = __shfl((int)toB, leader, 64) ? B : A;
         uint64_t area = wsum64(mine ? c.area : 0);
-        if (lane == leader) atomicAdd(&D.area[rl], (uint32_t)area);
-        for (int i = 0; i < D.nsum; i++) {
+        if (lane == leader) atomicAdd(&A.area[rl], (uint32_t)area);
+#pragma unroll
+        for (int i = 0; i < NS; i++) {
             uint64_t v = wsum64(mine ? (uint64_t)c.s[i] : 0);
-            if (lane == leader && v) atomicAdd((unsigned long long *)&D.s[i][rl], (unsigned long long)v);
+            if (lane == leader && v) atomicAdd((unsigned long long *)&A.s[i][rl], (unsigned long long)v);
         }
-        if (D.has_ext)
+        if constexpr (Contrib<G>::EXT)
             for (int i = 0; i < 3; i++) {
                 uint32_t lo = wmin32(mine ? c.mn[i] : 0xFFFFFFFFu), hi = wmax32(mine ? c.mx[i] : 0u);
-                if (lane == leader) { atomicMin(&D.mn[i][rl], lo); atomicMax(&D.mx[i][rl], hi); }
+                if (lane == leader) { atomicMin(&A.mn[i][rl], lo); atomicMax(&A.mx[i][rl], hi); }
             }
         pending &= ~m;
     }
 }
 
-__device__ __forceinline__ void wave_add(const DevAttr &A, bool valid, uint32_t r, const Contrib &c)
-{
-    wave_add2(A, A, valid, false, r, c);
-}
-
 /* zone pass: the coordinate / grey sums and extents of every zone into its representative's record
  * (the pixel counts come from k_tile_zones) */
-__global__ __launch_bounds__(256) void k_zone_totals(LevelArgs a, int grp, const uint32_t *__restrict__ zrep, DevAttr A, int64_t n)
+template <int G>
+__global__ __launch_bounds__(256) void k_zone_totals(LevelArgs a, const uint32_t *__restrict__ zrep, DevAttr A, int64_t n)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = i < n;
     uint32_t r = 0;
-    Contrib c;
+    Contrib<G> c;
     contrib_zero(c);
     if (valid) {
         r = zrep[i];
-        contrib_own(a, grp, (uint32_t)i, c);
+        contrib_own(a, (uint32_t)i, c);
         c.area = 0;
     }
     wave_add(A, valid, r, c);
@@ -919,7 +930,6 @@ __global__ __launch_bounds__(256) void k_zone_parents(ImgView iv, int pass, int 
 
 struct BlockScratch {
     uint32_t root[FW];
-    uint32_t dst[FW];
     uint64_t red[FW][1 + DCF_MAX_SUMS + 6];
 };
 
@@ -945,44 +955,41 @@ __device__ __forceinline__ uint32_t block_common_root(BlockScratch &bs, bool val
     return R;
 }
 
-/* block-wide wave_add2: one atomic per field when every valid lane of the block shares one root */
-__device__ void block_add2(const DevAttr &A0, const DevAttr &B, BlockScratch &bs, bool valid, bool toB, uint32_t r,
-                           const Contrib &c)
+/* block-wide wave_add: one atomic per field when every valid lane of the block shares one root */
+template <int G> __device__ void block_add(const DevAttr &A, BlockScratch &bs, bool valid, uint32_t r, const Contrib<G> &c)
 {
+    constexpr int NS = Contrib<G>::NS;
     const uint32_t R = block_common_root(bs, valid, r);
     if (R == R_EMPTY) return;
     if (R == R_MIXED) {
-        wave_add2(A0, B, valid, toB, r, c);
+        wave_add(A, valid, r, c);
         return;
     }
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const uint64_t tb = __ballot(valid && toB);
-    if (lane == 0) bs.dst[w] = tb ? 1u : 0u;
     const uint64_t area = wsum64(valid ? c.area : 0);
     if (lane == 0) bs.red[w][0] = area;
-    for (int i = 0; i < A0.nsum; i++) {
+#pragma unroll
+    for (int i = 0; i < NS; i++) {
         const uint64_t v = wsum64(valid ? (uint64_t)c.s[i] : 0);
         if (lane == 0) bs.red[w][1 + i] = v;
     }
-    if (A0.has_ext)
+    if constexpr (Contrib<G>::EXT)
         for (int i = 0; i < 3; i++) {
             const uint32_t lo = wmin32(valid ? c.mn[i] : 0xFFFFFFFFu), hi = wmax32(valid ? c.mx[i] : 0u);
             if (lane == 0) { bs.red[w][1 + DCF_MAX_SUMS + i] = lo; bs.red[w][1 + DCF_MAX_SUMS + 3 + i] = hi; }
         }
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint32_t anyb = 0;
-        for (int k = 0; k < FW; k++) anyb |= bs.dst[k];
-        const DevAttr &A = anyb ? B : A0;
         uint64_t t = 0;
         for (int k = 0; k < FW; k++) t += bs.red[k][0];
         atomicAdd(&A.area[R], (uint32_t)t);
-        for (int i = 0; i < A.nsum; i++) {
+#pragma unroll
+        for (int i = 0; i < NS; i++) {
             uint64_t v = 0;
             for (int k = 0; k < FW; k++) v += bs.red[k][1 + i];
             if (v) atomicAdd((unsigned long long *)&A.s[i][R], (unsigned long long)v);
         }
-        if (A.has_ext)
+        if constexpr (Contrib<G>::EXT)
             for (int i = 0; i < 3; i++) {
                 uint32_t lo = 0xFFFFFFFFu, hi = 0;
                 for (int k = 0; k < FW; k++) {
@@ -1035,8 +1042,9 @@ __device__ void block_canon_min(const LevelArgs &a, BlockScratch &bs, bool old, 
  * A set still rooted at an old root r gets its canonical levelroot computed here (minimum pixel of
  * the level, one atomicMin per wave and set; the first to touch a set queues r for k_fix); p's
  * contribution is kept pending at r (P) and its parent is left as HELD(r). */
+template <int G>
 __global__ __launch_bounds__(FB) void k_final_pixels(LevelArgs a, const uint32_t *__restrict__ lvl, uint32_t cnt, uint32_t lev,
-                                                     DevAttr A, DevAttr S, DevAttr P, int grp, int32_t *__restrict__ par,
+                                                     DevAttr A, DevAttr S, DevAttr P, int32_t *__restrict__ par,
                                                      int32_t *__restrict__ spar, uint32_t *__restrict__ marks)
 {
     __shared__ BlockScratch bs;
@@ -1045,7 +1053,7 @@ __global__ __launch_bounds__(FB) void k_final_pixels(LevelArgs a, const uint32_t
         const bool valid = i < cnt;
         uint32_t p = 0, r = 0;
         bool fresh = false, old = false;
-        Contrib c;
+        Contrib<G> c;
         contrib_zero(c);
         if (valid) {
             p = lvl[i];
@@ -1055,17 +1063,17 @@ __global__ __launch_bounds__(FB) void k_final_pixels(LevelArgs a, const uint32_t
             old = !fresh;
             if (fresh) {
                 if (r != p) par[p] = (int32_t)r;
-                if (!a.zone) contrib_own(a, grp, p, c);
+                if (!a.zone) contrib_own(a, p, c);
                 else if (r != p) contrib_rec(A, p, c); /* a root already holds its own zone's totals */
             } else {
                 par[p] = HELD(r);
-                if (!a.zone) contrib_own(a, grp, p, c);
+                if (!a.zone) contrib_own(a, p, c);
                 else contrib_rec(A, p, c);
             }
         }
-        /* a root of this level is fresh for all its pixels, an old one held for all: one pass, the
-         * fresh contributions to A, the held ones pending in P */
-        block_add2(A, P, bs, valid, old, r, c);
+        block_add(A, bs, fresh, r, c);
+        __syncthreads();
+        block_add(P, bs, old, r, c);
         __syncthreads();
         block_canon_min(a, bs, old, r, p);
     }
@@ -1079,7 +1087,7 @@ __global__ __launch_bounds__(FB) void k_final_pixels(LevelArgs a, const uint32_t
         const uint32_t j = base + threadIdx.x;
         const bool valid = j < end;
         uint32_t r = 0;
-        Contrib c;
+        Contrib<G> c;
         contrib_zero(c);
         if (valid) {
             const uint32_t h = a.absorbed[j];
@@ -1340,7 +1348,12 @@ static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAtt
         DCF_LAUNCH(ctx, "k_tile_zones", k_tile_zones, dim3((unsigned)ntiles), dim3(256), 0, la.iv, la.shift, tg, la.nb, zrep, la.uf,
                    A.area);
         if (grp != 0)
-            DCF_LAUNCH(ctx, "k_zone_totals", k_zone_totals, dim3(nblk(n, 256)), dim3(256), 0, la, grp, (const uint32_t *)zrep, A, n);
+            switch (grp) {
+            case 1: DCF_LAUNCH(ctx, "k_zone_totals", k_zone_totals<1>, dim3(nblk(n, 256)), dim3(256), 0, la, (const uint32_t *)zrep, A, n); break;
+            case 2: DCF_LAUNCH(ctx, "k_zone_totals", k_zone_totals<2>, dim3(nblk(n, 256)), dim3(256), 0, la, (const uint32_t *)zrep, A, n); break;
+            case 3: DCF_LAUNCH(ctx, "k_zone_totals", k_zone_totals<3>, dim3(nblk(n, 256)), dim3(256), 0, la, (const uint32_t *)zrep, A, n); break;
+            default: DCF_LAUNCH(ctx, "k_zone_totals", k_zone_totals<4>, dim3(nblk(n, 256)), dim3(256), 0, la, (const uint32_t *)zrep, A, n); break;
+            }
         DCF_LAUNCH(ctx, "k_lvl_hist", k_lvl_hist, dim3(hb), dim3(256), 0, la.iv, la.pass, la.shift, n, (const uint32_t *)nullptr,
                    hist_all);
     }
@@ -1467,8 +1480,17 @@ static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAtt
         unsigned rb = nblk(cnt, FB), ab = nblk((int64_t)cnt * la.nb.n / 4, FB);
         if (ab > 1024) ab = 1024;
         if (ab > rb) rb = ab;
-        DCF_LAUNCH(ctx, "k_final_pixels", k_final_pixels, dim3(rb), dim3(FB), 0, la, lvl, cnt, (uint32_t)lev, A, S, P, grp, par,
-                   spar, nabs + 1 + ord);
+#define DCF_FINAL(GV)                                                                                                        \
+    DCF_LAUNCH(ctx, "k_final_pixels", k_final_pixels<GV>, dim3(rb), dim3(FB), 0, la, lvl, cnt, (uint32_t)lev, A, S, P, par, spar, \
+               nabs + 1 + ord)
+        switch (grp) {
+        case 0: DCF_FINAL(0); break;
+        case 1: DCF_FINAL(1); break;
+        case 2: DCF_FINAL(2); break;
+        case 3: DCF_FINAL(3); break;
+        default: DCF_FINAL(4); break;
+        }
+#undef DCF_FINAL
         unsigned fb = nblk(cnt, 256);
         if (fb > 1024) fb = 1024;
         DCF_LAUNCH(ctx, "k_fix", k_fix, dim3(fb), dim3(256), 0, la, (uint32_t)ord, (uint32_t)lev, A, S, P, par, spar);
