@@ -2,8 +2,9 @@
 
 Workload (default): 8192x8192 u16 value-noise + ellipses image, 8-connectivity, the max-tree AND the
 min-tree, area attribute, area opening lambda=1000 with the subtractive rule.  One step = build
-+ filter of both trees (2 x 67.1 Mvoxel); the image and the output buffers are resident in HBM
-before the timed region.  `--workload spheres512` runs the 512^3 u8 volume (6-connectivity,
++ filter of both trees (2 x 67.1 Mvoxel), one build + filter call per tree (--paired times both trees
+in one batched call instead, with a tree type per image); the image and the output buffers are
+resident in HBM before the timed region.  `--workload spheres512` runs the 512^3 u8 volume (6-connectivity,
 max-tree, direct rule, lambda=500) instead.
 
 Printed: ONE JSON line with the metric, a roofline object for the dominant kernel (per-launch
@@ -121,7 +122,10 @@ def main():
                     help="only run a plain device-to-device copy of MIB MiB 5 times (the reference point for the "
                          "FETCH_SIZE / WRITE_SIZE counters) and print its bandwidth")
     ap.add_argument("--no-plain", action="store_true",
-                    help="batched workload: skip the per-image comparison loop (plain_form null)")
+                    help="batched forms: skip the per-image / per-tree comparison loop (plain_form null)")
+    ap.add_argument("--paired", action="store_true",
+                    help="a workload with a max-tree and a min-tree: time one batched call for both trees (a tree "
+                         "type per image) instead of one build + filter call per tree, with the per-tree form beside it")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -151,6 +155,7 @@ def main():
 
     w = WORKLOADS[args.workload]
     batch = w.get("kind") == "batch"
+    pair = not batch and len(w["trees"]) > 1 and args.paired
     dev = torch.device("cuda:0")
     ctx = dcf.Context(0)
     stream = torch.cuda.current_stream()
@@ -187,17 +192,29 @@ def main():
         passes = 2 if B == 2 else 1
         tdt = torch.int16 if B == 2 else torch.uint8
         d_img = torch.from_numpy(img.reshape(-1).view(np.int16 if B == 2 else np.uint8)).to(dev)
-        d_out = torch.empty(n, dtype=tdt, device=dev)
+        d_outs = [torch.empty(n, dtype=tdt, device=dev) for _ in w["trees"]]
         torch.cuda.synchronize()
         np_dtype = np.uint16 if B == 2 else np.uint8
         shape_desc = list(img.shape)
 
-        def step():
-            for tt in w["trees"]:
+        def plain_step():
+            for tt, d_out in zip(w["trees"], d_outs):
                 tr = ctx.build_device(d_img.data_ptr(), np_dtype, img.shape, w["conn"], tt, w["attr"])
                 tr.filter_device(w["attr"], w["lam"], w["decision"], d_out.data_ptr(), np_dtype)
                 state["K"] = tr.num_nodes
                 tr.free()
+
+        if pair:
+            # the trees of one image in one batched call (dcf_batch_filter with a tree type per image)
+            items = [(d_img.data_ptr(), np_dtype, img.shape, w["lam"], d_out.data_ptr(), tt)
+                     for tt, d_out in zip(w["trees"], d_outs)]
+
+            def step():
+                ctx.batch_filter_device(items, connectivity=w["conn"], attr_idx=w["attr"], decision=w["decision"])
+
+            plain_step()  # node count of the trees, for the filter kernels' algorithmic bytes
+        else:
+            step = plain_step
 
     for i in range(args.warmup):
         t1 = time.perf_counter()
@@ -268,16 +285,17 @@ def main():
         print("nodes K = %d (%.1f%% of pixels)" % (K, 100.0 * K / n), file=sys.stderr)
 
     plain = None
-    if batch and not args.no_plain:
-        # the plain form on the same device-resident images: one build + filter call per image
+    if (batch or pair) and not args.no_plain:
+        # the plain form on the same device-resident data: one build + filter call per image / tree
         plain_step()
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         plain_step()
         torch.cuda.synchronize()
         pms = (time.perf_counter() - t1) * 1e3
-        plain = {"value": round(n / (pms / 1e3) / 1e6, 3), "ms_per_step": round(pms, 3),
-                 "form": "one dcf_tree_build + dcf_filter per image, same images and lambdas, 1 timed step"}
+        plain = {"value": round(vox_per_step / (pms / 1e3) / 1e6, 3), "ms_per_step": round(pms, 3),
+                 "form": ("one dcf_tree_build + dcf_filter per image, same images and lambdas, 1 timed step" if batch
+                          else "one dcf_tree_build + dcf_filter per tree, 1 timed step")}
         print("plain per-image loop: %.1f ms (%.1f Mvox/s); batched %.1f ms" % (pms, plain["value"], ms_per_step),
               file=sys.stderr)
 
@@ -302,6 +320,9 @@ def main():
         "config": {"workload": w["desc"], "shape": shape_desc, "voxels_per_step": vox_per_step,
                    "connectivity": w["conn"], "trees": list(w["trees"]), "attribute": w["attr"],
                    "lambda": "per image" if batch else w["lam"], "decision": w["decision"],
+                   "form": ("batched: one dcf_batch_filter call for the whole batch" if batch else
+                            "batched: both trees of the image in one dcf_batch_filter call" if pair else
+                            "plain: one dcf_tree_build + dcf_filter call per tree"),
                    "bit_exact_vs_oracle": "tests/test_gpu_batch.py" if batch else "tests/test_gpu_fullsize.py"},
         "roofline": roof,
         "cpu_baseline": cpu,

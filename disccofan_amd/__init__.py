@@ -58,7 +58,20 @@ class DcfError(RuntimeError):
 class BatchImage(C.Structure):
     """dcf_batch_image of include/dcf.h"""
     _fields_ = [("img", C.c_void_p), ("dtype", C.c_int), ("dims", C.c_uint64 * 3), ("lam", C.c_double),
-                ("out", C.c_void_p)]
+                ("out", C.c_void_p), ("flags", C.c_uint)]
+
+
+DCF_IMAGE_MAX_TREE = 0x1
+DCF_IMAGE_MIN_TREE = 0x2
+
+
+def _image_flags(tree_type) -> int:
+    """per-image tree type of a batch item: None = the call's"""
+    if tree_type is None:
+        return 0
+    if tree_type not in ("max", "min"):
+        raise ValueError("tree_type must be 'max' or 'min'")
+    return DCF_IMAGE_MIN_TREE if tree_type == "min" else DCF_IMAGE_MAX_TREE
 
 
 _lib = None
@@ -248,31 +261,36 @@ class Context:
         return t
 
     def batch_filter(self, images, lambdas, connectivity: int = 4, tree_type: str = "max", attr_idx: int = 0,
-                     decision: int = 0, extract: bool = False, all_rules: bool = False):
+                     decision: int = 0, extract: bool = False, all_rules: bool = False, tree_types=None):
         """The batched form: every host image (u8 or u16, all 2D or all 3D) gets its own tree and is
-        filtered with its own lambda, all in one build over the batch.  Returns the outputs."""
+        filtered with its own lambda, all in one build over the batch.  tree_types: an optional
+        per-image 'max' / 'min' (None entries take tree_type).  Returns the outputs."""
         arrs = [np.ascontiguousarray(a) for a in images]
         outs = [np.empty_like(a) for a in arrs]
         items = (BatchImage * len(arrs))()
+        tts = list(tree_types) if tree_types is not None else [None] * len(arrs)
         for k, (a, o, lam) in enumerate(zip(arrs, outs, lambdas)):
             items[k].img = a.ctypes.data
             items[k].dtype = _dtype_code(a.dtype)
             items[k].dims = (C.c_uint64 * 3)(*_dims_of(a.shape))
             items[k].lam = float(lam)
             items[k].out = o.ctypes.data
+            items[k].flags = _image_flags(tts[k])
         self._batch(items, len(arrs), connectivity, tree_type, attr_idx, decision, extract, all_rules, 0)
         return outs
 
     def batch_filter_device(self, items, connectivity: int = 4, tree_type: str = "max", attr_idx: int = 0,
                             decision: int = 0, extract: bool = False, all_rules: bool = False):
-        """Same on device buffers: items = [(img_ptr, dtype, shape, lam, out_ptr), ...]."""
+        """Same on device buffers: items = [(img_ptr, dtype, shape, lam, out_ptr[, tree_type]), ...]."""
         arr = (BatchImage * len(items))()
-        for k, (ip, dt, shape, lam, op) in enumerate(items):
+        for k, it in enumerate(items):
+            ip, dt, shape, lam, op = it[:5]
             arr[k].img = ip
             arr[k].dtype = _dtype_code(dt)
             arr[k].dims = (C.c_uint64 * 3)(*_dims_of(shape))
             arr[k].lam = float(lam)
             arr[k].out = op
+            arr[k].flags = _image_flags(it[5] if len(it) > 5 else None)
         self._batch(arr, len(items), connectivity, tree_type, attr_idx, decision, extract, all_rules, DCF_DEVICE_PTR)
 
     def _batch(self, arr, count, connectivity, tree_type, attr_idx, decision, extract, all_rules, flags):

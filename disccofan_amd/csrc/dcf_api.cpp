@@ -378,6 +378,7 @@ static int batch_filter_same(dcf_ctx *ctx, const dcf_batch_image *images, int co
     std::vector<double> lam((size_t)count);
     uint64_t N = 0, tiles = 0;
     int is3d = -1, all_u8 = 1;
+    uint32_t any_min = 0;
     size_t max_u8 = 0;
     for (int i = 0; i < count; i++) {
         const dcf_batch_image &b = images[i];
@@ -395,6 +396,9 @@ static int batch_filter_same(dcf_ctx *ctx, const dcf_batch_image *images, int co
         tab[i].H = (uint32_t)b.dims[1];
         tab[i].D = (uint32_t)b.dims[2];
         tab[i].tile0 = (uint32_t)tiles;
+        tab[i].inv = (b.flags & DCF_IMAGE_MIN_TREE) ? 1u : ((b.flags & DCF_IMAGE_MAX_TREE) ? 0u : (tree_type == DCF_MIN_TREE));
+        tab[i].pad = 0;
+        any_min |= tab[i].inv;
         tiles += d3 ? ((b.dims[0] + ZONE_T3 - 1) / ZONE_T3) * ((b.dims[1] + ZONE_T3 - 1) / ZONE_T3) * ((b.dims[2] + ZONE_T3 - 1) / ZONE_T3)
                     : ((b.dims[0] + ZONE_T2 - 1) / ZONE_T2) * ((b.dims[1] + ZONE_T2 - 1) / ZONE_T2);
         lam[i] = b.lambda;
@@ -474,8 +478,8 @@ static int batch_filter_same(dcf_ctx *ctx, const dcf_batch_image *images, int co
         }
     }
     if (rc != DCF_OK) goto done;
-    /* min-trees: every image inverted against its own grey maximum (preprocessing.c:7-30,84-91) */
-    if (tree_type == DCF_MIN_TREE) {
+    /* min-trees: every min-tree image inverted against its own grey maximum (preprocessing.c:7-30,84-91) */
+    if (any_min) {
         gm = (uint32_t *)dcf_alloc(ctx, (size_t)count * 4);
         if (!gm) { rc = dcf_set_error(ctx, DCF_ENOMEM, "batch grey maxima"); goto done; }
         if ((rc = dcf_batch_invert(ctx, t->img, cdt, dtab, count, gm, (int64_t)N)) != DCF_OK) goto done;
@@ -543,6 +547,8 @@ extern "C" int dcf_batch_filter(dcf_ctx *ctx, const dcf_batch_image *images, int
         else if (is3d != d3) return dcf_set_error(ctx, DCF_EINVAL, "image %d: a batch is all 2D or all 3D", i);
         N += b.dims[0] * b.dims[1] * b.dims[2];
         if (N >= (1ull << 30)) return dcf_set_error(ctx, DCF_EUNSUPPORTED, "a batch is limited to 2^30 pixels");
+        if ((b.flags & ~(DCF_IMAGE_MAX_TREE | DCF_IMAGE_MIN_TREE)) || b.flags == (DCF_IMAGE_MAX_TREE | DCF_IMAGE_MIN_TREE))
+            return dcf_set_error(ctx, DCF_EINVAL, "image %d: bad flags 0x%x", i, b.flags);
     }
     /* u8 and u16 images in separate trees: a u8 image on 16-bit keys would spend a whole first pass
      * on one coarse level */

@@ -87,6 +87,8 @@ struct BatchImg {
     uint64_t off;     /* first pixel of the image */
     uint32_t W, H, D; /* its dimensions */
     uint32_t tile0;   /* its first tile in the batch's zone tiles (ZONE_T2^2 / ZONE_T3^3 tiles) */
+    uint32_t inv;     /* 1: a min-tree image, stored inverted against its own grey maximum */
+    uint32_t pad;
 };
 
 /* zone tiles of the first pass: 64x64 in 2D, 16x16x16 in 3D */

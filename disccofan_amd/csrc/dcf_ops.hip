@@ -365,7 +365,7 @@ __global__ __launch_bounds__(256) void k_batch_out(ImgView iv, const BatchImg *_
     const uint64_t local = (uint64_t)p - bimg[i].off;
     float f = val[p2n[p]];
     if (extract) f = px_grey_ops(iv, p) - f;
-    if (gmax_img) f = (float)gmax_img[i] - f;
+    if (gmax_img && bimg[i].inv) f = (float)gmax_img[i] - f;
     if (o.dtype == DCF_F32) ((float *)o.ptr)[local] = f;
     else if (o.dtype == DCF_U16) ((uint16_t *)o.ptr)[local] = (uint16_t)f;
     else ((uint8_t *)o.ptr)[local] = (uint8_t)f;
@@ -786,7 +786,9 @@ __global__ __launch_bounds__(256) void k_batch_invert(void *__restrict__ cat, in
 {
     int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
-    const uint32_t g = gmax_img[batch_locate(bimg, nb, (uint64_t)p)];
+    const int i = batch_locate(bimg, nb, (uint64_t)p);
+    if (!bimg[i].inv) return;
+    const uint32_t g = gmax_img[i];
     if (dtype == DCF_U8) ((uint8_t *)cat)[p] = (uint8_t)(g - ((uint8_t *)cat)[p]);
     else ((uint16_t *)cat)[p] = (uint16_t)(g - ((uint16_t *)cat)[p]);
 }

@@ -153,14 +153,19 @@ const char *dcf_decision_name(int decision);
  * max-tree or min-tree (no pixel of one image touches another) and all of them are built by one
  * level loop over the whole batch, then filtered with their own lambda.  Images must all be 2D or
  * all be 3D; u8 and u16 may be mixed; out receives the image's own dtype.  The batch holds at most
- * 2^30 pixels.  flags: DCF_DEVICE_PTR (img and out are device pointers), DCF_ALL_RULES. */
+ * 2^30 pixels.  flags: DCF_DEVICE_PTR (img and out are device pointers), DCF_ALL_RULES.
+ * tree_type is every image's tree unless the image's own flags name one (DCF_IMAGE_MAX_TREE /
+ * DCF_IMAGE_MIN_TREE): the max-tree and the min-tree of one image can share a call. */
 typedef struct {
     const void *img;  /* dims[0]*dims[1]*dims[2] values of dtype, x fastest */
     dcf_dtype dtype;  /* DCF_U8 or DCF_U16 */
     uint64_t dims[3];
     double lambda;    /* this image's lambda */
     void *out;        /* filtered (or extracted) image, same dtype and size */
+    unsigned flags;   /* 0: the call's tree type; DCF_IMAGE_MAX_TREE or DCF_IMAGE_MIN_TREE: this one */
 } dcf_batch_image;
+#define DCF_IMAGE_MAX_TREE 0x1u
+#define DCF_IMAGE_MIN_TREE 0x2u
 int dcf_batch_filter(dcf_ctx *ctx, const dcf_batch_image *images, int count, int connectivity, int tree_type, int attr_idx,
                      int decision, int extract, unsigned flags);
 

@@ -100,6 +100,31 @@ def test_uneven_workload_batch(ctx):
     check_batch(ctx, imgs, lams, 4, "max", label="workload")
 
 
+@pytest.mark.parametrize("conn", [4, 8])
+def test_per_image_tree_types(ctx, conn):
+    """every image with its own tree type (DCF_IMAGE_MAX_TREE / DCF_IMAGE_MIN_TREE) in one call,
+    u8 and u16 mixed, against the oracle's tree of that type for the image alone"""
+    rng = np.random.default_rng(310 + conn)
+    imgs, lams = random_batch(rng, 12)
+    tts = [("min" if rng.random() < 0.5 else "max") for _ in imgs]
+    for dec in (0, 3):
+        say("per-image tree types", conn, "decision", dec, tts)
+        outs = ctx.batch_filter(imgs, lams, connectivity=conn, tree_type="max", decision=dec, tree_types=tts)
+        for k, (img, lam, tt, out) in enumerate(zip(imgs, lams, tts, outs)):
+            want = OracleTree(img, conn, tt, 0).filter(0, lam, dec).astype(img.dtype)
+            bad = np.nonzero(out.reshape(-1) != want.reshape(-1))[0]
+            assert len(bad) == 0, "image %d %s %s: %d pixels differ" % (k, img.shape, tt, len(bad))
+
+
+def test_max_and_min_tree_of_one_image(ctx):
+    """the two trees of one u16 image in one call (the pair the 8192^2 workload builds), subtractive rule"""
+    img = synth.vnoise(300, 420)
+    outs = ctx.batch_filter([img, img], [1000.0, 1000.0], connectivity=8, decision=3, tree_types=["max", "min"])
+    for tt, out in zip(("max", "min"), outs):
+        want = OracleTree(img, 8, tt, 0).filter(0, 1000.0, 3).astype(np.uint16)
+        assert np.array_equal(out, want), tt
+
+
 def test_device_pointers(ctx):
     import torch
     rng = np.random.default_rng(97)
@@ -128,3 +153,13 @@ def test_batch_errors(ctx):
     with pytest.raises(dcf.DcfError) as e:
         ctx.batch_filter([], [])
     assert e.value.code == -1
+    # an image's flags may name one tree type, not both
+    items = (dcf.BatchImage * 1)()
+    out = np.empty_like(a2)
+    items[0].img = a2.ctypes.data
+    items[0].dtype = 0
+    items[0].dims = (dcf.C.c_uint64 * 3)(6, 5, 1)
+    items[0].lam = 1.0
+    items[0].out = out.ctypes.data
+    items[0].flags = dcf.DCF_IMAGE_MAX_TREE | dcf.DCF_IMAGE_MIN_TREE
+    assert dcf.lib().dcf_batch_filter(ctx._h, items, 1, 4, 0, 0, 0, 0, 0) == -1
