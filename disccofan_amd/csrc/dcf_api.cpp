@@ -174,6 +174,7 @@ extern "C" int dcf_ctx_create(int device, unsigned flags, dcf_ctx **out)
     };
     if (const char *z = getenv("DCF_ZONES")) c->zones = atoi(z);
     if (const char *z = getenv("DCF_TARGETS")) c->targets = atoi(z);
+    if (const char *z = getenv("DCF_RECORDS")) c->records = atoi(z);
     shape3("DCF_LEVEL_TILE2D", c->lt2);
     shape3("DCF_LEVEL_GROUP2D", c->lg2);
     shape3("DCF_LEVEL_TILE3D", c->lt3);

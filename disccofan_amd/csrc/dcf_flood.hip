@@ -76,29 +76,32 @@ struct Geo {
 #define UF_ROOT 0x80000000u
 #define UF_LVL 0x1FFu
 
-__device__ __forceinline__ uint32_t ld(const uint32_t *a, uint32_t i)
-{
-    return __hip_atomic_load(&a[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
+/* the union-find entries: element x's entry is p[x * s].  s = 1 in a first pass; in a refining pass
+ * every element has a 16-byte record {entry, key, coarse node, 0}, so that one line of a neighbour
+ * row holds all three for the k_union neighbour scan (s = 4). */
+struct UFV {
+    uint32_t *p;
+    uint32_t s;
+};
 
 /* workgroup-scope load: may be served by the CU's L1 and be stale.  Every value an entry ever holds
  * is an ancestor of the element (links only turn a root into a child, compression only moves a
  * pointer up), so a stale read still leads towards the root; a stale root tag is caught by the
  * link's CAS, which is performed in L2 and returns the fresh parent.  The root of a big component
  * is read by every thread that touches it, and this keeps those reads in the CUs' L1s. */
-__device__ __forceinline__ uint32_t ldc(const uint32_t *a, uint32_t i)
+__device__ __forceinline__ uint32_t ldc(const UFV &u, uint32_t x)
 {
-    return __hip_atomic_load(&a[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    return __hip_atomic_load(&u.p[(size_t)x * u.s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-__device__ __forceinline__ void st(uint32_t *a, uint32_t i, uint32_t v)
+__device__ __forceinline__ void st(const UFV &u, uint32_t x, uint32_t v)
 {
-    __hip_atomic_store(&a[i], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&u.p[(size_t)x * u.s], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 /* root of x with path splitting; tag = the root's entry.  Only non-root entries are rewritten, and
  * only ever to an ancestor, so these stores cannot race with a link (links CAS root tags). */
-__device__ __forceinline__ uint32_t uf_find(uint32_t *uf, uint32_t x, uint32_t &tag)
+__device__ __forceinline__ uint32_t uf_find(const UFV &uf, uint32_t x, uint32_t &tag)
 {
     uint32_t p = ldc(uf, x);
     if (p & UF_ROOT) {
@@ -118,7 +121,7 @@ __device__ __forceinline__ uint32_t uf_find(uint32_t *uf, uint32_t x, uint32_t &
 }
 
 /* the root reached from x whose (fresh) parent is known to be p */
-__device__ __forceinline__ uint32_t uf_find_from(uint32_t *uf, uint32_t x, uint32_t p, uint32_t &tag)
+__device__ __forceinline__ uint32_t uf_find_from(const UFV &uf, uint32_t x, uint32_t p, uint32_t &tag)
 {
     for (;;) {
         const uint32_t gp = ldc(uf, p);
@@ -146,7 +149,7 @@ struct AbsBuf {
  * recorded.  The canonical levelroot of a set that contains old roots (its minimum-index pixel of
  * level `lev`) is installed as its root after the unions (k_final_pixels, k_fix).  Returns the
  * surviving root (possibly stale) and its tag. */
-__device__ __forceinline__ uint32_t uf_unite(uint32_t *uf, uint32_t a, uint32_t ta, uint32_t b, uint32_t tb, uint32_t lev,
+__device__ __forceinline__ uint32_t uf_unite(const UFV &uf, uint32_t a, uint32_t ta, uint32_t b, uint32_t tb, uint32_t lev,
                                              const AbsBuf &ab, uint32_t &tout)
 {
     for (;;) {
@@ -162,7 +165,7 @@ __device__ __forceinline__ uint32_t uf_unite(uint32_t *uf, uint32_t a, uint32_t 
         uint32_t hi = a_hi ? a : b, lo = a_hi ? b : a;
         uint32_t th = a_hi ? ta : tb, tl = a_hi ? tb : ta;
         uint32_t exp = th;
-        if (__hip_atomic_compare_exchange_strong(&uf[hi], &exp, lo, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+        if (__hip_atomic_compare_exchange_strong(&uf.p[(size_t)hi * uf.s], &exp, lo, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                                  __HIP_MEMORY_SCOPE_AGENT)) {
             if ((th & UF_LVL) > lev) ab.buf[atomicAdd(ab.n, 1u)] = hi;
             tout = tl;
@@ -274,7 +277,7 @@ struct LevelArgs {
     Geo g;
     Nbr nb;
     int pass, shift;
-    uint32_t *uf;
+    UFV uf;
     uint32_t *absorbed;
     uint32_t *nabs;        /* [0] running count */
     uint32_t *canon;       /* per element: minimum pixel of the current level in its set (sets rooted at an old root) */
@@ -343,16 +346,17 @@ __device__ __forceinline__ void pixel_targets(const LevelArgs &a, uint32_t p, ui
     const uint32_t NONE = 0xFFFFFFFFu;
     const PixGeo pg = pix_geo(a, p);
     const int WH = pg.W * pg.H;
-    const uint32_t kp = fl_key(a.iv, p);
+    const bool rec = a.uf.s == 4; /* key and coarse node in the union-find record */
+    const uint32_t kp = rec ? a.uf.p[(size_t)p * 4 + 1] : fl_key(a.iv, p);
     const uint32_t cp = a.pass == 2 ? kp >> (a.shift + 8) : 0u;
-    const uint32_t nid = a.pass == 2 ? a.cnode[p] : 0u;
+    const uint32_t nid = a.pass == 2 ? (rec ? a.uf.p[(size_t)p * 4 + 2] : a.cnode[p]) : 0u;
 #pragma unroll
     for (int k = 0; k < NB; k++) {
         t[k] = NONE;
         const int nx = pg.x + a.nb.dx[k], ny = pg.y + a.nb.dy[k], nz = pg.z + a.nb.dz[k];
         if (nx < 0 || nx >= pg.W || ny < 0 || ny >= pg.H || nz < 0 || nz >= pg.D) continue;
         const uint32_t q = (uint32_t)((int)p + a.nb.dx[k] + a.nb.dy[k] * pg.W + a.nb.dz[k] * WH);
-        const uint32_t kq = fl_key(a.iv, q);
+        const uint32_t kq = rec ? a.uf.p[(size_t)q * 4 + 1] : fl_key(a.iv, q);
         const uint32_t dq = (kq >> a.shift) & 255u;
         if (a.pass == 1) {
             if (dq > lev || (dq == lev && q > p)) t[k] = q;
@@ -361,7 +365,8 @@ __device__ __forceinline__ void pixel_targets(const LevelArgs &a, uint32_t p, ui
             if (cq == cp) {
                 if (dq > lev || (dq == lev && q > p)) t[k] = q;
             } else if (cq > cp) {
-                const uint32_t sv = (uint32_t)a.g.n + child_toward(a, a.cnode[q], nid, kc_count);
+                const uint32_t cq_node = rec ? a.uf.p[(size_t)q * 4 + 2] : a.cnode[q];
+                const uint32_t sv = (uint32_t)a.g.n + child_toward(a, cq_node, nid, kc_count);
                 bool dup = false;
 #pragma unroll
                 for (int m = 0; m < k; m++) dup = dup || t[m] == sv;
@@ -605,7 +610,7 @@ __device__ __forceinline__ void lds_union(uint32_t *par, uint32_t a, uint32_t b)
 
 /* zones of every tile: zrep[p] = the zone's minimum pixel; uf[p] -> zrep for the other pixels */
 __global__ __launch_bounds__(256) void k_tile_zones(ImgView iv, int shift, TileGeo tg, Nbr nb, uint32_t *__restrict__ zrep,
-                                                    uint32_t *__restrict__ uf, uint32_t *__restrict__ zarea)
+                                                    UFV uf, uint32_t *__restrict__ zarea)
 {
     __shared__ uint32_t par[ZT_MAX];
     __shared__ uint32_t zc[ZT_MAX];
@@ -644,7 +649,7 @@ __global__ __launch_bounds__(256) void k_tile_zones(ImgView iv, int shift, TileG
         const uint32_t p = (uint32_t)(ta.off + x + y * W + z * WH);
         const uint32_t rp = (uint32_t)(ta.off + rx + ry * W + rz * WH);
         zrep[p] = rp;
-        if (p != rp) uf[p] = rp;
+        if (p != rp) uf.p[(size_t)p * uf.s] = rp;
         atomicAdd(&zc[r], 1u);
     }
     __syncthreads();
@@ -1158,7 +1163,7 @@ __global__ __launch_bounds__(256) void k_held_resolve(const uint32_t *__restrict
 /* ------------------------------------------------------------------------------------------------ */
 /* every element a singleton root: pixels tagged with their level digit, super-vertices (ids n and
  * above) with NLEV */
-__global__ __launch_bounds__(256) void k_uf_init(ImgView iv, int pass, int shift, int64_t n, int64_t total, uint32_t *uf)
+__global__ __launch_bounds__(256) void k_uf_init(ImgView iv, int pass, int shift, int64_t n, int64_t total, UFV uf)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
@@ -1167,7 +1172,22 @@ __global__ __launch_bounds__(256) void k_uf_init(ImgView iv, int pass, int shift
         uint32_t k = fl_key(iv, i);
         lv = (k >> shift) & 255u;
     }
-    uf[i] = UF_ROOT | lv;
+    uf.p[(size_t)i * uf.s] = UF_ROOT | lv;
+}
+
+/* the same for a refining pass with records: {singleton root tag, key, coarse node, 0} per pixel,
+ * {root tag of a super-vertex, 0, 0, 0} for the coarse nodes' super-vertices */
+__global__ __launch_bounds__(256) void k_rec_init(ImgView iv, int shift, const uint32_t *__restrict__ cnode, int64_t n, int64_t total,
+                                                  uint4 *__restrict__ rec)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    uint4 v = make_uint4(UF_ROOT | NLEV, 0u, 0u, 0u);
+    if (i < n) {
+        const uint32_t k = fl_key(iv, i);
+        v = make_uint4(UF_ROOT | ((k >> shift) & 255u), k, cnode[i], 0u);
+    }
+    rec[i] = v;
 }
 
 __global__ __launch_bounds__(256) void k_fill_i32(int32_t *a, int32_t v, int64_t n)
@@ -1238,7 +1258,7 @@ __global__ __launch_bounds__(256) void k_lift(int32_t *__restrict__ lift, int j,
 
 /* after pass 2: the lowest fine node of every coarse node hangs below the fine node its
  * super-vertex joined (or is the root) */
-__global__ __launch_bounds__(256) void k_fine_roots(uint32_t *uf, const uint32_t *__restrict__ cpix,
+__global__ __launch_bounds__(256) void k_fine_roots(UFV uf, const uint32_t *__restrict__ cpix,
                                                     const int32_t *__restrict__ spar, int32_t *__restrict__ par, int64_t kc)
 {
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1550,11 +1570,12 @@ int dcf_flood_build(dcf_tree *t, int grp)
     /* pixel-indexed attribute records (valid at roots) */
     DevAttr A;
     if ((rc = alloc_attr_n(ctx, &sc, A, grp, n))) return rc;
-    la.uf = (uint32_t *)sc.get((size_t)n * 4);
+    la.uf.p = (uint32_t *)sc.get((size_t)n * 4);
+    la.uf.s = 1;
     la.absorbed = (uint32_t *)sc.get((size_t)(n + 16) * 4);
     la.nabs = (uint32_t *)sc.get((NLEV + 4) * 4);
     int32_t *par = P == 1 ? t->parent : (int32_t *)sc.get((size_t)n * 4);
-    if (!la.uf || !la.absorbed || !la.nabs || !par) return dcf_set_error(ctx, DCF_ENOMEM, "flood scratch");
+    if (!la.uf.p || !la.absorbed || !la.nabs || !par) return dcf_set_error(ctx, DCF_ENOMEM, "flood scratch");
 
     /* ---- pass 1: the top digit over all pixels ---- */
     la.pass = 1;
@@ -1614,11 +1635,14 @@ int dcf_flood_build(dcf_tree *t, int grp)
                    cnode, n);
         DCF_LAUNCH(ctx, "k_fill", k_fill_i32, dim3(nblk(kc, 256)), dim3(256), 0, spar, -1, kc);
 
-        uint32_t *uf2 = (uint32_t *)sc.get((size_t)(n + kc) * 4);
+        /* union-find of the pass: 16-byte records {entry, key, coarse node, 0} (DCF_RECORDS), or entries */
+        const bool recs = ctx->records != 0;
+        uint32_t *uf2 = (uint32_t *)sc.get((size_t)(n + kc) * (recs ? 16 : 4));
         uint32_t *abs2 = (uint32_t *)sc.get((size_t)(n + kc + 16) * 4);
         int32_t *npar = pass == P ? t->parent : (int32_t *)sc.get((size_t)n * 4);
         if (!uf2 || !abs2 || !npar) return dcf_set_error(ctx, DCF_ENOMEM, "fine union-find");
-        la.uf = uf2;
+        la.uf.p = uf2;
+        la.uf.s = recs ? 4 : 1;
         la.absorbed = abs2;
         la.pass = 2;
         la.shift = 8 * (P - pass);
@@ -1626,7 +1650,11 @@ int dcf_flood_build(dcf_tree *t, int grp)
         la.lift = lift;
         la.cnkey = cnkey;
         la.nlift = nlift;
-        DCF_LAUNCH(ctx, "k_uf_init", k_uf_init, dim3(nblk(n + kc, 256)), dim3(256), 0, la.iv, 2, la.shift, n, n + kc, la.uf);
+        if (recs)
+            DCF_LAUNCH(ctx, "k_uf_init", k_rec_init, dim3(nblk(n + kc, 256)), dim3(256), 0, la.iv, la.shift, (const uint32_t *)cnode, n,
+                       n + kc, (uint4 *)uf2);
+        else
+            DCF_LAUNCH(ctx, "k_uf_init", k_uf_init, dim3(nblk(n + kc, 256)), dim3(256), 0, la.iv, 2, la.shift, n, n + kc, la.uf);
         DCF_LAUNCH(ctx, "k_fill", k_fill_i32, dim3(nblk(n, 256)), dim3(256), 0, npar, -1, n);
         if ((rc = clear_attr(ctx, A, n))) return rc;
         if ((rc = run_pass(ctx, la, sc, A, S, grp, npar, spar, kc, false, tgt, is3d, t->batch_tiles))) return rc;

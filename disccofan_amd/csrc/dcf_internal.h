@@ -74,6 +74,9 @@ struct dcf_ctx {
      * groups of tiles kept together in the list (DCF_LEVEL_GROUP2D / DCF_LEVEL_GROUP3D) */
     int lt2[3] = { 64, 64, 1 }, lg2[3] = { 1, 1, 1 };
     int lt3[3] = { 16, 16, 16 }, lg3[3] = { 1, 1, 1 };
+    /* refining passes keep {union-find entry, key, coarse node} in one 16-byte record per element
+     * (DCF_RECORDS, 0 off) */
+    int records = 1;
     /* union targets of pixel passes precomputed by the level scatter (DCF_TARGETS): 0 off, 1 on,
      * 2 batches only */
     int targets = 2;

@@ -122,16 +122,18 @@ def test_random_3d(ctx, seed):
             _compare_all(ctx, img, conn, tt, label="rand3d s%d" % seed)
 
 
-@pytest.mark.parametrize("zones,targets", [("0", "1"), ("1", "0"), ("1", "1"), ("0", "0")])
-def test_build_paths(monkeypatch, zones, targets):
-    """every combination of the build's optional paths on single images: the zone graph of the first
-    pass (DCF_ZONES, 1 forces it also for 3D u8 and zone-rich images) and the union targets precomputed
-    by the level scatter (DCF_TARGETS); the library reads both when a context is created"""
+@pytest.mark.parametrize("zones,targets,records", [("0", "1", "1"), ("1", "0", "1"), ("1", "1", "0"), ("0", "0", "0")])
+def test_build_paths(monkeypatch, zones, targets, records):
+    """the build's optional paths on single images, each on and off: the zone graph of the first pass
+    (DCF_ZONES, 1 forces it also for 3D u8 and zone-rich images), the union targets precomputed by the
+    level scatter (DCF_TARGETS) and the 16-byte union-find records of refining passes (DCF_RECORDS); the
+    library reads them when a context is created"""
     monkeypatch.setenv("DCF_ZONES", zones)
     monkeypatch.setenv("DCF_TARGETS", targets)
+    monkeypatch.setenv("DCF_RECORDS", records)
     c = dcf.Context(0)
     try:
-        rng = np.random.default_rng(700 + 2 * int(zones) + int(targets))
+        rng = np.random.default_rng(700 + 4 * int(zones) + 2 * int(targets) + int(records))
         cases = [
             (rng.integers(0, 65536, size=(67, 91)).astype(np.uint16), 8),
             (synth.vnoise(96, 128), 4),
@@ -142,7 +144,7 @@ def test_build_paths(monkeypatch, zones, targets):
         ]
         for img, conn in cases:
             for tt in ("max", "min"):
-                label = "zones=%s targets=%s %s %s" % (zones, targets, img.dtype, img.shape)
+                label = "zones=%s targets=%s records=%s %s %s" % (zones, targets, records, img.dtype, img.shape)
                 say("build path", label, conn, tt)
                 o = OracleTree(img, conn, tt, 0)
                 t = c.build(img, conn, tt, 0)
