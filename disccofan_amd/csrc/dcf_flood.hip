@@ -278,6 +278,8 @@ struct LevelArgs {
     Nbr nb;
     int pass, shift;
     UFV uf;
+    uint32_t *nrep;        /* per set root: its set's current node (levelroot pixel), NONE for a super-vertex
+                              whose set has no node in this pass yet */
     uint32_t *absorbed;
     uint32_t *nabs;        /* [0] running count */
     uint32_t *canon;       /* per element: minimum pixel of the current level in its set (sets rooted at an old root) */
@@ -1042,11 +1044,20 @@ __device__ void block_canon_min(const LevelArgs &a, BlockScratch &bs, bool old, 
     }
 }
 
+/* where a pending parent of r's set points until the pass ends: the set's current node, whose parent
+ * k_fix sets to the set's new node; for a super-vertex root with no node yet, the super-vertex itself,
+ * whose join k_fix records in spar */
+__device__ __forceinline__ uint32_t held_target(const LevelArgs &a, uint32_t r)
+{
+    const uint32_t x = a.nrep[r];
+    return x != 0xFFFFFFFFu ? x : r;
+}
+
 /* after the unions of a level, every pixel p of the level finds its set's root r.  A root of this
  * level is the canonical levelroot: p takes it as parent and adds its own attribute contribution.
  * A set still rooted at an old root r gets its canonical levelroot computed here (minimum pixel of
  * the level, one atomicMin per wave and set; the first to touch a set queues r for k_fix); p's
- * contribution is kept pending at r (P) and its parent is left as HELD(r). */
+ * contribution is kept pending at r (P) and its parent is left pending (HELD). */
 template <int G>
 __global__ __launch_bounds__(FB) void k_final_pixels(LevelArgs a, const uint32_t *__restrict__ lvl, uint32_t cnt, uint32_t lev,
                                                      DevAttr A, DevAttr S, DevAttr P, int32_t *__restrict__ par,
@@ -1071,7 +1082,7 @@ __global__ __launch_bounds__(FB) void k_final_pixels(LevelArgs a, const uint32_t
                 if (!a.zone) contrib_own(a, p, c);
                 else if (r != p) contrib_rec(A, p, c); /* a root already holds its own zone's totals */
             } else {
-                par[p] = HELD(r);
+                par[p] = HELD(held_target(a, r));
                 if (!a.zone) contrib_own(a, p, c);
                 else contrib_rec(A, p, c);
             }
@@ -1098,12 +1109,14 @@ __global__ __launch_bounds__(FB) void k_final_pixels(LevelArgs a, const uint32_t
             const uint32_t h = a.absorbed[j];
             uint32_t tag;
             r = uf_find(a.uf, h, tag);
-            if (h < (uint32_t)a.g.n) {
-                par[h] = HELD(r);
-                contrib_rec(A, h, c);
+            /* h's set's current node becomes a child of the node r's set gets at this level */
+            const uint32_t hn = a.nrep[h], to = HELD(held_target(a, r));
+            if (hn != 0xFFFFFFFFu) {
+                par[hn] = to;
+                contrib_rec(A, hn, c);
             } else {
                 const uint32_t cc = h - (uint32_t)a.g.n;
-                spar[cc] = HELD(r);
+                spar[cc] = to;
                 contrib_rec(S, cc, c);
             }
         }
@@ -1111,51 +1124,63 @@ __global__ __launch_bounds__(FB) void k_final_pixels(LevelArgs a, const uint32_t
     }
 }
 
-/* re-root every queued set at its canonical levelroot c: c becomes the root (tagged with this level),
- * the old root r hangs below it, and c's record becomes r's complete record plus everything pending at
- * r (the level's pixels of the set, the other old roots absorbed into it).  No other thread touches c's
- * record: c is no set's root before this, and the set's contributions were all sent to r. */
+/* every queued set r gets its node of this level, c = its minimum pixel of the level: the set's
+ * current node (or, for a super-vertex root with no node yet, the super-vertex's join) hangs below c,
+ * and c's record is that node's (or the super-vertex's) complete record plus everything pending at r
+ * (the set's pixels of this level, the old sets absorbed into it).  r stays the set's root (only its
+ * tag moves to this level), so its members' paths do not grow with every level the set gains.
+ * canon[r] and P[r] are cleared for the set's next level. */
+template <int G>
 __global__ __launch_bounds__(256) void k_fix(LevelArgs a, uint32_t ord, uint32_t lev, DevAttr A, DevAttr S, DevAttr P,
                                              int32_t *__restrict__ par, int32_t *__restrict__ spar)
 {
+    constexpr int NS = GroupShape<G>::NS;
     const uint32_t start = a.fcount[1 + ord], end = a.fcount[0];
     if (blockIdx.x == 0 && threadIdx.x == 0) a.fcount[2 + ord] = end;
     for (uint32_t j = start + blockIdx.x * blockDim.x + threadIdx.x; j < end; j += gridDim.x * blockDim.x) {
         const uint32_t r = a.fix[j];
         const uint32_t c = a.canon[r];
-        st(a.uf, c, UF_ROOT | lev);
-        st(a.uf, r, c);
-        const bool sv = r >= (uint32_t)a.g.n;
+        const uint32_t x = a.nrep[r];
+        const bool sv = x == 0xFFFFFFFFu;
         const DevAttr &R = sv ? S : A;
-        const uint32_t ri = sv ? r - (uint32_t)a.g.n : r;
+        const uint32_t ri = sv ? r - (uint32_t)a.g.n : x;
         if (sv) spar[ri] = (int32_t)c;
-        else par[r] = (int32_t)c;
+        else par[x] = (int32_t)c;
         A.area[c] = R.area[ri] + P.area[r];
-        for (int k = 0; k < A.nsum; k++) A.s[k][c] = R.s[k][ri] + P.s[k][r];
-        if (A.has_ext)
+        P.area[r] = 0;
+#pragma unroll
+        for (int k = 0; k < NS; k++) {
+            A.s[k][c] = R.s[k][ri] + P.s[k][r];
+            P.s[k][r] = 0;
+        }
+        if constexpr (GroupShape<G>::EXT)
             for (int k = 0; k < 3; k++) {
                 A.mn[k][c] = min(R.mn[k][ri], P.mn[k][r]);
                 A.mx[k][c] = max(R.mx[k][ri], P.mx[k][r]);
+                P.mn[k][r] = 0xFFFFFFFFu;
+                P.mx[k][r] = 0u;
             }
+        a.nrep[r] = c;
+        a.canon[r] = 0xFFFFFFFFu;
+        /* r's tag becomes this level: among old roots the most recently grown set wins the links (no
+         * CAS touches r during this kernel, and no thread of this level reads r's tag after it) */
+        st(a.uf, r, UF_ROOT | lev);
     }
 }
 
-/* after a pass: parents (and super-vertex joins) left as HELD(r) become the canonical levelroot of the
- * set rooted at r; that levelroot itself, still HELD, has no parent in this pass */
-__global__ __launch_bounds__(256) void k_held_resolve(const uint32_t *__restrict__ canon, int32_t *__restrict__ par, int64_t n,
-                                                      int32_t *__restrict__ spar, int64_t kc)
+/* after a pass: a parent (or super-vertex join) left pending as HELD(x) is the parent k_fix gave x:
+ * par[x] for a node x, spar for a super-vertex x; the node that is its own such parent has no parent
+ * in this pass */
+__global__ __launch_bounds__(256) void k_held_resolve(int32_t *__restrict__ par, int64_t n, int32_t *__restrict__ spar, int64_t kc)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
-        const int32_t v = par[i];
-        if (v <= -2) {
-            const uint32_t c = canon[HELD_ROOT(v)];
-            par[i] = c == (uint32_t)i ? -1 : (int32_t)c;
-        }
-    } else if (i < n + kc) {
-        const int32_t v = spar[i - n];
-        if (v <= -2) spar[i - n] = (int32_t)canon[HELD_ROOT(v)];
-    }
+    if (i >= n + kc) return;
+    int32_t *slot = i < n ? &par[i] : &spar[i - n];
+    const int32_t v = *slot;
+    if (v > -2) return;
+    const uint32_t x = HELD_ROOT(v);
+    const int32_t c = x < (uint32_t)n ? par[x] : spar[x - n];
+    *slot = (i < n && c == (int32_t)i) ? -1 : c;
 }
 
 /* ------------------------------------------------------------------------------------------------ */
@@ -1188,6 +1213,12 @@ __global__ __launch_bounds__(256) void k_rec_init(ImgView iv, int shift, const u
         v = make_uint4(UF_ROOT | ((k >> shift) & 255u), k, cnode[i], 0u);
     }
     rec[i] = v;
+}
+
+__global__ __launch_bounds__(256) void k_iota_or_none(uint32_t *a, int64_t n, int64_t total)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < total) a[i] = i < n ? (uint32_t)i : 0xFFFFFFFFu;
 }
 
 __global__ __launch_bounds__(256) void k_fill_i32(int32_t *a, int32_t v, int64_t n)
@@ -1258,13 +1289,13 @@ __global__ __launch_bounds__(256) void k_lift(int32_t *__restrict__ lift, int j,
 
 /* after pass 2: the lowest fine node of every coarse node hangs below the fine node its
  * super-vertex joined (or is the root) */
-__global__ __launch_bounds__(256) void k_fine_roots(UFV uf, const uint32_t *__restrict__ cpix,
+__global__ __launch_bounds__(256) void k_fine_roots(UFV uf, const uint32_t *__restrict__ nrep, const uint32_t *__restrict__ cpix,
                                                     const int32_t *__restrict__ spar, int32_t *__restrict__ par, int64_t kc)
 {
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= kc) return;
     uint32_t tag;
-    uint32_t fr = uf_find(uf, cpix[c], tag);
+    const uint32_t fr = nrep[uf_find(uf, cpix[c], tag)];
     par[fr] = spar[c];
 }
 
@@ -1467,6 +1498,10 @@ static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAtt
     la.fcount = (uint32_t *)sc.get((NLEV + 4) * 4);
     if (!la.canon || !la.fix || !la.fcount) return dcf_set_error(ctx, DCF_ENOMEM, "canonical root lists");
     DCF_LAUNCH(ctx, "k_fill", k_fill_u32f, dim3(nblk(nel, 256)), dim3(256), 0, la.canon, 0xFFFFFFFFu, nel);
+    /* every pixel is its own set's node until the set gains a level; super-vertices have no node */
+    la.nrep = (uint32_t *)sc.get((size_t)nel * 4);
+    if (!la.nrep) return dcf_set_error(ctx, DCF_ENOMEM, "set nodes");
+    DCF_LAUNCH(ctx, "k_iota", k_iota_or_none, dim3(nblk(nel, 256)), dim3(256), 0, la.nrep, n, nel);
     /* records pending at the old roots of the sets of a level (each element is such a root at one
      * level at most) */
     DevAttr P;
@@ -1513,11 +1548,16 @@ static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAtt
 #undef DCF_FINAL
         unsigned fb = nblk(cnt, 256);
         if (fb > 1024) fb = 1024;
-        DCF_LAUNCH(ctx, "k_fix", k_fix, dim3(fb), dim3(256), 0, la, (uint32_t)ord, (uint32_t)lev, A, S, P, par, spar);
+        switch (grp) {
+        case 0: DCF_LAUNCH(ctx, "k_fix", k_fix<0>, dim3(fb), dim3(256), 0, la, (uint32_t)ord, (uint32_t)lev, A, S, P, par, spar); break;
+        case 1: DCF_LAUNCH(ctx, "k_fix", k_fix<1>, dim3(fb), dim3(256), 0, la, (uint32_t)ord, (uint32_t)lev, A, S, P, par, spar); break;
+        case 2: DCF_LAUNCH(ctx, "k_fix", k_fix<2>, dim3(fb), dim3(256), 0, la, (uint32_t)ord, (uint32_t)lev, A, S, P, par, spar); break;
+        case 3: DCF_LAUNCH(ctx, "k_fix", k_fix<3>, dim3(fb), dim3(256), 0, la, (uint32_t)ord, (uint32_t)lev, A, S, P, par, spar); break;
+        default: DCF_LAUNCH(ctx, "k_fix", k_fix<4>, dim3(fb), dim3(256), 0, la, (uint32_t)ord, (uint32_t)lev, A, S, P, par, spar); break;
+        }
         ord++;
     }
-    DCF_LAUNCH(ctx, "k_held_resolve", k_held_resolve, dim3(nblk(n + kc, 256)), dim3(256), 0, (const uint32_t *)la.canon, par, n,
-               spar, kc);
+    DCF_LAUNCH(ctx, "k_held_resolve", k_held_resolve, dim3(nblk(n + kc, 256)), dim3(256), 0, par, n, spar, kc);
     if (zones)
         DCF_LAUNCH(ctx, "k_zone_parents", k_zone_parents, dim3(nblk(n, 256)), dim3(256), 0, la.iv, la.pass, la.shift,
                    (const uint32_t *)zrep, par, n);
@@ -1658,7 +1698,8 @@ int dcf_flood_build(dcf_tree *t, int grp)
         DCF_LAUNCH(ctx, "k_fill", k_fill_i32, dim3(nblk(n, 256)), dim3(256), 0, npar, -1, n);
         if ((rc = clear_attr(ctx, A, n))) return rc;
         if ((rc = run_pass(ctx, la, sc, A, S, grp, npar, spar, kc, false, tgt, is3d, t->batch_tiles))) return rc;
-        DCF_LAUNCH(ctx, "k_fine_roots", k_fine_roots, dim3(nblk(kc, 256)), dim3(256), 0, la.uf, cpix, spar, npar, kc);
+        DCF_LAUNCH(ctx, "k_fine_roots", k_fine_roots, dim3(nblk(kc, 256)), dim3(256), 0, la.uf, (const uint32_t *)la.nrep, cpix, spar,
+                   npar, kc);
         par = npar;
     }
 
