@@ -850,8 +850,18 @@ template <int G> __device__ __forceinline__ void contrib_zero(Contrib<G> &c)
         for (int i = 0; i < 3; i++) { c.mn[i] = 0xFFFFFFFFu; c.mx[i] = 0; }
 }
 
-/* adds every valid lane's contribution to record r of A, one atomic per distinct root and field */
-template <int G> __device__ void wave_add(const DevAttr &A, bool valid, uint32_t r, const Contrib<G> &c)
+/* the first lane to lower canon[r] from empty queues r for k_fix */
+__device__ __forceinline__ void canon_min(const LevelArgs &a, uint32_t r, uint32_t p)
+{
+    if (atomicMin(&a.canon[r], p) == 0xFFFFFFFFu) a.fix[atomicAdd(a.fcount, 1u)] = r;
+}
+
+/* adds every valid lane's contribution to record r of A, one atomic per distinct root and field;
+ * CANON: also lowers canon[r] to the lane's pixel p (the canonical levelroot candidates of the sets
+ * rooted at old roots) in the same pass over the distinct roots */
+template <int G, bool CANON = false>
+__device__ void wave_add(const DevAttr &A, bool valid, uint32_t r, const Contrib<G> &c, const LevelArgs *a = nullptr,
+                         uint32_t p = 0)
 {
     constexpr int NS = Contrib<G>::NS;
     const int lane = threadIdx.x & 63;
@@ -871,11 +881,16 @@ template <int G> __device__ void wave_add(const DevAttr &A, bool valid, uint32_t
                     if (c.s[i]) atomicAdd((unsigned long long *)&A.s[i][r], (unsigned long long)c.s[i]);
                 if constexpr (Contrib<G>::EXT)
                     for (int i = 0; i < 3; i++) { atomicMin(&A.mn[i][r], c.mn[i]); atomicMax(&A.mx[i][r], c.mx[i]); }
+                if constexpr (CANON) canon_min(*a, r, p);
             }
             break;
         }
         uint64_t area = wsum64(mine ? c.area : 0);
         if (lane == leader) atomicAdd(&A.area[rl], (uint32_t)area);
+        if constexpr (CANON) {
+            const uint32_t pm = wmin32(mine ? p : 0xFFFFFFFFu);
+            if (lane == leader) canon_min(*a, rl, pm);
+        }
 #pragma unroll
         for (int i = 0; i < NS; i++) {
             uint64_t v = wsum64(mine ? (uint64_t)c.s[i] : 0);
@@ -937,6 +952,7 @@ __global__ __launch_bounds__(256) void k_zone_parents(ImgView iv, int pass, int 
 
 struct BlockScratch {
     uint32_t root[FW];
+    uint32_t pmin[FW];
     uint64_t red[FW][1 + DCF_MAX_SUMS + 6];
 };
 
@@ -963,18 +979,24 @@ __device__ __forceinline__ uint32_t block_common_root(BlockScratch &bs, bool val
 }
 
 /* block-wide wave_add: one atomic per field when every valid lane of the block shares one root */
-template <int G> __device__ void block_add(const DevAttr &A, BlockScratch &bs, bool valid, uint32_t r, const Contrib<G> &c)
+template <int G, bool CANON = false>
+__device__ void block_add(const DevAttr &A, BlockScratch &bs, bool valid, uint32_t r, const Contrib<G> &c,
+                          const LevelArgs *a = nullptr, uint32_t p = 0)
 {
     constexpr int NS = Contrib<G>::NS;
     const uint32_t R = block_common_root(bs, valid, r);
     if (R == R_EMPTY) return;
     if (R == R_MIXED) {
-        wave_add(A, valid, r, c);
+        wave_add<G, CANON>(A, valid, r, c, a, p);
         return;
     }
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint64_t area = wsum64(valid ? c.area : 0);
     if (lane == 0) bs.red[w][0] = area;
+    if constexpr (CANON) {
+        const uint32_t pm = wmin32(valid ? p : 0xFFFFFFFFu);
+        if (lane == 0) bs.pmin[w] = pm;
+    }
 #pragma unroll
     for (int i = 0; i < NS; i++) {
         const uint64_t v = wsum64(valid ? (uint64_t)c.s[i] : 0);
@@ -990,6 +1012,11 @@ template <int G> __device__ void block_add(const DevAttr &A, BlockScratch &bs, b
         uint64_t t = 0;
         for (int k = 0; k < FW; k++) t += bs.red[k][0];
         atomicAdd(&A.area[R], (uint32_t)t);
+        if constexpr (CANON) {
+            uint32_t m = 0xFFFFFFFFu;
+            for (int k = 0; k < FW; k++) m = min(m, bs.pmin[k]);
+            canon_min(*a, R, m);
+        }
 #pragma unroll
         for (int i = 0; i < NS; i++) {
             uint64_t v = 0;
@@ -1006,41 +1033,6 @@ template <int G> __device__ void block_add(const DevAttr &A, BlockScratch &bs, b
                 atomicMin(&A.mn[i][R], lo);
                 atomicMax(&A.mx[i][R], hi);
             }
-    }
-}
-
-/* canonical levelroot candidates of sets still rooted at old roots (see k_final_pixels) */
-__device__ void block_canon_min(const LevelArgs &a, BlockScratch &bs, bool old, uint32_t r, uint32_t p)
-{
-    const uint32_t R = block_common_root(bs, old, r);
-    if (R == R_EMPTY) return;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (R != R_MIXED) {
-        const uint32_t pm = wmin32(old ? p : 0xFFFFFFFFu);
-        if (lane == 0) bs.red[w][0] = pm;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t m = 0xFFFFFFFFu;
-            for (int k = 0; k < FW; k++) m = min(m, (uint32_t)bs.red[k][0]);
-            if (atomicMin(&a.canon[R], m) == 0xFFFFFFFFu) a.fix[atomicAdd(a.fcount, 1u)] = R;
-        }
-        return;
-    }
-    uint64_t pend = __ballot(old);
-    while (pend) {
-        const int leader = __ffsll((unsigned long long)pend) - 1;
-        const uint32_t rl = __shfl(r, leader, 64);
-        const bool mine = old && r == rl;
-        const uint64_t m = __ballot(mine);
-        if (m == (1ull << leader)) {
-            if ((pend >> lane) & 1ull) {
-                if (atomicMin(&a.canon[r], p) == 0xFFFFFFFFu) a.fix[atomicAdd(a.fcount, 1u)] = r;
-            }
-            break;
-        }
-        const uint32_t pm = wmin32(mine ? p : 0xFFFFFFFFu);
-        if (lane == leader && atomicMin(&a.canon[rl], pm) == 0xFFFFFFFFu) a.fix[atomicAdd(a.fcount, 1u)] = rl;
-        pend &= ~m;
     }
 }
 
@@ -1089,9 +1081,8 @@ __global__ __launch_bounds__(FB) void k_final_pixels(LevelArgs a, const uint32_t
         }
         block_add(A, bs, fresh, r, c);
         __syncthreads();
-        block_add(P, bs, old, r, c);
-        __syncthreads();
-        block_canon_min(a, bs, old, r, p);
+        /* the held pixels: contributions pending at their old root, canonical levelroot candidates */
+        block_add<G, true>(P, bs, old, r, c, &a, p);
     }
     /* the old roots linked under another root at this level: parent = the set's canonical levelroot
      * (resolved after the pass), complete record pending at the set's root */
