@@ -1050,7 +1050,7 @@ __device__ __forceinline__ uint32_t held_target(const LevelArgs &a, uint32_t r)
  * A set still rooted at an old root r gets its canonical levelroot computed here (minimum pixel of
  * the level, one atomicMin per wave and set; the first to touch a set queues r for k_fix); p's
  * contribution is kept pending at r (P) and its parent is left pending (HELD). */
-template <int G>
+template <int G, bool BLK>
 __global__ __launch_bounds__(FB) void k_final_pixels(LevelArgs a, const uint32_t *__restrict__ lvl, uint32_t cnt, uint32_t lev,
                                                      DevAttr A, DevAttr S, DevAttr P, int32_t *__restrict__ par,
                                                      int32_t *__restrict__ spar, uint32_t *__restrict__ marks)
@@ -1079,10 +1079,18 @@ __global__ __launch_bounds__(FB) void k_final_pixels(LevelArgs a, const uint32_t
                 else contrib_rec(A, p, c);
             }
         }
-        block_add(A, bs, fresh, r, c);
-        __syncthreads();
-        /* the held pixels: contributions pending at their old root, canonical levelroot candidates */
-        block_add<G, true>(P, bs, old, r, c, &a, p);
+        /* BLK: a block whose pixels share one root (big flat level sets, as in a first pass over pixels)
+         * adds once; otherwise (zone representatives, a refining pass's scattered pixels) the waves
+         * aggregate on their own */
+        if constexpr (BLK) {
+            block_add(A, bs, fresh, r, c);
+            __syncthreads();
+            /* the held pixels: contributions pending at their old root, canonical levelroot candidates */
+            block_add<G, true>(P, bs, old, r, c, &a, p);
+        } else {
+            wave_add(A, fresh, r, c);
+            wave_add<G, true>(P, old, r, c, &a, p);
+        }
     }
     /* the old roots linked under another root at this level: parent = the set's canonical levelroot
      * (resolved after the pass), complete record pending at the set's root */
@@ -1527,8 +1535,12 @@ static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAtt
         if (ab > 1024) ab = 1024;
         if (ab > rb) rb = ab;
 #define DCF_FINAL(GV)                                                                                                        \
-    DCF_LAUNCH(ctx, "k_final_pixels", k_final_pixels<GV>, dim3(rb), dim3(FB), 0, la, lvl, cnt, (uint32_t)lev, A, S, P, par, spar, \
-               nabs + 1 + ord)
+    if (la.pass == 1 && !zones)                                                                                              \
+        DCF_LAUNCH(ctx, "k_final_pixels", (k_final_pixels<GV, true>), dim3(rb), dim3(FB), 0, la, lvl, cnt, (uint32_t)lev, A, S, P, \
+                   par, spar, nabs + 1 + ord);                                                                              \
+    else                                                                                                                     \
+        DCF_LAUNCH(ctx, "k_final_pixels", (k_final_pixels<GV, false>), dim3(rb), dim3(FB), 0, la, lvl, cnt, (uint32_t)lev, A, S, P, \
+                   par, spar, nabs + 1 + ord)
         switch (grp) {
         case 0: DCF_FINAL(0); break;
         case 1: DCF_FINAL(1); break;
