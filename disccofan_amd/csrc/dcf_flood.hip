@@ -343,10 +343,10 @@ __device__ __forceinline__ uint32_t child_toward(const LevelArgs &a, uint32_t e,
  * lev, or equal with a larger index); in a refining pass a neighbour in a coarse child region stands
  * for that child's super-vertex (duplicates of one super-vertex are dropped).  NONE marks no target. */
 template <int NB>
-__device__ __forceinline__ void pixel_targets(const LevelArgs &a, uint32_t p, uint32_t lev, uint32_t kc_count, uint32_t (&t)[NB])
+__device__ __forceinline__ void pixel_targets(const LevelArgs &a, const PixGeo &pg, uint32_t p, uint32_t lev, uint32_t kc_count,
+                                              uint32_t (&t)[NB])
 {
     const uint32_t NONE = 0xFFFFFFFFu;
-    const PixGeo pg = pix_geo(a, p);
     const int WH = pg.W * pg.H;
     const bool rec = a.uf.s == 4; /* key and coarse node in the union-find record */
     const uint32_t kp = rec ? a.uf.p[(size_t)p * 4 + 1] : fl_key(a.iv, p);
@@ -439,7 +439,7 @@ __global__ __launch_bounds__(256) void k_union(LevelArgs a, const uint32_t *__re
     if (i < cnt) {
         const uint32_t p = lvl[i];
         uint32_t t[NB];
-        pixel_targets<NB>(a, p, lev, kc_count, t);
+        pixel_targets<NB>(a, pix_geo(a, p), p, lev, kc_count, t);
         uint32_t tp = 0, rp = uf_find(a.uf, p, tp);
 #pragma unroll
         for (int k0 = 0; k0 < NB; k0 += CH) {
@@ -552,7 +552,11 @@ __global__ __launch_bounds__(256) void k_lvl_scatter(ImgView iv, int pass, int s
         order[base[d] + r] = (uint32_t)p;
         if constexpr (NB > 0) {
             uint32_t t[NB];
-            pixel_targets<NB>(a, (uint32_t)p, d, kc_count, t);
+            /* the block's tile knows its image: no lookup in the image table */
+            const int l = (int)(((int64_t)p - p0) % ((int64_t)iW * iH));
+            const int zz = (int)(((int64_t)p - p0) / ((int64_t)iW * iH));
+            const PixGeo pg{ l % iW, l / iW, zz, iW, iH, iD };
+            pixel_targets<NB>(a, pg, (uint32_t)p, d, kc_count, t);
             uint32_t *dst = tgt + (size_t)(base[d] + r) * NB;
             if (NB % 4 == 0) {
 #pragma unroll
