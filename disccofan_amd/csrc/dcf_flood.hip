@@ -860,52 +860,103 @@ __device__ __forceinline__ void canon_min(const LevelArgs &a, uint32_t r, uint32
     if (atomicMin(&a.canon[r], p) == 0xFFFFFFFFu) a.fix[atomicAdd(a.fcount, 1u)] = r;
 }
 
-/* adds every valid lane's contribution to record r of A, one atomic per distinct root and field;
- * CANON: also lowers canon[r] to the lane's pixel p (the canonical levelroot candidates of the sets
- * rooted at old roots) in the same pass over the distinct roots */
+/* adds every valid lane's contribution to record r of A: one reduction when the wave has one root,
+ * one atomic per lane when no two neighbouring lanes share a root, else segmented reductions over the
+ * runs of neighbouring lanes with one root (a fixed cost, whatever the number of roots) and one atomic
+ * per run and field.  CANON: also lowers canon[r] to the lane's pixel p (the canonical levelroot
+ * candidates of the sets rooted at old roots) in the same pass */
 template <int G, bool CANON = false>
 __device__ void wave_add(const DevAttr &A, bool valid, uint32_t r, const Contrib<G> &c, const LevelArgs *a = nullptr,
                          uint32_t p = 0)
 {
     constexpr int NS = Contrib<G>::NS;
     const int lane = threadIdx.x & 63;
-    uint64_t pending = __ballot(valid);
-    while (pending) {
-        int leader = __ffsll((unsigned long long)pending) - 1;
-        uint32_t rl = __shfl(r, leader, 64);
-        bool mine = ((pending >> lane) & 1ull) && r == rl;
-        uint64_t m = __ballot(mine);
-        if (__popcll(m) == 1) {
-            /* the leader's root is unique in the wave: the remaining roots are likely distinct too,
-             * so every pending lane updates its own root directly */
-            if ((pending >> lane) & 1ull) {
-                atomicAdd(&A.area[r], c.area);
+    const uint64_t vmask = __ballot(valid);
+    if (!vmask) return;
+    {
+        /* one root for the whole wave (big flat level sets): one reduction, one atomic per field */
+        const int leader = __ffsll((unsigned long long)vmask) - 1;
+        const uint32_t rl = __shfl(r, leader, 64);
+        if (__ballot(valid && r != rl) == 0) {
+            const uint64_t ar = wsum64(valid ? c.area : 0);
+            if (lane == leader) atomicAdd(&A.area[rl], (uint32_t)ar);
 #pragma unroll
-                for (int i = 0; i < NS; i++)
-                    if (c.s[i]) atomicAdd((unsigned long long *)&A.s[i][r], (unsigned long long)c.s[i]);
-                if constexpr (Contrib<G>::EXT)
-                    for (int i = 0; i < 3; i++) { atomicMin(&A.mn[i][r], c.mn[i]); atomicMax(&A.mx[i][r], c.mx[i]); }
-                if constexpr (CANON) canon_min(*a, r, p);
+            for (int i = 0; i < NS; i++) {
+                const uint64_t v = wsum64(valid ? (uint64_t)c.s[i] : 0);
+                if (lane == leader && v) atomicAdd((unsigned long long *)&A.s[i][rl], (unsigned long long)v);
             }
-            break;
+            if constexpr (Contrib<G>::EXT)
+                for (int i = 0; i < 3; i++) {
+                    const uint32_t lo = wmin32(valid ? c.mn[i] : 0xFFFFFFFFu), hi = wmax32(valid ? c.mx[i] : 0u);
+                    if (lane == leader) { atomicMin(&A.mn[i][rl], lo); atomicMax(&A.mx[i][rl], hi); }
+                }
+            if constexpr (CANON) {
+                const uint32_t pm = wmin32(valid ? p : 0xFFFFFFFFu);
+                if (lane == leader) canon_min(*a, rl, pm);
+            }
+            return;
         }
-        uint64_t area = wsum64(mine ? c.area : 0);
-        if (lane == leader) atomicAdd(&A.area[rl], (uint32_t)area);
-        if constexpr (CANON) {
-            const uint32_t pm = wmin32(mine ? p : 0xFFFFFFFFu);
-            if (lane == leader) canon_min(*a, rl, pm);
+    }
+    /* runs of consecutive valid lanes with one root (the level lists are tile-ordered, so a root's
+     * pixels tend to sit in neighbouring lanes) */
+    const uint32_t rprev = __shfl_up(r, 1, 64);
+    const bool cont = valid && lane > 0 && ((vmask >> (lane - 1)) & 1ull) && rprev == r;
+    const uint64_t cmask = __ballot(cont);
+    if (!cmask) {
+        /* no two neighbouring lanes share a root: every lane updates its own */
+        if (valid) {
+            atomicAdd(&A.area[r], c.area);
+#pragma unroll
+            for (int i = 0; i < NS; i++)
+                if (c.s[i]) atomicAdd((unsigned long long *)&A.s[i][r], (unsigned long long)c.s[i]);
+            if constexpr (Contrib<G>::EXT)
+                for (int i = 0; i < 3; i++) { atomicMin(&A.mn[i][r], c.mn[i]); atomicMax(&A.mx[i][r], c.mx[i]); }
+            if constexpr (CANON) canon_min(*a, r, p);
         }
+        return;
+    }
+    /* segmented suffix reductions: each run's first lane ends with the run's totals (lanes that are
+     * not valid carry neutral values and never start a run) */
+    const bool head = valid && !cont;
+    const uint64_t hmask = __ballot(head);
+    const uint32_t rid = (uint32_t)__popcll(hmask & (lane == 63 ? ~0ull : ((2ull << lane) - 1)));
+    uint32_t area = valid ? c.area : 0u;
+    int64_t sm[NS > 0 ? NS : 1];
+#pragma unroll
+    for (int i = 0; i < NS; i++) sm[i] = valid ? c.s[i] : 0;
+    uint32_t mn[3], mx[3];
+    if constexpr (Contrib<G>::EXT)
+        for (int i = 0; i < 3; i++) { mn[i] = valid ? c.mn[i] : 0xFFFFFFFFu; mx[i] = valid ? c.mx[i] : 0u; }
+    uint32_t pm = valid ? p : 0xFFFFFFFFu;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t rid2 = __shfl_down(rid, off, 64);
+        const bool same = lane + off < 64 && rid2 == rid;
+        const uint32_t a2 = __shfl_down(area, off, 64);
+        if (same) area += a2;
 #pragma unroll
         for (int i = 0; i < NS; i++) {
-            uint64_t v = wsum64(mine ? (uint64_t)c.s[i] : 0);
-            if (lane == leader && v) atomicAdd((unsigned long long *)&A.s[i][rl], (unsigned long long)v);
+            const uint32_t lo = __shfl_down((uint32_t)sm[i], off, 64), hi = __shfl_down((uint32_t)((uint64_t)sm[i] >> 32), off, 64);
+            if (same) sm[i] += (int64_t)(((uint64_t)hi << 32) | lo);
         }
         if constexpr (Contrib<G>::EXT)
             for (int i = 0; i < 3; i++) {
-                uint32_t lo = wmin32(mine ? c.mn[i] : 0xFFFFFFFFu), hi = wmax32(mine ? c.mx[i] : 0u);
-                if (lane == leader) { atomicMin(&A.mn[i][rl], lo); atomicMax(&A.mx[i][rl], hi); }
+                const uint32_t m1 = __shfl_down(mn[i], off, 64), m2 = __shfl_down(mx[i], off, 64);
+                if (same) { mn[i] = min(mn[i], m1); mx[i] = max(mx[i], m2); }
             }
-        pending &= ~m;
+        if constexpr (CANON) {
+            const uint32_t p2 = __shfl_down(pm, off, 64);
+            if (same) pm = min(pm, p2);
+        }
+    }
+    if (head) {
+        atomicAdd(&A.area[r], area);
+#pragma unroll
+        for (int i = 0; i < NS; i++)
+            if (sm[i]) atomicAdd((unsigned long long *)&A.s[i][r], (unsigned long long)sm[i]);
+        if constexpr (Contrib<G>::EXT)
+            for (int i = 0; i < 3; i++) { atomicMin(&A.mn[i][r], mn[i]); atomicMax(&A.mx[i][r], mx[i]); }
+        if constexpr (CANON) canon_min(*a, r, pm);
     }
 }
 
