@@ -16,10 +16,13 @@
  *   k_final_pixels  every pixel of level k finds its root.  A root of level k is canonical: the
  *             pixel takes it as parent and adds its attribute contribution to it.  A set rooted at
  *             an old root r gets its canonical levelroot c by atomicMin; the contributions of its
- *             pixels and the complete records of the old roots absorbed into it are kept pending
- *             at r, and their parents are left as "the levelroot of r's set".
- *   k_fix     every such set is re-rooted at c, and c's record becomes r's record plus what is
- *             pending at r.  After the pass, k_held_resolve replaces the pending parents by c.
+ *             pixels and the complete records of the old sets absorbed into it are kept pending
+ *             at r, and their parents are left pending on the set's current node nrep[r].
+ *   k_fix     every such set gets its node c: nrep[r] hangs below c, and c's record becomes
+ *             nrep[r]'s record plus what is pending at r.  r stays the set's union-find root (a
+ *             set's root changes only when the set is merged into an older one), so paths do not
+ *             grow with the levels a set gains.  After the pass, k_held_resolve resolves the
+ *             pending parents.
  * Attributes are therefore accumulated level by level, exactly like the reference's merges during
  * flooding, with no separate leaffix.
  *
@@ -146,9 +149,9 @@ struct AbsBuf {
  * completed (old) set is linked under that set's root, so the threads of a band of level-`lev`
  * pixels along a big old component each CAS their own root's entry and never a shared one; old
  * roots merge among themselves by (level, id), current-level roots by id.  A linked old root is
- * recorded.  The canonical levelroot of a set that contains old roots (its minimum-index pixel of
- * level `lev`) is installed as its root after the unions (k_final_pixels, k_fix).  Returns the
- * surviving root (possibly stale) and its tag. */
+ * recorded.  The node of a set that contains old roots (its levelroot, the minimum-index pixel of
+ * level `lev`) is found after the unions (k_final_pixels, k_fix); the set keeps its old root.
+ * Returns the surviving root (possibly stale) and its tag. */
 __device__ __forceinline__ uint32_t uf_unite(const UFV &uf, uint32_t a, uint32_t ta, uint32_t b, uint32_t tb, uint32_t lev,
                                              const AbsBuf &ab, uint32_t &tout)
 {
@@ -283,7 +286,7 @@ struct LevelArgs {
     uint32_t *absorbed;
     uint32_t *nabs;        /* [0] running count */
     uint32_t *canon;       /* per element: minimum pixel of the current level in its set (sets rooted at an old root) */
-    uint32_t *fix;         /* old set roots to re-root, all levels of the pass back to back */
+    uint32_t *fix;         /* old set roots that get a node, all levels of the pass back to back */
     uint32_t *fcount;      /* [0] running count of fix, [1 + i] where level i's entries start */
     /* pass 2 only */
     const uint32_t *cnode; /* coarse node id of every pixel */
@@ -1545,7 +1548,7 @@ static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAtt
      * entries start (written on the device, so the level loop never waits for the host) */
     uint32_t *nabs = la.nabs;
     DCF_LAUNCH(ctx, "k_fill", k_fill_u32f, dim3(2), dim3(256), 0, nabs, 0u, (int64_t)(NLEV + 2));
-    /* canonical-root bookkeeping for sets rooted at old roots (every element is re-rooted at most once) */
+    /* node bookkeeping for sets rooted at old roots (one fix entry per node made from an old set) */
     const int64_t nel = n + kc;
     la.canon = (uint32_t *)sc.get((size_t)nel * 4);
     la.fix = (uint32_t *)sc.get((size_t)(nel + 16) * 4);
