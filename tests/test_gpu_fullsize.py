@@ -122,3 +122,31 @@ def test_float_four_passes(ctx):
         gs, os_ = t.spectrum1d(0, lv), o.spectrum1d(0, lv)
         assert np.allclose(gs, os_, rtol=1e-12, atol=0), (gs, os_)
         t.free()
+
+
+@pytest.mark.parametrize("tree_type", ["max", "min"])
+def test_vnoise_2048_u16_4conn(ctx, tree_type):
+    """2048^2 u16 value noise at 4-connectivity (the default of the reference's 2D runs)"""
+    img = synth.vnoise(2048, 2048)
+    say("vnoise 2048^2 u16 4-conn", tree_type)
+    t = ctx.build(img, 4, tree_type, 0)
+    o = OracleTree(img, 4, tree_type, 0, threads=THREADS)
+    same(t.parent(), o.parent().astype(np.int32), "parent")
+    same(t.filter(0, 500, 3, out_dtype=np.uint16), o.filter(0, 500, 3).astype(np.uint16), "subtractive filter")
+    same(t.filter(0, 500, 0, out_dtype=np.uint16), o.filter(0, 500, 0).astype(np.uint16), "direct filter")
+
+
+def test_u16_volume_26conn(ctx):
+    """160^3 u16 volume (blurred noise + spheres scaled to 16 bits) at 26-connectivity: two digit passes in 3D"""
+    base = synth.spheres(160, 160, 160).astype(np.uint32)
+    rng = np.random.default_rng(26)
+    img = (base * 256 + rng.integers(0, 256, size=base.shape)).astype(np.uint16)
+    say("u16 160^3 26-conn")
+    for tt in ("max", "min"):
+        t = ctx.build(img, 26, tt, 0)
+        o = OracleTree(img, 26, tt, 0, threads=THREADS)
+        same(t.parent(), o.parent().astype(np.int32), "parent " + tt)
+        same(t.filter(0, 200, 0, out_dtype=np.uint16), o.filter(0, 200, 0).astype(np.uint16), "direct filter " + tt)
+        lv = synth.log_lambdas(32, 1, 2 ** 22)
+        same(t.spectrum1d(0, lv), o.spectrum1d(0, lv), "spectrum " + tt)
+        t.free()
