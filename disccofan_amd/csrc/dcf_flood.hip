@@ -1000,7 +1000,7 @@ __global__ __launch_bounds__(256) void k_zone_parents(ImgView iv, int pass, int 
  * LDS and issues one atomic per field; otherwise each wave aggregates on its own.  Same-address
  * atomics from thousands of waves were the cost of the final passes on levels that are one big
  * component.  Every thread of the block must call these. */
-#define FB 1024
+#define FB 256
 /* a parent left pending during a pass: the set rooted at r, whose canonical levelroot is not known yet */
 #define HELD(r) (-2 - (int32_t)(r))
 #define HELD_ROOT(v) ((uint32_t)(-2 - (v)))
