@@ -518,14 +518,60 @@ static int sub_float(dcf_tree *t, const AttrCtx &c, int attr_idx, double lambda,
     return rc;
 }
 
+/* ---- the min rule under a failing root (tree_filter_min, filtering.c:55-100) ------------------------ *
+ * The reference walks every pixel, in raster order, from its levelroot up to the first node already
+ * reached by an earlier walk, never testing the root.  When the root fails (an increasing attribute:
+ * then every node fails) the root is reached only by the walk of its first pixel c_root, which marks it
+ * 0; a walk that arrives at the root before that takes the root's grey.  The first walk into the
+ * subtree of a child a of the root is that of a's first pixel first(a), and every later walk into the
+ * subtree stops inside it and copies its value: every node under a gets 0 when c_root < first(a), else
+ * the root's grey; the root's own pixels get 0. */
+
+/* pass flags for the jumps to the root's children: J(k) -> the child of the root above k (-1 at a root) */
+__global__ __launch_bounds__(256) void k_rc_init(const int32_t *__restrict__ npar, int32_t *__restrict__ J, uint8_t *__restrict__ pass,
+                                                 int64_t K)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    const int32_t p = npar[k];
+    const bool rc = p >= 0 && npar[p] < 0;
+    pass[k] = rc;
+    J[k] = rc ? (int32_t)k : p;
+}
+
+/* first[x] = the first pixel (raster order) under each child of a root, and of each root's own level */
+__global__ __launch_bounds__(256) void k_first_pixel(const uint32_t *__restrict__ p2n, const int32_t *__restrict__ npar,
+                                                     const int32_t *__restrict__ J, uint32_t *__restrict__ first, int64_t n)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const uint32_t k = p2n[p];
+    const int32_t a = npar[k] < 0 ? (int32_t)k : J[k];
+    atomicMin(&first[a], (uint32_t)p);
+}
+
+__global__ __launch_bounds__(256) void k_min_rootfail(AttrCtx c, int idx, double lambda, const double *__restrict__ nlam,
+                                                      const int32_t *__restrict__ npar, const int32_t *__restrict__ J,
+                                                      const uint32_t *__restrict__ first, const float *__restrict__ ngrey,
+                                                      float *__restrict__ val, int64_t K)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    const int32_t a = npar[k] < 0 ? -1 : J[k];
+    const int32_t root = a < 0 ? (int32_t)k : npar[a];
+    if (attr_value(c, idx, root) >= (nlam ? nlam[root] : lambda)) return; /* the root passes: the direct rule */
+    val[k] = (a < 0 || first[root] < first[a]) ? 0.0f : ngrey[root];
+}
+
 static int rule_values(dcf_tree *t, int attr_idx, int decision, double lambda, unsigned flags, float *val)
 {
     dcf_ctx *ctx = t->ctx;
     const int64_t K = t->K;
     AttrCtx c = attr_ctx(t);
     /* For an increasing attribute (area / volume, idx 0) a node passes only if its parent passes, so
-     * the min, max and subtractive rules (filtering.c:55-261) all give the direct rule's output; the
-     * subtractive kernels can still be forced with DCF_ALL_RULES.  The min and max rules of a
+     * the min, max and subtractive rules (filtering.c:55-261) all give the direct rule's output, except
+     * the min rule under a failing root (k_min_rootfail above); the subtractive kernels can still be
+     * forced with DCF_ALL_RULES.  The min and max rules of a
      * non-increasing attribute depend on the reference's raster scan order (its max rule keeps stale
      * values, filtering.c:156-208) and are not implemented on this path. */
     /* float greys: the subtractive rule's float rounding differs from the direct rule's copies */
@@ -541,9 +587,30 @@ static int rule_values(dcf_tree *t, int attr_idx, int decision, double lambda, u
         for (int r = 0; r < t->max_rounds; r++)
             DCF_LAUNCH(ctx, "k_direct_jump", k_direct_jump, dim3(nblk(K, 256)), dim3(256), 0, J, pass, t->counters, r, K);
         DCF_LAUNCH(ctx, "k_direct_value", k_direct_value, dim3(nblk(K, 256)), dim3(256), 0, J, t->ngrey, val, K);
+        int rc = DCF_OK;
+        if (decision == DCF_MIN) {
+            /* J, pass reused for the jumps to the roots' children */
+            uint32_t *first = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4);
+            if (!first) rc = dcf_set_error(ctx, DCF_ENOMEM, "min rule scratch");
+            else {
+                hipLaunchKernelGGL(k_rc_init, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, (const int32_t *)t->npar, J, pass, K);
+                hipLaunchKernelGGL(k_fill_u32, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, first, 0xFFFFFFFFu, K);
+                hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, t->counters, 0u, (int64_t)64);
+                for (int r = 0; r < t->max_rounds; r++)
+                    hipLaunchKernelGGL(k_direct_jump, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, J, (const uint8_t *)pass, t->counters,
+                                       r, K);
+                hipLaunchKernelGGL(k_first_pixel, dim3(nblk(t->n, 256)), dim3(256), 0, ctx->stream, (const uint32_t *)t->p2n,
+                                   (const int32_t *)t->npar, (const int32_t *)J, first, t->n);
+                hipLaunchKernelGGL(k_min_rootfail, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, c, attr_idx, lambda,
+                                   (const double *)t->nlam, (const int32_t *)t->npar, (const int32_t *)J, (const uint32_t *)first,
+                                   (const float *)t->ngrey, val, K);
+                if (hipGetLastError() != hipSuccess) rc = dcf_set_error(ctx, DCF_EDEVICE, "min rule launches");
+                dcf_release(ctx, first);
+            }
+        }
         dcf_release(ctx, J);
         dcf_release(ctx, pass);
-        return DCF_OK;
+        return rc;
     }
     if (decision == DCF_SUBTRACTIVE && float_grey) return sub_float(t, c, attr_idx, lambda, val);
     if (decision == DCF_SUBTRACTIVE) {

@@ -125,6 +125,19 @@ def test_max_and_min_tree_of_one_image(ctx):
         assert np.array_equal(out, want), tt
 
 
+def test_min_rule_when_roots_fail(ctx):
+    """the min rule with per-image lambdas above some images' areas, every image's root grey above 0"""
+    rng = np.random.default_rng(321)
+    imgs = [(rng.integers(5, 300, size=tuple(int(v) for v in rng.integers(2, 60, size=2)))).astype(np.uint16)
+            for _ in range(6)] + [rng.integers(9, 200, size=(int(rng.integers(2, 50)), 33)).astype(np.uint8) for _ in range(4)]
+    lams = [float(a.size + 1) if k % 2 == 0 else float(a.size // 2) for k, a in enumerate(imgs)]
+    tts = [("max" if k % 3 else "min") for k in range(len(imgs))]
+    outs = ctx.batch_filter(imgs, lams, connectivity=4, decision=1, tree_types=tts)
+    for k, (img, lam, tt, out) in enumerate(zip(imgs, lams, tts, outs)):
+        want = OracleTree(img, 4, tt, 0).filter(0, lam, 1).astype(img.dtype)
+        assert np.array_equal(out, want), (k, img.shape, tt, lam)
+
+
 def test_device_pointers(ctx):
     import torch
     rng = np.random.default_rng(97)

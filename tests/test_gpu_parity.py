@@ -159,6 +159,23 @@ def test_build_paths(monkeypatch, zones, targets, records):
         c.close()
 
 
+@pytest.mark.parametrize("tree_type", ["max", "min"])
+def test_min_rule_when_the_root_fails(ctx, tree_type):
+    """the min rule with lambda above the image's area, on images whose root grey is not 0: the reference
+    keeps the root's grey above the root's level (filtering.c:64-80), the direct rule gives 0"""
+    rng = np.random.default_rng(81)
+    for img, conn in ((rng.integers(100, 4000, size=(37, 29)).astype(np.uint16), 8),
+                      (rng.integers(7, 40, size=(23, 31)).astype(np.uint8), 4),
+                      (rng.integers(3, 20, size=(6, 9, 11)).astype(np.uint8), 26)):
+        o = OracleTree(img, conn, tree_type, 0)
+        t = ctx.build(img, conn, tree_type, 0)
+        for lam in (img.size + 1, 1e9, img.size, img.size // 3):
+            say("min rule", img.dtype, img.shape, conn, tree_type, lam)
+            assert_same(t.filter(0, lam, 1), o.filter(0, lam, 1), "min rule lambda %g" % lam)
+            assert_same(t.filter(0, lam, 1, extract=True), o.filter(0, lam, 1, extract=True), "min extract %g" % lam)
+        t.free()
+
+
 @pytest.mark.parametrize("shape,conn", [((57, 83), 4), ((61, 47), 8), ((9, 23, 31), 6), ((7, 19, 17), 26)])
 def test_every_attribute(ctx, shape, conn):
     """check() of all 18 attributes (every group of AttribsArray) against the oracle; the eigenvalue

@@ -129,3 +129,35 @@ def test_find_scale_is_searchsorted():
     for a in vals:
         want = max(int(np.searchsorted(lv.astype(np.float64), a, side="right")) - 1, 0)
         assert find_scale(lv, a) == want, a
+
+
+@pytest.mark.parametrize("shape,conn,root_first", [((9, 13), 4, False), ((16, 5), 8, False), ((4, 5, 6), 6, False),
+                                                    ((9, 13), 4, True), ((12, 11), 8, True), ((3, 1, 1), 6, True)])
+def test_min_rule_when_the_root_fails(shape, conn, root_first):
+    """tree_filter_min (filtering.c:55-100) walks the pixels in raster order from their levelroot up to the
+    first node an earlier walk reached, and never tests the root, which only the walk of a root-level pixel
+    reaches (and sets to 0).  With lambda above the image's area every node fails, and the output has a
+    closed form over the root's children, the connected components of {g > gmin}: a component whose first
+    pixel precedes the first root-level pixel keeps gmin, any other 0, and the root-level pixels 0 (the
+    direct rule gives 0 everywhere).  One lambda below, the root passes and the two rules agree."""
+    from scipy import ndimage
+    from oracle.oracle import _structure
+    rng = np.random.default_rng(60 + conn + 7 * root_first)
+    img = (rng.integers(0, 6, size=shape) + 3).astype(np.uint8)
+    if root_first:
+        img.reshape(-1)[0] = 3  # a root-level pixel first in raster order: every component gets 0
+    elif len(shape) == 2:
+        img[shape[0] // 2, :] = 3  # a band of the root's level: the components below it start after c_root
+        img[: shape[0] // 2, :] = np.maximum(img[: shape[0] // 2, :], 4)
+    m = img.min()
+    flat = img.reshape(-1)
+    lab, nl = ndimage.label(img > m, structure=_structure(img.ndim, conn))
+    lab = lab.reshape(-1)
+    first = np.full(nl + 1, flat.size, dtype=np.int64)
+    np.minimum.at(first, lab, np.arange(flat.size))
+    c_root = int(np.nonzero(flat == m)[0][0])
+    want = np.where(flat == m, 0, np.where(first[lab] < c_root, m, 0)).astype(np.float32).reshape(img.shape)
+    o = OracleTree(img, conn, "max", 0, threads=1)
+    assert np.array_equal(o.filter(0, img.size + 1, 1), want)
+    assert np.array_equal(o.filter(0, img.size + 1, 0), np.zeros(img.shape, np.float32))
+    assert np.array_equal(o.filter(0, img.size, 1), o.filter(0, img.size, 0))
