@@ -274,13 +274,29 @@ __device__ __forceinline__ float px_grey_ops(const ImgView &iv, int64_t p)
     return iv.invert ? *iv.gmax - v : v;
 }
 
-/* out[p] = post(val[p2n[p]]), post = extract (g - f) then min-tree re-inversion (gmax - f) */
+/* the rule's value at pixel p of node k: val[k], except under the max rule of a non-increasing
+ * attribute (kq != nullptr), where a pixel of a node fixed before its turn, and the levelroot of a node
+ * ever fixed, keep the node's grey */
+__device__ __forceinline__ float pixel_value(int64_t p, uint32_t k, const float *__restrict__ val, const uint32_t *__restrict__ kq,
+                                             const uint32_t *__restrict__ klr, const float *__restrict__ ngrey)
+{
+    if (kq) {
+        const uint32_t q = kq[k];
+        if (q != 0xFFFFFFFFu && ((uint64_t)p > q || (uint64_t)p == klr[k])) return ngrey[k];
+    }
+    return val[k];
+}
+
+/* out[p] = post(value at p), post = extract (g - f) then min-tree re-inversion (gmax - f) */
 __global__ __launch_bounds__(256) void k_filter_out(ImgView iv, const uint32_t *__restrict__ p2n, const float *__restrict__ val,
-                                                    int extract, int reinvert, void *__restrict__ out, int out_dtype, int64_t n)
+                                                    const uint32_t *__restrict__ kq, const uint32_t *__restrict__ klr,
+                                                    const float *__restrict__ ngrey, int extract, int reinvert, void *__restrict__ out,
+                                                    int out_dtype, int64_t n)
 {
     int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
-    float f = val[p2n[p]];
+    const uint32_t k = p2n[p];
+    float f = pixel_value(p, k, val, kq, klr, ngrey);
     if (extract) f = px_grey_ops(iv, p) - f;
     if (reinvert && iv.invert) f = *iv.gmax - f;
     if (out_dtype == DCF_F32) ((float *)out)[p] = f;
@@ -355,15 +371,16 @@ __global__ __launch_bounds__(256) void k_node_lambda(const BatchImg *__restrict_
  * inverted greys, then min-tree re-inversion with the image's own grey maximum, as k_filter_out) */
 __global__ __launch_bounds__(256) void k_batch_out(ImgView iv, const BatchImg *__restrict__ bimg, int nb,
                                                    const BatchOut *__restrict__ outs, const uint32_t *__restrict__ gmax_img,
-                                                   const uint32_t *__restrict__ p2n, const float *__restrict__ val, int extract,
-                                                   int64_t n)
+                                                   const uint32_t *__restrict__ p2n, const float *__restrict__ val,
+                                                   const uint32_t *__restrict__ kq, const uint32_t *__restrict__ klr,
+                                                   const float *__restrict__ ngrey, int extract, int64_t n)
 {
     int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int i = batch_locate(bimg, nb, (uint64_t)p);
     const BatchOut o = outs[i];
     const uint64_t local = (uint64_t)p - bimg[i].off;
-    float f = val[p2n[p]];
+    float f = pixel_value(p, p2n[p], val, kq, klr, ngrey);
     if (extract) f = px_grey_ops(iv, p) - f;
     if (gmax_img && bimg[i].inv) f = (float)gmax_img[i] - f;
     if (o.dtype == DCF_F32) ((float *)o.ptr)[local] = f;
@@ -451,25 +468,25 @@ static AttrCtx attr_ctx(dcf_tree *t)
     return c;
 }
 
-/* node values of the rule (float per node) into val */
-/* the subtractive rule on float greys, exactly as the reference rounds it */
-static int sub_float(dcf_tree *t, const AttrCtx &c, int attr_idx, double lambda, float *val)
+/* Nodes in order of depth: order[] lists the nodes by increasing depth, start[d] is the first
+ * position of depth d (start[maxd + 1] = K).  Depths are sums of ones up the tree by doubling (exact
+ * in double), then a radix sort by depth.  The caller releases *order and *start. */
+static int depth_order(dcf_tree *t, uint32_t **order, uint32_t **start, uint32_t *maxd_out)
 {
     dcf_ctx *ctx = t->ctx;
     const int64_t K = t->K;
+    *order = nullptr;
+    *start = nullptr;
     double *F0 = (double *)dcf_alloc(ctx, (size_t)K * 8), *F1 = (double *)dcf_alloc(ctx, (size_t)K * 8);
     int32_t *A0 = (int32_t *)dcf_alloc(ctx, (size_t)K * 4), *A1 = (int32_t *)dcf_alloc(ctx, (size_t)K * 4);
     uint32_t *k0 = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4), *k1 = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4);
     uint32_t *v0 = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4), *v1 = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4);
-    uint8_t *pass = (uint8_t *)dcf_alloc(ctx, (size_t)K);
-    int32_t *J = (int32_t *)A1;
     void *tmp = nullptr;
-    uint32_t *start = nullptr;
+    uint32_t *st = nullptr;
     int rc = DCF_OK;
     std::vector<uint32_t> act(64);
-    if (!F0 || !F1 || !A0 || !A1 || !k0 || !k1 || !v0 || !v1 || !pass) rc = dcf_set_error(ctx, DCF_ENOMEM, "subtractive scratch");
+    if (!F0 || !F1 || !A0 || !A1 || !k0 || !k1 || !v0 || !v1) rc = dcf_set_error(ctx, DCF_ENOMEM, "depth order scratch");
     if (rc == DCF_OK) {
-        /* depth of every node: sums of ones up the tree by doubling (exact in double) */
         hipLaunchKernelGGL(k_depth_init, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, t->npar, F0, A0, K);
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, t->counters, 0u, (int64_t)64);
         double *fi = F0, *fo = F1;
@@ -486,11 +503,8 @@ static int sub_float(dcf_tree *t, const AttrCtx &c, int attr_idx, double lambda,
         for (int r = 0; r < t->max_rounds; r++) { ran = r + 1; if (act[r] == 0) break; }
         if (rc == DCF_OK && act[t->max_rounds - 1] != 0) rc = dcf_set_error(ctx, DCF_EDEVICE, "tree deeper than the doubling rounds");
         const double *depth = (ran & 1) ? F1 : F0;
-        /* pass flags (the J pointers of k_direct_init are not used here) */
         if (rc == DCF_OK) {
             hipLaunchKernelGGL(k_depth_keys, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, depth, k0, v0, K);
-            hipLaunchKernelGGL(k_direct_init, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, c, attr_idx, lambda, t->nlam, t->npar, J,
-                               pass, K);
             size_t need = 0;
             if (hipcub::DeviceRadixSort::SortPairs(nullptr, need, k0, k1, v0, v1, (int)K, 0, 32, ctx->stream) != hipSuccess)
                 rc = dcf_set_error(ctx, DCF_EDEVICE, "depth sort size");
@@ -503,18 +517,46 @@ static int sub_float(dcf_tree *t, const AttrCtx &c, int attr_idx, double lambda,
         if (rc == DCF_OK && (hipMemcpyAsync(&maxd, k1 + (K - 1), 4, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
                              hipStreamSynchronize(ctx->stream) != hipSuccess))
             rc = dcf_set_error(ctx, DCF_EDEVICE, "maximum depth");
-        if (rc == DCF_OK && !(start = (uint32_t *)dcf_alloc(ctx, ((size_t)maxd + 2) * 4)))
+        if (rc == DCF_OK && !(st = (uint32_t *)dcf_alloc(ctx, ((size_t)maxd + 2) * 4)))
             rc = dcf_set_error(ctx, DCF_ENOMEM, "depth starts");
         if (rc == DCF_OK) {
-            hipLaunchKernelGGL(k_depth_starts, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, (const uint32_t *)k1, start, K);
-            hipLaunchKernelGGL(k_sub_sweep, dim3(1), dim3(1024), 0, ctx->stream, (const uint32_t *)v1, (const uint32_t *)start, maxd,
-                               t->npar, t->ngrey, (const uint8_t *)pass, val);
-            if (hipGetLastError() != hipSuccess) rc = dcf_set_error(ctx, DCF_EDEVICE, "launch k_sub_sweep");
+            hipLaunchKernelGGL(k_depth_starts, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, (const uint32_t *)k1, st, K);
+            if (hipGetLastError() != hipSuccess) rc = dcf_set_error(ctx, DCF_EDEVICE, "launch k_depth_starts");
         }
+        *maxd_out = maxd;
     }
     dcf_release(ctx, F0); dcf_release(ctx, F1); dcf_release(ctx, A0); dcf_release(ctx, A1);
-    dcf_release(ctx, k0); dcf_release(ctx, k1); dcf_release(ctx, v0); dcf_release(ctx, v1);
-    dcf_release(ctx, pass); dcf_release(ctx, tmp); dcf_release(ctx, start);
+    dcf_release(ctx, k0); dcf_release(ctx, k1); dcf_release(ctx, v0); dcf_release(ctx, tmp);
+    if (rc == DCF_OK) {
+        *order = v1;
+        *start = st;
+    } else {
+        dcf_release(ctx, v1);
+        dcf_release(ctx, st);
+    }
+    return rc;
+}
+
+/* the subtractive rule on float greys, exactly as the reference rounds it */
+static int sub_float(dcf_tree *t, const AttrCtx &c, int attr_idx, double lambda, float *val)
+{
+    dcf_ctx *ctx = t->ctx;
+    const int64_t K = t->K;
+    uint8_t *pass = (uint8_t *)dcf_alloc(ctx, (size_t)K);
+    int32_t *J = (int32_t *)dcf_alloc(ctx, (size_t)K * 4);
+    uint32_t *order = nullptr, *start = nullptr, maxd = 0;
+    int rc = DCF_OK;
+    if (!pass || !J) rc = dcf_set_error(ctx, DCF_ENOMEM, "subtractive scratch");
+    /* pass flags (the J pointers of k_direct_init are not used here) */
+    if (rc == DCF_OK)
+        hipLaunchKernelGGL(k_direct_init, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, c, attr_idx, lambda, t->nlam, t->npar, J, pass, K);
+    if (rc == DCF_OK) rc = depth_order(t, &order, &start, &maxd);
+    if (rc == DCF_OK) {
+        hipLaunchKernelGGL(k_sub_sweep, dim3(1), dim3(1024), 0, ctx->stream, (const uint32_t *)order, (const uint32_t *)start, maxd,
+                           t->npar, t->ngrey, (const uint8_t *)pass, val);
+        if (hipGetLastError() != hipSuccess) rc = dcf_set_error(ctx, DCF_EDEVICE, "launch k_sub_sweep");
+    }
+    dcf_release(ctx, pass); dcf_release(ctx, J); dcf_release(ctx, order); dcf_release(ctx, start);
     return rc;
 }
 
@@ -563,20 +605,166 @@ __global__ __launch_bounds__(256) void k_min_rootfail(AttrCtx c, int idx, double
     val[k] = (a < 0 || first[root] < first[a]) ? 0.0f : ngrey[root];
 }
 
-static int rule_values(dcf_tree *t, int attr_idx, int decision, double lambda, unsigned flags, float *val)
+/* ---- the max rule of a non-increasing attribute (tree_filter_max, filtering.c:156-208) ------------ *
+ * The reference walks the pixels in raster order.  A pixel climbs from its node through failing,
+ * unreached nodes; at the first passing node w it keeps g(w) and sets out(u) = g(u) on every ancestor u
+ * of w (a "fix", stopping at the first ancestor already fixed); at an already reached node it copies
+ * that node's out; at a failing root it writes 0.  Every node on the climb takes the same value.
+ * Solved per node, with first(x) = the first pixel of x's subtree (the walk of that pixel is the first
+ * to enter x, and it reaches x) and Q(x) = min first(w) over the passing nodes w in x's subtree (the
+ * moment x is fixed; none when x keeps no passing node):
+ *   - the walk of first(x) through a node x that is not fixed then (x fails, Q(x) > first(x)) writes
+ *     V(x) = g(a) for the parent a of x when a passes or is already fixed (Q(a) < first(x)), and V(a)
+ *     otherwise (the same walk continues, or a was reached by an earlier walk with the value V(a));
+ *     a failing root gives 0.  V(x) is g(parent(c)) for the nearest ancestor-or-self c of x whose
+ *     parent passes or has Q(parent) < first(c): pointer jumping as for the direct rule.
+ *   - the levelroot pixel of x ends with g(x) when x is ever fixed, else V(x); any other pixel v of x
+ *     reads out(x) at its own turn: g(x) when x was fixed before it (Q(x) < v), else V(x).
+ * The reference's out buffer starts zeroed (fresh pages), so a fix also stops at a node whose grey is
+ * 0.  Only a float max-tree with negative greys has such a node below its root; that case is
+ * refused. */
+#define DCF_NONE32 0xFFFFFFFFu
+
+/* a node other than a root with grey 0 */
+__global__ __launch_bounds__(256) void k_zero_grey(const int32_t *__restrict__ npar, const float *__restrict__ ngrey,
+                                                   uint32_t *__restrict__ flag, int64_t K)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < K && npar[k] >= 0 && ngrey[k] == 0.0f) __hip_atomic_store(flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+/* lr[k] = the smallest pixel index of node k's own level (its levelroot) */
+__global__ __launch_bounds__(256) void k_levelroot_pixel(const uint32_t *__restrict__ p2n, uint32_t *__restrict__ lr, int64_t n)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) atomicMin(&lr[p2n[p]], (uint32_t)p);
+}
+
+/* bottom-up over the depths, one workgroup: first[] holds the levelroot pixels on entry and the
+ * subtree minima on exit; Q[] holds DCF_NONE32 on entry.  A node's children are one depth deeper, so
+ * its first and Q are complete when its depth comes up.  The minima are atomics in L2; the reads go
+ * there too, and a device fence and a barrier separate consecutive depths. */
+__global__ __launch_bounds__(1024) void k_first_sweep(const uint32_t *__restrict__ order, const uint32_t *__restrict__ start,
+                                                      uint32_t maxd, const int32_t *__restrict__ npar, const uint8_t *__restrict__ pass,
+                                                      uint32_t *first, uint32_t *Q)
+{
+    for (int64_t d = maxd; d >= 0; d--) {
+        const uint32_t a = start[d], b = start[d + 1];
+        for (uint32_t i = a + threadIdx.x; i < b; i += blockDim.x) {
+            const uint32_t u = order[i];
+            const uint32_t f = __hip_atomic_load(&first[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint32_t q = __hip_atomic_load(&Q[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (pass[u]) q = f; /* first(u) is below every first of its subtree */
+            __hip_atomic_store(&Q[u], q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const int32_t p = npar[u];
+            if (p >= 0) {
+                atomicMin(&first[p], f);
+                if (q != DCF_NONE32) atomicMin(&Q[p], q);
+            }
+        }
+        __threadfence();
+        __syncthreads();
+    }
+}
+
+/* stop flags for V: J(k) -> the nearest ancestor-or-self c whose parent passes or is fixed before first(c) */
+__global__ __launch_bounds__(256) void k_max_stop(const int32_t *__restrict__ npar, const uint8_t *__restrict__ pass,
+                                                  const uint32_t *__restrict__ first, const uint32_t *__restrict__ Q,
+                                                  int32_t *__restrict__ J, uint8_t *__restrict__ stop, int64_t K)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    const int32_t p = npar[k];
+    const bool s = p >= 0 && (pass[p] || Q[p] < first[k]);
+    stop[k] = s;
+    J[k] = s ? (int32_t)k : p;
+}
+
+__global__ __launch_bounds__(256) void k_max_value(const int32_t *__restrict__ J, const int32_t *__restrict__ npar,
+                                                   const float *__restrict__ ngrey, float *__restrict__ val, int64_t K)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    const int32_t j = J[k];
+    val[k] = j < 0 ? 0.0f : ngrey[npar[j]];
+}
+
+/* node values V into val, plus Q and the levelroot pixels for the per-pixel choice (*kq, *klr: the
+ * caller releases them) */
+static int max_general(dcf_tree *t, const AttrCtx &c, int attr_idx, double lambda, float *val, uint32_t **kq, uint32_t **klr)
+{
+    dcf_ctx *ctx = t->ctx;
+    const int64_t K = t->K;
+    *kq = *klr = nullptr;
+    uint8_t *pass = (uint8_t *)dcf_alloc(ctx, (size_t)K), *stop = (uint8_t *)dcf_alloc(ctx, (size_t)K);
+    int32_t *J = (int32_t *)dcf_alloc(ctx, (size_t)K * 4);
+    uint32_t *first = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4), *Q = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4);
+    uint32_t *lr = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4);
+    uint32_t *order = nullptr, *start = nullptr, maxd = 0;
+    int rc = DCF_OK;
+    if (!pass || !stop || !J || !first || !Q || !lr) rc = dcf_set_error(ctx, DCF_ENOMEM, "max rule scratch");
+    if (rc == DCF_OK && t->dtype == DCF_F32 && !t->tree_min) {
+        uint32_t zero = 0;
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, t->counters, 0u, (int64_t)64);
+        hipLaunchKernelGGL(k_zero_grey, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, (const int32_t *)t->npar, (const float *)t->ngrey,
+                           t->counters, K);
+        if (hipMemcpyAsync(&zero, t->counters, 4, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess)
+            rc = dcf_set_error(ctx, DCF_EDEVICE, "zero-grey check");
+        else if (zero)
+            rc = dcf_set_error(ctx, DCF_EUNSUPPORTED,
+                               "max rule of a non-increasing attribute on a float max-tree with a node of grey 0 below its root");
+    }
+    if (rc == DCF_OK) {
+        hipLaunchKernelGGL(k_direct_init, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, c, attr_idx, lambda, t->nlam, t->npar, J, pass, K);
+        hipLaunchKernelGGL(k_fill_u32, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, lr, DCF_NONE32, K);
+        hipLaunchKernelGGL(k_fill_u32, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, Q, DCF_NONE32, K);
+        hipLaunchKernelGGL(k_levelroot_pixel, dim3(nblk(t->n, 256)), dim3(256), 0, ctx->stream, (const uint32_t *)t->p2n, lr, t->n);
+        if (hipMemcpyAsync(first, lr, (size_t)K * 4, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
+            rc = dcf_set_error(ctx, DCF_EDEVICE, "levelroot copy");
+    }
+    if (rc == DCF_OK) rc = depth_order(t, &order, &start, &maxd);
+    if (rc == DCF_OK) {
+        hipLaunchKernelGGL(k_first_sweep, dim3(1), dim3(1024), 0, ctx->stream, (const uint32_t *)order, (const uint32_t *)start, maxd,
+                           (const int32_t *)t->npar, (const uint8_t *)pass, first, Q);
+        hipLaunchKernelGGL(k_max_stop, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, (const int32_t *)t->npar, (const uint8_t *)pass,
+                           (const uint32_t *)first, (const uint32_t *)Q, J, stop, K);
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, t->counters, 0u, (int64_t)64);
+        for (int r = 0; r < t->max_rounds; r++)
+            hipLaunchKernelGGL(k_direct_jump, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, J, (const uint8_t *)stop, t->counters, r, K);
+        hipLaunchKernelGGL(k_max_value, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, (const int32_t *)J, (const int32_t *)t->npar,
+                           (const float *)t->ngrey, val, K);
+        if (hipGetLastError() != hipSuccess) rc = dcf_set_error(ctx, DCF_EDEVICE, "max rule launches");
+    }
+    dcf_release(ctx, pass); dcf_release(ctx, stop); dcf_release(ctx, J); dcf_release(ctx, first);
+    dcf_release(ctx, order); dcf_release(ctx, start);
+    if (rc == DCF_OK) {
+        *kq = Q;
+        *klr = lr;
+    } else {
+        dcf_release(ctx, Q);
+        dcf_release(ctx, lr);
+    }
+    return rc;
+}
+
+static int rule_values(dcf_tree *t, int attr_idx, int decision, double lambda, unsigned flags, float *val, uint32_t **kq,
+                       uint32_t **klr)
 {
     dcf_ctx *ctx = t->ctx;
     const int64_t K = t->K;
     AttrCtx c = attr_ctx(t);
+    *kq = *klr = nullptr;
     /* For an increasing attribute (area / volume, idx 0) a node passes only if its parent passes, so
      * the min, max and subtractive rules (filtering.c:55-261) all give the direct rule's output, except
      * the min rule under a failing root (k_min_rootfail above); the subtractive kernels can still be
-     * forced with DCF_ALL_RULES.  The min and max rules of a
-     * non-increasing attribute depend on the reference's raster scan order (its max rule keeps stale
-     * values, filtering.c:156-208) and are not implemented on this path. */
+     * forced with DCF_ALL_RULES.  The max rule of a non-increasing attribute (or of any attribute
+     * under DCF_ALL_RULES) follows the reference's raster scan, stale values included (max_general
+     * above).  The min rule of a non-increasing attribute is not implemented on this path. */
     /* float greys: the subtractive rule's float rounding differs from the direct rule's copies */
     const bool float_grey = t->dtype == DCF_F32;
     bool general_sub = decision == DCF_SUBTRACTIVE && (float_grey || !(attr_idx == 0 && !(flags & DCF_ALL_RULES)));
+    if (decision == DCF_MAX && (attr_idx != 0 || (flags & DCF_ALL_RULES))) return max_general(t, c, attr_idx, lambda, val, kq, klr);
     bool min_max_inc = (decision == DCF_MIN || decision == DCF_MAX) && attr_idx == 0;
     if (decision == DCF_DIRECT || (decision == DCF_SUBTRACTIVE && !general_sub) || min_max_inc) {
         int32_t *J = (int32_t *)dcf_alloc(ctx, (size_t)K * 4);
@@ -650,20 +838,23 @@ int dcf_ops_filter(dcf_tree *t, int attr_idx, int decision, double lambda, int e
     dcf_ctx *ctx = t->ctx;
     float *val = (float *)dcf_alloc(ctx, (size_t)t->K * 4);
     if (!val) return dcf_set_error(ctx, DCF_ENOMEM, "filter values");
-    int rc = rule_values(t, attr_idx, decision, lambda, flags, val);
+    uint32_t *kq = nullptr, *klr = nullptr;
+    int rc = rule_values(t, attr_idx, decision, lambda, flags, val, &kq, &klr);
     if (rc == DCF_OK) {
         ImgView iv = view_of(t);
         hipEvent_t ea = nullptr, eb = nullptr;
         if (ctx->profiling && (ctx->prof_only.empty() || strstr("k_filter_out", ctx->prof_only.c_str())))
             dcf_prof_begin(ctx, "k_filter_out", &ea, &eb);
         const int reinvert = (flags & DCF_KEEP_INVERTED) ? 0 : 1;
-        hipLaunchKernelGGL(k_filter_out, dim3(nblk(t->n, 256)), dim3(256), 0, ctx->stream, iv, t->p2n, val, extract, reinvert,
-                           out_dev, (int)out_dtype, t->n);
+        hipLaunchKernelGGL(k_filter_out, dim3(nblk(t->n, 256)), dim3(256), 0, ctx->stream, iv, t->p2n, val, (const uint32_t *)kq,
+                           (const uint32_t *)klr, (const float *)t->ngrey, extract, reinvert, out_dev, (int)out_dtype, t->n);
         hipError_t le = hipGetLastError();
         if (ea) dcf_prof_end(ctx, "k_filter_out", ea, eb);
         if (le != hipSuccess) rc = dcf_set_error(ctx, DCF_EDEVICE, "launch k_filter_out: %s", hipGetErrorString(le));
     }
     dcf_release(ctx, val);
+    dcf_release(ctx, kq);
+    dcf_release(ctx, klr);
     return rc;
 }
 
@@ -682,18 +873,21 @@ int dcf_ops_batch_filter(dcf_tree *t, int attr_idx, int decision, const double *
     hipLaunchKernelGGL(k_node_lambda, dim3(nblk(t->n, 256)), dim3(256), 0, ctx->stream, t->bimg, t->nbatch, lam_img, t->p2n, nlam,
                        t->n);
     if (hipGetLastError() != hipSuccess) rc = dcf_set_error(ctx, DCF_EDEVICE, "launch k_node_lambda");
+    uint32_t *kq = nullptr, *klr = nullptr;
     if (rc == DCF_OK) {
         t->nlam = nlam;
-        rc = rule_values(t, attr_idx, decision, 0.0, flags, val);
+        rc = rule_values(t, attr_idx, decision, 0.0, flags, val, &kq, &klr);
         t->nlam = nullptr;
     }
     if (rc == DCF_OK) {
         hipLaunchKernelGGL(k_batch_out, dim3(nblk(t->n, 256)), dim3(256), 0, ctx->stream, view_of(t), t->bimg, t->nbatch, outs, gmax_img,
-                           t->p2n, val, extract, t->n);
+                           t->p2n, val, (const uint32_t *)kq, (const uint32_t *)klr, (const float *)t->ngrey, extract, t->n);
         if (hipGetLastError() != hipSuccess) rc = dcf_set_error(ctx, DCF_EDEVICE, "launch k_batch_out");
     }
     dcf_release(ctx, nlam);
     dcf_release(ctx, val);
+    dcf_release(ctx, kq);
+    dcf_release(ctx, klr);
     return rc;
 }
 

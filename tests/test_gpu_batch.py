@@ -92,6 +92,15 @@ def test_non_area_attribute(ctx):
     check_batch(ctx, imgs, lams, 4, "max", decision=0, attr=17, label="ncomp")
 
 
+def test_max_rule_non_increasing(ctx):
+    """the max rule of non-compactness with per-image lambdas (the reference's raster-scan result per image)"""
+    rng = np.random.default_rng(96)
+    imgs, _ = random_batch(rng, 8, maxside=50)
+    lams = [0.2, 0.3, 0.5, 0.25, 1.0, 0.18, 0.4, 0.22]
+    check_batch(ctx, imgs, lams, 8, "max", decision=2, attr=17, label="max ncomp")
+    check_batch(ctx, imgs, lams, 4, "min", decision=2, attr=17, extract=True, label="max ncomp min-tree")
+
+
 def test_uneven_workload_batch(ctx):
     """the uneven-size batch generator of the batched workload, first 24 images, sides 64..512"""
     shapes = synth.batch_shapes(24, 64, 512)

@@ -84,6 +84,7 @@ def _compare_all(ctx, img, conn, tree_type, lam_area=20, lam_nc=0.25, label=""):
         assert_same(t.filter(0, lam_area, dec, all_rules=True), o.filter(0, lam_area, dec), "general d%d %s" % (dec, label))
     for dec in (1, 2):
         assert_same(t.filter(0, lam_area, dec), o.filter(0, lam_area, dec), "filter area d%d %s" % (dec, label))
+    assert_same(t.filter(0, lam_area, 2, all_rules=True), o.filter(0, lam_area, 2), "general max " + label)
     assert_same(t.filter(0, lam_area, 0, extract=True), o.filter(0, lam_area, 0, extract=True), "extract " + label)
     lv = synth.log_lambdas(16, 1, 2 ** 14)
     assert_same(t.spectrum1d(0, lv), o.spectrum1d(0, lv), "spectrum " + label)
@@ -91,7 +92,7 @@ def _compare_all(ctx, img, conn, tree_type, lam_area=20, lam_nc=0.25, label=""):
     t.accumulate(17)
     assert_same(t.check(17), o.check(17), "ncomp " + label)
     assert_same(t.check(13), o.check(13), "trace " + label)
-    for dec in (0, 3):
+    for dec in (0, 2, 3):
         assert_same(t.filter(17, lam_nc, dec), o.filter(17, lam_nc, dec), "filter ncomp d%d %s" % (dec, label))
     l1 = synth.log_lambdas(8, 0.05, 2.0)
     l2 = synth.log_lambdas(8, 1, 4096)
@@ -173,6 +174,63 @@ def test_min_rule_when_the_root_fails(ctx, tree_type):
             say("min rule", img.dtype, img.shape, conn, tree_type, lam)
             assert_same(t.filter(0, lam, 1), o.filter(0, lam, 1), "min rule lambda %g" % lam)
             assert_same(t.filter(0, lam, 1, extract=True), o.filter(0, lam, 1, extract=True), "min extract %g" % lam)
+        t.free()
+
+
+def _node_values(o, aidx):
+    par, g = o.parent(), o.grey()
+    a, _ = o.attributes(aidx)
+    return np.unique(a[(par < 0) | (g[np.maximum(par, 0)] != g)])
+
+
+@pytest.mark.parametrize("tree_type", ["max", "min"])
+def test_max_rule_of_non_increasing_attributes(ctx, tree_type):
+    """the max rule of non-increasing attributes follows the reference's raster scan (filtering.c:156-208),
+    including the stale values of pixels visited before their node is kept; lambdas sit exactly on node
+    values"""
+    rng = np.random.default_rng(91 if tree_type == "max" else 92)
+    cases = [(rng.integers(0, 12, size=(37, 29)).astype(np.uint8), 4),
+             (rng.integers(0, 9, size=(41, 23)).astype(np.uint8), 8),
+             (rng.integers(100, 140, size=(33, 35)).astype(np.uint16), 8),
+             (rng.integers(0, 6, size=(7, 9, 11)).astype(np.uint8), 6),
+             (rng.integers(0, 5, size=(6, 8, 9)).astype(np.uint8), 26)]
+    for img, conn in cases:
+        for aidx in (17, 13, 2, 6):
+            o = OracleTree(img, conn, tree_type, aidx)
+            t = ctx.build(img, conn, tree_type, aidx)
+            vals = _node_values(o, aidx)
+            for lam in (vals[len(vals) // 2], vals[len(vals) // 4], vals[-1] + 1, rng.choice(vals)):
+                say("max rule", img.dtype, img.shape, conn, tree_type, aidx, lam)
+                assert_same(t.filter(aidx, lam, 2), o.filter(aidx, lam, 2), "max rule a%d lambda %r" % (aidx, lam))
+            lam = vals[len(vals) // 3]
+            assert_same(t.filter(aidx, lam, 2, extract=True), o.filter(aidx, lam, 2, extract=True), "max extract a%d" % aidx)
+            t.free()
+
+
+@pytest.mark.parametrize("tree_type", ["max", "min"])
+def test_max_rule_vnoise_1024(ctx, tree_type):
+    """the max rule of non-compactness on a 1024^2 u16 value-noise image (deep tree, many nodes)"""
+    img = synth.vnoise(1024, 1024)
+    o = OracleTree(img, 8, tree_type, 17, threads=8)
+    t = ctx.build(img, 8, tree_type, 17)
+    for lam in (0.2, 0.5, 3.0):
+        say("max rule vnoise 1024", tree_type, lam)
+        assert_same(t.filter(17, lam, 2), o.filter(17, lam, 2), "max rule vnoise lambda %g" % lam)
+    t.free()
+
+
+def test_max_rule_float(ctx):
+    """float greys: positive noise as max- and min-tree; negatives with zeros as a min-tree"""
+    rng = np.random.default_rng(93)
+    pos = (rng.random((31, 37)) * 50 + 1).astype(np.float32)
+    neg = rng.integers(-3, 4, size=(29, 33)).astype(np.float32)
+    for img, tt in ((pos, "max"), (pos, "min"), (neg, "min")):
+        o = OracleTree(img, 4, tt, 17)
+        t = ctx.build(img, 4, tt, 17)
+        vals = _node_values(o, 17)
+        for lam in (vals[len(vals) // 2], vals[len(vals) // 5]):
+            say("max rule float", tt, lam)
+            assert_same(t.filter(17, lam, 2), o.filter(17, lam, 2), "max rule float %s lambda %r" % (tt, lam))
         t.free()
 
 
@@ -433,12 +491,17 @@ def test_errors(ctx, tmp_path):
     with pytest.raises(dcf.DcfError) as e:
         ctx.run_operations(img, [("csl", 0, "x")])
     assert e.value.code == -4
-    # min / max rules of a non-increasing attribute depend on the reference's scan order: refused
+    # the min rule of a non-increasing attribute depends on the reference's scan order: refused
     t17 = ctx.build(img, 4, "max", 17)
-    for dec in (1, 2):
-        with pytest.raises(dcf.DcfError) as e:
-            t17.filter(17, 0.3, dec)
-        assert e.value.code == -4
+    with pytest.raises(dcf.DcfError) as e:
+        t17.filter(17, 0.3, 1)
+    assert e.value.code == -4
+    # the max rule on a float max-tree with a node of grey 0 above its negative root: refused
+    fimg = np.array([[-2.0, 0.0, 3.0], [0.0, 1.5, -2.0]], np.float32)
+    tf = ctx.build(fimg, 4, "max", 17)
+    with pytest.raises(dcf.DcfError) as e:
+        tf.filter(17, 0.3, 2)
+    assert e.value.code == -4
 
 
 def test_c_caller_on_gpu(tmp_path):

@@ -10,7 +10,11 @@ restate the oracle's own code:
   walked on the brute-force tree;
 * the 2D pattern spectrum against the 1D spectra of its two axes (marginals), which fails for a
   transposed bin index;
-* find_scale against numpy's searchsorted.
+* find_scale against numpy's searchsorted;
+* the max rule of non-increasing attributes, whose reference result depends on the raster scan
+  (stale values), against a closed form of that scan derived per node (first pixel of each subtree,
+  first pixel of its passing subtrees); the cases are checked to include stale values, where the
+  textbook max rule differs.
 """
 import numpy as np
 import pytest
@@ -161,3 +165,90 @@ def test_min_rule_when_the_root_fails(shape, conn, root_first):
     assert np.array_equal(o.filter(0, img.size + 1, 1), want)
     assert np.array_equal(o.filter(0, img.size + 1, 0), np.zeros(img.shape, np.float32))
     assert np.array_equal(o.filter(0, img.size, 1), o.filter(0, img.size, 0))
+
+
+NONE = 1 << 62
+
+
+def max_rule_closed_form(parent, g, attr, lam):
+    """tree_filter_max (filtering.c:156-208) over one raster scan, solved per node.  first(x) is the
+    first pixel of x's subtree, Q(x) the smallest first(w) over passing nodes w under x (when x is
+    fixed, i.e. set to its own grey by a passing descendant's walk).  The walk of first(x) writes
+    V(x) = g(a) for the parent a of x if a passes or Q(a) < first(x), else V(a); a failing root gives
+    0.  A levelroot pixel ends with g when its node is ever fixed; another pixel v of node y reads
+    g(y) when Q(y) < v, else V(y)."""
+    n = len(parent)
+    is_lr = (parent < 0) | (g[np.maximum(parent, 0)] != g)
+    node = np.where(is_lr, np.arange(n), parent)
+    lrs = np.nonzero(is_lr)[0]
+    depth = {}
+    for x in lrs:
+        d, y = 0, x
+        while parent[y] >= 0:
+            y, d = parent[y], d + 1
+        depth[x] = d
+    order = sorted(lrs, key=lambda x: depth[x])
+    passes = {x: attr[x] >= lam for x in lrs}
+    first = {x: x for x in lrs}
+    qc = {x: NONE for x in lrs}
+    Q = {}
+    for x in reversed(order):
+        Q[x] = first[x] if passes[x] else qc[x]
+        a = parent[x]
+        if a >= 0:
+            first[a] = min(first[a], first[x])
+            qc[a] = min(qc[a], Q[x])
+    V = {}
+    for x in order:
+        a = parent[x]
+        V[x] = 0.0 if a < 0 else (g[a] if (passes[a] or Q[a] < first[x]) else V[a])
+    out = np.empty(n, np.float32)
+    for v in range(n):
+        y = node[v]
+        kept = Q[y] < NONE if v == y else Q[y] < v
+        out[v] = g[y] if kept else V[y]
+    return out
+
+
+def textbook_max_rule(parent, g, attr, lam):
+    """keep every node with a passing node in its subtree; a pixel takes its nearest kept node's grey"""
+    n = len(parent)
+    is_lr = (parent < 0) | (g[np.maximum(parent, 0)] != g)
+    kept = np.zeros(n, bool)
+    for x in np.nonzero(is_lr & (attr >= lam))[0]:
+        while x >= 0 and not kept[x]:
+            kept[x] = True
+            x = parent[x]
+    out = np.zeros(n, np.float32)
+    for v in range(n):
+        y = v if is_lr[v] else parent[v]
+        while y >= 0 and not kept[y]:
+            y = parent[y]
+        out[v] = g[y] if y >= 0 else 0
+    return out
+
+
+def test_max_rule_of_non_increasing_attributes():
+    rng = np.random.default_rng(77)
+    stale = 0
+    for it in range(120):
+        if it % 3 == 2:
+            shape, conn = tuple(int(s) for s in rng.integers(2, 7, size=3)), int(rng.choice([6, 26]))
+        else:
+            shape, conn = tuple(int(s) for s in rng.integers(2, 14, size=2)), int(rng.choice([4, 8]))
+        img = rand_img(rng, shape, int(rng.integers(3, 10)))
+        tt = "max" if it % 2 == 0 else "min"
+        aidx = int(rng.choice([17, 13, 6, 14, 2]))
+        o = OracleTree(img, conn, tt, aidx, threads=1)
+        par, g = o.parent(), o.grey()
+        a, _ = o.attributes(aidx)
+        lr = (par < 0) | (g[np.maximum(par, 0)] != g)
+        lam = float(rng.choice(np.unique(a[lr])))
+        got = o.filter(aidx, lam, 2).reshape(-1)
+        want = max_rule_closed_form(par, g, a, lam)
+        text = textbook_max_rule(par, g, a, lam)
+        if tt == "min":
+            want, text = o.gmax - want, o.gmax - text
+        assert np.array_equal(got, want), (it, shape, conn, tt, aidx, lam)
+        stale += not np.array_equal(got, text)
+    assert stale >= 20, stale  # the scan's stale values are exercised, not only the textbook rule
