@@ -689,6 +689,26 @@ __global__ __launch_bounds__(256) void k_max_value(const int32_t *__restrict__ J
     val[k] = j < 0 ? 0.0f : ngrey[npar[j]];
 }
 
+/* the reference's out buffer starts zeroed, so a node of grey 0 below the root reads as already
+ * filtered to its own grey: DCF_EUNSUPPORTED for such a tree (only a float max-tree with negative
+ * greys has one) */
+static int refuse_zero_grey(dcf_tree *t, const char *rule)
+{
+    dcf_ctx *ctx = t->ctx;
+    if (t->dtype != DCF_F32 || t->tree_min) return DCF_OK;
+    uint32_t zero = 0;
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, t->counters, 0u, (int64_t)64);
+    hipLaunchKernelGGL(k_zero_grey, dim3(nblk(t->K, 256)), dim3(256), 0, ctx->stream, (const int32_t *)t->npar, (const float *)t->ngrey,
+                       t->counters, t->K);
+    if (hipMemcpyAsync(&zero, t->counters, 4, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+        return dcf_set_error(ctx, DCF_EDEVICE, "zero-grey check");
+    if (zero)
+        return dcf_set_error(ctx, DCF_EUNSUPPORTED,
+                             "%s rule of a non-increasing attribute on a float max-tree with a node of grey 0 below its root", rule);
+    return DCF_OK;
+}
+
 /* node values V into val, plus Q and the levelroot pixels for the per-pixel choice (*kq, *klr: the
  * caller releases them) */
 static int max_general(dcf_tree *t, const AttrCtx &c, int attr_idx, double lambda, float *val, uint32_t **kq, uint32_t **klr)
@@ -703,18 +723,7 @@ static int max_general(dcf_tree *t, const AttrCtx &c, int attr_idx, double lambd
     uint32_t *order = nullptr, *start = nullptr, maxd = 0;
     int rc = DCF_OK;
     if (!pass || !stop || !J || !first || !Q || !lr) rc = dcf_set_error(ctx, DCF_ENOMEM, "max rule scratch");
-    if (rc == DCF_OK && t->dtype == DCF_F32 && !t->tree_min) {
-        uint32_t zero = 0;
-        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, t->counters, 0u, (int64_t)64);
-        hipLaunchKernelGGL(k_zero_grey, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, (const int32_t *)t->npar, (const float *)t->ngrey,
-                           t->counters, K);
-        if (hipMemcpyAsync(&zero, t->counters, 4, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
-            hipStreamSynchronize(ctx->stream) != hipSuccess)
-            rc = dcf_set_error(ctx, DCF_EDEVICE, "zero-grey check");
-        else if (zero)
-            rc = dcf_set_error(ctx, DCF_EUNSUPPORTED,
-                               "max rule of a non-increasing attribute on a float max-tree with a node of grey 0 below its root");
-    }
+    if (rc == DCF_OK) rc = refuse_zero_grey(t, "max");
     if (rc == DCF_OK) {
         hipLaunchKernelGGL(k_direct_init, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, c, attr_idx, lambda, t->nlam, t->npar, J, pass, K);
         hipLaunchKernelGGL(k_fill_u32, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, lr, DCF_NONE32, K);
@@ -748,6 +757,112 @@ static int max_general(dcf_tree *t, const AttrCtx &c, int attr_idx, double lambd
     return rc;
 }
 
+/* ---- the min rule of any attribute (tree_filter_min, filtering.c:55-100) ------------------------- *
+ * A walk starts at the pixel's levelroot, climbs to the first node already reached (the root when
+ * none is), never testing the root, and writes the pixels and nodes up to the highest failing node it
+ * passed: g(parent) of that node.  A walk that passed no failing node writes only its own levelroot,
+ * with its grey.  A walk that stops at a reached node whose out differs from its grey copies that
+ * out.  The result is the textbook min rule with the root left out (a node's value is g(parent(h))
+ * for the highest failing node h below the root on its path, its own grey when there is none), except
+ * for zeros spread by a failing root of nonzero grey.  Such a root is written 0 by its own levelroot
+ * pixel, at time R.  A walk that later stops at it, or at a node already zeroed, writes 0.
+ * Solved top-down, with t(x) the time x is written and tau(x) = min t over x's strict ancestors:
+ *   - a node with a failing node at or above it (below the root) is written by the first walk that
+ *     enters it, that of first(x); t = first(x);
+ *   - a node without one is written at its own levelroot pixel, or earlier by a walk that stops at a
+ *     zeroed node;
+ *   - a newly reached node copies the status of the lowest reached ancestor above it.  So every node
+ *     reached on a path below x after tau(x) has the status W(x) of x's first-reached ancestor;
+ *   - zeroed: W(x) and tau(x) < first(x) for the first kind, W(x) and tau(x) < levelroot(x) for the
+ *     second.  For a zeroed node of the second kind, t = max(first, tau + 1) stands in for its write
+ *     time: its descendants' answers are the same for any time from there to the true one. */
+__global__ __launch_bounds__(1024) void k_min_sweep(const uint32_t *__restrict__ order, const uint32_t *__restrict__ start,
+                                                    uint32_t maxd, const int32_t *__restrict__ npar, const uint8_t *__restrict__ pass,
+                                                    const float *__restrict__ ngrey, const uint32_t *__restrict__ first,
+                                                    const uint32_t *__restrict__ lr, uint32_t *tw, uint32_t *tau, int32_t *hf,
+                                                    uint32_t *st, float *__restrict__ val)
+{
+    for (uint32_t d = 0; d <= maxd; d++) {
+        const uint32_t a = start[d], b = start[d + 1];
+        for (uint32_t i = a + threadIdx.x; i < b; i += blockDim.x) {
+            const uint32_t u = order[i];
+            const int32_t p = npar[u];
+            const bool fail = !pass[u];
+            uint32_t t_u, tau_u, s_u; /* s: bit 0 W (first-reached ancestor zeroed), bit 1 zeroed */
+            int32_t h_u;
+            float v;
+            if (p < 0) {
+                t_u = lr[u];
+                tau_u = DCF_NONE32;
+                s_u = (fail && ngrey[u] != 0.0f) ? 2u : 0u;
+                h_u = -1;
+                v = fail ? 0.0f : ngrey[u];
+            } else {
+                const uint32_t tp = __hip_atomic_load(&tw[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const uint32_t taup = __hip_atomic_load(&tau[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const uint32_t sp = __hip_atomic_load(&st[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const int32_t hp = __hip_atomic_load(&hf[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                tau_u = min(tp, taup);
+                const bool W = tp <= taup ? (sp & 2u) != 0 : (sp & 1u) != 0;
+                h_u = hp >= 0 ? hp : (fail ? (int32_t)u : -1);
+                bool Z;
+                if (h_u >= 0) {
+                    Z = W && tau_u < first[u];
+                    t_u = first[u];
+                } else {
+                    Z = W && tau_u < lr[u];
+                    t_u = Z ? max(first[u], tau_u + 1) : lr[u];
+                }
+                s_u = (W ? 1u : 0u) | (Z ? 2u : 0u);
+                v = Z ? 0.0f : (h_u >= 0 ? ngrey[npar[h_u]] : ngrey[u]);
+            }
+            __hip_atomic_store(&tw[u], t_u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&tau[u], tau_u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&st[u], s_u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&hf[u], h_u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            val[u] = v;
+        }
+        __threadfence();
+        __syncthreads();
+    }
+}
+
+static int min_general(dcf_tree *t, const AttrCtx &c, int attr_idx, double lambda, float *val)
+{
+    dcf_ctx *ctx = t->ctx;
+    const int64_t K = t->K;
+    uint8_t *pass = (uint8_t *)dcf_alloc(ctx, (size_t)K);
+    int32_t *J = (int32_t *)dcf_alloc(ctx, (size_t)K * 4), *hf = (int32_t *)dcf_alloc(ctx, (size_t)K * 4);
+    uint32_t *first = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4), *Q = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4);
+    uint32_t *lr = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4), *tw = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4);
+    uint32_t *tau = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4), *st = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4);
+    uint32_t *order = nullptr, *start = nullptr, maxd = 0;
+    int rc = DCF_OK;
+    if (!pass || !J || !hf || !first || !Q || !lr || !tw || !tau || !st) rc = dcf_set_error(ctx, DCF_ENOMEM, "min rule scratch");
+    if (rc == DCF_OK) rc = refuse_zero_grey(t, "min");
+    if (rc == DCF_OK) {
+        hipLaunchKernelGGL(k_direct_init, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, c, attr_idx, lambda, t->nlam, t->npar, J, pass, K);
+        hipLaunchKernelGGL(k_fill_u32, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, lr, DCF_NONE32, K);
+        hipLaunchKernelGGL(k_fill_u32, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, Q, DCF_NONE32, K);
+        hipLaunchKernelGGL(k_levelroot_pixel, dim3(nblk(t->n, 256)), dim3(256), 0, ctx->stream, (const uint32_t *)t->p2n, lr, t->n);
+        if (hipMemcpyAsync(first, lr, (size_t)K * 4, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
+            rc = dcf_set_error(ctx, DCF_EDEVICE, "levelroot copy");
+    }
+    if (rc == DCF_OK) rc = depth_order(t, &order, &start, &maxd);
+    if (rc == DCF_OK) {
+        hipLaunchKernelGGL(k_first_sweep, dim3(1), dim3(1024), 0, ctx->stream, (const uint32_t *)order, (const uint32_t *)start, maxd,
+                           (const int32_t *)t->npar, (const uint8_t *)pass, first, Q);
+        hipLaunchKernelGGL(k_min_sweep, dim3(1), dim3(1024), 0, ctx->stream, (const uint32_t *)order, (const uint32_t *)start, maxd,
+                           (const int32_t *)t->npar, (const uint8_t *)pass, (const float *)t->ngrey, (const uint32_t *)first,
+                           (const uint32_t *)lr, tw, tau, hf, st, val);
+        if (hipGetLastError() != hipSuccess) rc = dcf_set_error(ctx, DCF_EDEVICE, "min rule launches");
+    }
+    dcf_release(ctx, pass); dcf_release(ctx, J); dcf_release(ctx, hf); dcf_release(ctx, first); dcf_release(ctx, Q);
+    dcf_release(ctx, lr); dcf_release(ctx, tw); dcf_release(ctx, tau); dcf_release(ctx, st);
+    dcf_release(ctx, order); dcf_release(ctx, start);
+    return rc;
+}
+
 static int rule_values(dcf_tree *t, int attr_idx, int decision, double lambda, unsigned flags, float *val, uint32_t **kq,
                        uint32_t **klr)
 {
@@ -758,13 +873,14 @@ static int rule_values(dcf_tree *t, int attr_idx, int decision, double lambda, u
     /* For an increasing attribute (area / volume, idx 0) a node passes only if its parent passes, so
      * the min, max and subtractive rules (filtering.c:55-261) all give the direct rule's output, except
      * the min rule under a failing root (k_min_rootfail above); the subtractive kernels can still be
-     * forced with DCF_ALL_RULES.  The max rule of a non-increasing attribute (or of any attribute
-     * under DCF_ALL_RULES) follows the reference's raster scan, stale values included (max_general
-     * above).  The min rule of a non-increasing attribute is not implemented on this path. */
+     * forced with DCF_ALL_RULES.  The max and min rules of a non-increasing attribute (or of any
+     * attribute under DCF_ALL_RULES) follow the reference's raster scan (max_general, min_general
+     * above). */
     /* float greys: the subtractive rule's float rounding differs from the direct rule's copies */
     const bool float_grey = t->dtype == DCF_F32;
     bool general_sub = decision == DCF_SUBTRACTIVE && (float_grey || !(attr_idx == 0 && !(flags & DCF_ALL_RULES)));
     if (decision == DCF_MAX && (attr_idx != 0 || (flags & DCF_ALL_RULES))) return max_general(t, c, attr_idx, lambda, val, kq, klr);
+    if (decision == DCF_MIN && (attr_idx != 0 || (flags & DCF_ALL_RULES))) return min_general(t, c, attr_idx, lambda, val);
     bool min_max_inc = (decision == DCF_MIN || decision == DCF_MAX) && attr_idx == 0;
     if (decision == DCF_DIRECT || (decision == DCF_SUBTRACTIVE && !general_sub) || min_max_inc) {
         int32_t *J = (int32_t *)dcf_alloc(ctx, (size_t)K * 4);
@@ -827,9 +943,7 @@ static int rule_values(dcf_tree *t, int attr_idx, int decision, double lambda, u
         dcf_release(ctx, F0); dcf_release(ctx, F1); dcf_release(ctx, A0); dcf_release(ctx, A1);
         return DCF_OK;
     }
-    return dcf_set_error(ctx, DCF_EUNSUPPORTED,
-                         "decision %d (%s) is implemented on the GPU path for the increasing area attribute only", decision,
-                         dcf_decision_name(decision));
+    return dcf_set_error(ctx, DCF_EINVAL, "decision %d", decision);
 }
 
 int dcf_ops_filter(dcf_tree *t, int attr_idx, int decision, double lambda, int extract, void *out_dev, dcf_dtype out_dtype,

@@ -92,13 +92,16 @@ def test_non_area_attribute(ctx):
     check_batch(ctx, imgs, lams, 4, "max", decision=0, attr=17, label="ncomp")
 
 
-def test_max_rule_non_increasing(ctx):
-    """the max rule of non-compactness with per-image lambdas (the reference's raster-scan result per image)"""
+def test_max_and_min_rules_non_increasing(ctx):
+    """the max and min rules of non-compactness with per-image lambdas (the reference's raster-scan result
+    per image); the images' greys start at 1, so a failing max-tree root spreads zeros"""
     rng = np.random.default_rng(96)
     imgs, _ = random_batch(rng, 8, maxside=50)
+    imgs = [np.maximum(a, 1) for a in imgs]
     lams = [0.2, 0.3, 0.5, 0.25, 1.0, 0.18, 0.4, 0.22]
-    check_batch(ctx, imgs, lams, 8, "max", decision=2, attr=17, label="max ncomp")
-    check_batch(ctx, imgs, lams, 4, "min", decision=2, attr=17, extract=True, label="max ncomp min-tree")
+    for dec in (1, 2):
+        check_batch(ctx, imgs, lams, 8, "max", decision=dec, attr=17, label="rule %d ncomp" % dec)
+        check_batch(ctx, imgs, lams, 4, "min", decision=dec, attr=17, extract=True, label="rule %d ncomp min-tree" % dec)
 
 
 def test_uneven_workload_batch(ctx):

@@ -85,6 +85,7 @@ def _compare_all(ctx, img, conn, tree_type, lam_area=20, lam_nc=0.25, label=""):
     for dec in (1, 2):
         assert_same(t.filter(0, lam_area, dec), o.filter(0, lam_area, dec), "filter area d%d %s" % (dec, label))
     assert_same(t.filter(0, lam_area, 2, all_rules=True), o.filter(0, lam_area, 2), "general max " + label)
+    assert_same(t.filter(0, lam_area, 1, all_rules=True), o.filter(0, lam_area, 1), "general min " + label)
     assert_same(t.filter(0, lam_area, 0, extract=True), o.filter(0, lam_area, 0, extract=True), "extract " + label)
     lv = synth.log_lambdas(16, 1, 2 ** 14)
     assert_same(t.spectrum1d(0, lv), o.spectrum1d(0, lv), "spectrum " + label)
@@ -92,7 +93,7 @@ def _compare_all(ctx, img, conn, tree_type, lam_area=20, lam_nc=0.25, label=""):
     t.accumulate(17)
     assert_same(t.check(17), o.check(17), "ncomp " + label)
     assert_same(t.check(13), o.check(13), "trace " + label)
-    for dec in (0, 2, 3):
+    for dec in (0, 1, 2, 3):
         assert_same(t.filter(17, lam_nc, dec), o.filter(17, lam_nc, dec), "filter ncomp d%d %s" % (dec, label))
     l1 = synth.log_lambdas(8, 0.05, 2.0)
     l2 = synth.log_lambdas(8, 1, 4096)
@@ -177,17 +178,21 @@ def test_min_rule_when_the_root_fails(ctx, tree_type):
         t.free()
 
 
-def _node_values(o, aidx):
+def _node_values(o, aidx, exact=True):
+    """the distinct attribute values of the nodes; exact=False gives the midpoints between them, for an
+    attribute whose GPU and CPU values may differ in the last bit (3D non-compactness divides by
+    pow(A, 5/3), whose last bit differs between math libraries)"""
     par, g = o.parent(), o.grey()
     a, _ = o.attributes(aidx)
-    return np.unique(a[(par < 0) | (g[np.maximum(par, 0)] != g)])
+    vals = np.unique(a[(par < 0) | (g[np.maximum(par, 0)] != g)])
+    return vals if exact or len(vals) < 2 else (vals[:-1] + vals[1:]) / 2
 
 
 @pytest.mark.parametrize("tree_type", ["max", "min"])
 def test_max_rule_of_non_increasing_attributes(ctx, tree_type):
     """the max rule of non-increasing attributes follows the reference's raster scan (filtering.c:156-208),
     including the stale values of pixels visited before their node is kept; lambdas sit exactly on node
-    values"""
+    values (between them for 3D non-compactness)"""
     rng = np.random.default_rng(91 if tree_type == "max" else 92)
     cases = [(rng.integers(0, 12, size=(37, 29)).astype(np.uint8), 4),
              (rng.integers(0, 9, size=(41, 23)).astype(np.uint8), 8),
@@ -198,7 +203,7 @@ def test_max_rule_of_non_increasing_attributes(ctx, tree_type):
         for aidx in (17, 13, 2, 6):
             o = OracleTree(img, conn, tree_type, aidx)
             t = ctx.build(img, conn, tree_type, aidx)
-            vals = _node_values(o, aidx)
+            vals = _node_values(o, aidx, exact=not (img.ndim == 3 and aidx == 17))
             for lam in (vals[len(vals) // 2], vals[len(vals) // 4], vals[-1] + 1, rng.choice(vals)):
                 say("max rule", img.dtype, img.shape, conn, tree_type, aidx, lam)
                 assert_same(t.filter(aidx, lam, 2), o.filter(aidx, lam, 2), "max rule a%d lambda %r" % (aidx, lam))
@@ -208,18 +213,48 @@ def test_max_rule_of_non_increasing_attributes(ctx, tree_type):
 
 
 @pytest.mark.parametrize("tree_type", ["max", "min"])
-def test_max_rule_vnoise_1024(ctx, tree_type):
-    """the max rule of non-compactness on a 1024^2 u16 value-noise image (deep tree, many nodes)"""
+def test_min_rule_of_non_increasing_attributes(ctx, tree_type):
+    """the min rule of non-increasing attributes follows the reference's raster scan (filtering.c:55-100),
+    including the zeros a failing root of nonzero grey spreads; half the lambdas make the root fail"""
+    rng = np.random.default_rng(94 if tree_type == "max" else 95)
+    cases = [(rng.integers(1, 12, size=(37, 29)).astype(np.uint8), 4),
+             (rng.integers(2, 9, size=(41, 23)).astype(np.uint8), 8),
+             (rng.integers(100, 140, size=(33, 35)).astype(np.uint16), 8),
+             (rng.integers(1, 6, size=(7, 9, 11)).astype(np.uint8), 6),
+             (rng.integers(1, 5, size=(6, 8, 9)).astype(np.uint8), 26)]
+    for img, conn in cases:
+        for aidx in (17, 13, 2, 6):
+            o = OracleTree(img, conn, tree_type, aidx)
+            t = ctx.build(img, conn, tree_type, aidx)
+            vals = _node_values(o, aidx, exact=not (img.ndim == 3 and aidx == 17))
+            par = o.parent()
+            a, _ = o.attributes(aidx)
+            above = vals[vals > a[par < 0][0]]
+            lams = [vals[len(vals) // 2], vals[len(vals) // 4], vals[-1] + 1]
+            if len(above):
+                lams += [above[0], above[len(above) // 2]]
+            for lam in lams:
+                say("min rule", img.dtype, img.shape, conn, tree_type, aidx, lam)
+                assert_same(t.filter(aidx, lam, 1), o.filter(aidx, lam, 1), "min rule a%d lambda %r" % (aidx, lam))
+            lam = lams[-1]
+            assert_same(t.filter(aidx, lam, 1, extract=True), o.filter(aidx, lam, 1, extract=True), "min extract a%d" % aidx)
+            t.free()
+
+
+@pytest.mark.parametrize("tree_type", ["max", "min"])
+def test_max_and_min_rules_vnoise_1024(ctx, tree_type):
+    """the max and min rules of non-compactness on a 1024^2 u16 value-noise image (deep tree, many nodes)"""
     img = synth.vnoise(1024, 1024)
     o = OracleTree(img, 8, tree_type, 17, threads=8)
     t = ctx.build(img, 8, tree_type, 17)
-    for lam in (0.2, 0.5, 3.0):
-        say("max rule vnoise 1024", tree_type, lam)
-        assert_same(t.filter(17, lam, 2), o.filter(17, lam, 2), "max rule vnoise lambda %g" % lam)
+    for dec in (1, 2):
+        for lam in (0.2, 0.5, 3.0):
+            say("rule vnoise 1024", tree_type, dec, lam)
+            assert_same(t.filter(17, lam, dec), o.filter(17, lam, dec), "rule %d vnoise lambda %g" % (dec, lam))
     t.free()
 
 
-def test_max_rule_float(ctx):
+def test_min_and_max_rules_float(ctx):
     """float greys: positive noise as max- and min-tree; negatives with zeros as a min-tree"""
     rng = np.random.default_rng(93)
     pos = (rng.random((31, 37)) * 50 + 1).astype(np.float32)
@@ -228,9 +263,10 @@ def test_max_rule_float(ctx):
         o = OracleTree(img, 4, tt, 17)
         t = ctx.build(img, 4, tt, 17)
         vals = _node_values(o, 17)
-        for lam in (vals[len(vals) // 2], vals[len(vals) // 5]):
-            say("max rule float", tt, lam)
-            assert_same(t.filter(17, lam, 2), o.filter(17, lam, 2), "max rule float %s lambda %r" % (tt, lam))
+        for dec in (1, 2):
+            for lam in (vals[len(vals) // 2], vals[len(vals) // 5], vals[-1]):
+                say("rule float", tt, dec, lam)
+                assert_same(t.filter(17, lam, dec), o.filter(17, lam, dec), "rule %d float %s lambda %r" % (dec, tt, lam))
         t.free()
 
 
@@ -491,17 +527,14 @@ def test_errors(ctx, tmp_path):
     with pytest.raises(dcf.DcfError) as e:
         ctx.run_operations(img, [("csl", 0, "x")])
     assert e.value.code == -4
-    # the min rule of a non-increasing attribute depends on the reference's scan order: refused
-    t17 = ctx.build(img, 4, "max", 17)
-    with pytest.raises(dcf.DcfError) as e:
-        t17.filter(17, 0.3, 1)
-    assert e.value.code == -4
-    # the max rule on a float max-tree with a node of grey 0 above its negative root: refused
+    # the min and max rules of a non-increasing attribute on a float max-tree with a node of grey 0
+    # above its negative root: refused
     fimg = np.array([[-2.0, 0.0, 3.0], [0.0, 1.5, -2.0]], np.float32)
     tf = ctx.build(fimg, 4, "max", 17)
-    with pytest.raises(dcf.DcfError) as e:
-        tf.filter(17, 0.3, 2)
-    assert e.value.code == -4
+    for dec in (1, 2):
+        with pytest.raises(dcf.DcfError) as e:
+            tf.filter(17, 0.3, dec)
+        assert e.value.code == -4
 
 
 def test_c_caller_on_gpu(tmp_path):

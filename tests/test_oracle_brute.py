@@ -14,7 +14,10 @@ restate the oracle's own code:
 * the max rule of non-increasing attributes, whose reference result depends on the raster scan
   (stale values), against a closed form of that scan derived per node (first pixel of each subtree,
   first pixel of its passing subtrees); the cases are checked to include stale values, where the
-  textbook max rule differs.
+  textbook max rule differs;
+* the min rule of any attribute against a closed form of its raster scan: the textbook min rule with
+  the root left out, except for the zeros a failing root spreads, found top-down from write times; the
+  cases are checked to include such zeros.
 """
 import numpy as np
 import pytest
@@ -252,3 +255,93 @@ def test_max_rule_of_non_increasing_attributes():
         assert np.array_equal(got, want), (it, shape, conn, tt, aidx, lam)
         stale += not np.array_equal(got, text)
     assert stale >= 20, stale  # the scan's stale values are exercised, not only the textbook rule
+
+
+def min_rule_closed_form(parent, g, attr, lam):
+    """tree_filter_min (filtering.c:55-100) over one raster scan, solved per node, top-down.  Without
+    its root test the min rule gives g(parent(h)) for the highest failing node h below the root on a
+    node's path (its own grey when there is none; the root: g or 0).  A failing root of nonzero grey is
+    written 0 at its levelroot pixel R, and later walks stopping at it, or at a node already zeroed, copy
+    the 0.  t(x) is when x is written, tau(x) the earliest t of its strict ancestors, W(x) whether its
+    first-reached ancestor is zeroed; a node below a failing node is written at first(x) and zeroed when
+    W and tau < first(x); any other node is zeroed when W and tau < its levelroot, and else written at its
+    levelroot."""
+    n = len(parent)
+    is_lr = (parent < 0) | (g[np.maximum(parent, 0)] != g)
+    node = np.where(is_lr, np.arange(n), parent)
+    lrs = np.nonzero(is_lr)[0]
+    depth = {}
+    for x in lrs:
+        d, y = 0, x
+        while parent[y] >= 0:
+            y, d = parent[y], d + 1
+        depth[x] = d
+    order = sorted(lrs, key=lambda x: depth[x])
+    first = {x: x for x in lrs}
+    for x in reversed(order):
+        if parent[x] >= 0:
+            first[parent[x]] = min(first[parent[x]], first[x])
+    fail = {x: attr[x] < lam for x in lrs}
+    t, tau, W, Z, h, val = {}, {}, {}, {}, {}, {}
+    for x in order:
+        p = parent[x]
+        if p < 0:
+            t[x], tau[x], W[x], h[x] = x, NONE, False, -1
+            Z[x] = bool(fail[x] and g[x] != 0)
+            val[x] = 0.0 if fail[x] else g[x]
+            continue
+        tau[x] = min(t[p], tau[p])
+        W[x] = Z[p] if t[p] <= tau[p] else W[p]
+        h[x] = h[p] if h[p] >= 0 else (x if fail[x] else -1)
+        if h[x] >= 0:
+            Z[x] = W[x] and tau[x] < first[x]
+            t[x] = first[x]
+        else:
+            Z[x] = W[x] and tau[x] < x
+            t[x] = max(first[x], tau[x] + 1) if Z[x] else x
+        val[x] = 0.0 if Z[x] else (g[parent[h[x]]] if h[x] >= 0 else g[x])
+    return np.array([val[node[v]] for v in range(n)], np.float32)
+
+
+def textbook_min_rule_without_root(parent, g, attr, lam):
+    n = len(parent)
+    is_lr = (parent < 0) | (g[np.maximum(parent, 0)] != g)
+    out = np.empty(n, np.float32)
+    for v in range(n):
+        y = v if is_lr[v] else parent[v]
+        h, x = -1, y
+        while parent[x] >= 0:
+            if attr[x] < lam:
+                h = x
+            x = parent[x]
+        out[v] = g[parent[h]] if h >= 0 else (g[y] if (y != x or attr[y] >= lam) else 0)
+    return out
+
+
+def test_min_rule_of_any_attribute():
+    rng = np.random.default_rng(78)
+    zeros = 0
+    for it in range(150):
+        if it % 3 == 2:
+            shape, conn = tuple(int(s) for s in rng.integers(2, 7, size=3)), int(rng.choice([6, 26]))
+        else:
+            shape, conn = tuple(int(s) for s in rng.integers(2, 14, size=2)), int(rng.choice([4, 8]))
+        tt = "max" if it % 4 else "min"
+        img = rand_img(rng, shape, int(rng.integers(3, 10))) + np.uint8(tt == "max")  # a max-tree root above 0
+        aidx = int(rng.choice([17, 13, 6, 2, 0]))
+        o = OracleTree(img, conn, tt, aidx, threads=1)
+        par, g = o.parent(), o.grey()
+        a, _ = o.attributes(aidx)
+        lr = (par < 0) | (g[np.maximum(par, 0)] != g)
+        vals = np.unique(a[lr])
+        above_root = vals[vals > a[par < 0][0]]
+        # half the lambdas make the root fail
+        lam = float(rng.choice(above_root if len(above_root) and it % 2 else vals))
+        got = o.filter(aidx, lam, 1).reshape(-1)
+        want = min_rule_closed_form(par, g, a, lam)
+        text = textbook_min_rule_without_root(par, g, a, lam)
+        if tt == "min":
+            want, text = o.gmax - want, o.gmax - text
+        assert np.array_equal(got, want), (it, shape, conn, tt, aidx, lam)
+        zeros += not np.array_equal(got, text)
+    assert zeros >= 10, zeros  # the zeros of a failing root are exercised
