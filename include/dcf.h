@@ -107,10 +107,11 @@ int dcf_tree_num_nodes(dcf_tree *tree, int64_t *num_nodes);
 /* ---- filter (tree_filtering, filtering.c:18; perform_filter_or_extract, operation.c:10-29) ------ *
  * out: one value per pixel in out_dtype (DCF_F32 gives the reference's float output; DCF_U8/U16
  * are exact for integer images of that range).  extract != 0 gives g - f; a min-tree's output is
- * re-inverted gmax - f, as the reference.  The max rule gives the reference's single-thread scan
- * result (tree_filter_max, filtering.c:156-208, stale values included).  DCF_EUNSUPPORTED: the min
- * rule of a non-increasing attribute (any attribute but area / volume), and the max rule of a
- * non-increasing attribute on a float max-tree with a node of grey 0 below a negative root. */
+ * re-inverted gmax - f, as the reference.  The min and max rules give the reference's single-thread
+ * scan result (tree_filter_min / tree_filter_max, filtering.c:55-208: the max rule's stale values,
+ * the min rule's zeros below a failing root).  DCF_EUNSUPPORTED: either rule of a non-increasing
+ * attribute (any attribute but area / volume) on a float max-tree with a node of grey 0 below a
+ * negative root. */
 int dcf_filter(dcf_tree *tree, int attr_idx, int decision, double lambda, int extract, void *out,
                dcf_dtype out_dtype, unsigned flags);
 
