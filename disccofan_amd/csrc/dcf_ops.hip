@@ -633,38 +633,260 @@ __global__ __launch_bounds__(256) void k_zero_grey(const int32_t *__restrict__ n
     if (k < K && npar[k] >= 0 && ngrey[k] == 0.0f) __hip_atomic_store(flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-/* lr[k] = the smallest pixel index of node k's own level (its levelroot) */
-__global__ __launch_bounds__(256) void k_levelroot_pixel(const uint32_t *__restrict__ p2n, uint32_t *__restrict__ lr, int64_t n)
+/* lr[k] = the levelroot pixel of node k (the pixel whose canonical parent lies in another node) */
+__global__ __launch_bounds__(256) void k_levelroot_pixel(const int32_t *__restrict__ parent, const uint32_t *__restrict__ p2n,
+                                                         uint32_t *__restrict__ lr, int64_t n)
 {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < n) atomicMin(&lr[p2n[p]], (uint32_t)p);
+    if (p >= n) return;
+    const int32_t r = parent[p];
+    const uint32_t k = p2n[p];
+    if (r < 0 || p2n[r] != k) lr[k] = (uint32_t)p;
 }
 
-/* bottom-up over the depths, one workgroup: first[] holds the levelroot pixels on entry and the
- * subtree minima on exit; Q[] holds DCF_NONE32 on entry.  A node's children are one depth deeper, so
- * its first and Q are complete when its depth comes up.  The minima are atomics in L2; the reads go
- * there too, and a device fence and a barrier separate consecutive depths. */
-__global__ __launch_bounds__(1024) void k_first_sweep(const uint32_t *__restrict__ order, const uint32_t *__restrict__ start,
-                                                      uint32_t maxd, const int32_t *__restrict__ npar, const uint8_t *__restrict__ pass,
-                                                      uint32_t *first, uint32_t *Q)
+/* ---- subtree minima in log depth ---------------------------------------------------------------- *
+ * The pixels are laid out in preorder: a node's own pixels, then its children's subtrees in order of
+ * node id.  Subtree sizes are the accumulated areas, so a node's position is the sum, up its path, of
+ * its parent's own-pixel count plus the areas of its earlier siblings (a sort by parent, a scan, then
+ * pointer doubling).  Sorted by position, the nodes of every subtree are one contiguous run, and a
+ * minimum over a subtree is a range minimum (sparse table, O(1) per node).  No step depends on the
+ * depth of the tree. */
+__global__ __launch_bounds__(256) void k_child_area(const int32_t *__restrict__ npar, const uint32_t *__restrict__ area,
+                                                    uint32_t *__restrict__ csum, int64_t K)
 {
-    for (int64_t d = maxd; d >= 0; d--) {
-        const uint32_t a = start[d], b = start[d + 1];
-        for (uint32_t i = a + threadIdx.x; i < b; i += blockDim.x) {
-            const uint32_t u = order[i];
-            const uint32_t f = __hip_atomic_load(&first[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            uint32_t q = __hip_atomic_load(&Q[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (pass[u]) q = f; /* first(u) is below every first of its subtree */
-            __hip_atomic_store(&Q[u], q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const int32_t p = npar[u];
-            if (p >= 0) {
-                atomicMin(&first[p], f);
-                if (q != DCF_NONE32) atomicMin(&Q[p], q);
-            }
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < K && npar[k] >= 0) atomicAdd(&csum[npar[k]], area[k]);
+}
+
+/* sort keys: parent + 1 (0 for the roots, siblings under one virtual root) */
+__global__ __launch_bounds__(256) void k_parent_keys(const int32_t *__restrict__ npar, uint32_t *__restrict__ key, uint32_t *__restrict__ id,
+                                                     int64_t K)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    key[k] = (uint32_t)(npar[k] + 1);
+    id[k] = (uint32_t)k;
+}
+
+__global__ __launch_bounds__(256) void k_gather_area(const uint32_t *__restrict__ id, const uint32_t *__restrict__ area,
+                                                     uint64_t *__restrict__ out, int64_t K)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < K) out[i] = area[id[i]];
+}
+
+/* base[key] = the scan value at the start of the key's run of siblings */
+__global__ __launch_bounds__(256) void k_seg_base(const uint32_t *__restrict__ key, const uint64_t *__restrict__ E, uint64_t *__restrict__ base,
+                                                  int64_t K)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < K && (i == 0 || key[i - 1] != key[i])) base[key[i]] = E[i];
+}
+
+/* off(x) = own pixels of x's parent + areas of x's earlier siblings; A = the parent, for the path sum */
+__global__ __launch_bounds__(256) void k_offsets(const uint32_t *__restrict__ key, const uint32_t *__restrict__ id,
+                                                 const uint64_t *__restrict__ E, const uint64_t *__restrict__ base,
+                                                 const uint32_t *__restrict__ area, const uint32_t *__restrict__ csum,
+                                                 const int32_t *__restrict__ npar, uint32_t *__restrict__ off, int32_t *__restrict__ A,
+                                                 int64_t K)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= K) return;
+    const uint32_t kk = key[i], x = id[i];
+    const uint32_t own_parent = kk == 0 ? 0u : area[kk - 1] - csum[kk - 1];
+    off[x] = own_parent + (uint32_t)(E[i] - base[kk]);
+    A[x] = npar[x];
+}
+
+/* the "another round is needed" flag, set once per round */
+__device__ __forceinline__ void round_flag(uint32_t *active, int round, bool more)
+{
+    const unsigned long long b = __ballot(more);
+    if ((threadIdx.x & (warpSize - 1)) == 0 && b &&
+        __hip_atomic_load(&active[round], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0u)
+        __hip_atomic_store(&active[round], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+/* path sums: S covers [k, A(k)) */
+__global__ __launch_bounds__(256) void k_psum_round(const uint32_t *__restrict__ S, uint32_t *__restrict__ S2, const int32_t *__restrict__ A,
+                                                    int32_t *__restrict__ A2, uint32_t *__restrict__ active, int round, int64_t K)
+{
+    if (round > 0 && active[round - 1] == 0) return;
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int32_t a2 = -1;
+    if (k < K) {
+        const int32_t a = A[k];
+        uint32_t v = S[k];
+        if (a >= 0) {
+            v += S[a];
+            a2 = A[a];
         }
-        __threadfence();
-        __syncthreads();
+        S2[k] = v;
+        A2[k] = a2;
     }
+    round_flag(active, round, a2 >= 0);
+}
+
+__global__ __launch_bounds__(256) void k_iota_u32(uint32_t *__restrict__ id, int64_t K)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < K) id[k] = (uint32_t)k;
+}
+
+/* rank of every node in preorder, the length of its subtree's run, and level 0 of the table (vals
+ * gathered into preorder) */
+__global__ __launch_bounds__(256) void k_preorder(const uint32_t *__restrict__ spos, const uint32_t *__restrict__ order,
+                                                  const uint32_t *__restrict__ area, uint32_t *__restrict__ rank, uint32_t *__restrict__ cnt,
+                                                  int64_t K)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= K) return;
+    const uint32_t x = order[i];
+    const uint64_t end = (uint64_t)spos[i] + area[x];
+    /* first position at or past the subtree's end */
+    uint64_t lo = (uint64_t)i + 1, hi = (uint64_t)K;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if ((uint64_t)spos[mid] < end) lo = mid + 1;
+        else hi = mid;
+    }
+    rank[x] = (uint32_t)i;
+    cnt[x] = (uint32_t)(lo - (uint64_t)i);
+}
+
+__global__ __launch_bounds__(256) void k_gather_pre(const uint32_t *__restrict__ order, const uint32_t *__restrict__ v,
+                                                    const uint8_t *__restrict__ pass, uint32_t *__restrict__ T0, int64_t K)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= K) return;
+    const uint32_t x = order[i];
+    T0[i] = (pass == nullptr || pass[x]) ? v[x] : DCF_NONE32;
+}
+
+/* level j of the sparse table: the minimum over [i, i + 2^j) */
+__global__ __launch_bounds__(256) void k_rmq_level(const uint32_t *__restrict__ prev, uint32_t *__restrict__ cur, int64_t half, int64_t m)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) cur[i] = min(prev[i], prev[i + half]);
+}
+
+__global__ __launch_bounds__(256) void k_rmq_query(const uint32_t *__restrict__ T, const uint32_t *__restrict__ rank,
+                                                   const uint32_t *__restrict__ cnt, uint32_t *__restrict__ out, int64_t K)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    const uint32_t l = rank[k], c = cnt[k];
+    const int j = 31 - __clz(c);
+    const uint32_t *Tj = T + (size_t)j * (size_t)K;
+    out[k] = min(Tj[l], Tj[l + c - (1u << j)]);
+}
+
+/* runs a ping-pong doubling to convergence: launch(r, in, out) for rounds r = 0.. (a round returns at
+ * once after the flags say the previous one changed nothing); returns which buffer (0 / 1) holds the
+ * result */
+template <typename F>
+static int doubling(dcf_tree *t, F launch, int *which)
+{
+    dcf_ctx *ctx = t->ctx;
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, t->counters, 0u, (int64_t)64);
+    for (int r = 0; r < t->max_rounds; r++) launch(r, r & 1);
+    uint32_t act[64];
+    if (hipGetLastError() != hipSuccess) return dcf_set_error(ctx, DCF_EDEVICE, "doubling launches");
+    if (hipMemcpyAsync(act, t->counters, sizeof act, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+        return dcf_set_error(ctx, DCF_EDEVICE, "doubling rounds");
+    int ran = 0;
+    for (int r = 0; r < t->max_rounds; r++) { ran = r + 1; if (act[r] == 0) break; }
+    if (act[t->max_rounds - 1] != 0) return dcf_set_error(ctx, DCF_EDEVICE, "tree deeper than the doubling rounds");
+    *which = ran & 1; /* round r reads buffer r & 1 and writes the other */
+    return DCF_OK;
+}
+
+/* lr = the levelroot pixel of every node, first = the first pixel of its subtree, Q = the smallest
+ * first of a passing node in its subtree (DCF_NONE32: none); pass flags given */
+static int subtree_firsts(dcf_tree *t, const uint8_t *pass, uint32_t *lr, uint32_t *first, uint32_t *Q)
+{
+    dcf_ctx *ctx = t->ctx;
+    const int64_t K = t->K;
+    const dim3 g(nblk(K, 256)), b(256);
+    int levels = 1;
+    while (((int64_t)1 << levels) <= K) levels++; /* levels j = 0 .. levels - 1, 2^j <= K */
+    uint32_t *csum = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4);
+    uint32_t *k0 = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4), *k1 = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4);
+    uint32_t *v0 = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4), *v1 = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4);
+    uint64_t *sa = (uint64_t *)dcf_alloc(ctx, (size_t)K * 8), *E = (uint64_t *)dcf_alloc(ctx, (size_t)K * 8);
+    uint64_t *base = (uint64_t *)dcf_alloc(ctx, ((size_t)K + 1) * 8);
+    uint32_t *S[2] = { (uint32_t *)dcf_alloc(ctx, (size_t)K * 4), (uint32_t *)dcf_alloc(ctx, (size_t)K * 4) };
+    int32_t *A[2] = { (int32_t *)dcf_alloc(ctx, (size_t)K * 4), (int32_t *)dcf_alloc(ctx, (size_t)K * 4) };
+    uint32_t *rank = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4), *cnt = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4);
+    uint32_t *T = (uint32_t *)dcf_alloc(ctx, (size_t)levels * (size_t)K * 4);
+    void *tmp = nullptr;
+    const uint32_t *area = t->attr.area;
+    int rc = DCF_OK, sw = 0;
+    if (!csum || !k0 || !k1 || !v0 || !v1 || !sa || !E || !base || !S[0] || !S[1] || !A[0] || !A[1] || !rank || !cnt || !T)
+        rc = dcf_set_error(ctx, DCF_ENOMEM, "subtree scratch");
+    /* scratch for the larger of the radix sort and the scan */
+    size_t need_sort = 0, need_scan = 0;
+    if (rc == DCF_OK &&
+        (hipcub::DeviceRadixSort::SortPairs(nullptr, need_sort, k0, k1, v0, v1, (int)K, 0, 32, ctx->stream) != hipSuccess ||
+         hipcub::DeviceScan::ExclusiveSum(nullptr, need_scan, sa, E, (int)K, ctx->stream) != hipSuccess))
+        rc = dcf_set_error(ctx, DCF_EDEVICE, "subtree scratch size");
+    const size_t need = need_sort > need_scan ? need_sort : need_scan;
+    if (rc == DCF_OK && !(tmp = dcf_alloc(ctx, need ? need : 16))) rc = dcf_set_error(ctx, DCF_ENOMEM, "subtree sort scratch");
+    if (rc == DCF_OK) {
+        hipLaunchKernelGGL(k_levelroot_pixel, dim3(nblk(t->n, 256)), b, 0, ctx->stream, (const int32_t *)t->parent, (const uint32_t *)t->p2n,
+                           lr, t->n);
+        hipLaunchKernelGGL(k_fill_u32, g, b, 0, ctx->stream, csum, 0u, K);
+        hipLaunchKernelGGL(k_child_area, g, b, 0, ctx->stream, (const int32_t *)t->npar, area, csum, K);
+        /* siblings together, in order of node id (the sort is stable) */
+        hipLaunchKernelGGL(k_parent_keys, g, b, 0, ctx->stream, (const int32_t *)t->npar, k0, v0, K);
+        size_t ns = need;
+        if (hipcub::DeviceRadixSort::SortPairs(tmp, ns, k0, k1, v0, v1, (int)K, 0, 32, ctx->stream) != hipSuccess)
+            rc = dcf_set_error(ctx, DCF_EDEVICE, "sort by parent");
+    }
+    if (rc == DCF_OK) {
+        hipLaunchKernelGGL(k_gather_area, g, b, 0, ctx->stream, (const uint32_t *)v1, area, sa, K);
+        size_t ns = need;
+        if (hipcub::DeviceScan::ExclusiveSum(tmp, ns, sa, E, (int)K, ctx->stream) != hipSuccess)
+            rc = dcf_set_error(ctx, DCF_EDEVICE, "sibling scan");
+    }
+    if (rc == DCF_OK) {
+        hipLaunchKernelGGL(k_seg_base, g, b, 0, ctx->stream, (const uint32_t *)k1, (const uint64_t *)E, base, K);
+        hipLaunchKernelGGL(k_offsets, g, b, 0, ctx->stream, (const uint32_t *)k1, (const uint32_t *)v1, (const uint64_t *)E,
+                           (const uint64_t *)base, area, (const uint32_t *)csum, (const int32_t *)t->npar, S[0], A[0], K);
+        rc = doubling(t, [&](int r, int i) {
+            hipLaunchKernelGGL(k_psum_round, g, b, 0, ctx->stream, (const uint32_t *)S[i], S[1 - i], (const int32_t *)A[i], A[1 - i],
+                               t->counters, r, K);
+        }, &sw);
+    }
+    if (rc == DCF_OK) {
+        /* preorder: nodes sorted by position */
+        hipLaunchKernelGGL(k_iota_u32, g, b, 0, ctx->stream, v0, K);
+        size_t ns = need;
+        if (hipcub::DeviceRadixSort::SortPairs(tmp, ns, S[sw], k1, v0, v1, (int)K, 0, 32, ctx->stream) != hipSuccess)
+            rc = dcf_set_error(ctx, DCF_EDEVICE, "sort by position");
+    }
+    if (rc == DCF_OK) {
+        hipLaunchKernelGGL(k_preorder, g, b, 0, ctx->stream, (const uint32_t *)k1, (const uint32_t *)v1, area, rank, cnt, K);
+        /* first: range minima of the levelroot pixels; Q: of first over the passing nodes */
+        for (int pass_q = 0; pass_q < 2 && rc == DCF_OK; pass_q++) {
+            hipLaunchKernelGGL(k_gather_pre, g, b, 0, ctx->stream, (const uint32_t *)v1, pass_q ? (const uint32_t *)first : (const uint32_t *)lr,
+                               pass_q ? pass : (const uint8_t *)nullptr, T, K);
+            for (int j = 1; j < levels; j++) {
+                const int64_t half = (int64_t)1 << (j - 1), m = K - ((int64_t)1 << j) + 1;
+                hipLaunchKernelGGL(k_rmq_level, dim3(nblk(m, 256)), b, 0, ctx->stream, (const uint32_t *)(T + (size_t)(j - 1) * K),
+                                   T + (size_t)j * K, half, m);
+            }
+            hipLaunchKernelGGL(k_rmq_query, g, b, 0, ctx->stream, (const uint32_t *)T, (const uint32_t *)rank, (const uint32_t *)cnt,
+                               pass_q ? Q : first, K);
+            if (hipGetLastError() != hipSuccess) rc = dcf_set_error(ctx, DCF_EDEVICE, "subtree minima launches");
+        }
+    }
+    dcf_release(ctx, csum); dcf_release(ctx, k0); dcf_release(ctx, k1); dcf_release(ctx, v0); dcf_release(ctx, v1);
+    dcf_release(ctx, sa); dcf_release(ctx, E); dcf_release(ctx, base); dcf_release(ctx, rank); dcf_release(ctx, cnt);
+    dcf_release(ctx, T); dcf_release(ctx, tmp);
+    for (int i = 0; i < 2; i++) { dcf_release(ctx, S[i]); dcf_release(ctx, A[i]); }
+    return rc;
 }
 
 /* stop flags for V: J(k) -> the nearest ancestor-or-self c whose parent passes or is fixed before first(c) */
@@ -720,22 +942,14 @@ static int max_general(dcf_tree *t, const AttrCtx &c, int attr_idx, double lambd
     int32_t *J = (int32_t *)dcf_alloc(ctx, (size_t)K * 4);
     uint32_t *first = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4), *Q = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4);
     uint32_t *lr = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4);
-    uint32_t *order = nullptr, *start = nullptr, maxd = 0;
     int rc = DCF_OK;
     if (!pass || !stop || !J || !first || !Q || !lr) rc = dcf_set_error(ctx, DCF_ENOMEM, "max rule scratch");
     if (rc == DCF_OK) rc = refuse_zero_grey(t, "max");
     if (rc == DCF_OK) {
         hipLaunchKernelGGL(k_direct_init, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, c, attr_idx, lambda, t->nlam, t->npar, J, pass, K);
-        hipLaunchKernelGGL(k_fill_u32, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, lr, DCF_NONE32, K);
-        hipLaunchKernelGGL(k_fill_u32, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, Q, DCF_NONE32, K);
-        hipLaunchKernelGGL(k_levelroot_pixel, dim3(nblk(t->n, 256)), dim3(256), 0, ctx->stream, (const uint32_t *)t->p2n, lr, t->n);
-        if (hipMemcpyAsync(first, lr, (size_t)K * 4, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
-            rc = dcf_set_error(ctx, DCF_EDEVICE, "levelroot copy");
+        rc = subtree_firsts(t, pass, lr, first, Q);
     }
-    if (rc == DCF_OK) rc = depth_order(t, &order, &start, &maxd);
     if (rc == DCF_OK) {
-        hipLaunchKernelGGL(k_first_sweep, dim3(1), dim3(1024), 0, ctx->stream, (const uint32_t *)order, (const uint32_t *)start, maxd,
-                           (const int32_t *)t->npar, (const uint8_t *)pass, first, Q);
         hipLaunchKernelGGL(k_max_stop, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, (const int32_t *)t->npar, (const uint8_t *)pass,
                            (const uint32_t *)first, (const uint32_t *)Q, J, stop, K);
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, t->counters, 0u, (int64_t)64);
@@ -746,7 +960,6 @@ static int max_general(dcf_tree *t, const AttrCtx &c, int attr_idx, double lambd
         if (hipGetLastError() != hipSuccess) rc = dcf_set_error(ctx, DCF_EDEVICE, "max rule launches");
     }
     dcf_release(ctx, pass); dcf_release(ctx, stop); dcf_release(ctx, J); dcf_release(ctx, first);
-    dcf_release(ctx, order); dcf_release(ctx, start);
     if (rc == DCF_OK) {
         *kq = Q;
         *klr = lr;
@@ -765,101 +978,195 @@ static int max_general(dcf_tree *t, const AttrCtx &c, int attr_idx, double lambd
  * out.  The result is the textbook min rule with the root left out (a node's value is g(parent(h))
  * for the highest failing node h below the root on its path, its own grey when there is none), except
  * for zeros spread by a failing root of nonzero grey.  Such a root is written 0 by its own levelroot
- * pixel, at time R.  A walk that later stops at it, or at a node already zeroed, writes 0.
- * Solved top-down, with t(x) the time x is written and tau(x) = min t over x's strict ancestors:
- *   - a node with a failing node at or above it (below the root) is written by the first walk that
- *     enters it, that of first(x); t = first(x);
- *   - a node without one is written at its own levelroot pixel, or earlier by a walk that stops at a
- *     zeroed node;
- *   - a newly reached node copies the status of the lowest reached ancestor above it.  So every node
- *     reached on a path below x after tau(x) has the status W(x) of x's first-reached ancestor;
- *   - zeroed: W(x) and tau(x) < first(x) for the first kind, W(x) and tau(x) < levelroot(x) for the
- *     second.  For a zeroed node of the second kind, t = max(first, tau + 1) stands in for its write
- *     time: its descendants' answers are the same for any time from there to the true one. */
-__global__ __launch_bounds__(1024) void k_min_sweep(const uint32_t *__restrict__ order, const uint32_t *__restrict__ start,
-                                                    uint32_t maxd, const int32_t *__restrict__ npar, const uint8_t *__restrict__ pass,
-                                                    const float *__restrict__ ngrey, const uint32_t *__restrict__ first,
-                                                    const uint32_t *__restrict__ lr, uint32_t *tw, uint32_t *tau, int32_t *hf,
-                                                    uint32_t *st, float *__restrict__ val)
+ * pixel, and a walk that later stops at it, or at a node already zeroed, writes 0.
+ * Which nodes are zeroed, with t(x) = first(x) for a node with a failing node h at or above it (the
+ * walk of first(x) is the first to enter x, and it writes x) and t(x) = its levelroot pixel otherwise
+ * (for a node that gets zeroed earlier, any time from its first zeroing chance to its true write time
+ * gives its descendants the same answers, and the levelroot is one), D(x) = the strict ancestor with
+ * the smallest t (the lowest on ties) and tau(x) = t(D(x)):
+ *   a newly reached node takes the status of the lowest reached node above it, so every node reached
+ *   on x's path after tau(x) has D(x)'s status, and x is zeroed iff D(x) is and tau(x) < first(x) (h
+ *   exists) or tau(x) < its levelroot (no h).  The root starts the chains: zeroed iff it fails with a
+ *   nonzero grey.
+ * Each step is a path aggregate solved by pointer doubling (ping-pong buffers, per-round flags that
+ * end the rounds early): h by "the topmost flagged", (tau, D) by a path minimum with argmin, the zero
+ * status by an AND along the D chain. */
+__global__ __launch_bounds__(256) void k_hf_init(const int32_t *__restrict__ npar, const uint8_t *__restrict__ pass,
+                                                 int32_t *__restrict__ H, int32_t *__restrict__ A, int64_t K)
 {
-    for (uint32_t d = 0; d <= maxd; d++) {
-        const uint32_t a = start[d], b = start[d + 1];
-        for (uint32_t i = a + threadIdx.x; i < b; i += blockDim.x) {
-            const uint32_t u = order[i];
-            const int32_t p = npar[u];
-            const bool fail = !pass[u];
-            uint32_t t_u, tau_u, s_u; /* s: bit 0 W (first-reached ancestor zeroed), bit 1 zeroed */
-            int32_t h_u;
-            float v;
-            if (p < 0) {
-                t_u = lr[u];
-                tau_u = DCF_NONE32;
-                s_u = (fail && ngrey[u] != 0.0f) ? 2u : 0u;
-                h_u = -1;
-                v = fail ? 0.0f : ngrey[u];
-            } else {
-                const uint32_t tp = __hip_atomic_load(&tw[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                const uint32_t taup = __hip_atomic_load(&tau[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                const uint32_t sp = __hip_atomic_load(&st[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                const int32_t hp = __hip_atomic_load(&hf[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                tau_u = min(tp, taup);
-                const bool W = tp <= taup ? (sp & 2u) != 0 : (sp & 1u) != 0;
-                h_u = hp >= 0 ? hp : (fail ? (int32_t)u : -1);
-                bool Z;
-                if (h_u >= 0) {
-                    Z = W && tau_u < first[u];
-                    t_u = first[u];
-                } else {
-                    Z = W && tau_u < lr[u];
-                    t_u = Z ? max(first[u], tau_u + 1) : lr[u];
-                }
-                s_u = (W ? 1u : 0u) | (Z ? 2u : 0u);
-                v = Z ? 0.0f : (h_u >= 0 ? ngrey[npar[h_u]] : ngrey[u]);
-            }
-            __hip_atomic_store(&tw[u], t_u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&tau[u], tau_u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&st[u], s_u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&hf[u], h_u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            val[u] = v;
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    const int32_t p = npar[k];
+    H[k] = (p >= 0 && !pass[k]) ? (int32_t)k : -1;
+    A[k] = p;
+}
+
+/* H covers [k, A(k)): the topmost failing node there; an ancestor's answer overrides */
+__global__ __launch_bounds__(256) void k_hf_round(const int32_t *__restrict__ H, int32_t *__restrict__ H2, const int32_t *__restrict__ A,
+                                                  int32_t *__restrict__ A2, uint32_t *__restrict__ active, int round, int64_t K)
+{
+    if (round > 0 && active[round - 1] == 0) return;
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int32_t a2 = -1;
+    if (k < K) {
+        const int32_t a = A[k];
+        int32_t h = H[k];
+        if (a >= 0) {
+            const int32_t ha = H[a];
+            if (ha >= 0) h = ha;
+            a2 = A[a];
         }
-        __threadfence();
-        __syncthreads();
+        H2[k] = h;
+        A2[k] = a2;
     }
+    round_flag(active, round, a2 >= 0);
+}
+
+/* write times t, then the path minimum N over [k, A(k)) with its argmin Dn */
+__global__ __launch_bounds__(256) void k_pm_init(const int32_t *__restrict__ npar, const int32_t *__restrict__ H,
+                                                 const uint32_t *__restrict__ first, const uint32_t *__restrict__ lr,
+                                                 uint32_t *__restrict__ N, int32_t *__restrict__ Dn, int32_t *__restrict__ A, int64_t K)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    const int32_t p = npar[k];
+    N[k] = (p >= 0 && H[k] >= 0) ? first[k] : lr[k];
+    Dn[k] = (int32_t)k;
+    A[k] = p;
+}
+
+__global__ __launch_bounds__(256) void k_pm_round(const uint32_t *__restrict__ N, uint32_t *__restrict__ N2, const int32_t *__restrict__ Dn,
+                                                  int32_t *__restrict__ Dn2, const int32_t *__restrict__ A, int32_t *__restrict__ A2,
+                                                  uint32_t *__restrict__ active, int round, int64_t K)
+{
+    if (round > 0 && active[round - 1] == 0) return;
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int32_t a2 = -1;
+    if (k < K) {
+        const int32_t a = A[k];
+        uint32_t nv = N[k];
+        int32_t d = Dn[k];
+        if (a >= 0) {
+            const uint32_t na = N[a];
+            if (na < nv) { /* ties keep the lower node */
+                nv = na;
+                d = Dn[a];
+            }
+            a2 = A[a];
+        }
+        N2[k] = nv;
+        Dn2[k] = d;
+        A2[k] = a2;
+    }
+    round_flag(active, round, a2 >= 0);
+}
+
+/* zero status: C covers the D chain from k up to (not including) P(k); the root's is its own */
+__global__ __launch_bounds__(256) void k_z_init(const int32_t *__restrict__ npar, const uint8_t *__restrict__ pass,
+                                                const float *__restrict__ ngrey, const uint32_t *__restrict__ first,
+                                                const uint32_t *__restrict__ lr, const int32_t *__restrict__ H,
+                                                const uint32_t *__restrict__ N, const int32_t *__restrict__ Dn, uint8_t *__restrict__ C,
+                                                int32_t *__restrict__ P, int64_t K)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    const int32_t p = npar[k];
+    if (p < 0) {
+        C[k] = !pass[k] && ngrey[k] != 0.0f;
+        P[k] = -1;
+        return;
+    }
+    const uint32_t tau = N[p];
+    const bool c = tau < (H[k] >= 0 ? first[k] : lr[k]);
+    C[k] = c;
+    P[k] = c ? Dn[p] : -1;
+}
+
+__global__ __launch_bounds__(256) void k_z_round(const uint8_t *__restrict__ C, uint8_t *__restrict__ C2, const int32_t *__restrict__ P,
+                                                 int32_t *__restrict__ P2, uint32_t *__restrict__ active, int round, int64_t K)
+{
+    if (round > 0 && active[round - 1] == 0) return;
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int32_t p2 = -1;
+    if (k < K) {
+        const int32_t pp = P[k];
+        bool c = C[k] != 0;
+        if (pp >= 0) {
+            c = c && C[pp] != 0;
+            p2 = c ? P[pp] : -1; /* a false link settles the chain */
+        }
+        C2[k] = c;
+        P2[k] = p2;
+    }
+    round_flag(active, round, p2 >= 0);
+}
+
+__global__ __launch_bounds__(256) void k_min_value(const int32_t *__restrict__ npar, const uint8_t *__restrict__ pass,
+                                                   const float *__restrict__ ngrey, const int32_t *__restrict__ H,
+                                                   const uint8_t *__restrict__ Z, float *__restrict__ val, int64_t K)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    if (npar[k] < 0) val[k] = pass[k] ? ngrey[k] : 0.0f;
+    else val[k] = Z[k] ? 0.0f : (H[k] >= 0 ? ngrey[npar[H[k]]] : ngrey[k]);
 }
 
 static int min_general(dcf_tree *t, const AttrCtx &c, int attr_idx, double lambda, float *val)
 {
     dcf_ctx *ctx = t->ctx;
     const int64_t K = t->K;
+    const dim3 g(nblk(K, 256)), b(256);
     uint8_t *pass = (uint8_t *)dcf_alloc(ctx, (size_t)K);
-    int32_t *J = (int32_t *)dcf_alloc(ctx, (size_t)K * 4), *hf = (int32_t *)dcf_alloc(ctx, (size_t)K * 4);
+    uint8_t *C[2] = { (uint8_t *)dcf_alloc(ctx, (size_t)K), (uint8_t *)dcf_alloc(ctx, (size_t)K) };
+    int32_t *H[2] = { (int32_t *)dcf_alloc(ctx, (size_t)K * 4), (int32_t *)dcf_alloc(ctx, (size_t)K * 4) };
+    int32_t *A[2] = { (int32_t *)dcf_alloc(ctx, (size_t)K * 4), (int32_t *)dcf_alloc(ctx, (size_t)K * 4) };
+    int32_t *D[2] = { (int32_t *)dcf_alloc(ctx, (size_t)K * 4), (int32_t *)dcf_alloc(ctx, (size_t)K * 4) };
+    uint32_t *N[2] = { (uint32_t *)dcf_alloc(ctx, (size_t)K * 4), (uint32_t *)dcf_alloc(ctx, (size_t)K * 4) };
     uint32_t *first = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4), *Q = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4);
-    uint32_t *lr = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4), *tw = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4);
-    uint32_t *tau = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4), *st = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4);
-    uint32_t *order = nullptr, *start = nullptr, maxd = 0;
-    int rc = DCF_OK;
-    if (!pass || !J || !hf || !first || !Q || !lr || !tw || !tau || !st) rc = dcf_set_error(ctx, DCF_ENOMEM, "min rule scratch");
+    uint32_t *lr = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4);
+    int rc = DCF_OK, hw = 0, nw = 0, zw = 0;
+    if (!pass || !C[0] || !C[1] || !H[0] || !H[1] || !A[0] || !A[1] || !D[0] || !D[1] || !N[0] || !N[1] || !first || !Q || !lr)
+        rc = dcf_set_error(ctx, DCF_ENOMEM, "min rule scratch");
     if (rc == DCF_OK) rc = refuse_zero_grey(t, "min");
     if (rc == DCF_OK) {
-        hipLaunchKernelGGL(k_direct_init, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, c, attr_idx, lambda, t->nlam, t->npar, J, pass, K);
-        hipLaunchKernelGGL(k_fill_u32, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, lr, DCF_NONE32, K);
-        hipLaunchKernelGGL(k_fill_u32, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, Q, DCF_NONE32, K);
-        hipLaunchKernelGGL(k_levelroot_pixel, dim3(nblk(t->n, 256)), dim3(256), 0, ctx->stream, (const uint32_t *)t->p2n, lr, t->n);
-        if (hipMemcpyAsync(first, lr, (size_t)K * 4, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
-            rc = dcf_set_error(ctx, DCF_EDEVICE, "levelroot copy");
+        /* pass flags (k_direct_init's J goes to scratch that is overwritten below) */
+        hipLaunchKernelGGL(k_direct_init, g, b, 0, ctx->stream, c, attr_idx, lambda, t->nlam, t->npar, A[0], pass, K);
+        rc = subtree_firsts(t, pass, lr, first, Q);
     }
-    if (rc == DCF_OK) rc = depth_order(t, &order, &start, &maxd);
     if (rc == DCF_OK) {
-        hipLaunchKernelGGL(k_first_sweep, dim3(1), dim3(1024), 0, ctx->stream, (const uint32_t *)order, (const uint32_t *)start, maxd,
-                           (const int32_t *)t->npar, (const uint8_t *)pass, first, Q);
-        hipLaunchKernelGGL(k_min_sweep, dim3(1), dim3(1024), 0, ctx->stream, (const uint32_t *)order, (const uint32_t *)start, maxd,
-                           (const int32_t *)t->npar, (const uint8_t *)pass, (const float *)t->ngrey, (const uint32_t *)first,
-                           (const uint32_t *)lr, tw, tau, hf, st, val);
-        if (hipGetLastError() != hipSuccess) rc = dcf_set_error(ctx, DCF_EDEVICE, "min rule launches");
+        hipLaunchKernelGGL(k_hf_init, g, b, 0, ctx->stream, (const int32_t *)t->npar, (const uint8_t *)pass, H[0], A[0], K);
+        rc = doubling(t, [&](int r, int i) {
+            hipLaunchKernelGGL(k_hf_round, g, b, 0, ctx->stream, (const int32_t *)H[i], H[1 - i], (const int32_t *)A[i], A[1 - i],
+                               t->counters, r, K);
+        }, &hw);
     }
-    dcf_release(ctx, pass); dcf_release(ctx, J); dcf_release(ctx, hf); dcf_release(ctx, first); dcf_release(ctx, Q);
-    dcf_release(ctx, lr); dcf_release(ctx, tw); dcf_release(ctx, tau); dcf_release(ctx, st);
-    dcf_release(ctx, order); dcf_release(ctx, start);
+    if (rc == DCF_OK) {
+        hipLaunchKernelGGL(k_pm_init, g, b, 0, ctx->stream, (const int32_t *)t->npar, (const int32_t *)H[hw], (const uint32_t *)first,
+                           (const uint32_t *)lr, N[0], D[0], A[0], K);
+        rc = doubling(t, [&](int r, int i) {
+            hipLaunchKernelGGL(k_pm_round, g, b, 0, ctx->stream, (const uint32_t *)N[i], N[1 - i], (const int32_t *)D[i], D[1 - i],
+                               (const int32_t *)A[i], A[1 - i], t->counters, r, K);
+        }, &nw);
+    }
+    if (rc == DCF_OK) {
+        hipLaunchKernelGGL(k_z_init, g, b, 0, ctx->stream, (const int32_t *)t->npar, (const uint8_t *)pass, (const float *)t->ngrey,
+                           (const uint32_t *)first, (const uint32_t *)lr, (const int32_t *)H[hw], (const uint32_t *)N[nw],
+                           (const int32_t *)D[nw], C[0], A[0], K);
+        rc = doubling(t, [&](int r, int i) {
+            hipLaunchKernelGGL(k_z_round, g, b, 0, ctx->stream, (const uint8_t *)C[i], C[1 - i], (const int32_t *)A[i], A[1 - i],
+                               t->counters, r, K);
+        }, &zw);
+    }
+    if (rc == DCF_OK) {
+        hipLaunchKernelGGL(k_min_value, g, b, 0, ctx->stream, (const int32_t *)t->npar, (const uint8_t *)pass, (const float *)t->ngrey,
+                           (const int32_t *)H[hw], (const uint8_t *)C[zw], val, K);
+        if (hipGetLastError() != hipSuccess) rc = dcf_set_error(ctx, DCF_EDEVICE, "launch k_min_value");
+    }
+    dcf_release(ctx, pass);
+    for (int i = 0; i < 2; i++) {
+        dcf_release(ctx, C[i]); dcf_release(ctx, H[i]); dcf_release(ctx, A[i]); dcf_release(ctx, D[i]); dcf_release(ctx, N[i]);
+    }
+    dcf_release(ctx, first); dcf_release(ctx, Q); dcf_release(ctx, lr);
     return rc;
 }
 
