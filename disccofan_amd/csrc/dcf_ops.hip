@@ -1,8 +1,9 @@
 /*
  * dcf_ops.hip -- attribute evaluation, filter rules, check images and pattern spectra on gfx950.
  *
- * Replaces tree_filtering / tree_filter_direct / tree_filter_subtractive / tree_attribute_check
- * (src/sources/filtering.c:18-261), tree_pattern_spectrum[2d] (src/sources/pattern.c:27-203) and
+ * Replaces tree_filtering / tree_filter_direct / tree_filter_min / tree_filter_max /
+ * tree_filter_subtractive / tree_attribute_check (src/sources/filtering.c:18-261),
+ * tree_pattern_spectrum[2d] (src/sources/pattern.c:27-203) and
  * the attribute functions of AttribsArray (src/sources/attributes.c:47-874).
  *
  * The reference walks parent chains per pixel with a `reached` memo.  Here every rule is first
@@ -11,9 +12,12 @@
  *                -> nearest-passing-ancestor by in-place pointer jumping
  *   subtractive  out(root) = pass ? g : 0;  out(n) = out(par) + pass(n) * (g(n) - g(par))
  *                -> rootfix sum by pointer doubling (ping-pong buffers)
+ *   min, max    the reference's single-thread raster scan, solved per node from subtree minima
+ *                (preorder range minima) and pointer doubling (max_general, min_general)
  *   For the area attribute (increasing: a parent's area is never smaller) every ancestor of a
- *   passing node passes, so the subtractive closed form collapses to the direct one and the
- *   cheaper direct kernels give the identical result (DCF_ALL_RULES forces the general kernels).
+ *   passing node passes, so the subtractive, min and max rules collapse to the direct one (the min
+ *   rule under a failing root aside) and the cheaper direct kernels give the identical result
+ *   (DCF_ALL_RULES forces the general kernels).
  * Attribute values are evaluated in FP64 from exact integer moments with the reference's
  * operation order, including its float evaluation of area_attribute (attributes.c:105: the
  * ulong area is converted to float before the product), so bins and thresholds match bit for bit.
