@@ -308,59 +308,6 @@ __global__ __launch_bounds__(256) void k_filter_out(ImgView iv, const uint32_t *
     else ((uint8_t *)out)[p] = (uint8_t)f;
 }
 
-/* ---- subtractive rule on float greys ---------------------------------------------------------- *
- * tree_filter_subtractive (filtering.c:211-261) evaluates, top-down, f(u) = g(u) + f(p) - g(p) in float
- * for a passing node u with parent p (f(p) for a failing one; the root: g or 0).  Float sums do not
- * reassociate, so the nodes are processed in order of depth: depths by doubling, a radix sort by
- * depth, then one workgroup walks the depths (a barrier between consecutive depths). */
-__global__ __launch_bounds__(256) void k_depth_init(const int32_t *__restrict__ npar, double *__restrict__ F, int32_t *__restrict__ A,
-                                                    int64_t K)
-{
-    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= K) return;
-    F[k] = npar[k] >= 0 ? 1.0 : 0.0;
-    A[k] = npar[k];
-}
-
-__global__ __launch_bounds__(256) void k_depth_keys(const double *__restrict__ F, uint32_t *__restrict__ key, uint32_t *__restrict__ id,
-                                                    int64_t K)
-{
-    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= K) return;
-    key[k] = (uint32_t)F[k];
-    id[k] = (uint32_t)k;
-}
-
-/* start[d] = first position of depth d in the depth-sorted order (depths 0..max all occur) */
-__global__ __launch_bounds__(256) void k_depth_starts(const uint32_t *__restrict__ key, uint32_t *__restrict__ start, int64_t K)
-{
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= K) return;
-    if (i == 0 || key[i - 1] != key[i]) start[key[i]] = (uint32_t)i;
-    if (i == K - 1) start[key[i] + 1] = (uint32_t)K;
-}
-
-__global__ __launch_bounds__(1024) void k_sub_sweep(const uint32_t *__restrict__ order, const uint32_t *__restrict__ start,
-                                                    uint32_t maxd, const int32_t *__restrict__ npar, const float *__restrict__ ngrey,
-                                                    const uint8_t *__restrict__ pass, float *__restrict__ val)
-{
-    for (uint32_t d = 0; d <= maxd; d++) {
-        const uint32_t a = start[d], b = start[d + 1];
-        for (uint32_t i = a + threadIdx.x; i < b; i += blockDim.x) {
-            const uint32_t u = order[i];
-            const int32_t p = npar[u];
-            float f;
-            if (p < 0) f = pass[u] ? ngrey[u] : 0.0f;
-            else if (pass[u]) {
-                const float t = ngrey[u] + val[p];
-                f = t - ngrey[p];
-            } else f = val[p];
-            val[u] = f;
-        }
-        __syncthreads();
-    }
-}
-
 /* batched form: every node takes the lambda of its image */
 __global__ __launch_bounds__(256) void k_node_lambda(const BatchImg *__restrict__ bimg, int nb, const double *__restrict__ lam,
                                                      const uint32_t *__restrict__ p2n, double *__restrict__ nlam, int64_t n)
@@ -472,98 +419,6 @@ static AttrCtx attr_ctx(dcf_tree *t)
     return c;
 }
 
-/* Nodes in order of depth: order[] lists the nodes by increasing depth, start[d] is the first
- * position of depth d (start[maxd + 1] = K).  Depths are sums of ones up the tree by doubling (exact
- * in double), then a radix sort by depth.  The caller releases *order and *start. */
-static int depth_order(dcf_tree *t, uint32_t **order, uint32_t **start, uint32_t *maxd_out)
-{
-    dcf_ctx *ctx = t->ctx;
-    const int64_t K = t->K;
-    *order = nullptr;
-    *start = nullptr;
-    double *F0 = (double *)dcf_alloc(ctx, (size_t)K * 8), *F1 = (double *)dcf_alloc(ctx, (size_t)K * 8);
-    int32_t *A0 = (int32_t *)dcf_alloc(ctx, (size_t)K * 4), *A1 = (int32_t *)dcf_alloc(ctx, (size_t)K * 4);
-    uint32_t *k0 = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4), *k1 = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4);
-    uint32_t *v0 = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4), *v1 = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4);
-    void *tmp = nullptr;
-    uint32_t *st = nullptr;
-    int rc = DCF_OK;
-    std::vector<uint32_t> act(64);
-    if (!F0 || !F1 || !A0 || !A1 || !k0 || !k1 || !v0 || !v1) rc = dcf_set_error(ctx, DCF_ENOMEM, "depth order scratch");
-    if (rc == DCF_OK) {
-        hipLaunchKernelGGL(k_depth_init, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, t->npar, F0, A0, K);
-        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, t->counters, 0u, (int64_t)64);
-        double *fi = F0, *fo = F1;
-        int32_t *ai = A0, *ao = A1;
-        for (int r = 0; r < t->max_rounds; r++) {
-            hipLaunchKernelGGL(k_sub_round, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, fi, fo, ai, ao, t->counters, r, K);
-            double *tf = fi; fi = fo; fo = tf;
-            int32_t *ta = ai; ai = ao; ao = ta;
-        }
-        if (hipMemcpyAsync(act.data(), t->counters, 64 * 4, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
-            hipStreamSynchronize(ctx->stream) != hipSuccess)
-            rc = dcf_set_error(ctx, DCF_EDEVICE, "depth rounds");
-        int ran = 0;
-        for (int r = 0; r < t->max_rounds; r++) { ran = r + 1; if (act[r] == 0) break; }
-        if (rc == DCF_OK && act[t->max_rounds - 1] != 0) rc = dcf_set_error(ctx, DCF_EDEVICE, "tree deeper than the doubling rounds");
-        const double *depth = (ran & 1) ? F1 : F0;
-        if (rc == DCF_OK) {
-            hipLaunchKernelGGL(k_depth_keys, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, depth, k0, v0, K);
-            size_t need = 0;
-            if (hipcub::DeviceRadixSort::SortPairs(nullptr, need, k0, k1, v0, v1, (int)K, 0, 32, ctx->stream) != hipSuccess)
-                rc = dcf_set_error(ctx, DCF_EDEVICE, "depth sort size");
-            else if (!(tmp = dcf_alloc(ctx, need ? need : 16)))
-                rc = dcf_set_error(ctx, DCF_ENOMEM, "depth sort scratch");
-            else if (hipcub::DeviceRadixSort::SortPairs(tmp, need, k0, k1, v0, v1, (int)K, 0, 32, ctx->stream) != hipSuccess)
-                rc = dcf_set_error(ctx, DCF_EDEVICE, "depth sort");
-        }
-        uint32_t maxd = 0;
-        if (rc == DCF_OK && (hipMemcpyAsync(&maxd, k1 + (K - 1), 4, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
-                             hipStreamSynchronize(ctx->stream) != hipSuccess))
-            rc = dcf_set_error(ctx, DCF_EDEVICE, "maximum depth");
-        if (rc == DCF_OK && !(st = (uint32_t *)dcf_alloc(ctx, ((size_t)maxd + 2) * 4)))
-            rc = dcf_set_error(ctx, DCF_ENOMEM, "depth starts");
-        if (rc == DCF_OK) {
-            hipLaunchKernelGGL(k_depth_starts, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, (const uint32_t *)k1, st, K);
-            if (hipGetLastError() != hipSuccess) rc = dcf_set_error(ctx, DCF_EDEVICE, "launch k_depth_starts");
-        }
-        *maxd_out = maxd;
-    }
-    dcf_release(ctx, F0); dcf_release(ctx, F1); dcf_release(ctx, A0); dcf_release(ctx, A1);
-    dcf_release(ctx, k0); dcf_release(ctx, k1); dcf_release(ctx, v0); dcf_release(ctx, tmp);
-    if (rc == DCF_OK) {
-        *order = v1;
-        *start = st;
-    } else {
-        dcf_release(ctx, v1);
-        dcf_release(ctx, st);
-    }
-    return rc;
-}
-
-/* the subtractive rule on float greys, exactly as the reference rounds it */
-static int sub_float(dcf_tree *t, const AttrCtx &c, int attr_idx, double lambda, float *val)
-{
-    dcf_ctx *ctx = t->ctx;
-    const int64_t K = t->K;
-    uint8_t *pass = (uint8_t *)dcf_alloc(ctx, (size_t)K);
-    int32_t *J = (int32_t *)dcf_alloc(ctx, (size_t)K * 4);
-    uint32_t *order = nullptr, *start = nullptr, maxd = 0;
-    int rc = DCF_OK;
-    if (!pass || !J) rc = dcf_set_error(ctx, DCF_ENOMEM, "subtractive scratch");
-    /* pass flags (the J pointers of k_direct_init are not used here) */
-    if (rc == DCF_OK)
-        hipLaunchKernelGGL(k_direct_init, dim3(nblk(K, 256)), dim3(256), 0, ctx->stream, c, attr_idx, lambda, t->nlam, t->npar, J, pass, K);
-    if (rc == DCF_OK) rc = depth_order(t, &order, &start, &maxd);
-    if (rc == DCF_OK) {
-        hipLaunchKernelGGL(k_sub_sweep, dim3(1), dim3(1024), 0, ctx->stream, (const uint32_t *)order, (const uint32_t *)start, maxd,
-                           t->npar, t->ngrey, (const uint8_t *)pass, val);
-        if (hipGetLastError() != hipSuccess) rc = dcf_set_error(ctx, DCF_EDEVICE, "launch k_sub_sweep");
-    }
-    dcf_release(ctx, pass); dcf_release(ctx, J); dcf_release(ctx, order); dcf_release(ctx, start);
-    return rc;
-}
-
 /* ---- the min rule under a failing root (tree_filter_min, filtering.c:55-100) ------------------------ *
  * The reference walks every pixel, in raster order, from its levelroot up to the first node already
  * reached by an earlier walk, never testing the root.  When the root fails (an increasing attribute:
@@ -590,10 +445,23 @@ __global__ __launch_bounds__(256) void k_first_pixel(const uint32_t *__restrict_
                                                      const int32_t *__restrict__ J, uint32_t *__restrict__ first, int64_t n)
 {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const uint32_t k = p2n[p];
-    const int32_t a = npar[k] < 0 ? (int32_t)k : J[k];
-    atomicMin(&first[a], (uint32_t)p);
+    const bool valid = p < n;
+    int32_t a = -1;
+    if (valid) {
+        const uint32_t k = p2n[p];
+        a = npar[k] < 0 ? (int32_t)k : J[k];
+    }
+    /* consecutive pixels mostly lie under one child of the root: one atomic per wave then (the
+     * wave's smallest pixel is its first valid lane's) */
+    const uint64_t vm = __ballot(valid);
+    if (!vm) return;
+    const int leader = __ffsll((unsigned long long)vm) - 1;
+    const int32_t al = __shfl(a, leader, 64);
+    if (__ballot(valid && a != al) == 0) {
+        if ((int)(threadIdx.x & 63) == leader) atomicMin(&first[al], (uint32_t)p);
+        return;
+    }
+    if (valid) atomicMin(&first[a], (uint32_t)p);
 }
 
 __global__ __launch_bounds__(256) void k_min_rootfail(AttrCtx c, int idx, double lambda, const double *__restrict__ nlam,
@@ -662,13 +530,15 @@ __global__ __launch_bounds__(256) void k_child_area(const int32_t *__restrict__ 
     if (k < K && npar[k] >= 0) atomicAdd(&csum[npar[k]], area[k]);
 }
 
-/* sort keys: parent + 1 (0 for the roots, siblings under one virtual root) */
-__global__ __launch_bounds__(256) void k_parent_keys(const int32_t *__restrict__ npar, uint32_t *__restrict__ key, uint32_t *__restrict__ id,
-                                                     int64_t K)
+/* sort keys: parent + 1 (0 for the roots, siblings under one virtual root); with heavy flags, one more
+ * bit puts each node's heavy child first among its siblings */
+__global__ __launch_bounds__(256) void k_parent_keys(const int32_t *__restrict__ npar, const uint8_t *__restrict__ heavy,
+                                                     uint32_t *__restrict__ key, uint32_t *__restrict__ id, int64_t K)
 {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= K) return;
-    key[k] = (uint32_t)(npar[k] + 1);
+    const uint32_t pk = (uint32_t)(npar[k] + 1);
+    key[k] = heavy ? (pk << 1) | (heavy[k] ? 0u : 1u) : pk;
     id[k] = (uint32_t)k;
 }
 
@@ -680,15 +550,15 @@ __global__ __launch_bounds__(256) void k_gather_area(const uint32_t *__restrict_
 }
 
 /* base[key] = the scan value at the start of the key's run of siblings */
-__global__ __launch_bounds__(256) void k_seg_base(const uint32_t *__restrict__ key, const uint64_t *__restrict__ E, uint64_t *__restrict__ base,
-                                                  int64_t K)
+__global__ __launch_bounds__(256) void k_seg_base(const uint32_t *__restrict__ key, int shift, const uint64_t *__restrict__ E,
+                                                  uint64_t *__restrict__ base, int64_t K)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < K && (i == 0 || key[i - 1] != key[i])) base[key[i]] = E[i];
+    if (i < K && (i == 0 || (key[i - 1] >> shift) != (key[i] >> shift))) base[key[i] >> shift] = E[i];
 }
 
 /* off(x) = own pixels of x's parent + areas of x's earlier siblings; A = the parent, for the path sum */
-__global__ __launch_bounds__(256) void k_offsets(const uint32_t *__restrict__ key, const uint32_t *__restrict__ id,
+__global__ __launch_bounds__(256) void k_offsets(const uint32_t *__restrict__ key, int shift, const uint32_t *__restrict__ id,
                                                  const uint64_t *__restrict__ E, const uint64_t *__restrict__ base,
                                                  const uint32_t *__restrict__ area, const uint32_t *__restrict__ csum,
                                                  const int32_t *__restrict__ npar, uint32_t *__restrict__ off, int32_t *__restrict__ A,
@@ -696,7 +566,7 @@ __global__ __launch_bounds__(256) void k_offsets(const uint32_t *__restrict__ ke
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= K) return;
-    const uint32_t kk = key[i], x = id[i];
+    const uint32_t kk = key[i] >> shift, x = id[i];
     const uint32_t own_parent = kk == 0 ? 0u : area[kk - 1] - csum[kk - 1];
     off[x] = own_parent + (uint32_t)(E[i] - base[kk]);
     A[x] = npar[x];
@@ -806,15 +676,15 @@ static int doubling(dcf_tree *t, F launch, int *which)
     return DCF_OK;
 }
 
-/* lr = the levelroot pixel of every node, first = the first pixel of its subtree, Q = the smallest
- * first of a passing node in its subtree (DCF_NONE32: none); pass flags given */
-static int subtree_firsts(dcf_tree *t, const uint8_t *pass, uint32_t *lr, uint32_t *first, uint32_t *Q)
+/* the nodes in preorder of the pixel layout: order[i] = the node at position rank i, spos[i] = its
+ * position (strictly increasing).  Siblings follow in order of node id, the heavy child first when
+ * heavy flags are given. */
+static int node_preorder(dcf_tree *t, const uint8_t *heavy, uint32_t *spos, uint32_t *order)
 {
     dcf_ctx *ctx = t->ctx;
     const int64_t K = t->K;
     const dim3 g(nblk(K, 256)), b(256);
-    int levels = 1;
-    while (((int64_t)1 << levels) <= K) levels++; /* levels j = 0 .. levels - 1, 2^j <= K */
+    const int shift = heavy ? 1 : 0;
     uint32_t *csum = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4);
     uint32_t *k0 = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4), *k1 = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4);
     uint32_t *v0 = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4), *v1 = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4);
@@ -822,28 +692,24 @@ static int subtree_firsts(dcf_tree *t, const uint8_t *pass, uint32_t *lr, uint32
     uint64_t *base = (uint64_t *)dcf_alloc(ctx, ((size_t)K + 1) * 8);
     uint32_t *S[2] = { (uint32_t *)dcf_alloc(ctx, (size_t)K * 4), (uint32_t *)dcf_alloc(ctx, (size_t)K * 4) };
     int32_t *A[2] = { (int32_t *)dcf_alloc(ctx, (size_t)K * 4), (int32_t *)dcf_alloc(ctx, (size_t)K * 4) };
-    uint32_t *rank = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4), *cnt = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4);
-    uint32_t *T = (uint32_t *)dcf_alloc(ctx, (size_t)levels * (size_t)K * 4);
     void *tmp = nullptr;
     const uint32_t *area = t->attr.area;
     int rc = DCF_OK, sw = 0;
-    if (!csum || !k0 || !k1 || !v0 || !v1 || !sa || !E || !base || !S[0] || !S[1] || !A[0] || !A[1] || !rank || !cnt || !T)
-        rc = dcf_set_error(ctx, DCF_ENOMEM, "subtree scratch");
+    if (!csum || !k0 || !k1 || !v0 || !v1 || !sa || !E || !base || !S[0] || !S[1] || !A[0] || !A[1])
+        rc = dcf_set_error(ctx, DCF_ENOMEM, "preorder scratch");
     /* scratch for the larger of the radix sort and the scan */
     size_t need_sort = 0, need_scan = 0;
     if (rc == DCF_OK &&
         (hipcub::DeviceRadixSort::SortPairs(nullptr, need_sort, k0, k1, v0, v1, (int)K, 0, 32, ctx->stream) != hipSuccess ||
          hipcub::DeviceScan::ExclusiveSum(nullptr, need_scan, sa, E, (int)K, ctx->stream) != hipSuccess))
-        rc = dcf_set_error(ctx, DCF_EDEVICE, "subtree scratch size");
+        rc = dcf_set_error(ctx, DCF_EDEVICE, "preorder scratch size");
     const size_t need = need_sort > need_scan ? need_sort : need_scan;
-    if (rc == DCF_OK && !(tmp = dcf_alloc(ctx, need ? need : 16))) rc = dcf_set_error(ctx, DCF_ENOMEM, "subtree sort scratch");
+    if (rc == DCF_OK && !(tmp = dcf_alloc(ctx, need ? need : 16))) rc = dcf_set_error(ctx, DCF_ENOMEM, "preorder sort scratch");
     if (rc == DCF_OK) {
-        hipLaunchKernelGGL(k_levelroot_pixel, dim3(nblk(t->n, 256)), b, 0, ctx->stream, (const int32_t *)t->parent, (const uint32_t *)t->p2n,
-                           lr, t->n);
         hipLaunchKernelGGL(k_fill_u32, g, b, 0, ctx->stream, csum, 0u, K);
         hipLaunchKernelGGL(k_child_area, g, b, 0, ctx->stream, (const int32_t *)t->npar, area, csum, K);
         /* siblings together, in order of node id (the sort is stable) */
-        hipLaunchKernelGGL(k_parent_keys, g, b, 0, ctx->stream, (const int32_t *)t->npar, k0, v0, K);
+        hipLaunchKernelGGL(k_parent_keys, g, b, 0, ctx->stream, (const int32_t *)t->npar, heavy, k0, v0, K);
         size_t ns = need;
         if (hipcub::DeviceRadixSort::SortPairs(tmp, ns, k0, k1, v0, v1, (int)K, 0, 32, ctx->stream) != hipSuccess)
             rc = dcf_set_error(ctx, DCF_EDEVICE, "sort by parent");
@@ -855,8 +721,8 @@ static int subtree_firsts(dcf_tree *t, const uint8_t *pass, uint32_t *lr, uint32
             rc = dcf_set_error(ctx, DCF_EDEVICE, "sibling scan");
     }
     if (rc == DCF_OK) {
-        hipLaunchKernelGGL(k_seg_base, g, b, 0, ctx->stream, (const uint32_t *)k1, (const uint64_t *)E, base, K);
-        hipLaunchKernelGGL(k_offsets, g, b, 0, ctx->stream, (const uint32_t *)k1, (const uint32_t *)v1, (const uint64_t *)E,
+        hipLaunchKernelGGL(k_seg_base, g, b, 0, ctx->stream, (const uint32_t *)k1, shift, (const uint64_t *)E, base, K);
+        hipLaunchKernelGGL(k_offsets, g, b, 0, ctx->stream, (const uint32_t *)k1, shift, (const uint32_t *)v1, (const uint64_t *)E,
                            (const uint64_t *)base, area, (const uint32_t *)csum, (const int32_t *)t->npar, S[0], A[0], K);
         rc = doubling(t, [&](int r, int i) {
             hipLaunchKernelGGL(k_psum_round, g, b, 0, ctx->stream, (const uint32_t *)S[i], S[1 - i], (const int32_t *)A[i], A[1 - i],
@@ -867,14 +733,39 @@ static int subtree_firsts(dcf_tree *t, const uint8_t *pass, uint32_t *lr, uint32
         /* preorder: nodes sorted by position */
         hipLaunchKernelGGL(k_iota_u32, g, b, 0, ctx->stream, v0, K);
         size_t ns = need;
-        if (hipcub::DeviceRadixSort::SortPairs(tmp, ns, S[sw], k1, v0, v1, (int)K, 0, 32, ctx->stream) != hipSuccess)
+        if (hipcub::DeviceRadixSort::SortPairs(tmp, ns, S[sw], spos, v0, order, (int)K, 0, 32, ctx->stream) != hipSuccess)
             rc = dcf_set_error(ctx, DCF_EDEVICE, "sort by position");
     }
+    dcf_release(ctx, csum); dcf_release(ctx, k0); dcf_release(ctx, k1); dcf_release(ctx, v0); dcf_release(ctx, v1);
+    dcf_release(ctx, sa); dcf_release(ctx, E); dcf_release(ctx, base); dcf_release(ctx, tmp);
+    for (int i = 0; i < 2; i++) { dcf_release(ctx, S[i]); dcf_release(ctx, A[i]); }
+    return rc;
+}
+
+/* lr = the levelroot pixel of every node, first = the first pixel of its subtree, Q = the smallest
+ * first of a passing node in its subtree (DCF_NONE32: none); pass flags given */
+static int subtree_firsts(dcf_tree *t, const uint8_t *pass, uint32_t *lr, uint32_t *first, uint32_t *Q)
+{
+    dcf_ctx *ctx = t->ctx;
+    const int64_t K = t->K;
+    const dim3 g(nblk(K, 256)), b(256);
+    int levels = 1;
+    while (((int64_t)1 << levels) <= K) levels++; /* levels j = 0 .. levels - 1, 2^j <= K */
+    uint32_t *spos = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4), *order = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4);
+    uint32_t *rank = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4), *cnt = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4);
+    uint32_t *T = (uint32_t *)dcf_alloc(ctx, (size_t)levels * (size_t)K * 4);
+    int rc = DCF_OK;
+    if (!spos || !order || !rank || !cnt || !T) rc = dcf_set_error(ctx, DCF_ENOMEM, "subtree scratch");
     if (rc == DCF_OK) {
-        hipLaunchKernelGGL(k_preorder, g, b, 0, ctx->stream, (const uint32_t *)k1, (const uint32_t *)v1, area, rank, cnt, K);
+        hipLaunchKernelGGL(k_levelroot_pixel, dim3(nblk(t->n, 256)), b, 0, ctx->stream, (const int32_t *)t->parent, (const uint32_t *)t->p2n,
+                           lr, t->n);
+        rc = node_preorder(t, nullptr, spos, order);
+    }
+    if (rc == DCF_OK) {
+        hipLaunchKernelGGL(k_preorder, g, b, 0, ctx->stream, (const uint32_t *)spos, (const uint32_t *)order, t->attr.area, rank, cnt, K);
         /* first: range minima of the levelroot pixels; Q: of first over the passing nodes */
         for (int pass_q = 0; pass_q < 2 && rc == DCF_OK; pass_q++) {
-            hipLaunchKernelGGL(k_gather_pre, g, b, 0, ctx->stream, (const uint32_t *)v1, pass_q ? (const uint32_t *)first : (const uint32_t *)lr,
+            hipLaunchKernelGGL(k_gather_pre, g, b, 0, ctx->stream, (const uint32_t *)order, pass_q ? (const uint32_t *)first : (const uint32_t *)lr,
                                pass_q ? pass : (const uint8_t *)nullptr, T, K);
             for (int j = 1; j < levels; j++) {
                 const int64_t half = (int64_t)1 << (j - 1), m = K - ((int64_t)1 << j) + 1;
@@ -886,9 +777,166 @@ static int subtree_firsts(dcf_tree *t, const uint8_t *pass, uint32_t *lr, uint32
             if (hipGetLastError() != hipSuccess) rc = dcf_set_error(ctx, DCF_EDEVICE, "subtree minima launches");
         }
     }
-    dcf_release(ctx, csum); dcf_release(ctx, k0); dcf_release(ctx, k1); dcf_release(ctx, v0); dcf_release(ctx, v1);
-    dcf_release(ctx, sa); dcf_release(ctx, E); dcf_release(ctx, base); dcf_release(ctx, rank); dcf_release(ctx, cnt);
-    dcf_release(ctx, T); dcf_release(ctx, tmp);
+    dcf_release(ctx, spos); dcf_release(ctx, order); dcf_release(ctx, rank); dcf_release(ctx, cnt); dcf_release(ctx, T);
+    return rc;
+}
+
+/* ---- subtractive rule on float greys ------------------------------------------------------------ *
+ * tree_filter_subtractive (filtering.c:211-261) evaluates, top-down, f(u) = g(u) + f(p) - g(p) in float
+ * for a passing node u with parent p (f(p) for a failing one; the root: g or 0).  Float sums do not
+ * reassociate, so every path is evaluated in order.  The tree is cut into heavy paths (each node
+ * continues into its child of largest area), laid out so that each path is one run of the preorder:
+ * one thread walks a path with f in a register, streaming the run's greys and flags (gathered into
+ * preorder beforehand) 32 at a time.  A path
+ * starts from its head's parent, which lies on a path with one light edge fewer; a light child has at
+ * most half its parent's area, so at most log2(n) + 1 rounds of launches cover the tree. */
+__global__ __launch_bounds__(256) void k_fill_u64(uint64_t *a, uint64_t v, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+/* hv[p] = max over p's children of (area, -id): the heavy child, the smallest id among equal areas */
+__global__ __launch_bounds__(256) void k_heavy_key(const int32_t *__restrict__ npar, const uint32_t *__restrict__ area,
+                                                   uint64_t *__restrict__ hv, int64_t K)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K || npar[k] < 0) return;
+    atomicMax((unsigned long long *)&hv[npar[k]], ((unsigned long long)area[k] << 32) | (0xFFFFFFFFull - (unsigned long long)k));
+}
+
+/* heavy / head flags; S, A start the path count of heads (the light depth) */
+__global__ __launch_bounds__(256) void k_heavy_flag(const int32_t *__restrict__ npar, const uint64_t *__restrict__ hv,
+                                                    uint8_t *__restrict__ heavy, uint8_t *__restrict__ head, uint32_t *__restrict__ S,
+                                                    int32_t *__restrict__ A, int64_t K)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    const int32_t p = npar[k];
+    const bool h = p >= 0 && (uint32_t)hv[p] == 0xFFFFFFFFu - (uint32_t)k;
+    heavy[k] = h;
+    head[k] = !h;
+    S[k] = h ? 0u : 1u;
+    A[k] = p;
+}
+
+__global__ __launch_bounds__(256) void k_rank_of(const uint32_t *__restrict__ order, uint32_t *__restrict__ rank, int64_t K)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < K) rank[order[i]] = (uint32_t)i;
+}
+
+/* the walk's inputs in preorder: grey, and flags bit 0 pass, bit 1 head of a path */
+__global__ __launch_bounds__(256) void k_chain_gather(const uint32_t *__restrict__ order, const float *__restrict__ ngrey,
+                                                      const uint8_t *__restrict__ pass, const uint8_t *__restrict__ head,
+                                                      float *__restrict__ Gr, uint8_t *__restrict__ Fr, int64_t K)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= K) return;
+    const uint32_t x = order[i];
+    Gr[i] = ngrey[x];
+    Fr[i] = (uint8_t)((pass[x] ? 1 : 0) | (head[x] ? 2 : 0));
+}
+
+/* the heavy paths whose heads have light depth round + 1 (L counts the heads from the root down); a
+ * path is the run of preorder ranks from its head to the next head, read 32 at a time; values go to
+ * Vr by rank */
+__global__ __launch_bounds__(256) void k_sub_chain(int round, const uint32_t *__restrict__ rank, const uint8_t *__restrict__ head,
+                                                   const uint32_t *__restrict__ L, const int32_t *__restrict__ npar,
+                                                   const float *__restrict__ ngrey, const uint8_t *__restrict__ pass,
+                                                   const float *__restrict__ Gr, const uint8_t *__restrict__ Fr, float *__restrict__ Vr,
+                                                   int64_t K)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K || !head[k] || L[k] != (uint32_t)round + 1u) return;
+    const int32_t p = npar[k];
+    const int64_t r0 = rank[k];
+    const float gk = ngrey[k];
+    float f;
+    if (p < 0) f = pass[k] ? gk : 0.0f;
+    else {
+        const float fp = Vr[rank[p]];
+        if (pass[k]) {
+            const float t = gk + fp;
+            f = t - ngrey[p];
+        } else f = fp;
+    }
+    Vr[r0] = f;
+    float gprev = gk;
+    constexpr int B = 32;
+    for (int64_t r = r0 + 1; r < K; r += B) {
+        float gv[B];
+        uint8_t fv[B];
+        const int m = (int)((K - r) < B ? (K - r) : B);
+#pragma unroll
+        for (int j = 0; j < B; j++) {
+            gv[j] = j < m ? Gr[r + j] : 0.0f;
+            fv[j] = j < m ? Fr[r + j] : (uint8_t)2;
+        }
+        bool end = false;
+#pragma unroll
+        for (int j = 0; j < B; j++) {
+            if (end || (fv[j] & 2)) { end = true; continue; }
+            if (fv[j] & 1) {
+                const float t = gv[j] + f;
+                f = t - gprev;
+            }
+            Vr[r + j] = f;
+            gprev = gv[j];
+        }
+        if (end) break;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_chain_scatter(const uint32_t *__restrict__ order, const float *__restrict__ Vr,
+                                                       float *__restrict__ val, int64_t K)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < K) val[order[i]] = Vr[i];
+}
+
+static int sub_float(dcf_tree *t, const AttrCtx &c, int attr_idx, double lambda, float *val)
+{
+    dcf_ctx *ctx = t->ctx;
+    const int64_t K = t->K;
+    const dim3 g(nblk(K, 256)), b(256);
+    uint8_t *pass = (uint8_t *)dcf_alloc(ctx, (size_t)K), *heavy = (uint8_t *)dcf_alloc(ctx, (size_t)K), *head = (uint8_t *)dcf_alloc(ctx, (size_t)K);
+    uint64_t *hv = (uint64_t *)dcf_alloc(ctx, (size_t)K * 8);
+    uint32_t *spos = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4), *order = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4);
+    uint32_t *rank = (uint32_t *)dcf_alloc(ctx, (size_t)K * 4);
+    uint32_t *S[2] = { (uint32_t *)dcf_alloc(ctx, (size_t)K * 4), (uint32_t *)dcf_alloc(ctx, (size_t)K * 4) };
+    int32_t *A[2] = { (int32_t *)dcf_alloc(ctx, (size_t)K * 4), (int32_t *)dcf_alloc(ctx, (size_t)K * 4) };
+    int rc = DCF_OK, sw = 0;
+    if (!pass || !heavy || !head || !hv || !spos || !order || !rank || !S[0] || !S[1] || !A[0] || !A[1])
+        rc = dcf_set_error(ctx, DCF_ENOMEM, "subtractive scratch");
+    if (rc == DCF_OK) {
+        /* pass flags (k_direct_init's J goes to A[1], overwritten below) */
+        hipLaunchKernelGGL(k_direct_init, g, b, 0, ctx->stream, c, attr_idx, lambda, t->nlam, t->npar, A[1], pass, K);
+        hipLaunchKernelGGL(k_fill_u64, g, b, 0, ctx->stream, hv, (uint64_t)0, K);
+        hipLaunchKernelGGL(k_heavy_key, g, b, 0, ctx->stream, (const int32_t *)t->npar, (const uint32_t *)t->attr.area, hv, K);
+        hipLaunchKernelGGL(k_heavy_flag, g, b, 0, ctx->stream, (const int32_t *)t->npar, (const uint64_t *)hv, heavy, head, S[0], A[0], K);
+        rc = doubling(t, [&](int r, int i) {
+            hipLaunchKernelGGL(k_psum_round, g, b, 0, ctx->stream, (const uint32_t *)S[i], S[1 - i], (const int32_t *)A[i], A[1 - i],
+                               t->counters, r, K);
+        }, &sw);
+    }
+    if (rc == DCF_OK) rc = node_preorder(t, heavy, spos, order);
+    if (rc == DCF_OK) {
+        /* spos and the heavy flags are free again: Vr, Gr and Fr use their memory */
+        float *Vr = (float *)spos, *Gr = (float *)hv;
+        uint8_t *Fr = heavy;
+        hipLaunchKernelGGL(k_rank_of, g, b, 0, ctx->stream, (const uint32_t *)order, rank, K);
+        hipLaunchKernelGGL(k_chain_gather, g, b, 0, ctx->stream, (const uint32_t *)order, (const float *)t->ngrey, (const uint8_t *)pass,
+                           (const uint8_t *)head, Gr, Fr, K);
+        for (int r = 0; r < 32; r++)
+            hipLaunchKernelGGL(k_sub_chain, g, b, 0, ctx->stream, r, (const uint32_t *)rank, (const uint8_t *)head, (const uint32_t *)S[sw],
+                               (const int32_t *)t->npar, (const float *)t->ngrey, (const uint8_t *)pass, (const float *)Gr,
+                               (const uint8_t *)Fr, Vr, K);
+        hipLaunchKernelGGL(k_chain_scatter, g, b, 0, ctx->stream, (const uint32_t *)order, (const float *)Vr, val, K);
+        if (hipGetLastError() != hipSuccess) rc = dcf_set_error(ctx, DCF_EDEVICE, "launch k_sub_chain");
+    }
+    dcf_release(ctx, pass); dcf_release(ctx, heavy); dcf_release(ctx, head); dcf_release(ctx, hv); dcf_release(ctx, spos);
+    dcf_release(ctx, order); dcf_release(ctx, rank);
     for (int i = 0; i < 2; i++) { dcf_release(ctx, S[i]); dcf_release(ctx, A[i]); }
     return rc;
 }
