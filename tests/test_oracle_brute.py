@@ -17,7 +17,10 @@ restate the oracle's own code:
   textbook max rule differs;
 * the min rule of any attribute against a closed form of its raster scan: the textbook min rule with
   the root left out, except for the zeros a failing root spreads, found top-down from write times; the
-  cases are checked to include such zeros.
+  cases are checked to include such zeros;
+* the subtractive rule of any attribute, on integer and float images, against its per-node recurrence
+  evaluated top-down in float32 in the reference's operation order (the oracle follows the reference's
+  path stack instead).
 """
 import numpy as np
 import pytest
@@ -345,3 +348,54 @@ def test_min_rule_of_any_attribute():
         assert np.array_equal(got, want), (it, shape, conn, tt, aidx, lam)
         zeros += not np.array_equal(got, text)
     assert zeros >= 10, zeros  # the zeros of a failing root are exercised
+
+
+def subtractive_recurrence(parent, g, attr, lam):
+    """tree_filter_subtractive (filtering.c:211-261) per node, top-down: the root keeps g when it passes
+    (0 otherwise); a passing node u below p takes (g(u) + f(p)) - g(p), a failing one f(p), each sum
+    rounded to float32 as the reference's float arithmetic does"""
+    n = len(parent)
+    is_lr = (parent < 0) | (g[np.maximum(parent, 0)] != g)
+    lrs = np.nonzero(is_lr)[0]
+    depth = {}
+    for x in lrs:
+        d, y = 0, x
+        while parent[y] >= 0:
+            y, d = parent[y], d + 1
+        depth[x] = d
+    f = {}
+    for x in sorted(lrs, key=lambda x: depth[x]):
+        p = parent[x]
+        if p < 0:
+            f[x] = g[x] if attr[x] >= lam else np.float32(0)
+        elif attr[x] >= lam:
+            f[x] = np.float32(np.float32(g[x] + f[p]) - g[p])
+        else:
+            f[x] = f[p]
+    node = np.where(is_lr, np.arange(n), parent)
+    return np.array([f[node[v]] for v in range(n)], np.float32)
+
+
+def test_subtractive_rule_of_any_attribute():
+    rng = np.random.default_rng(79)
+    differs_from_direct = 0
+    for it in range(120):
+        shape = tuple(int(s) for s in rng.integers(2, 14, size=2))
+        if it % 3 == 2:
+            img = (rng.random(shape) * 9 - 3).astype(np.float32)
+        else:
+            img = rand_img(rng, shape, int(rng.integers(3, 12)))
+        tt = "max" if it % 2 else "min"
+        aidx = int(rng.choice([17, 13, 6, 2]))
+        o = OracleTree(img, int(rng.choice([4, 8])), tt, aidx, threads=1)
+        par, g = o.parent(), o.grey()
+        a, _ = o.attributes(aidx)
+        lr = (par < 0) | (g[np.maximum(par, 0)] != g)
+        lam = float(rng.choice(np.unique(a[lr])))
+        got = o.filter(aidx, lam, 3).reshape(-1)
+        want = subtractive_recurrence(par, g, a, lam)
+        if tt == "min":
+            want = np.float32(o.gmax) - want
+        assert np.array_equal(got, want), (it, shape, tt, aidx, lam)
+        differs_from_direct += not np.array_equal(got, o.filter(aidx, lam, 0).reshape(-1))
+    assert differs_from_direct >= 20, differs_from_direct  # non-increasing attributes: not the direct rule
