@@ -20,7 +20,10 @@ restate the oracle's own code:
   cases are checked to include such zeros;
 * the subtractive rule of any attribute, on integer and float images, against its per-node recurrence
   evaluated top-down in float32 in the reference's operation order (the oracle follows the reference's
-  path stack instead).
+  path stack instead);
+* the eigenvalue attributes (elongation, flatness, sparseness), whose closed-form 2x2 / 3x3 solvers
+  replace EISPACK rs(), against LAPACK's symmetric eigenvalues of each node's moment matrix, built from
+  the node's pixel coordinates.
 """
 import numpy as np
 import pytest
@@ -399,3 +402,40 @@ def test_subtractive_rule_of_any_attribute():
         assert np.array_equal(got, want), (it, shape, tt, aidx, lam)
         differs_from_direct += not np.array_equal(got, o.filter(aidx, lam, 0).reshape(-1))
     assert differs_from_direct >= 20, differs_from_direct  # non-increasing attributes: not the direct rule
+
+
+@pytest.mark.parametrize("shape,conn", [((5, 6, 7), 6), ((4, 7, 6), 26), ((1, 9, 11), 6), ((13, 17), 4), ((11, 9), 8)])
+def test_eigen_attributes_against_lapack(shape, conn):
+    """inertia_elon / flat / spar (attributes.c:726-849): eigenvalues of the central moment matrix with
+    the pixel-size correction A/12 on the diagonal; a node whose z sum is 0 uses the 2D formulas
+    (flatness -1), as the reference tests inertia[3] == 0"""
+    rng = np.random.default_rng(sum(shape) + conn)
+    img = rand_img(rng, shape, 5)
+    o = OracleTree(img, conn, "max", 14, threads=1)
+    par, g = o.parent(), o.grey()
+    n = len(par)
+    is_lr = (par < 0) | (g[np.maximum(par, 0)] != g)
+    dims = (shape[-1], shape[-2], shape[0] if len(shape) == 3 else 1)
+    idx = np.arange(n)
+    xyz = np.stack([idx % dims[0], (idx // dims[0]) % dims[1], idx // (dims[0] * dims[1])], axis=1).astype(np.float64)
+    members = {x: [] for x in np.nonzero(is_lr)[0]}
+    for v in range(n):
+        y = v if is_lr[v] else par[v]
+        while y >= 0:
+            members[y].append(v)
+            y = par[y]
+    got = {a: o.attributes(a)[0] for a in (14, 15, 16)}
+    for x, pix in members.items():
+        c = xyz[pix]
+        A = float(len(pix))
+        s1 = c.sum(axis=0)
+        M = c.T @ c - np.outer(s1, s1) / A + np.eye(3) * (A / 12)
+        if s1[2] == 0:
+            ev = np.linalg.eigvalsh(M[:2, :2])
+            want = (ev[1] / ev[0], -1.0, np.pi * np.sqrt(4 * ev[1] / A) * np.sqrt(4 * ev[0] / A) / A)
+        else:
+            ev = np.linalg.eigvalsh(M)
+            want = (ev[2] / ev[1], ev[1] / ev[0],
+                    np.pi * np.sqrt(20 * ev[2] / A) * np.sqrt(20 * ev[1] / A) * np.sqrt(20 * ev[0] / A) / (6 * A))
+        for a, w in zip((14, 15, 16), want):
+            assert abs(got[a][x] - w) <= 1e-9 * abs(w), (x, a, got[a][x], w)
