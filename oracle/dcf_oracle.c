@@ -24,6 +24,17 @@
  * Eigen-based attributes (14 elong, 15 flat, 16 spars) use a closed-form symmetric
  * 2x2 / 3x3 eigen-solver instead of EISPACK rs() (attributes.c:654,679); their parity
  * is within FP64 rounding, not bit-exact.
+ *
+ * How each part is pinned (tests/):
+ *   test_oracle_golden.py  the reference's own fixtures: area and non-compactness check images,
+ *                          direct-rule filters, GT_tree.txt (parents, moments, flux), lambda files,
+ *                          thread-count invariance of the slab fuse
+ *   test_oracle_brute.py   parents and areas against connected components of every level set (2D,
+ *                          3D, all connectivities, integer and float greys); min-tree = max-tree of
+ *                          the inverse; the direct rule against its definition; closed forms of the
+ *                          min, max and subtractive scans for every attribute; the eigenvalue
+ *                          attributes against LAPACK; 2D spectrum marginals; find_scale
+ * No function is left unpinned.
  */
 #include <limits.h>
 #include <math.h>
