@@ -125,6 +125,21 @@ def test_float_four_passes(ctx):
 
 
 @pytest.mark.parametrize("tree_type", ["max", "min"])
+def test_vnoise_4096_rules_of_ncomp(ctx, tree_type):
+    """4096^2 u16 value noise, 8-conn, non-compactness: all four rules, the min and max rules being the
+    reference's raster-scan results (millions of nodes: the preorder range minima and doubling at size)"""
+    img = synth.vnoise(4096, 4096)
+    say("vnoise 4096 rules", tree_type)
+    t = ctx.build(img, 8, tree_type, 17)
+    o = OracleTree(img, 8, tree_type, 17, threads=THREADS)
+    for dec in (0, 1, 2, 3):
+        for lam in (0.3, 2.0):
+            say("vnoise 4096 rule", tree_type, dec, lam)
+            same(t.filter(17, lam, dec), o.filter(17, lam, dec), "rule %d lambda %g %s" % (dec, lam, tree_type))
+    t.free()
+
+
+@pytest.mark.parametrize("tree_type", ["max", "min"])
 def test_vnoise_2048_u16_4conn(ctx, tree_type):
     """2048^2 u16 value noise at 4-connectivity (the default of the reference's 2D runs)"""
     img = synth.vnoise(2048, 2048)
