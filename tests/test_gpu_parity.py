@@ -270,6 +270,37 @@ def test_min_and_max_rules_float(ctx):
         t.free()
 
 
+def test_random_sweep_all_rules(ctx):
+    """300 seeded random cases: 2D / 3D shapes from 1 pixel up, u8 / u16 / f32 greys of few to many levels,
+    every connectivity and tree type, several attribute groups, the four rules at the median check value
+    and above every node's value; parent arrays and outputs bit for bit"""
+    rng = np.random.default_rng(2026)
+    for it in range(300):
+        is3d = rng.random() < 0.35
+        shape = tuple(int(v) for v in (rng.integers(1, 24, size=3) if is3d else rng.integers(1, 160, size=2)))
+        kind = str(rng.choice(["u8", "u16", "f32"]))
+        levels = int(rng.choice([2, 5, 40, 256, 4000, 65536]))
+        if kind == "u8":
+            img = rng.integers(0, min(levels, 256), size=shape).astype(np.uint8)
+        elif kind == "u16":
+            img = rng.integers(0, levels, size=shape).astype(np.uint16)
+        else:
+            img = (rng.integers(0, levels, size=shape) * rng.choice([0.25, 1.5, -0.75])).astype(np.float32)
+            img[img == 0] = 0.5  # no zero grey below a negative root (refused for the scan rules)
+        conn = int(rng.choice([6, 26] if is3d else [4, 8]))
+        tt = str(rng.choice(["max", "min"]))
+        attr = int(rng.choice([0, 17, 13, 1, 6, 9] if kind != "f32" else [0, 17, 1, 6]))
+        label = "sweep %d %s %s conn %d %s attr %d" % (it, kind, shape, conn, tt, attr)
+        o = OracleTree(img, conn, tt, attr)
+        t = ctx.build(img, conn, tt, attr)
+        assert_same(t.parent(), o.parent().astype(np.int32), "parent " + label)
+        chk = o.check(attr)
+        for lam in (float(np.median(chk)), float(np.max(chk)) * 1.01 + 1e-9):
+            for dec in (0, 1, 2, 3):
+                assert_same(t.filter(attr, lam, dec), o.filter(attr, lam, dec), "%s rule %d lambda %r" % (label, dec, lam))
+        t.free()
+
+
 @pytest.mark.parametrize("shape,conn", [((57, 83), 4), ((61, 47), 8), ((9, 23, 31), 6), ((7, 19, 17), 26)])
 def test_every_attribute(ctx, shape, conn):
     """check() of all 18 attributes (every group of AttribsArray) against the oracle; the eigenvalue
