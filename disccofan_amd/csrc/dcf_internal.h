@@ -75,11 +75,12 @@ struct dcf_ctx {
     int lt2[3] = { 64, 64, 1 }, lg2[3] = { 1, 1, 1 };
     int lt3[3] = { 16, 16, 16 }, lg3[3] = { 1, 1, 1 };
     /* refining passes keep {union-find entry, key, coarse node} in one 16-byte record per element
-     * (DCF_RECORDS, 0 off) */
-    int records = 1;
+     * (DCF_RECORDS, 0 off: the default, since the precomputed targets already carry what the records
+     * would) */
+    int records = 0;
     /* union targets of pixel passes precomputed by the level scatter (DCF_TARGETS): 0 off, 1 on,
      * 2 batches only */
-    int targets = 2;
+    int targets = 1;
     /* zone graph of the first pass (DCF_ZONES): 0 off, 1 every single-image first pass,
      * 2 the coarse pass of u16 images and 2D u8 images */
     int zones = 2;

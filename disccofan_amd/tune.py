@@ -1,0 +1,176 @@
+"""Autotuner for the build's tunables: the zone graph, precomputed union targets, 16-byte records and the
+level-list tile shapes (the DCF_* variables that a context reads when it is created).
+
+Each knob is tried in turn with the others held at their current best (coordinate descent from the
+defaults), timing build + filter of a sample workload on the GPU: a single image, or a batch through
+the batched form.  The outputs of every candidate must agree bit for bit (the tunables change the
+schedule, never the result); a difference raises.  The result is the fastest setting, as environment
+assignments to export before the process creates its contexts.
+
+    python -m disccofan_amd.tune --workload vnoise --size 2048
+    python -m disccofan_amd.tune --workload batch --count 64
+"""
+import argparse
+import os
+import time
+
+import numpy as np
+
+from . import Context, synth
+
+# knob -> candidate values, the library's default first
+KNOBS_2D = {
+    "DCF_ZONES": ["2", "1", "0"],
+    "DCF_RECORDS": ["0", "1"],
+    "DCF_TARGETS": ["1", "2", "0"],
+    "DCF_LEVEL_TILE2D": ["64x64x1", "32x32x1", "128x32x1", "128x128x1", "256x16x1"],
+    "DCF_LEVEL_GROUP2D": ["1x1x1", "2x2x1", "4x1x1"],
+}
+KNOBS_3D = {
+    "DCF_ZONES": ["2", "1", "0"],
+    "DCF_RECORDS": ["0", "1"],
+    "DCF_TARGETS": ["1", "2", "0"],
+    "DCF_LEVEL_TILE3D": ["16x16x16", "32x32x8", "64x16x4", "8x8x8"],
+    "DCF_LEVEL_GROUP3D": ["1x1x1", "2x2x2"],
+}
+# the batched form orders its level lists by each image's zone tiles: the level-tile knobs do not apply
+KNOBS_BATCH = {
+    "DCF_ZONES": ["2", "1", "0"],
+    "DCF_RECORDS": ["0", "1"],
+    "DCF_TARGETS": ["1", "2", "0"],
+}
+
+
+def knobs_for(is3d: bool, batched: bool) -> dict:
+    if batched:
+        return KNOBS_BATCH
+    return KNOBS_3D if is3d else KNOBS_2D
+
+
+def env_line(setting: dict) -> str:
+    return " ".join("export %s=%s;" % (k, v) for k, v in sorted(setting.items()))
+
+
+class _Runner:
+    """times one setting: a fresh context with the setting in the environment, one warm-up, then the
+    best of `repeats` runs; returns (seconds, outputs)"""
+
+    def __init__(self, images, lambdas, connectivity, tree_type, attr_idx, decision, batched, repeats, device):
+        self.images, self.lambdas = images, lambdas
+        self.conn, self.tt, self.attr, self.dec = connectivity, tree_type, attr_idx, decision
+        self.batched, self.repeats, self.device = batched, repeats, device
+        try:
+            import torch
+            self.torch = torch if torch.cuda.is_available() else None
+        except ImportError:
+            self.torch = None
+
+    def _once(self, ctx):
+        if self.batched:
+            return ctx.batch_filter(self.images, self.lambdas, connectivity=self.conn, tree_type=self.tt,
+                                    attr_idx=self.attr, decision=self.dec)
+        outs = []
+        for img, lam in zip(self.images, self.lambdas):
+            if self.torch is not None:
+                th = self.torch
+                dev = th.device("cuda:%d" % self.device)
+                src = th.from_numpy(img.view(np.int16) if img.dtype == np.uint16 else img).to(dev)
+                out = th.empty(img.shape, dtype=th.float32, device=dev)
+                t = ctx.build_device(src.data_ptr(), img.dtype, img.shape, self.conn, self.tt, self.attr)
+                t.filter_device(self.attr, lam, self.dec, out.data_ptr(), np.float32)
+                ctx.synchronize()
+                outs.append(out)
+                t.free()
+            else:
+                t = ctx.build(img, self.conn, self.tt, self.attr)
+                outs.append(t.filter(self.attr, lam, self.dec))
+                t.free()
+        return outs
+
+    def run(self, setting: dict):
+        saved = {k: os.environ.get(k) for k in setting}
+        os.environ.update(setting)
+        try:
+            ctx = Context(self.device)
+        finally:
+            for k, v in saved.items():
+                if v is None:
+                    os.environ.pop(k, None)
+                else:
+                    os.environ[k] = v
+        try:
+            self._once(ctx)
+            best = float("inf")
+            outs = None
+            for _ in range(self.repeats):
+                ctx.synchronize()
+                t0 = time.perf_counter()
+                outs = self._once(ctx)
+                ctx.synchronize()
+                best = min(best, time.perf_counter() - t0)
+            outs = [o.cpu().numpy() if hasattr(o, "cpu") else np.asarray(o) for o in outs]
+            return best, outs
+        finally:
+            ctx.close()
+
+
+def tune(images, lambdas, connectivity=4, tree_type="max", attr_idx=0, decision=0, batched=False, repeats=3,
+         sweeps=1, device=0, knobs=None, log=None):
+    """Coordinate descent over the knobs.  Returns (best setting, best seconds, table of (setting,
+    seconds)).  Raises RuntimeError when a candidate's output differs from the first candidate's."""
+    images = [np.ascontiguousarray(a) for a in images]
+    is3d = images[0].ndim == 3
+    knobs = knobs or knobs_for(is3d, batched)
+    runner = _Runner(images, lambdas, connectivity, tree_type, attr_idx, decision, batched, repeats, device)
+    best = {k: v[0] for k, v in knobs.items()}
+    best_s, ref = runner.run(best)
+    table = [(dict(best), best_s)]
+    if log:
+        log("default %s: %.2f ms" % (env_line(best), best_s * 1e3))
+    for _ in range(sweeps):
+        for k, values in knobs.items():
+            for v in values:
+                if v == best[k]:
+                    continue
+                cand = dict(best)
+                cand[k] = v
+                s, outs = runner.run(cand)
+                if any(not np.array_equal(a, b) for a, b in zip(outs, ref)):
+                    raise RuntimeError("setting %s changed the output" % env_line(cand))
+                table.append((cand, s))
+                if log:
+                    log("%s: %.2f ms" % (env_line(cand), s * 1e3))
+                if s < best_s:
+                    best, best_s = cand, s
+    return best, best_s, table
+
+
+def _workload(args):
+    if args.workload == "vnoise":
+        return [synth.vnoise(args.size, args.size)], [1000.0], 8, False
+    if args.workload == "spheres":
+        return [synth.spheres(args.size, args.size, args.size)], [500.0], 6, False
+    shapes = synth.batch_shapes(args.count, 64, args.size)
+    imgs = [synth.disks(H, W, is16, seed) for (W, H, is16, seed, lam) in shapes]
+    lams = [lam for (W, H, is16, seed, lam) in shapes]
+    return imgs, lams, 4, True
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--workload", choices=["vnoise", "spheres", "batch"], default="vnoise")
+    ap.add_argument("--size", type=int, default=2048, help="side of the image (vnoise, spheres) or the largest batch side")
+    ap.add_argument("--count", type=int, default=64, help="images in the batch")
+    ap.add_argument("--repeats", type=int, default=3)
+    ap.add_argument("--sweeps", type=int, default=1)
+    ap.add_argument("--device", type=int, default=0)
+    args = ap.parse_args(argv)
+    imgs, lams, conn, batched = _workload(args)
+    best, s, _ = tune(imgs, lams, connectivity=conn, decision=0, batched=batched, repeats=args.repeats,
+                      sweeps=args.sweeps, device=args.device, log=lambda m: print(m, flush=True))
+    print("best: %.2f ms" % (s * 1e3))
+    print(env_line(best))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,42 @@
+"""The tuner of the build's tunables (disccofan_amd/tune.py): its knob tables and output on the CPU,
+a short tuning run on the GPU (every candidate's output must agree bit for bit with the default's)."""
+import numpy as np
+import pytest
+
+from disccofan_amd import synth, tune
+
+
+def test_knob_tables():
+    for is3d, batched in ((False, False), (True, False), (False, True)):
+        knobs = tune.knobs_for(is3d, batched)
+        for name, values in knobs.items():
+            assert name.startswith("DCF_")
+            assert len(values) == len(set(values)) >= 2
+        assert ("DCF_LEVEL_TILE2D" in knobs) == (not is3d and not batched)
+        assert ("DCF_LEVEL_TILE3D" in knobs) == (is3d and not batched)
+    # the defaults come first: the library's own (zones 2, records 0, targets 1, tiles 64x64 / 16^3)
+    assert tune.KNOBS_2D["DCF_ZONES"][0] == "2" and tune.KNOBS_2D["DCF_LEVEL_TILE2D"][0] == "64x64x1"
+    assert tune.KNOBS_2D["DCF_RECORDS"][0] == "0" and tune.KNOBS_2D["DCF_TARGETS"][0] == "1"
+    assert tune.KNOBS_3D["DCF_LEVEL_TILE3D"][0] == "16x16x16"
+    for shape in tune.KNOBS_2D["DCF_LEVEL_TILE2D"] + tune.KNOBS_3D["DCF_LEVEL_TILE3D"]:
+        a, b, d = (int(v) for v in shape.split("x"))
+        assert a * b * d <= 1 << 20
+
+
+def test_env_line():
+    assert tune.env_line({"DCF_ZONES": "1", "DCF_RECORDS": "0"}) == "export DCF_RECORDS=0; export DCF_ZONES=1;"
+
+
+@pytest.mark.gpu
+def test_tune_single_image_and_batch():
+    img = synth.vnoise(384, 320)
+    best, s, table = tune.tune([img], [300.0], connectivity=8, repeats=1,
+                               knobs={"DCF_ZONES": ["2", "0"], "DCF_LEVEL_TILE2D": ["64x64x1", "32x32x1"]})
+    assert s > 0 and len(table) == 3
+    assert best["DCF_ZONES"] in ("2", "0") and best["DCF_LEVEL_TILE2D"] in ("64x64x1", "32x32x1")
+    shapes = synth.batch_shapes(6, 64, 200)
+    imgs = [synth.disks(H, W, is16, seed) for (W, H, is16, seed, lam) in shapes]
+    lams = [lam for (W, H, is16, seed, lam) in shapes]
+    best, s, table = tune.tune(imgs, lams, connectivity=4, batched=True, repeats=1)
+    assert len(table) == 1 + sum(len(v) - 1 for v in tune.KNOBS_BATCH.values())
+    assert set(best) == set(tune.KNOBS_BATCH)
