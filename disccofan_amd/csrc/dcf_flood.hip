@@ -670,6 +670,40 @@ __global__ __launch_bounds__(256) void k_tile_zones(ImgView iv, int shift, TileG
     }
 }
 
+/* the neighbour pairs of equal digit inside every tile (forward half of the neighbourhood): each zone
+ * merge needs one, so a tiling has at least n - pairs zones.  Counted before labelling, it tells when
+ * the zones are sure to exceed half the pixels and the labelling can be skipped. */
+__global__ __launch_bounds__(256) void k_zone_pairs(ImgView iv, int shift, TileGeo tg, Nbr nb, unsigned long long *count)
+{
+    __shared__ unsigned long long part[4];
+    const TileAt ta = tile_at(tg, blockIdx.x);
+    const int tsz = tg.TW * tg.TH * tg.TD;
+    const int64_t W = ta.W, WH = (int64_t)ta.W * ta.H;
+    unsigned long long c = 0;
+    for (int l = threadIdx.x; l < tsz; l += blockDim.x) {
+        const int lx = l % tg.TW, ly = (l / tg.TW) % tg.TH, lz = l / (tg.TW * tg.TH);
+        const int x = ta.x0 + lx, y = ta.y0 + ly, z = ta.z0 + lz;
+        if (x >= ta.W || y >= ta.H || z >= ta.D) continue;
+        const uint32_t d = fl_key(iv, ta.off + x + y * W + z * WH) >> shift;
+        for (int k = 0; k < nb.n; k++) {
+            const int dx = nb.dx[k], dy = nb.dy[k], dz = nb.dz[k];
+            if (!(dz > 0 || (dz == 0 && (dy > 0 || (dy == 0 && dx > 0))))) continue;
+            const int nx = lx + dx, ny = ly + dy, nz = lz + dz;
+            if (nx < 0 || nx >= tg.TW || ny < 0 || ny >= tg.TH || nz < 0 || nz >= tg.TD) continue;
+            const int gx = ta.x0 + nx, gy = ta.y0 + ny, gz = ta.z0 + nz;
+            if (gx >= ta.W || gy >= ta.H || gz >= ta.D) continue;
+            c += (fl_key(iv, ta.off + gx + gy * W + gz * WH) >> shift) == d;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t lo = __shfl_xor((uint32_t)c, o, 64), hi = __shfl_xor((uint32_t)(c >> 32), o, 64);
+        c += ((unsigned long long)hi << 32) | lo;
+    }
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(count, part[0] + part[1] + part[2] + part[3]);
+}
+
 __device__ __forceinline__ uint32_t zh_hash(unsigned long long k)
 {
     k ^= k >> 31;
@@ -1451,6 +1485,23 @@ static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAtt
         tg.bimg = la.bimg;
         tg.nbatch = la.nbatch;
         ntiles = la.bimg ? batch_tiles : (int64_t)tg.ntx * tg.nty * ((tg.D + tg.TD - 1) / tg.TD);
+    }
+    if (zones && la.bimg) {
+        /* a batch (often of noisy images): at least n - pairs zones; when that is already more than half
+         * the pixels, skip the labelling (on the 8192^2 u16 image the count cost more than it saved) */
+        unsigned long long *pairs = (unsigned long long *)sc.get(8);
+        if (!pairs) return dcf_set_error(ctx, DCF_ENOMEM, "zone pair count");
+        unsigned long long npairs = 0;
+        DCF_LAUNCH(ctx, "k_fill", k_fill_u32f, dim3(1), dim3(64), 0, (uint32_t *)pairs, 0u, (int64_t)2);
+        DCF_LAUNCH(ctx, "k_zone_pairs", k_zone_pairs, dim3((unsigned)ntiles), dim3(256), 0, la.iv, la.shift, tg, la.nb, pairs);
+        DCF_HIP(ctx, hipMemcpyAsync(&npairs, pairs, 8, hipMemcpyDeviceToHost, ctx->stream));
+        DCF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (((uint64_t)n - (npairs < (uint64_t)n ? npairs : (uint64_t)n)) * 2 > (uint64_t)n) {
+            zones = false;
+            la.zone = 0;
+        }
+    }
+    if (zones) {
         zrep = (uint32_t *)sc.get((size_t)n * 4);
         if (!zrep) return dcf_set_error(ctx, DCF_ENOMEM, "zone representatives");
         DCF_LAUNCH(ctx, "k_tile_zones", k_tile_zones, dim3((unsigned)ntiles), dim3(256), 0, la.iv, la.shift, tg, la.nb, zrep, la.uf,
