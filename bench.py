@@ -113,9 +113,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--kernels", action="store_true", help="print the per-kernel event times to stderr")
-    ap.add_argument("--roofline-kernel", default="k_union",
-                    help="the kernel bracketed by HIP events in the timed region (the dominant one); "
-                         "--kernels brackets every kernel instead")
+    ap.add_argument("--roofline-kernel", default="k_final_pixels",
+                    help="the kernel bracketed by HIP events in the timed region (the dominant one: k_final_pixels, "
+                         "the largest share of the kernel time in every workload's profile; k_union for the union "
+                         "kernels); --kernels brackets every kernel instead")
     ap.add_argument("--no-profile", action="store_true",
                     help="time the steps without per-launch HIP events (no roofline object)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
