@@ -787,7 +787,7 @@ static int subtree_firsts(dcf_tree *t, const uint8_t *pass, uint32_t *lr, uint32
  * reassociate, so every path is evaluated in order.  The tree is cut into heavy paths (each node
  * continues into its child of largest area), laid out so that each path is one run of the preorder:
  * one thread walks a path with f in a register, streaming the run's greys and flags (gathered into
- * preorder beforehand) 32 at a time.  A path
+ * preorder beforehand) 64 at a time.  A path
  * starts from its head's parent, which lies on a path with one light edge fewer; a light child has at
  * most half its parent's area, so at most log2(n) + 1 rounds of launches cover the tree. */
 __global__ __launch_bounds__(256) void k_fill_u64(uint64_t *a, uint64_t v, int64_t n)
@@ -839,7 +839,7 @@ __global__ __launch_bounds__(256) void k_chain_gather(const uint32_t *__restrict
 }
 
 /* the heavy paths whose heads have light depth round + 1 (L counts the heads from the root down); a
- * path is the run of preorder ranks from its head to the next head, read 32 at a time; values go to
+ * path is the run of preorder ranks from its head to the next head, read 64 at a time; values go to
  * Vr by rank */
 __global__ __launch_bounds__(256) void k_sub_chain(int round, const uint32_t *__restrict__ rank, const uint8_t *__restrict__ head,
                                                    const uint32_t *__restrict__ L, const int32_t *__restrict__ npar,
@@ -863,7 +863,7 @@ __global__ __launch_bounds__(256) void k_sub_chain(int round, const uint32_t *__
     }
     Vr[r0] = f;
     float gprev = gk;
-    constexpr int B = 32;
+    constexpr int B = 64;
     for (int64_t r = r0 + 1; r < K; r += B) {
         float gv[B];
         uint8_t fv[B];
