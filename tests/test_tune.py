@@ -40,3 +40,28 @@ def test_tune_single_image_and_batch():
     best, s, table = tune.tune(imgs, lams, connectivity=4, batched=True, repeats=1)
     assert len(table) == 1 + sum(len(v) - 1 for v in tune.KNOBS_BATCH.values())
     assert set(best) == set(tune.KNOBS_BATCH)
+
+
+def _fake_runner(monkeypatch, cost, outputs=None):
+    """replace the GPU runner: the time of a setting is cost(setting), its output outputs(setting)"""
+    def run(self, setting):
+        out = outputs(setting) if outputs else [np.zeros(3)]
+        return cost(setting), out
+    monkeypatch.setattr(tune._Runner, "run", run)
+
+
+def test_coordinate_descent_keeps_the_best_of_each_knob(monkeypatch):
+    knobs = {"A": ["0", "1", "2"], "B": ["x", "y"]}
+    table = {("0", "x"): 5.0, ("1", "x"): 3.0, ("2", "x"): 4.0, ("1", "y"): 2.0, ("0", "y"): 9.0, ("2", "y"): 9.0}
+    _fake_runner(monkeypatch, lambda st: table[(st["A"], st["B"])])
+    best, s, rows = tune.tune([np.zeros((4, 4), np.uint8)], [1.0], knobs=knobs)
+    assert best == {"A": "1", "B": "y"} and s == 2.0
+    # the default, two other values of A, then one other value of B with A at its best
+    assert [r[0] for r in rows] == [{"A": "0", "B": "x"}, {"A": "1", "B": "x"}, {"A": "2", "B": "x"}, {"A": "1", "B": "y"}]
+
+
+def test_a_setting_that_changes_the_output_is_refused(monkeypatch):
+    knobs = {"A": ["0", "1"]}
+    _fake_runner(monkeypatch, lambda st: 1.0, lambda st: [np.full(3, int(st["A"]))])
+    with pytest.raises(RuntimeError, match="changed the output"):
+        tune.tune([np.zeros((4, 4), np.uint8)], [1.0], knobs=knobs)
