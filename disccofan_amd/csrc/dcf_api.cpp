@@ -175,6 +175,14 @@ extern "C" int dcf_ctx_create(int device, unsigned flags, dcf_ctx **out)
     if (const char *z = getenv("DCF_ZONES")) c->zones = atoi(z);
     if (const char *z = getenv("DCF_TARGETS")) c->targets = atoi(z);
     if (const char *z = getenv("DCF_RECORDS")) c->records = atoi(z);
+    if (const char *e = getenv("DCF_ZONE_TILE2D")) {
+        int a = 0, b = 0;
+        if (sscanf(e, "%dx%d", &a, &b) == 2 && a > 0 && b > 0 && a * b <= ZONE_TILE_MAX) { c->zt2[0] = a; c->zt2[1] = b; }
+    }
+    if (const char *e = getenv("DCF_ZONE_TILE3D")) {
+        int a = 0;
+        if (sscanf(e, "%d", &a) == 1 && a > 0 && a * a * a <= ZONE_TILE_MAX) c->zt3 = a;
+    }
     shape3("DCF_LEVEL_TILE2D", c->lt2);
     shape3("DCF_LEVEL_GROUP2D", c->lg2);
     shape3("DCF_LEVEL_TILE3D", c->lt3);
@@ -400,8 +408,9 @@ static int batch_filter_same(dcf_ctx *ctx, const dcf_batch_image *images, int co
         tab[i].inv = (b.flags & DCF_IMAGE_MIN_TREE) ? 1u : ((b.flags & DCF_IMAGE_MAX_TREE) ? 0u : (tree_type == DCF_MIN_TREE));
         tab[i].pad = 0;
         any_min |= tab[i].inv;
-        tiles += d3 ? ((b.dims[0] + ZONE_T3 - 1) / ZONE_T3) * ((b.dims[1] + ZONE_T3 - 1) / ZONE_T3) * ((b.dims[2] + ZONE_T3 - 1) / ZONE_T3)
-                    : ((b.dims[0] + ZONE_T2 - 1) / ZONE_T2) * ((b.dims[1] + ZONE_T2 - 1) / ZONE_T2);
+        const uint64_t t3 = (uint64_t)ctx->zt3, tw = (uint64_t)ctx->zt2[0], th = (uint64_t)ctx->zt2[1];
+        tiles += d3 ? ((b.dims[0] + t3 - 1) / t3) * ((b.dims[1] + t3 - 1) / t3) * ((b.dims[2] + t3 - 1) / t3)
+                    : ((b.dims[0] + tw - 1) / tw) * ((b.dims[1] + th - 1) / th);
         lam[i] = b.lambda;
         N += n;
         if (N >= (1ull << 30)) return dcf_set_error(ctx, DCF_EUNSUPPORTED, "a batch is limited to 2^30 pixels");

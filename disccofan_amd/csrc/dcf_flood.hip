@@ -581,7 +581,7 @@ __global__ __launch_bounds__(256) void k_lvl_scatter(ImgView iv, int pass, int s
  * (zone, neighbouring zone) pairs, de-duplicated per tile in LDS, instead of one attempt per pixel
  * and neighbour.  The canonical levelroot of a set is its minimum pixel of the level, which is its
  * minimum zone representative of the level, so the level kernels work unchanged. */
-#define ZT_MAX 4096
+#define ZT_MAX ZONE_TILE_MAX
 #define ZH_CAP 4096
 #define ZH_EMPTY 0xFFFFFFFFFFFFFFFFull
 
@@ -1478,8 +1478,8 @@ static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAtt
     la.zone = zones ? 1 : 0;
     if (zones) {
         tg.W = la.g.W; tg.H = la.g.H; tg.D = la.g.D;
-        if (is3d) { tg.TW = ZONE_T3; tg.TH = ZONE_T3; tg.TD = ZONE_T3; }
-        else { tg.TW = ZONE_T2; tg.TH = ZONE_T2; tg.TD = 1; }
+        if (is3d) { tg.TW = ctx->zt3; tg.TH = ctx->zt3; tg.TD = ctx->zt3; }
+        else { tg.TW = ctx->zt2[0]; tg.TH = ctx->zt2[1]; tg.TD = 1; }
         tg.ntx = (tg.W + tg.TW - 1) / tg.TW;
         tg.nty = (tg.H + tg.TH - 1) / tg.TH;
         tg.bimg = la.bimg;
@@ -1550,8 +1550,9 @@ static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAtt
     sg.GX = gs[0]; sg.GY = gs[1]; sg.GZ = gs[2];
     if (la.bimg) {
         /* a batch: every image's own zone tiles (their first indices are in the image table) */
-        const int tz = is3d ? ZONE_T3 : ZONE_T2;
-        sg.TW = tz; sg.TH = tz; sg.TD = is3d ? tz : 1;
+        sg.TW = is3d ? ctx->zt3 : ctx->zt2[0];
+        sg.TH = is3d ? ctx->zt3 : ctx->zt2[1];
+        sg.TD = is3d ? ctx->zt3 : 1;
         sg.GX = sg.GY = sg.GZ = 1;
     }
     sg.ntx = (sg.W + sg.TW - 1) / sg.TW;

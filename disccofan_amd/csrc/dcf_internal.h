@@ -55,6 +55,12 @@ struct KernelStat {
     double ms = 0;
 };
 
+/* zone tiles of the first pass by default: 64x64 in 2D, 16x16x16 in 3D (a context may set others, see
+ * dcf_ctx::zt2 / zt3); a tile holds at most ZONE_TILE_MAX pixels */
+#define ZONE_T2 64
+#define ZONE_T3 16
+#define ZONE_TILE_MAX 4096
+
 struct dcf_ctx {
     int device = 0;
     hipStream_t own = nullptr;
@@ -74,6 +80,9 @@ struct dcf_ctx {
      * groups of tiles kept together in the list (DCF_LEVEL_GROUP2D / DCF_LEVEL_GROUP3D) */
     int lt2[3] = { 64, 64, 1 }, lg2[3] = { 1, 1, 1 };
     int lt3[3] = { 16, 16, 16 }, lg3[3] = { 1, 1, 1 };
+    /* zone tiles of the first pass (DCF_ZONE_TILE2D "WxH", DCF_ZONE_TILE3D "N"; at most ZONE_TILE_MAX pixels, the
+     * size of the labelling's LDS tables), which are also a batch's level-list tiles */
+    int zt2[2] = { ZONE_T2, ZONE_T2 }, zt3 = ZONE_T3;
     /* refining passes keep {union-find entry, key, coarse node} in one 16-byte record per element
      * (DCF_RECORDS, 0 off: the default, since the precomputed targets already carry what the records
      * would) */
@@ -90,14 +99,12 @@ struct dcf_ctx {
 struct BatchImg {
     uint64_t off;     /* first pixel of the image */
     uint32_t W, H, D; /* its dimensions */
-    uint32_t tile0;   /* its first tile in the batch's zone tiles (ZONE_T2^2 / ZONE_T3^3 tiles) */
+    uint32_t tile0;   /* its first tile in the batch's zone tiles (the context's zone tile shape) */
     uint32_t inv;     /* 1: a min-tree image, stored inverted against its own grey maximum */
     uint32_t pad;
 };
 
-/* zone tiles of the first pass: 64x64 in 2D, 16x16x16 in 3D */
-#define ZONE_T2 64
-#define ZONE_T3 16
+
 
 /* output of one batch image: device pointer and element type */
 struct BatchOut {

@@ -1,5 +1,5 @@
-"""Autotuner for the build's tunables: the zone graph, precomputed union targets, 16-byte records and the
-level-list tile shapes (the DCF_* variables that a context reads when it is created).
+"""Autotuner for the build's tunables: the zone graph and its tile shape, precomputed union targets, 16-byte
+records and the level-list tile shapes (the DCF_* variables that a context reads when it is created).
 
 Each knob is tried in turn with the others held at their current best (coordinate descent from the
 defaults), timing build + filter of a sample workload on the GPU: a single image, or a batch through
@@ -25,6 +25,7 @@ KNOBS_2D = {
     "DCF_TARGETS": ["1", "2", "0"],
     "DCF_LEVEL_TILE2D": ["64x64x1", "32x32x1", "128x32x1", "128x128x1", "256x16x1"],
     "DCF_LEVEL_GROUP2D": ["1x1x1", "2x2x1", "4x1x1"],
+    "DCF_ZONE_TILE2D": ["64x64", "32x32", "128x32", "32x128"],
 }
 KNOBS_3D = {
     "DCF_ZONES": ["2", "1", "0"],
@@ -32,12 +33,14 @@ KNOBS_3D = {
     "DCF_TARGETS": ["1", "2", "0"],
     "DCF_LEVEL_TILE3D": ["16x16x16", "32x32x8", "64x16x4", "8x8x8"],
     "DCF_LEVEL_GROUP3D": ["1x1x1", "2x2x2"],
+    "DCF_ZONE_TILE3D": ["16", "8", "12"],
 }
-# the batched form orders its level lists by each image's zone tiles: the level-tile knobs do not apply
+# the batched form orders its level lists by each image's zone tiles: the zone tiles are its tile shape
 KNOBS_BATCH = {
     "DCF_ZONES": ["2", "1", "0"],
     "DCF_RECORDS": ["0", "1"],
     "DCF_TARGETS": ["1", "2", "0"],
+    "DCF_ZONE_TILE2D": ["64x64", "32x32", "128x32", "32x128"],
 }
 
 

@@ -124,6 +124,40 @@ def test_random_3d(ctx, seed):
             _compare_all(ctx, img, conn, tt, label="rand3d s%d" % seed)
 
 
+@pytest.mark.parametrize("tile2d,tile3d", [("32x32", "8"), ("128x32", "12"), ("16x128", "16")])
+def test_zone_tiles(monkeypatch, tile2d, tile3d):
+    """other zone tiles (DCF_ZONE_TILE2D / DCF_ZONE_TILE3D, non-square included), with the zone graph forced
+    on: single images, and a batch, whose level lists and per-image tile tables follow the same tiles"""
+    monkeypatch.setenv("DCF_ZONES", "1")
+    monkeypatch.setenv("DCF_ZONE_TILE2D", tile2d)
+    monkeypatch.setenv("DCF_ZONE_TILE3D", tile3d)
+    c = dcf.Context(0)
+    try:
+        rng = np.random.default_rng(900 + len(tile2d))
+        cases = [(synth.vnoise(150, 203), 8), (rng.integers(0, 9, size=(97, 61)).astype(np.uint8), 4),
+                 (rng.integers(0, 4, size=(19, 23, 29)).astype(np.uint8), 6),
+                 (rng.integers(0, 300, size=(13, 17, 21)).astype(np.uint16), 26)]
+        for img, conn in cases:
+            for tt in ("max", "min"):
+                label = "zone tiles %s / %s %s %s" % (tile2d, tile3d, img.dtype, img.shape)
+                say(label, conn, tt)
+                o = OracleTree(img, conn, tt, 0)
+                t = c.build(img, conn, tt, 0)
+                assert_same(t.parent(), o.parent().astype(np.int32), "parent " + label)
+                assert_same(t.filter(0, 15, 3), o.filter(0, 15, 3), "subtractive " + label)
+                t.free()
+        imgs = [synth.vnoise(int(rng.integers(20, 200)), int(rng.integers(20, 200))) for _ in range(5)]
+        imgs += [rng.integers(0, 6, size=(int(rng.integers(5, 150)), int(rng.integers(5, 150)))).astype(np.uint8) for _ in range(5)]
+        lams = [float(rng.choice([3, 20, 100])) for _ in imgs]
+        say("zone tiles batch", tile2d)
+        outs = c.batch_filter(imgs, lams, connectivity=4, decision=0)
+        for img, lam, out in zip(imgs, lams, outs):
+            want = OracleTree(img, 4, "max", 0).filter(0, lam, 0).astype(img.dtype)
+            assert_same(out, want, "batch zone tiles %s %s" % (tile2d, img.shape))
+    finally:
+        c.close()
+
+
 @pytest.mark.parametrize("zones,targets,records", [("0", "1", "1"), ("1", "0", "1"), ("1", "1", "0"), ("0", "0", "0")])
 def test_build_paths(monkeypatch, zones, targets, records):
     """the build's optional paths on single images, each on and off: the zone graph of the first pass

@@ -21,6 +21,11 @@ def test_knob_tables():
     for shape in tune.KNOBS_2D["DCF_LEVEL_TILE2D"] + tune.KNOBS_3D["DCF_LEVEL_TILE3D"]:
         a, b, d = (int(v) for v in shape.split("x"))
         assert a * b * d <= 1 << 20
+    # zone tiles fit the labelling's LDS tables (4096 pixels)
+    for shape in tune.KNOBS_2D["DCF_ZONE_TILE2D"] + tune.KNOBS_BATCH["DCF_ZONE_TILE2D"]:
+        a, b = (int(v) for v in shape.split("x"))
+        assert a * b <= 4096
+    assert all(int(v) ** 3 <= 4096 for v in tune.KNOBS_3D["DCF_ZONE_TILE3D"])
 
 
 def test_env_line():
