@@ -168,7 +168,8 @@ extern "C" int dcf_ctx_create(int device, unsigned flags, dcf_ctx **out)
     auto shape3 = [](const char *env, int v[3]) {
         const char *e = getenv(env);
         int a = 0, b = 0, d = 0;
-        if (e && sscanf(e, "%dx%dx%d", &a, &b, &d) == 3 && a > 0 && b > 0 && d > 0 && (int64_t)a * b * d <= (1 << 20)) {
+        if (e && sscanf(e, "%dx%dx%d", &a, &b, &d) == 3 && a > 0 && b > 0 && d > 0 && a <= (1 << 20) && b <= (1 << 20) &&
+            d <= (1 << 20) && (int64_t)a * b * d <= (1 << 20)) {
             v[0] = a; v[1] = b; v[2] = d;
         }
     };
@@ -177,11 +178,15 @@ extern "C" int dcf_ctx_create(int device, unsigned flags, dcf_ctx **out)
     if (const char *z = getenv("DCF_RECORDS")) c->records = atoi(z);
     if (const char *e = getenv("DCF_ZONE_TILE2D")) {
         int a = 0, b = 0;
-        if (sscanf(e, "%dx%d", &a, &b) == 2 && a > 0 && b > 0 && a * b <= ZONE_TILE_MAX) { c->zt2[0] = a; c->zt2[1] = b; }
+        if (sscanf(e, "%dx%d", &a, &b) == 2 && a > 0 && b > 0 && a <= ZONE_TILE_MAX && b <= ZONE_TILE_MAX &&
+            (int64_t)a * b <= ZONE_TILE_MAX) {
+            c->zt2[0] = a;
+            c->zt2[1] = b;
+        }
     }
     if (const char *e = getenv("DCF_ZONE_TILE3D")) {
         int a = 0;
-        if (sscanf(e, "%d", &a) == 1 && a > 0 && a * a * a <= ZONE_TILE_MAX) c->zt3 = a;
+        if (sscanf(e, "%d", &a) == 1 && a > 0 && a <= ZONE_TILE_MAX && (int64_t)a * a * a <= ZONE_TILE_MAX) c->zt3 = a;
     }
     shape3("DCF_LEVEL_TILE2D", c->lt2);
     shape3("DCF_LEVEL_GROUP2D", c->lg2);
