@@ -83,7 +83,13 @@ void dcf_prof_begin(dcf_ctx *ctx, const char *name, hipEvent_t *a, hipEvent_t *b
     (void)name;
     *a = take_event(ctx);
     *b = take_event(ctx);
-    if (*a) (void)hipEventRecord(*a, ctx->stream);
+    if (!*a || !*b) { /* one of the pair could not be created: this launch goes untimed, the other waits in the pool */
+        if (*a) ctx->event_pool.push_back(*a);
+        if (*b) ctx->event_pool.push_back(*b);
+        *a = *b = nullptr;
+        return;
+    }
+    (void)hipEventRecord(*a, ctx->stream);
 }
 
 void dcf_prof_end(dcf_ctx *ctx, const char *name, hipEvent_t a, hipEvent_t b)
@@ -285,6 +291,18 @@ extern "C" const char *dcf_kernel_name(dcf_ctx *ctx, int i)
 /* ------------------------------------------------------------------------------------------------ */
 static size_t dtype_size(dcf_dtype d) { return d == DCF_U8 ? 1 : (d == DCF_U16 ? 2 : 4); }
 
+/* pixels of an image whose dimensions are each below 2^31, without wrap-around: the product of the
+ * first two is checked before the third multiplies it (both partial products stay below 2^62);
+ * UINT64_MAX when the count reaches 2^31 - 1 */
+static uint64_t pixel_count(const uint64_t d[3])
+{
+    if (d[0] > 0x7FFFFFFFull || d[1] > 0x7FFFFFFFull || d[2] > 0x7FFFFFFFull) return UINT64_MAX;
+    const uint64_t a = d[0] * d[1];
+    if (a >= 0x7FFFFFFFull) return UINT64_MAX;
+    const uint64_t n = a * d[2];
+    return n >= 0x7FFFFFFFull ? UINT64_MAX : n;
+}
+
 static void tree_release(dcf_tree *t)
 {
     dcf_ctx *ctx = t->ctx;
@@ -317,10 +335,9 @@ extern "C" int dcf_tree_build(dcf_ctx *ctx, const void *img, dcf_dtype dtype, co
     *out = nullptr;
     if (dtype != DCF_U8 && dtype != DCF_U16 && dtype != DCF_F32) return dcf_set_error(ctx, DCF_EINVAL, "bad dtype %d", (int)dtype);
     if (dims[0] == 0 || dims[1] == 0 || dims[2] == 0) return dcf_set_error(ctx, DCF_EINVAL, "empty image");
-    if (dims[0] > 0x7FFFFFFFull || dims[1] > 0x7FFFFFFFull || dims[2] > 0x7FFFFFFFull)
-        return dcf_set_error(ctx, DCF_EINVAL, "dimension too large");
-    uint64_t n = dims[0] * dims[1] * dims[2];
-    if (n >= 0x7FFFFFFFull) return dcf_set_error(ctx, DCF_EINVAL, "image of %llu pixels exceeds 2^31-1", (unsigned long long)n);
+    const uint64_t n = pixel_count(dims);
+    if (n == UINT64_MAX) return dcf_set_error(ctx, DCF_EINVAL, "image of %llux%llux%llu pixels: the count exceeds 2^31-1",
+                                              (unsigned long long)dims[0], (unsigned long long)dims[1], (unsigned long long)dims[2]);
     if (tree_type != DCF_MAX_TREE && tree_type != DCF_MIN_TREE) return dcf_set_error(ctx, DCF_EINVAL, "bad tree type %d", tree_type);
     if (attr_idx < 0 || attr_idx >= DCF_NUM_ATTR) return dcf_set_error(ctx, DCF_EINVAL, "bad attribute index %d", attr_idx);
     /* check_connectivity (io.c:64-79) */
@@ -399,12 +416,11 @@ static int batch_filter_same(dcf_ctx *ctx, const dcf_batch_image *images, int co
         if (!b.img || !b.out) return dcf_set_error(ctx, DCF_EINVAL, "image %d: null pointer", i);
         if (b.dtype != DCF_U8 && b.dtype != DCF_U16) return dcf_set_error(ctx, DCF_EINVAL, "image %d: dtype must be u8 or u16", i);
         if (!b.dims[0] || !b.dims[1] || !b.dims[2]) return dcf_set_error(ctx, DCF_EINVAL, "image %d: empty", i);
-        if (b.dims[0] > 0x7FFFFFFFull || b.dims[1] > 0x7FFFFFFFull || b.dims[2] > 0x7FFFFFFFull)
-            return dcf_set_error(ctx, DCF_EINVAL, "image %d: dimension too large", i);
+        const uint64_t n = pixel_count(b.dims);
+        if (n == UINT64_MAX) return dcf_set_error(ctx, DCF_EINVAL, "image %d: the pixel count exceeds 2^31-1", i);
         const int d3 = b.dims[2] > 1;
         if (is3d < 0) is3d = d3;
         else if (is3d != d3) return dcf_set_error(ctx, DCF_EINVAL, "image %d: a batch is all 2D or all 3D", i);
-        const uint64_t n = b.dims[0] * b.dims[1] * b.dims[2];
         tab[i].off = N;
         tab[i].W = (uint32_t)b.dims[0];
         tab[i].H = (uint32_t)b.dims[1];
@@ -555,12 +571,12 @@ extern "C" int dcf_batch_filter(dcf_ctx *ctx, const dcf_batch_image *images, int
         if (!b.img || !b.out) return dcf_set_error(ctx, DCF_EINVAL, "image %d: null pointer", i);
         if (b.dtype != DCF_U8 && b.dtype != DCF_U16) return dcf_set_error(ctx, DCF_EINVAL, "image %d: dtype must be u8 or u16", i);
         if (!b.dims[0] || !b.dims[1] || !b.dims[2]) return dcf_set_error(ctx, DCF_EINVAL, "image %d: empty", i);
-        if (b.dims[0] > 0x7FFFFFFFull || b.dims[1] > 0x7FFFFFFFull || b.dims[2] > 0x7FFFFFFFull)
-            return dcf_set_error(ctx, DCF_EINVAL, "image %d: dimension too large", i);
+        const uint64_t n = pixel_count(b.dims);
+        if (n == UINT64_MAX) return dcf_set_error(ctx, DCF_EINVAL, "image %d: the pixel count exceeds 2^31-1", i);
         const int d3 = b.dims[2] > 1;
         if (is3d < 0) is3d = d3;
         else if (is3d != d3) return dcf_set_error(ctx, DCF_EINVAL, "image %d: a batch is all 2D or all 3D", i);
-        N += b.dims[0] * b.dims[1] * b.dims[2];
+        N += n;
         if (N >= (1ull << 30)) return dcf_set_error(ctx, DCF_EUNSUPPORTED, "a batch is limited to 2^30 pixels");
         if ((b.flags & ~(DCF_IMAGE_MAX_TREE | DCF_IMAGE_MIN_TREE)) || b.flags == (DCF_IMAGE_MAX_TREE | DCF_IMAGE_MIN_TREE))
             return dcf_set_error(ctx, DCF_EINVAL, "image %d: bad flags 0x%x", i, b.flags);
@@ -695,7 +711,9 @@ static int tree_csv_text(dcf_tree *t, int attr_idx, std::string &text)
     else if (grp == 4) text += "x_mean,y_mean,z_mean,x_wmean,y_wmean,z_wmean,elong,flat,spars,ncomp";
     text += "\n";
     const uint64_t W = t->dims[0], WH = t->dims[0] * t->dims[1];
-    char buf[512];
+    /* the longest row: 6 longs of 20 characters, 2 "%f" and 10 "%.2f" of floats of at most 39 integer
+     * digits (48 characters each with sign and fraction), commas: under 800 */
+    char buf[1024];
     for (int64_t k = 0; k < t->K; k++) {
         const uint64_t i = n2p[(size_t)k];
         const long x = (long)((i % WH) % W), y = (long)((i % WH) / W), z = (long)(i / WH);
@@ -708,13 +726,19 @@ static int tree_csv_text(dcf_tree *t, int attr_idx, std::string &text)
                            (double)(dh * (float)sz), sz);
         const double *c = cols.data() + (size_t)k * ncol;
         auto f = [&](int j) { return (double)(float)c[j]; };
-        if (grp == 1)
-            len += snprintf(buf + len, sizeof buf - len, "%.2f,%.2f,%.2f,%.2f,%.2f", f(0), f(1), f(2), f(3), f(4));
-        else if (grp == 2 || grp == 3)
-            len += snprintf(buf + len, sizeof buf - len, "%.2f,%.2f,%.2f", f(0), f(1), f(2));
-        else if (grp == 4)
-            len += snprintf(buf + len, sizeof buf - len, "%.2f,%.2f,%.2f,%.2f,%.2f,%.2f,%.3f,%.3f,%.3f,%.4f ", f(0), f(1), f(2), f(3),
-                            f(4), f(5), f(6), f(7), f(8), f(9));
+        int more = 0;
+        if (len >= 0 && len < (int)sizeof buf) {
+            if (grp == 1)
+                more = snprintf(buf + len, sizeof buf - len, "%.2f,%.2f,%.2f,%.2f,%.2f", f(0), f(1), f(2), f(3), f(4));
+            else if (grp == 2 || grp == 3)
+                more = snprintf(buf + len, sizeof buf - len, "%.2f,%.2f,%.2f", f(0), f(1), f(2));
+            else if (grp == 4)
+                more = snprintf(buf + len, sizeof buf - len, "%.2f,%.2f,%.2f,%.2f,%.2f,%.2f,%.3f,%.3f,%.3f,%.4f ", f(0), f(1), f(2),
+                                f(3), f(4), f(5), f(6), f(7), f(8), f(9));
+        }
+        if (len < 0 || more < 0 || len + more >= (int)sizeof buf)
+            return dcf_set_error(ctx, DCF_EINVAL, "tree row %lld does not fit its %d-byte buffer", (long long)k, (int)sizeof buf);
+        len += more;
         text.append(buf, (size_t)len);
         text += "\n";
     }
@@ -933,7 +957,8 @@ static int to_ops(dcf_ctx *ctx, const std::vector<OpTuple> &tp, std::vector<Op> 
 extern "C" int dcf_run_operations(dcf_ctx *ctx, const void *img, dcf_dtype dtype, const uint64_t dims[3], int connectivity,
                                   int tree_type, const float pixdim[3], const char *opstr, dcf_output_fn cb, void *user)
 {
-    if (!ctx || !opstr) return DCF_EINVAL;
+    if (!ctx) return DCF_EINVAL;
+    if (!opstr || !img || !dims) return dcf_set_error(ctx, DCF_EINVAL, "null argument");
     std::vector<OpTuple> tp;
     std::string perr;
     if (parse_tuples(opstr, tp, perr) != 0) return dcf_set_error(ctx, DCF_EINVAL, "operation list: %s", perr.c_str());
@@ -942,13 +967,14 @@ extern "C" int dcf_run_operations(dcf_ctx *ctx, const void *img, dcf_dtype dtype
     if (rc) return rc;
     if (ops.empty()) return DCF_OK;
     dcf_tree *t = nullptr;
-    const int64_t n = (int64_t)(dims[0] * dims[1] * dims[2]);
+    int64_t n = 0; /* pixels, from the tree once the build has checked the dimensions */
     std::vector<float> fbuf;
     for (size_t i = 0; i < ops.size() && rc == DCF_OK; i++) {
         const Op &o = ops[i];
         if (i == 0) {
             rc = dcf_tree_build(ctx, img, dtype, dims, connectivity, tree_type, o.attr, pixdim, 0, &t);
             if (rc) break;
+            n = t->n;
         } else if (dcf_group_of(o.attr) != dcf_group_of(ops[i - 1].attr)) {
             rc = dcf_tree_accumulate(t, o.attr); /* disccofan.c:179-195 */
             if (rc) break;

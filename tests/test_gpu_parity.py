@@ -602,6 +602,30 @@ def test_errors(ctx, tmp_path):
         assert e.value.code == -4
 
 
+def test_dimension_errors(ctx):
+    """dimensions whose product wraps around 2^64 (2^22 x 2^22 x 2^20 gives 0) or passes 2^31 are
+    refused before any copy or launch, in the single and the batched form; NULL dimensions are refused
+    by the operator driver.  The arguments are raw C-ABI calls: the Python wrapper takes its
+    dimensions from numpy shapes and cannot express these."""
+    import ctypes as C
+    L = dcf.lib()
+    img = np.zeros(64, np.uint8)
+    out = np.zeros(64, np.uint8)
+    for dims in ((1 << 22, 1 << 22, 1 << 20), (1 << 30, 1 << 30, 1 << 30), (1 << 16, 1 << 15, 1),
+                 (0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF), (1 << 32, 1, 1)):
+        say("dims", dims)
+        d = (C.c_uint64 * 3)(*dims)
+        h = C.c_void_p()
+        rc = L.dcf_tree_build(ctx._h, img.ctypes.data, dcf.DCF_U8, d, 4, 0, 0, None, 0, C.byref(h))
+        assert rc == -1 and not h.value, (dims, rc)
+        b = (dcf.BatchImage * 1)()
+        b[0].img, b[0].dtype, b[0].lam, b[0].out, b[0].flags = img.ctypes.data, dcf.DCF_U8, 1.0, out.ctypes.data, 0
+        b[0].dims[:] = list(dims)
+        assert L.dcf_batch_filter(ctx._h, b, 1, 4, 0, 0, 0, 0, 0) == -1, dims
+    assert L.dcf_run_operations(ctx._h, img.ctypes.data, dcf.DCF_U8, None, 4, 0, None, b'("filter",0,5)', None,
+                                None) == -1
+
+
 def test_c_caller_on_gpu(tmp_path):
     """a plain C program (tests/c/c_caller.c) drives build + filter + spectrum through the C ABI;
     its outputs agree with the oracle"""
