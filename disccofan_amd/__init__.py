@@ -69,8 +69,7 @@ def _image_flags(tree_type) -> int:
     """per-image tree type of a batch item: None = the call's"""
     if tree_type is None:
         return 0
-    if tree_type not in ("max", "min"):
-        raise ValueError("tree_type must be 'max' or 'min'")
+    _tree_type_code(tree_type)
     return DCF_IMAGE_MIN_TREE if tree_type == "min" else DCF_IMAGE_MAX_TREE
 
 
@@ -158,6 +157,31 @@ def _dims_of(shape):
     if len(shape) == 3:
         return (shape[2], shape[1], shape[0])
     raise ValueError("2D (H,W) or 3D (D,H,W) image expected")
+
+
+def _pixdim3(pixdim) -> np.ndarray:
+    """data_properties.pixdim (x, y, z) as the three floats the library reads"""
+    pd = np.ascontiguousarray(pixdim, dtype=np.float32).reshape(-1)
+    if pd.size != 3:
+        raise ValueError("pixdim must hold 3 values (x, y, z), got %d" % pd.size)
+    return pd
+
+
+def _tree_type_code(tree_type) -> int:
+    if tree_type not in ("max", "min"):
+        raise ValueError("tree_type must be 'max' or 'min', got %r" % (tree_type,))
+    return 1 if tree_type == "min" else 0
+
+
+def _batch_args(count: int, lambdas, tree_types):
+    """one lambda and one optional tree type per image"""
+    lams = [float(x) for x in lambdas]
+    if len(lams) != count:
+        raise ValueError("%d images but %d lambdas" % (count, len(lams)))
+    tts = list(tree_types) if tree_types is not None else [None] * count
+    if len(tts) != count:
+        raise ValueError("%d images but %d tree types" % (count, len(tts)))
+    return lams, tts
 
 
 def lvec_read(path: str, imscale: float = 1.0) -> np.ndarray:
@@ -248,12 +272,11 @@ class Context:
                            attr_idx, pixdim, DCF_DEVICE_PTR, keep=None)
 
     def _build(self, p, code, dims, shape, connectivity, tree_type, attr_idx, pixdim, flags, keep):
-        if tree_type not in ("max", "min"):
-            raise ValueError("tree_type must be 'max' or 'min'")
+        tt = _tree_type_code(tree_type)
         d = (C.c_uint64 * 3)(*dims)
-        pd = np.asarray(pixdim, dtype=np.float32)
+        pd = _pixdim3(pixdim)
         h = C.c_void_p()
-        rc = lib().dcf_tree_build(self._h, p, code, d, int(connectivity), 1 if tree_type == "min" else 0,
+        rc = lib().dcf_tree_build(self._h, p, code, d, int(connectivity), tt,
                                   int(attr_idx), _ptr(pd), flags, C.byref(h))
         self._check(rc)
         t = Tree(self, h, shape, code, keep)
@@ -266,14 +289,14 @@ class Context:
         filtered with its own lambda, all in one build over the batch.  tree_types: an optional
         per-image 'max' / 'min' (None entries take tree_type).  Returns the outputs."""
         arrs = [np.ascontiguousarray(a) for a in images]
+        lams, tts = _batch_args(len(arrs), lambdas, tree_types)
         outs = [np.empty_like(a) for a in arrs]
         items = (BatchImage * len(arrs))()
-        tts = list(tree_types) if tree_types is not None else [None] * len(arrs)
-        for k, (a, o, lam) in enumerate(zip(arrs, outs, lambdas)):
+        for k, (a, o, lam) in enumerate(zip(arrs, outs, lams)):
             items[k].img = a.ctypes.data
             items[k].dtype = _dtype_code(a.dtype)
             items[k].dims = (C.c_uint64 * 3)(*_dims_of(a.shape))
-            items[k].lam = float(lam)
+            items[k].lam = lam
             items[k].out = o.ctypes.data
             items[k].flags = _image_flags(tts[k])
         self._batch(items, len(arrs), connectivity, tree_type, attr_idx, decision, extract, all_rules, 0)
@@ -294,10 +317,9 @@ class Context:
         self._batch(arr, len(items), connectivity, tree_type, attr_idx, decision, extract, all_rules, DCF_DEVICE_PTR)
 
     def _batch(self, arr, count, connectivity, tree_type, attr_idx, decision, extract, all_rules, flags):
-        if tree_type not in ("max", "min"):
-            raise ValueError("tree_type must be 'max' or 'min'")
+        tt = _tree_type_code(tree_type)
         fl = flags | (DCF_ALL_RULES if all_rules else 0)
-        self._check(lib().dcf_batch_filter(self._h, arr, int(count), int(connectivity), 1 if tree_type == "min" else 0,
+        self._check(lib().dcf_batch_filter(self._h, arr, int(count), int(connectivity), tt,
                                            int(attr_idx), int(decision), 1 if extract else 0, fl))
 
     def run_operations(self, img, operations, connectivity: int = 4, tree_type: str = "max", pixdim=(1.0, 1.0, 1.0)):
@@ -324,9 +346,9 @@ class Context:
                 outs[idx] = np.ctypeslib.as_array(C.cast(data, C.POINTER(C.c_double)), shape=(count,)).copy()
 
         d = (C.c_uint64 * 3)(*_dims_of(a.shape))
-        pd = np.asarray(pixdim, dtype=np.float32)
+        pd = _pixdim3(pixdim)
         rc = lib().dcf_run_operations(self._h, _ptr(a), _dtype_code(a.dtype), d, int(connectivity),
-                                      1 if tree_type == "min" else 0, _ptr(pd), operations.encode(),
+                                      _tree_type_code(tree_type), _ptr(pd), operations.encode(),
                                       C.cast(cb, C.c_void_p), None)
         self._check(rc)
         return [outs[i] for i in sorted(outs)]
@@ -398,7 +420,7 @@ class Tree:
     def dmp(self, attr_idx: int, lambdas, decision: int = 0, out_dtype=np.float32) -> np.ndarray:
         """perform_dmp_operation (operation.c:63-77): g - filter(lambda_i) for every lambda after
         lambda_0, stacked along a leading axis (min-trees stay in the inverted grey space)."""
-        lv = np.ascontiguousarray(lambdas, dtype=np.float32)
+        lv = np.ascontiguousarray(lambdas, dtype=np.float32).reshape(-1)
         out = np.empty((max(len(lv) - 1, 0),) + tuple(self.shape), dtype=out_dtype)
         self.ctx._check(lib().dcf_dmp(self._h, int(attr_idx), int(decision), _ptr(lv), len(lv), _ptr(out),
                                       _dtype_code(out_dtype), 0))
@@ -414,14 +436,14 @@ class Tree:
         return out
 
     def spectrum1d(self, attr_idx: int, lambdas) -> np.ndarray:
-        lv = np.ascontiguousarray(lambdas, dtype=np.float32)
+        lv = np.ascontiguousarray(lambdas, dtype=np.float32).reshape(-1)
         out = np.empty(len(lv), dtype=np.float64)
         self.ctx._check(lib().dcf_spectrum1d(self._h, int(attr_idx), _ptr(lv), len(lv), _ptr(out)))
         return out
 
     def spectrum2d(self, attr1: int, l1, attr2: int, l2) -> np.ndarray:
-        a = np.ascontiguousarray(l1, dtype=np.float32)
-        b = np.ascontiguousarray(l2, dtype=np.float32)
+        a = np.ascontiguousarray(l1, dtype=np.float32).reshape(-1)
+        b = np.ascontiguousarray(l2, dtype=np.float32).reshape(-1)
         out = np.empty(len(a) * len(b), dtype=np.float64)
         self.ctx._check(lib().dcf_spectrum2d(self._h, int(attr1), _ptr(a), len(a), int(attr2), _ptr(b), len(b),
                                              _ptr(out)))

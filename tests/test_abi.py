@@ -78,6 +78,31 @@ def test_context_without_gpu_fails_loudly():
         d.Context(0)
 
 
+def test_wrapper_argument_checks():
+    """the Python wrapper refuses, before any library call, what the C side would read out of bounds
+    (a pixdim of other than 3 values) or take for a null image (a batch with fewer lambdas or tree
+    types than images), and tree types other than max / min"""
+    import numpy as np
+    import disccofan_amd as d
+    assert d._pixdim3((1, 2, 3)).dtype == np.float32 and d._pixdim3([[1, 2, 3]]).tolist() == [1, 2, 3]
+    for bad in ((1.0, 1.0), (1, 2, 3, 4), ()):
+        with pytest.raises(ValueError):
+            d._pixdim3(bad)
+    assert d._tree_type_code("max") == 0 and d._tree_type_code("min") == 1
+    for bad in ("Max", "minimum", None, 1):
+        with pytest.raises(ValueError):
+            d._tree_type_code(bad)
+    assert d._image_flags(None) == 0 and d._image_flags("min") == d.DCF_IMAGE_MIN_TREE
+    with pytest.raises(ValueError):
+        d._image_flags("both")
+    assert d._batch_args(2, [3, 4.5], None) == ([3.0, 4.5], [None, None])
+    assert d._batch_args(2, (1, 2), ["min", None]) == ([1.0, 2.0], ["min", None])
+    with pytest.raises(ValueError):
+        d._batch_args(3, [1, 2], None)
+    with pytest.raises(ValueError):
+        d._batch_args(2, [1, 2], ["max"])
+
+
 def _build_c_caller(tmp_path):
     import subprocess
     exe = str(tmp_path / "c_caller")
