@@ -88,7 +88,9 @@ const char *dcf_kernel_name(dcf_ctx *ctx, int i);
  * connectivity: 4 or 8 (2D), 6 or 26 (3D); mapped like check_connectivity (io.c:64-79).
  * tree_type: DCF_MAX_TREE or DCF_MIN_TREE (min-tree = max-tree of gmax - g).
  * attr_idx: the first operation's attribute; its group is accumulated (disccofan.c:145-148).
- * pixdim: physical pixel size (NULL = 1,1,1), as data_properties.pixdim. */
+ * pixdim: physical pixel size (NULL = 1,1,1), as data_properties.pixdim.
+ * Size: at most 2^30 - 1 pixels (DCF_EUNSUPPORTED from 2^30); DCF_EINVAL when a dimension is 0 or
+ * the pixel count reaches 2^31 - 1, a count taken without 64-bit wrap-around of the product. */
 int dcf_tree_build(dcf_ctx *ctx, const void *img, dcf_dtype dtype, const uint64_t dims[3], int connectivity,
                    int tree_type, int attr_idx, const float pixdim[3], unsigned flags, dcf_tree **out);
 int dcf_tree_free(dcf_tree *tree);
