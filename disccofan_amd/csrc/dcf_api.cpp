@@ -15,6 +15,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <exception>
 #include <string>
 #include <vector>
 
@@ -61,7 +62,10 @@ int dcf_set_error(dcf_ctx *ctx, int code, const char *fmt, ...)
         va_start(ap, fmt);
         vsnprintf(buf, sizeof buf, fmt, ap);
         va_end(ap);
-        ctx->err = buf;
+        try {
+            ctx->err = buf;
+        } catch (const std::exception &) { /* no memory for the message: the code still goes back */
+        }
     }
     return code;
 }
@@ -156,7 +160,7 @@ void dcf_release(dcf_ctx *ctx, void *p)
 /* context                                                                                          */
 /* ------------------------------------------------------------------------------------------------ */
 extern "C" int dcf_ctx_create(int device, unsigned flags, dcf_ctx **out)
-{
+try {
     (void)flags;
     if (!out) return DCF_EINVAL;
     *out = nullptr;
@@ -200,6 +204,8 @@ extern "C" int dcf_ctx_create(int device, unsigned flags, dcf_ctx **out)
     shape3("DCF_LEVEL_GROUP3D", c->lg3);
     *out = c;
     return DCF_OK;
+} catch (const std::exception &) {
+    return DCF_ENOMEM;
 }
 
 static void tree_release(dcf_tree *t);
@@ -329,7 +335,7 @@ extern "C" int dcf_tree_free(dcf_tree *t)
 
 extern "C" int dcf_tree_build(dcf_ctx *ctx, const void *img, dcf_dtype dtype, const uint64_t dims[3], int connectivity,
                               int tree_type, int attr_idx, const float pixdim[3], unsigned flags, dcf_tree **out)
-{
+try {
     if (!ctx) return DCF_EINVAL;
     if (!out || !img || !dims) return dcf_set_error(ctx, DCF_EINVAL, "null argument");
     *out = nullptr;
@@ -390,11 +396,20 @@ fail:
     tree_release(t);
     delete t;
     return rc;
+} catch (const std::exception &e) {
+    return dcf_set_error(ctx, DCF_ENOMEM, "out of host memory (%s)", e.what());
 }
 
 /* ------------------------------------------------------------------------------------------------ */
 /* batched form                                                                                     */
 /* ------------------------------------------------------------------------------------------------ */
+/* waits for the stream when it goes out of scope: host tables read by asynchronous copies are
+ * declared before it, so they are freed only after those copies */
+struct StreamWait {
+    hipStream_t s;
+    ~StreamWait() { (void)hipStreamSynchronize(s); }
+};
+
 /* one batch tree over images that are all u8 or all u16 (keys of that width: one digit pass for u8,
  * two for u16) */
 static int batch_filter_same(dcf_ctx *ctx, const dcf_batch_image *images, int count, int connectivity, int tree_type,
@@ -467,6 +482,8 @@ static int batch_filter_same(dcf_ctx *ctx, const dcf_batch_image *images, int co
     uint8_t *staging = nullptr, *obuf = nullptr;
     std::vector<BatchOut> outs((size_t)count);
     std::vector<size_t> boff((size_t)count);
+    /* destroyed before the tables above, also when an exception unwinds: they outlive their copies */
+    StreamWait wait_for_copies{ ctx->stream };
     int rc = DCF_OK;
     t->img = dcf_alloc(ctx, (size_t)N * csz);
     t->own_img = 1;
@@ -560,7 +577,7 @@ done:
 
 extern "C" int dcf_batch_filter(dcf_ctx *ctx, const dcf_batch_image *images, int count, int connectivity, int tree_type,
                                 int attr_idx, int decision, int extract, unsigned flags)
-{
+try {
     if (!ctx) return DCF_EINVAL;
     if (!images || count <= 0) return dcf_set_error(ctx, DCF_EINVAL, "empty batch");
     /* every image is checked here, so that errors name the caller's index */
@@ -591,10 +608,12 @@ extern "C" int dcf_batch_filter(dcf_ctx *ctx, const dcf_batch_image *images, int
     if (rc == DCF_OK && !g16.empty())
         rc = batch_filter_same(ctx, g16.data(), (int)g16.size(), connectivity, tree_type, attr_idx, decision, extract, flags);
     return rc;
+} catch (const std::exception &e) {
+    return dcf_set_error(ctx, DCF_ENOMEM, "out of host memory (%s)", e.what());
 }
 
 extern "C" int dcf_tree_accumulate(dcf_tree *t, int attr_idx)
-{
+try {
     if (!t) return DCF_EINVAL;
     int grp = dcf_group_of(attr_idx);
     if (grp < 0) return dcf_set_error(t->ctx, DCF_EINVAL, "bad attribute index %d", attr_idx);
@@ -603,6 +622,8 @@ extern "C" int dcf_tree_accumulate(dcf_tree *t, int attr_idx)
         return dcf_set_error(t->ctx, DCF_EUNSUPPORTED, "grey-weighted attributes on float images are not implemented");
     (void)hipSetDevice(t->ctx->device);
     return dcf_accumulate_group(t, grp);
+} catch (const std::exception &e) {
+    return dcf_set_error(t ? t->ctx : nullptr, DCF_ENOMEM, "out of host memory (%s)", e.what());
 }
 
 extern "C" int dcf_tree_group(const dcf_tree *t) { return t ? t->grp : -1; }
@@ -636,7 +657,7 @@ static int need_group(dcf_tree *t, int attr_idx)
 
 extern "C" int dcf_filter(dcf_tree *t, int attr_idx, int decision, double lambda, int extract, void *out, dcf_dtype out_dtype,
                           unsigned flags)
-{
+try {
     if (!t || !out) return DCF_EINVAL;
     dcf_ctx *ctx = t->ctx;
     int rc = need_group(t, attr_idx);
@@ -658,11 +679,13 @@ extern "C" int dcf_filter(dcf_tree *t, int attr_idx, int decision, double lambda
     }
     if (!(flags & DCF_DEVICE_PTR)) dcf_release(ctx, dst);
     return rc;
+} catch (const std::exception &e) {
+    return dcf_set_error(t ? t->ctx : nullptr, DCF_ENOMEM, "out of host memory (%s)", e.what());
 }
 
 extern "C" int dcf_dmp(dcf_tree *t, int attr_idx, int decision, const float *lambdas, int n, void *out, dcf_dtype out_dtype,
                        unsigned flags)
-{
+try {
     if (!t || !out || !lambdas) return DCF_EINVAL;
     dcf_ctx *ctx = t->ctx;
     if (n < 2) return dcf_set_error(ctx, DCF_EINVAL, "a differential profile needs at least 2 lambdas (lambda_0 = 0 first)");
@@ -687,6 +710,8 @@ extern "C" int dcf_dmp(dcf_tree *t, int attr_idx, int decision, const float *lam
     }
     if (!dev) dcf_release(ctx, dst);
     return rc;
+} catch (const std::exception &e) {
+    return dcf_set_error(t ? t->ctx : nullptr, DCF_ENOMEM, "out of host memory (%s)", e.what());
 }
 
 /* write_tree (io.c:1431-1573): one row per levelroot in raster order, the reference's printf formats */
@@ -746,7 +771,7 @@ static int tree_csv_text(dcf_tree *t, int attr_idx, std::string &text)
 }
 
 extern "C" int dcf_tree_write_csv(dcf_tree *t, int attr_idx, const char *path)
-{
+try {
     if (!t || !path) return DCF_EINVAL;
     std::string text;
     int rc = tree_csv_text(t, attr_idx, text);
@@ -757,10 +782,12 @@ extern "C" int dcf_tree_write_csv(dcf_tree *t, int attr_idx, const char *path)
     const int cl = fclose(f);
     if (w != text.size() || cl != 0) return dcf_set_error(t->ctx, DCF_EIO, "cannot write %s", path);
     return DCF_OK;
+} catch (const std::exception &e) {
+    return dcf_set_error(t ? t->ctx : nullptr, DCF_ENOMEM, "out of host memory (%s)", e.what());
 }
 
 extern "C" int dcf_check(dcf_tree *t, int attr_idx, float *out, unsigned flags)
-{
+try {
     if (!t || !out) return DCF_EINVAL;
     dcf_ctx *ctx = t->ctx;
     int rc = need_group(t, attr_idx);
@@ -780,21 +807,29 @@ extern "C" int dcf_check(dcf_tree *t, int attr_idx, float *out, unsigned flags)
     }
     if (!(flags & DCF_DEVICE_PTR)) dcf_release(ctx, dst);
     return rc;
+} catch (const std::exception &e) {
+    return dcf_set_error(t ? t->ctx : nullptr, DCF_ENOMEM, "out of host memory (%s)", e.what());
 }
 
 extern "C" int dcf_spectrum1d(dcf_tree *t, int attr_idx, const float *lambdas, int n, double *out)
-{
+try {
     if (!t || !lambdas || !out || n <= 0) return t ? dcf_set_error(t->ctx, DCF_EINVAL, "bad spectrum arguments") : DCF_EINVAL;
+    if (n > DCF_MAX_SPECTRUM_BINS) return dcf_set_error(t->ctx, DCF_EINVAL, "%d bins exceed %d", n, DCF_MAX_SPECTRUM_BINS);
     int rc = need_group(t, attr_idx);
     if (rc) return rc;
     (void)hipSetDevice(t->ctx->device);
     return dcf_ops_spectrum(t, attr_idx, lambdas, n, 0, nullptr, 1, 0, out);
+} catch (const std::exception &e) {
+    return dcf_set_error(t ? t->ctx : nullptr, DCF_ENOMEM, "out of host memory (%s)", e.what());
 }
 
 extern "C" int dcf_spectrum2d(dcf_tree *t, int attr1, const float *l1, int n1, int attr2, const float *l2, int n2, double *out)
-{
+try {
     if (!t || !l1 || !l2 || !out || n1 <= 0 || n2 <= 0) return t ? dcf_set_error(t->ctx, DCF_EINVAL, "bad spectrum arguments") : DCF_EINVAL;
     dcf_ctx *ctx = t->ctx;
+    /* the bins are indexed s1 + s2*n1 in 32-bit arithmetic on both sides: bound the product first */
+    if ((int64_t)n1 * n2 > DCF_MAX_SPECTRUM_BINS)
+        return dcf_set_error(ctx, DCF_EINVAL, "%d x %d bins exceed %d", n1, n2, DCF_MAX_SPECTRUM_BINS);
     if (attr1 < 0 || attr1 >= DCF_NUM_ATTR || attr2 < 0 || attr2 >= DCF_NUM_ATTR)
         return dcf_set_error(ctx, DCF_EINVAL, "bad attribute index");
     /* operation.c:93-95: the area attribute must be the second axis */
@@ -806,6 +841,8 @@ extern "C" int dcf_spectrum2d(dcf_tree *t, int attr1, const float *l1, int n1, i
     if (!second_is_area && (rc = need_group(t, attr2)) != 0) return rc;
     (void)hipSetDevice(ctx->device);
     return dcf_ops_spectrum(t, attr1, l1, n1, attr2, l2, n2, second_is_area, out);
+} catch (const std::exception &e) {
+    return dcf_set_error(t ? t->ctx : nullptr, DCF_ENOMEM, "out of host memory (%s)", e.what());
 }
 
 /* ------------------------------------------------------------------------------------------------ */
@@ -833,10 +870,17 @@ extern "C" int dcf_lvec_read(const char *path, double imscale, float **lambdas, 
         p++;
     }
     if (c != EOF) ungetc(c, f);
-    std::vector<float> v(1, 0.0f);
-    float x;
-    while (fscanf(f, "%f", &x) == 1) v.push_back(x);
+    std::vector<float> v;
+    try {
+        v.push_back(0.0f);
+        float x;
+        while (fscanf(f, "%f", &x) == 1) v.push_back(x);
+    } catch (const std::exception &) {
+        fclose(f);
+        return DCF_ENOMEM;
+    }
     fclose(f);
+    if (v.size() > (size_t)DCF_MAX_SPECTRUM_BINS) return DCF_EINVAL; /* more thresholds than any spectrum takes */
     for (size_t i = 1; i < v.size(); i++) v[i] /= imscale;
     float *out = (float *)malloc(v.size() * sizeof(float));
     if (!out) return DCF_ENOMEM;
@@ -956,7 +1000,7 @@ static int to_ops(dcf_ctx *ctx, const std::vector<OpTuple> &tp, std::vector<Op> 
 
 extern "C" int dcf_run_operations(dcf_ctx *ctx, const void *img, dcf_dtype dtype, const uint64_t dims[3], int connectivity,
                                   int tree_type, const float pixdim[3], const char *opstr, dcf_output_fn cb, void *user)
-{
+try {
     if (!ctx) return DCF_EINVAL;
     if (!opstr || !img || !dims) return dcf_set_error(ctx, DCF_EINVAL, "null argument");
     std::vector<OpTuple> tp;
@@ -1017,6 +1061,12 @@ extern "C" int dcf_run_operations(dcf_ctx *ctx, const void *img, dcf_dtype dtype
             if (rc) { rc = dcf_set_error(ctx, rc, "couldn't read the lvec file: %s", o.lvec.c_str()); break; }
             rc = dcf_lvec_read(o.lvec2.c_str(), o.imscale2, &l2, &n2);
             if (rc) { dcf_lvec_free(l1); rc = dcf_set_error(ctx, rc, "couldn't read the lvec file: %s", o.lvec2.c_str()); break; }
+            if ((int64_t)n1 * n2 > DCF_MAX_SPECTRUM_BINS) {
+                dcf_lvec_free(l1);
+                dcf_lvec_free(l2);
+                rc = dcf_set_error(ctx, DCF_EINVAL, "%d x %d bins exceed %d", n1, n2, DCF_MAX_SPECTRUM_BINS);
+                break;
+            }
             std::vector<double> sp((size_t)n1 * n2);
             rc = dcf_spectrum2d(t, o.attr, l1, n1, o.attr2, l2, n2, sp.data());
             dcf_lvec_free(l1);
@@ -1026,4 +1076,6 @@ extern "C" int dcf_run_operations(dcf_ctx *ctx, const void *img, dcf_dtype dtype
     }
     if (t) dcf_tree_free(t);
     return rc;
+} catch (const std::exception &e) {
+    return dcf_set_error(ctx, DCF_ENOMEM, "out of host memory (%s)", e.what());
 }

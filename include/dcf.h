@@ -130,7 +130,9 @@ int dcf_check(dcf_tree *tree, int attr_idx, float *out, unsigned flags);
 /* ---- spectra (pattern.c:27, 110), background = 1 (operation.c:82) ------------------------------ *
  * lambdas: thresholds with lambda_0 = 0 included, as returned by dcf_lvec_read.
  * 2D: bin index s1 + s2*n1; attr2 == 0 with attr1 != 0 means "area of attr1's group"
- * (operation.c:97-98); attr1 == 0 with attr2 != 0 is refused (operation.c:93-95). */
+ * (operation.c:97-98); attr1 == 0 with attr2 != 0 is refused (operation.c:93-95).
+ * A spectrum has at most DCF_MAX_SPECTRUM_BINS bins (n, or n1*n2); more is DCF_EINVAL. */
+#define DCF_MAX_SPECTRUM_BINS (1 << 28)
 int dcf_spectrum1d(dcf_tree *tree, int attr_idx, const float *lambdas, int n, double *out);
 int dcf_spectrum2d(dcf_tree *tree, int attr1, const float *l1, int n1, int attr2, const float *l2, int n2,
                    double *out);

@@ -624,6 +624,13 @@ def test_dimension_errors(ctx):
         assert L.dcf_batch_filter(ctx._h, b, 1, 4, 0, 0, 0, 0, 0) == -1, dims
     assert L.dcf_run_operations(ctx._h, img.ctypes.data, dcf.DCF_U8, None, 4, 0, None, b'("filter",0,5)', None,
                                 None) == -1
+    # spectra of more than DCF_MAX_SPECTRUM_BINS (2^28) bins are refused before anything is read
+    say("spectrum bin limits")
+    t = ctx.build(np.zeros((4, 4), np.uint8), 4, "max", 0)
+    l1, l2, sp = np.zeros(1 << 15, np.float32), np.zeros(1 << 14, np.float32), np.zeros(4, np.float64)
+    assert L.dcf_spectrum2d(t._h, 0, l1.ctypes.data, 1 << 15, 0, l2.ctypes.data, 1 << 14, sp.ctypes.data) == -1
+    assert L.dcf_spectrum1d(t._h, 0, l1.ctypes.data, (1 << 28) + 1, sp.ctypes.data) == -1
+    assert np.array_equal(t.spectrum1d(0, [0, 1, 4]), t.spectrum1d(0, [0, 1, 4]))
 
 
 def test_c_caller_on_gpu(tmp_path):
