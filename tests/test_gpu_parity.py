@@ -603,7 +603,7 @@ def test_errors(ctx, tmp_path):
 
 
 def test_dimension_errors(ctx):
-    """dimensions whose product wraps around 2^64 (2^22 x 2^22 x 2^20 gives 0) or passes 2^31 are
+    """dimensions whose product overflows 64 bits (2^22 x 2^22 x 2^20 wraps to 0) or passes 2^31 are
     refused before any copy or launch, in the single and the batched form; NULL dimensions are refused
     by the operator driver.  The arguments are raw C-ABI calls: the Python wrapper takes its
     dimensions from numpy shapes and cannot express these."""
