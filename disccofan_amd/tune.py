@@ -148,12 +148,17 @@ def tune(images, lambdas, connectivity=4, tree_type="max", attr_idx=0, decision=
     return best, best_s, table
 
 
+# side of the sample when --size is not given: a 2048^2 image, a 256^3 volume, batch sides up to 2048
+DEFAULT_SIZE = {"vnoise": 2048, "spheres": 256, "batch": 2048}
+
+
 def _workload(args):
+    size = args.size or DEFAULT_SIZE[args.workload]
     if args.workload == "vnoise":
-        return [synth.vnoise(args.size, args.size)], [1000.0], 8, False
+        return [synth.vnoise(size, size)], [1000.0], 8, False
     if args.workload == "spheres":
-        return [synth.spheres(args.size, args.size, args.size)], [500.0], 6, False
-    shapes = synth.batch_shapes(args.count, 64, args.size)
+        return [synth.spheres(size, size, size)], [500.0], 6, False
+    shapes = synth.batch_shapes(args.count, 64, size)
     imgs = [synth.disks(H, W, is16, seed) for (W, H, is16, seed, lam) in shapes]
     lams = [lam for (W, H, is16, seed, lam) in shapes]
     return imgs, lams, 4, True
@@ -162,7 +167,8 @@ def _workload(args):
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--workload", choices=["vnoise", "spheres", "batch"], default="vnoise")
-    ap.add_argument("--size", type=int, default=2048, help="side of the image (vnoise, spheres) or the largest batch side")
+    ap.add_argument("--size", type=int, default=0,
+                    help="side of the image (vnoise: 2048), of the volume (spheres: 256) or the largest batch side (2048)")
     ap.add_argument("--count", type=int, default=64, help="images in the batch")
     ap.add_argument("--repeats", type=int, default=3)
     ap.add_argument("--sweeps", type=int, default=1)

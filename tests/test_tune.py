@@ -70,3 +70,17 @@ def test_a_setting_that_changes_the_output_is_refused(monkeypatch):
     _fake_runner(monkeypatch, lambda st: 1.0, lambda st: [np.full(3, int(st["A"]))])
     with pytest.raises(RuntimeError, match="changed the output"):
         tune.tune([np.zeros((4, 4), np.uint8)], [1.0], knobs=knobs)
+
+
+def test_cli_sample_sizes(monkeypatch):
+    """--size defaults per workload: a 2048^2 image, a 256^3 volume (2048^3 would pass the 2^30-voxel limit), batch
+    sides up to 2048; an explicit --size is taken as given"""
+    import argparse
+    calls = []
+    monkeypatch.setattr(tune.synth, "vnoise", lambda H, W: calls.append(("vnoise", H, W)) or np.zeros((1, 1), np.uint16))
+    monkeypatch.setattr(tune.synth, "spheres", lambda D, H, W: calls.append(("spheres", D, H, W)) or np.zeros((1, 1, 1), np.uint8))
+    monkeypatch.setattr(tune.synth, "batch_shapes", lambda n, lo, hi: calls.append(("batch", n, lo, hi)) or [])
+    for wl, size in (("vnoise", 0), ("spheres", 0), ("batch", 0), ("spheres", 64)):
+        tune._workload(argparse.Namespace(workload=wl, size=size, count=8))
+    assert calls == [("vnoise", 2048, 2048), ("spheres", 256, 256, 256), ("batch", 8, 64, 2048), ("spheres", 64, 64, 64)]
+    assert 256 ** 3 < (1 << 30)
