@@ -20,3 +20,11 @@ def test_workloads_are_listed():
     assert r.returncode == 0
     for w in ("vnoise8192", "spheres512", "batch256"):
         assert w in r.stdout
+
+
+def test_step_counts_are_checked():
+    for argv in (["--steps", "0"], ["--warmup", "-1"]):
+        r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py")] + argv, capture_output=True, text=True,
+                           timeout=120)
+        assert r.returncode == 2, argv
+        assert "--steps must be at least 1" in r.stderr
