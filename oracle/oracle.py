@@ -22,8 +22,9 @@ _lib = None
 
 
 def build() -> str:
-    """Compile the oracle with its Makefile (gcc, no reference sources involved)."""
-    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    """Compile the oracle with its Makefile (gcc, no reference sources involved), always from the source:
+    an existing library, however recent, is rebuilt."""
+    subprocess.run(["make", "-s", "-B", "-C", _HERE], check=True)
     return _LIB_PATH
 
 
