@@ -19,6 +19,8 @@
 #include <string>
 #include <vector>
 
+#include <memory>
+
 #include "dcf_internal.h"
 
 /* ------------------------------------------------------------------------------------------------ */
@@ -323,6 +325,34 @@ static void tree_release(dcf_tree *t)
     dcf_free_attr(ctx, t->attr_pix);
 }
 
+/* a tree under construction: released (with its device arrays) unless handed to the caller, also
+ * when an exception unwinds */
+struct TreeDel {
+    void operator()(dcf_tree *t) const
+    {
+        if (!t) return;
+        t->bimg = nullptr; /* a batch's image table belongs to the batch call */
+        tree_release(t);
+        delete t;
+    }
+};
+using TreePtr = std::unique_ptr<dcf_tree, TreeDel>;
+
+/* device blocks of one call, given back to the context's cache when the call returns */
+struct DevBlocks {
+    dcf_ctx *ctx;
+    std::vector<void *> p;
+    template <class T> T *add(T *q)
+    {
+        if (q) p.push_back((void *)q);
+        return q;
+    }
+    ~DevBlocks()
+    {
+        for (void *q : p) dcf_release(ctx, q);
+    }
+};
+
 extern "C" int dcf_tree_free(dcf_tree *t)
 {
     if (!t) return DCF_EINVAL;
@@ -359,7 +389,7 @@ try {
         return dcf_set_error(ctx, DCF_EUNSUPPORTED, "grey-weighted attributes on float images are not implemented");
     if (hipSetDevice(ctx->device) != hipSuccess) return dcf_set_error(ctx, DCF_EDEVICE, "hipSetDevice");
 
-    dcf_tree *t = new dcf_tree();
+    TreePtr t(new dcf_tree());
     t->ctx = ctx;
     memcpy(t->dims, dims, sizeof t->dims);
     t->n = (int64_t)n;
@@ -375,27 +405,21 @@ try {
     } else {
         t->img = dcf_alloc(ctx, (size_t)n * isz);
         t->own_img = 1;
-        if (!t->img) { rc = dcf_set_error(ctx, DCF_ENOMEM, "image copy"); goto fail; }
-        if (hipMemcpyAsync(t->img, img, (size_t)n * isz, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) {
-            rc = dcf_set_error(ctx, DCF_EDEVICE, "image upload");
-            goto fail;
-        }
+        if (!t->img) return dcf_set_error(ctx, DCF_ENOMEM, "image copy");
+        if (hipMemcpyAsync(t->img, img, (size_t)n * isz, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+            return dcf_set_error(ctx, DCF_EDEVICE, "image upload");
     }
     t->gmax = (float *)dcf_alloc(ctx, 16);
     t->counters = (uint32_t *)dcf_alloc(ctx, 64 * 4);
     t->parent = (int32_t *)dcf_alloc(ctx, (size_t)n * 4);
     t->p2n = (uint32_t *)dcf_alloc(ctx, (size_t)n * 4);
-    if (!t->gmax || !t->counters || !t->parent || !t->p2n) { rc = dcf_set_error(ctx, DCF_ENOMEM, "tree arrays"); goto fail; }
+    if (!t->gmax || !t->counters || !t->parent || !t->p2n) return dcf_set_error(ctx, DCF_ENOMEM, "tree arrays");
     /* find_extremas (preprocessing.c:7-30): only a min-tree reads gmax */
-    if (t->tree_min && (rc = dcf_ops_minmax(t)) != DCF_OK) goto fail;
-    if ((rc = dcf_build_tree(t, dcf_group_of(attr_idx))) != DCF_OK) goto fail;
-    ctx->trees.insert(t);
-    *out = t;
+    if (t->tree_min && (rc = dcf_ops_minmax(t.get())) != DCF_OK) return rc;
+    if ((rc = dcf_build_tree(t.get(), dcf_group_of(attr_idx))) != DCF_OK) return rc;
+    ctx->trees.insert(t.get());
+    *out = t.release();
     return DCF_OK;
-fail:
-    tree_release(t);
-    delete t;
-    return rc;
 } catch (const std::exception &e) {
     return dcf_set_error(ctx, DCF_ENOMEM, "out of host memory (%s)", e.what());
 }
@@ -466,7 +490,14 @@ static int batch_filter_same(dcf_ctx *ctx, const dcf_batch_image *images, int co
     const dcf_dtype cdt = all_u8 ? DCF_U8 : DCF_U16;
     const size_t csz = all_u8 ? 1 : 2;
     const bool dev = (flags & DCF_DEVICE_PTR) != 0;
-    dcf_tree *t = new dcf_tree();
+    std::vector<BatchOut> outs((size_t)count);
+    std::vector<size_t> boff((size_t)count);
+    /* device blocks and the tree are released when the call returns, also when an exception unwinds;
+     * wait_for_copies is destroyed first, so nothing is released (and no host table above freed) while
+     * an asynchronous copy or kernel of this call may still read it */
+    DevBlocks blocks{ ctx, {} };
+    TreePtr t(new dcf_tree());
+    StreamWait wait_for_copies{ ctx->stream };
     t->ctx = ctx;
     t->dims[0] = N;
     t->n = (int64_t)N;
@@ -475,39 +506,28 @@ static int batch_filter_same(dcf_ctx *ctx, const dcf_batch_image *images, int co
     t->nbatch = count;
     t->batch3d = is3d;
     t->batch_tiles = (int64_t)tiles;
-    BatchImg *dtab = nullptr;
-    double *dlam = nullptr;
-    BatchOut *douts = nullptr;
-    uint32_t *gm = nullptr;
-    uint8_t *staging = nullptr, *obuf = nullptr;
-    std::vector<BatchOut> outs((size_t)count);
-    std::vector<size_t> boff((size_t)count);
-    /* destroyed before the tables above, also when an exception unwinds: they outlive their copies */
-    StreamWait wait_for_copies{ ctx->stream };
     int rc = DCF_OK;
     t->img = dcf_alloc(ctx, (size_t)N * csz);
     t->own_img = 1;
-    dtab = (BatchImg *)dcf_alloc(ctx, (size_t)count * sizeof(BatchImg));
-    dlam = (double *)dcf_alloc(ctx, (size_t)count * 8);
-    douts = (BatchOut *)dcf_alloc(ctx, (size_t)count * sizeof(BatchOut));
+    BatchImg *dtab = blocks.add((BatchImg *)dcf_alloc(ctx, (size_t)count * sizeof(BatchImg)));
+    double *dlam = blocks.add((double *)dcf_alloc(ctx, (size_t)count * 8));
+    BatchOut *douts = blocks.add((BatchOut *)dcf_alloc(ctx, (size_t)count * sizeof(BatchOut)));
+    uint32_t *gm = nullptr;
+    uint8_t *staging = nullptr, *obuf = nullptr;
     t->gmax = (float *)dcf_alloc(ctx, 16);
     t->counters = (uint32_t *)dcf_alloc(ctx, 64 * 4);
     t->parent = (int32_t *)dcf_alloc(ctx, (size_t)N * 4);
     t->p2n = (uint32_t *)dcf_alloc(ctx, (size_t)N * 4);
-    if (!t->img || !dtab || !dlam || !douts || !t->gmax || !t->counters || !t->parent || !t->p2n) {
-        rc = dcf_set_error(ctx, DCF_ENOMEM, "batch arrays");
-        goto done;
-    }
+    if (!t->img || !dtab || !dlam || !douts || !t->gmax || !t->counters || !t->parent || !t->p2n)
+        return dcf_set_error(ctx, DCF_ENOMEM, "batch arrays");
     t->bimg = dtab;
     if (hipMemcpyAsync(dtab, tab.data(), tab.size() * sizeof(BatchImg), hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
-        hipMemcpyAsync(dlam, lam.data(), lam.size() * 8, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) {
-        rc = dcf_set_error(ctx, DCF_EDEVICE, "batch table upload");
-        goto done;
-    }
+        hipMemcpyAsync(dlam, lam.data(), lam.size() * 8, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+        return dcf_set_error(ctx, DCF_EDEVICE, "batch table upload");
     /* the images, back to back in the batch's key type */
     if (!dev && !all_u8 && max_u8) {
-        staging = (uint8_t *)dcf_alloc(ctx, max_u8);
-        if (!staging) { rc = dcf_set_error(ctx, DCF_ENOMEM, "batch staging"); goto done; }
+        staging = blocks.add((uint8_t *)dcf_alloc(ctx, max_u8));
+        if (!staging) return dcf_set_error(ctx, DCF_ENOMEM, "batch staging");
     }
     for (int i = 0; i < count && rc == DCF_OK; i++) {
         const dcf_batch_image &b = images[i];
@@ -525,53 +545,38 @@ static int batch_filter_same(dcf_ctx *ctx, const dcf_batch_image *images, int co
                 rc = dcf_batch_copy_in(ctx, staging, DCF_U8, dst, cdt, n);
         }
     }
-    if (rc != DCF_OK) goto done;
+    if (rc != DCF_OK) return rc;
     /* min-trees: every min-tree image inverted against its own grey maximum (preprocessing.c:7-30,84-91) */
     if (any_min) {
-        gm = (uint32_t *)dcf_alloc(ctx, (size_t)count * 4);
-        if (!gm) { rc = dcf_set_error(ctx, DCF_ENOMEM, "batch grey maxima"); goto done; }
-        if ((rc = dcf_batch_invert(ctx, t->img, cdt, dtab, count, gm, (int64_t)N)) != DCF_OK) goto done;
+        gm = blocks.add((uint32_t *)dcf_alloc(ctx, (size_t)count * 4));
+        if (!gm) return dcf_set_error(ctx, DCF_ENOMEM, "batch grey maxima");
+        if ((rc = dcf_batch_invert(ctx, t->img, cdt, dtab, count, gm, (int64_t)N)) != DCF_OK) return rc;
     }
-    if ((rc = dcf_build_tree(t, dcf_group_of(attr_idx))) != DCF_OK) goto done;
+    if ((rc = dcf_build_tree(t.get(), dcf_group_of(attr_idx))) != DCF_OK) return rc;
     /* outputs: straight into device buffers, or into one staging buffer copied back per image */
-    {
-        size_t ob = 0;
-        for (int i = 0; i < count; i++) {
-            boff[i] = ob;
-            ob += (size_t)(images[i].dims[0] * images[i].dims[1] * images[i].dims[2]) * (images[i].dtype == DCF_U8 ? 1 : 2);
-        }
-        if (!dev) {
-            obuf = (uint8_t *)dcf_alloc(ctx, ob);
-            if (!obuf) { rc = dcf_set_error(ctx, DCF_ENOMEM, "batch output staging"); goto done; }
-        }
-        for (int i = 0; i < count; i++) {
-            outs[i].ptr = dev ? images[i].out : (void *)(obuf + boff[i]);
-            outs[i].dtype = images[i].dtype;
-        }
-        if (hipMemcpyAsync(douts, outs.data(), outs.size() * sizeof(BatchOut), hipMemcpyHostToDevice, ctx->stream) != hipSuccess) {
-            rc = dcf_set_error(ctx, DCF_EDEVICE, "batch output table upload");
-            goto done;
-        }
+    size_t ob = 0;
+    for (int i = 0; i < count; i++) {
+        boff[i] = ob;
+        ob += (size_t)(images[i].dims[0] * images[i].dims[1] * images[i].dims[2]) * (images[i].dtype == DCF_U8 ? 1 : 2);
     }
-    if ((rc = dcf_ops_batch_filter(t, attr_idx, decision, dlam, extract, douts, gm, flags)) != DCF_OK) goto done;
+    if (!dev) {
+        obuf = blocks.add((uint8_t *)dcf_alloc(ctx, ob));
+        if (!obuf) return dcf_set_error(ctx, DCF_ENOMEM, "batch output staging");
+    }
+    for (int i = 0; i < count; i++) {
+        outs[i].ptr = dev ? images[i].out : (void *)(obuf + boff[i]);
+        outs[i].dtype = images[i].dtype;
+    }
+    if (hipMemcpyAsync(douts, outs.data(), outs.size() * sizeof(BatchOut), hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+        return dcf_set_error(ctx, DCF_EDEVICE, "batch output table upload");
+    if ((rc = dcf_ops_batch_filter(t.get(), attr_idx, decision, dlam, extract, douts, gm, flags)) != DCF_OK) return rc;
     if (!dev)
         for (int i = 0; i < count && rc == DCF_OK; i++) {
             const size_t bytes = (size_t)(images[i].dims[0] * images[i].dims[1] * images[i].dims[2]) * (images[i].dtype == DCF_U8 ? 1 : 2);
             if (hipMemcpyAsync(images[i].out, obuf + boff[i], bytes, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
                 rc = dcf_set_error(ctx, DCF_EDEVICE, "image %d download", i);
         }
-done:
-    /* the host tables above are read by asynchronous copies: wait before they go out of scope */
     if (hipStreamSynchronize(ctx->stream) != hipSuccess && rc == DCF_OK) rc = dcf_set_error(ctx, DCF_EDEVICE, "batch synchronize");
-    t->bimg = nullptr;
-    tree_release(t);
-    delete t;
-    dcf_release(ctx, dtab);
-    dcf_release(ctx, dlam);
-    dcf_release(ctx, douts);
-    dcf_release(ctx, gm);
-    dcf_release(ctx, staging);
-    dcf_release(ctx, obuf);
     return rc;
 }
 

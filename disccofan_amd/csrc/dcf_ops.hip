@@ -351,6 +351,28 @@ __global__ __launch_bounds__(256) void k_check_out(const uint32_t *__restrict__ 
 /* ------------------------------------------------------------------------------------------------ */
 #define SPEC_LDS_BINS 2048
 
+/* sum over the wave's lanes (every lane gets the total) */
+__device__ __forceinline__ unsigned long long spec_wsum(unsigned long long v)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t lo = __shfl_xor((uint32_t)v, o, 64), hi = __shfl_xor((uint32_t)(v >> 32), o, 64);
+        v += ((unsigned long long)hi << 32) | lo;
+    }
+    return v;
+}
+
+__device__ __forceinline__ double spec_wsum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+/* Every levelroot adds (g - g_parent) * area to the bin of its attribute (pattern.c:80-96, 170-191).
+ * With at most SPEC_LDS_BINS bins each block keeps the histogram in LDS, and each wave first sums its
+ * lanes' contributions per distinct bin (a ballot picks the lanes of the leader's bin), so one LDS atomic
+ * per distinct bin and wave replaces one per node: most nodes are small and share the first bins.
+ * Integer greys at unit pixel size add exact 64-bit integers, so the order of the additions does not
+ * matter; float greys add doubles (their sum order differs from the reference's thread split anyway). */
 __global__ __launch_bounds__(256) void k_spectrum(AttrCtx c, int grp, int attr1, const float *__restrict__ l1, int n1, int attr2,
                                                   const float *__restrict__ l2, int n2, int second_is_area,
                                                   const int32_t *__restrict__ npar, const float *__restrict__ ngrey, int exact,
@@ -364,25 +386,48 @@ __global__ __launch_bounds__(256) void k_spectrum(AttrCtx c, int grp, int attr1,
         for (int i = threadIdx.x; i < nb; i += blockDim.x) { sh_i[i] = 0; sh_d[i] = 0; }
         __syncthreads();
     }
-    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < K; k += (int64_t)gridDim.x * blockDim.x) {
-        int32_t p = npar[k];
-        float dh = p >= 0 ? ngrey[k] - ngrey[p] : ngrey[k];
-        if (dh == 0) continue;
-        int s1 = find_scale(l1, n1, attr_value(c, attr1, k));
-        int bin = s1;
-        if (l2) {
-            double a2 = second_is_area ? attr_area(c, grp, k) : attr_value(c, attr2, k);
-            bin = s1 + find_scale(l2, n2, a2) * n1;
+    const int lane = threadIdx.x & 63;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    /* every lane of a wave runs the same iterations (the ballots below need the whole wave) */
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < K; base += stride) {
+        const int64_t k = base + threadIdx.x;
+        int bin = -1;
+        unsigned long long vi = 0;
+        double vd = 0;
+        if (k < K) {
+            const int32_t p = npar[k];
+            const float dh = p >= 0 ? ngrey[k] - ngrey[p] : ngrey[k];
+            if (dh != 0) {
+                bin = find_scale(l1, n1, attr_value(c, attr1, k));
+                if (l2) {
+                    const double a2 = second_is_area ? attr_area(c, grp, k) : attr_value(c, attr2, k);
+                    bin += find_scale(l2, n2, a2) * n1;
+                }
+                const double ar = attr_area(c, grp, k);
+                if (exact) vi = (unsigned long long)(long long)dh * (unsigned long long)(long long)ar;
+                else vd = (double)dh * ar;
+            }
         }
-        double ar = attr_area(c, grp, k);
-        if (exact) {
-            unsigned long long v = (unsigned long long)(long long)dh * (unsigned long long)(long long)ar;
-            if (use_lds) atomicAdd(&sh_i[bin], v);
-            else atomicAdd(&hist_i[bin], v);
-        } else {
-            double v = (double)dh * ar;
-            if (use_lds) atomicAdd(&sh_d[bin], v);
-            else atomicAdd(&hist_d[bin], v);
+        if (!use_lds) {
+            if (bin >= 0) {
+                if (exact) atomicAdd(&hist_i[bin], vi);
+                else atomicAdd(&hist_d[bin], vd);
+            }
+            continue;
+        }
+        uint64_t pending = __ballot(bin >= 0);
+        while (pending) {
+            const int leader = __ffsll((unsigned long long)pending) - 1;
+            const int b = __shfl(bin, leader, 64);
+            const bool mine = bin == b;
+            pending &= ~__ballot(mine);
+            if (exact) {
+                const unsigned long long s = spec_wsum(mine ? vi : 0ull);
+                if (lane == leader) atomicAdd(&sh_i[b], s);
+            } else {
+                const double s = spec_wsum(mine ? vd : 0.0);
+                if (lane == leader) atomicAdd(&sh_d[b], s);
+            }
         }
     }
     if (use_lds) {

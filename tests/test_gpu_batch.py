@@ -104,6 +104,22 @@ def test_max_and_min_rules_non_increasing(ctx):
         check_batch(ctx, imgs, lams, 4, "min", decision=dec, attr=17, extract=True, label="rule %d ncomp min-tree" % dec)
 
 
+def test_full_batch256_workload(ctx):
+    """the batched workload at its real size: all 256 images (sides 64..2048, u8 and u16, 289 Mpixel) in
+    one call with their own lambdas, every output against the oracle's tree of that image alone"""
+    shapes = synth.batch_shapes(256, 64, 2048)
+    imgs = [synth.disks(H, W, is16, seed) for (W, H, is16, seed, lam) in shapes]
+    lams = [lam for (W, H, is16, seed, lam) in shapes]
+    say("batch256: %d images, %d pixels" % (len(imgs), sum(a.size for a in imgs)))
+    outs = ctx.batch_filter(imgs, lams, connectivity=4, tree_type="max", attr_idx=0, decision=0)
+    for k, (img, lam, out) in enumerate(zip(imgs, lams, outs)):
+        want = OracleTree(img, 4, "max", 0, threads=16).filter(0, lam, 0).astype(img.dtype)
+        bad = np.nonzero(out.reshape(-1) != want.reshape(-1))[0]
+        assert len(bad) == 0, "batch256 image %d %s %s lam %g: %d pixels differ, first %s" % (
+            k, img.shape, img.dtype, lam, len(bad), bad[:5])
+    say("batch256: 256 images bit-exact")
+
+
 def test_uneven_workload_batch(ctx):
     """the uneven-size batch generator of the batched workload, first 24 images, sides 64..512"""
     shapes = synth.batch_shapes(24, 64, 512)

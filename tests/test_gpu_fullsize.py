@@ -64,8 +64,22 @@ def test_spheres_256_u8(ctx):
     same(t.spectrum1d(0, lv), o.spectrum1d(0, lv), "spectrum")
 
 
+def test_spheres_512_u8(ctx):
+    """512^3 u8, 6-conn max-tree at full size against the oracle: parent array, the volume opening
+    lambda=500 with the direct rule and the 48-bin volume spectrum"""
+    img = synth.spheres(512, 512, 512)
+    say("spheres 512^3 u8 6-conn vs oracle, sha", synth.sha256(img)[:16])
+    t = ctx.build(img, 6, "max", 0)
+    o = OracleTree(img, 6, "max", 0, threads=THREADS)
+    same(t.parent(), o.parent().astype(np.int32), "parent")
+    same(t.filter(0, 500, 0, out_dtype=np.uint8), o.filter(0, 500, 0).astype(np.uint8), "direct filter")
+    lv = synth.log_lambdas(48, 1, 2 ** 27)
+    same(t.spectrum1d(0, lv), o.spectrum1d(0, lv), "spectrum")
+    t.free()
+
+
 def test_spheres_512_u8_properties(ctx):
-    """512^3 u8: the full volume against closed-form properties (the oracle is exercised at 256^3)."""
+    """512^3 u8: the full volume against closed-form properties."""
     img = synth.spheres(512, 512, 512)
     say("spheres 512^3 u8 6-conn properties")
     t = ctx.build(img, 6, "max", 0)
