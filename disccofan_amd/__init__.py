@@ -44,6 +44,7 @@ EXPORTS = [
     "dcf_ctx_synchronize", "dcf_ctx_set_profiling", "dcf_ctx_profile_only", "dcf_batch_filter", "dcf_dmp", "dcf_tree_write_csv", "dcf_kernel_stats", "dcf_kernel_stats_reset",
     "dcf_kernel_name", "dcf_tree_build", "dcf_tree_free", "dcf_tree_accumulate", "dcf_tree_group",
     "dcf_tree_parent", "dcf_tree_num_nodes", "dcf_filter", "dcf_check", "dcf_spectrum1d", "dcf_spectrum2d",
+    "dcf_spectrum1d_noalloc",
     "dcf_lvec_read", "dcf_lvec_free", "dcf_attr_name", "dcf_attr_short_name", "dcf_attr_group",
     "dcf_decision_name", "dcf_run_operations",
 ]
@@ -118,6 +119,7 @@ def lib():
     L.dcf_tree_num_nodes.argtypes = [vp, C.POINTER(C.c_int64)]
     L.dcf_filter.argtypes = [vp, i, i, d, i, vp, i, u]
     L.dcf_check.argtypes = [vp, i, vp, u]
+    L.dcf_spectrum1d_noalloc.argtypes = [vp, i, vp, i, vp]
     L.dcf_dmp.argtypes = [vp, i, i, vp, i, vp, i, u]
     L.dcf_tree_write_csv.argtypes = [vp, i, C.c_char_p]
     L.dcf_spectrum1d.argtypes = [vp, i, vp, i, vp]

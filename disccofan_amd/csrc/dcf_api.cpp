@@ -204,6 +204,10 @@ try {
     shape3("DCF_LEVEL_GROUP2D", c->lg2);
     shape3("DCF_LEVEL_TILE3D", c->lt3);
     shape3("DCF_LEVEL_GROUP3D", c->lg3);
+    /* the allocation-free entry points' scratch address on this device (their calls fail with
+     * DCF_EDEVICE when it is unavailable; the context's other calls do not need it) */
+    (void)dcf_noalloc_init(c);
+    c->err.clear();
     *out = c;
     return DCF_OK;
 } catch (const std::exception &) {
@@ -826,6 +830,18 @@ try {
     return dcf_ops_spectrum(t, attr_idx, lambdas, n, 0, nullptr, 1, 0, out);
 } catch (const std::exception &e) {
     return dcf_set_error(t ? t->ctx : nullptr, DCF_ENOMEM, "out of host memory (%s)", e.what());
+}
+
+extern "C" int dcf_spectrum1d_noalloc(dcf_tree *t, int attr_idx, const float *lambdas, int n, double *out)
+{
+    if (!t) return DCF_EINVAL;
+    if (!lambdas || !out || n <= 0) return dcf_set_error(t->ctx, DCF_EINVAL, "bad spectrum arguments");
+    if (n > DCF_NOALLOC_MAX_BINS)
+        return dcf_set_error(t->ctx, DCF_EINVAL, "%d bins exceed the allocation-free maximum of %d", n, DCF_NOALLOC_MAX_BINS);
+    const int rc = need_group(t, attr_idx);
+    if (rc) return rc;
+    if (hipSetDevice(t->ctx->device) != hipSuccess) return dcf_set_error(t->ctx, DCF_EDEVICE, "hipSetDevice");
+    return dcf_ops_spectrum_noalloc(t, attr_idx, lambdas, n, out);
 }
 
 extern "C" int dcf_spectrum2d(dcf_tree *t, int attr1, const float *l1, int n1, int attr2, const float *l2, int n2, double *out)

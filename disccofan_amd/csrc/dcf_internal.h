@@ -93,6 +93,8 @@ struct dcf_ctx {
     /* zone graph of the first pass (DCF_ZONES): 0 off, 1 every single-image first pass,
      * 2 the coarse pass of u16 images and 2D u8 images */
     int zones = 2;
+    /* device address of the allocation-free entry points' scratch array on this context's device */
+    void *noalloc_scratch = nullptr;
 };
 
 /* batched form: one entry per image of a batch laid out back to back in one pixel space */
@@ -216,6 +218,10 @@ int dcf_ops_check(dcf_tree *t, int attr_idx, float *out_dev);
 int dcf_ops_spectrum(dcf_tree *t, int attr1, const float *l1, int n1, int attr2, const float *l2, int n2,
                      int second_is_area, double *out_host);
 int dcf_ops_minmax(dcf_tree *t);
+/* allocation-free spectrum: the scratch array's address for ctx's device (ctx->device current), and the
+ * 1D spectrum with its buffers in that array */
+int dcf_noalloc_init(dcf_ctx *ctx);
+int dcf_ops_spectrum_noalloc(dcf_tree *t, int attr1, const float *l1, int n1, double *out_host);
 void dcf_free_attr(dcf_ctx *ctx, DevAttr &a);
 
 #endif

@@ -23,6 +23,7 @@
  * ulong area is converted to float before the product), so bins and thresholds match bit for bit.
  */
 #include <hipcub/hipcub.hpp>
+#include <mutex>
 #include <vector>
 
 #include "dcf_internal.h"
@@ -1452,6 +1453,74 @@ int dcf_ops_spectrum(dcf_tree *t, int attr1, const float *l1, int n1, int attr2,
     DCF_HIP(ctx, hipStreamSynchronize(ctx->stream));
     for (int i = 0; i < nb; i++) out_host[i] = exact ? (double)vi[i] : vd[i];
     dcf_release(ctx, buf);
+    return DCF_OK;
+}
+
+/* ------------------------------------------------------------------------------------------------ */
+/* the allocation-free 1D spectrum: its scratch is a file-scope device array                      */
+/* ------------------------------------------------------------------------------------------------ */
+/* Layout per call of n bins: u64 integer bins [n], double bins [n], double output [n], float
+ * thresholds [n], 28 bytes per bin.  Every call on a device shares that device's array and leaves
+ * its contents behind, so calls that overlap on one device race. */
+#define NOALLOC_BYTES ((size_t)DCF_NOALLOC_MAX_BINS * 28)
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+/* the array's device address per device (the symbol's host name is no device address) */
+static void *g_scratch_ptr[64];
+static std::mutex g_scratch_mu;
+
+int dcf_noalloc_init(dcf_ctx *ctx)
+{
+    if (ctx->device < 0 || ctx->device >= 64) return dcf_set_error(ctx, DCF_EDEVICE, "device %d: no scratch slot", ctx->device);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[ctx->device]) {
+        void *ptr = nullptr;
+        /* the caller made ctx->device current */
+        const hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return dcf_set_error(ctx, DCF_EDEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        g_scratch_ptr[ctx->device] = ptr;
+    }
+    ctx->noalloc_scratch = g_scratch_ptr[ctx->device];
+    return DCF_OK;
+}
+
+__global__ __launch_bounds__(256) void k_spec_out(int exact, const unsigned long long *__restrict__ hi, const double *__restrict__ hd,
+                                                  double *__restrict__ out, int nb)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nb) out[i] = exact ? (double)hi[i] : hd[i];
+}
+
+/* dcf_ops_spectrum for one axis, with every device buffer in the scratch array: no allocation */
+int dcf_ops_spectrum_noalloc(dcf_tree *t, int attr1, const float *l1, int n1, double *out_host)
+{
+    dcf_ctx *ctx = t->ctx;
+    if (!ctx->noalloc_scratch) return dcf_set_error(ctx, DCF_EDEVICE, "no allocation-free scratch on device %d", ctx->device);
+    if (n1 <= 0 || n1 > DCF_NOALLOC_MAX_BINS)
+        return dcf_set_error(ctx, DCF_EINVAL, "%d bins: the allocation-free spectrum takes 1 to %d", n1, DCF_NOALLOC_MAX_BINS);
+    const int nb = n1;
+    const bool unit = t->pixdim[0] == 1.0f && t->pixdim[1] == 1.0f && t->pixdim[2] == 1.0f;
+    const int exact = (t->dtype != DCF_F32) && unit;
+    char *buf = (char *)ctx->noalloc_scratch;
+    unsigned long long *hi = (unsigned long long *)buf;
+    double *hd = (double *)(buf + (size_t)nb * 8);
+    double *dout = (double *)(buf + (size_t)nb * 16);
+    float *dl1 = (float *)(buf + (size_t)nb * 24);
+    /* the integer and double bins start at zero: a fill kernel (no memset node if a caller captures) */
+    DCF_LAUNCH(ctx, "k_fill", k_fill_u32, dim3(nblk((int64_t)nb * 4, 256)), dim3(256), 0, (uint32_t *)buf, 0u, (int64_t)nb * 4);
+    DCF_HIP(ctx, hipMemcpyAsync(dl1, l1, (size_t)n1 * 4, hipMemcpyHostToDevice, ctx->stream));
+    AttrCtx c = attr_ctx(t);
+    const int grp = dcf_group_of(attr1);
+    unsigned blocks = nblk(t->K, 256);
+    if (blocks > 4096) blocks = 4096;
+    if (blocks < 1) blocks = 1;
+    DCF_LAUNCH(ctx, "k_spectrum", k_spectrum, dim3(blocks), dim3(256), 0, c, grp, attr1, (const float *)dl1, n1, 0,
+               (const float *)nullptr, 1, 0, t->npar, t->ngrey, exact, hi, hd, t->K);
+    DCF_LAUNCH(ctx, "k_spec_out", k_spec_out, dim3(nblk(nb, 256)), dim3(256), 0, exact, (const unsigned long long *)hi,
+               (const double *)hd, dout, nb);
+    DCF_HIP(ctx, hipMemcpyAsync(out_host, dout, (size_t)nb * 8, hipMemcpyDeviceToHost, ctx->stream));
+    DCF_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return DCF_OK;
 }
 

@@ -137,6 +137,16 @@ int dcf_spectrum1d(dcf_tree *tree, int attr_idx, const float *lambdas, int n, do
 int dcf_spectrum2d(dcf_tree *tree, int attr1, const float *l1, int n1, int attr2, const float *l2, int n2,
                    double *out);
 
+/* ---- allocation-free 1D spectrum ------------------------------------------------------------------ *
+ * No counterpart in the reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): dcf_spectrum1d with every
+ * device buffer in a fixed array of the library (28 bytes per bin, DCF_NOALLOC_MAX_BINS bins at most,
+ * held on each device the library runs on until the process ends); it calls no allocator.  More than
+ * DCF_NOALLOC_MAX_BINS bins is DCF_EINVAL.  Every call on one device shares that device's array, so
+ * calls that overlap on one device (several host threads, several contexts) race; serialise them. */
+#define DCF_NOALLOC_MAX_BINS 65536
+int dcf_spectrum1d_noalloc(dcf_tree *tree, int attr_idx, const float *lambdas, int n, double *out);
+
 /* ---- tree export (write_tree, io.c:1431-1573) --------------------------------------------------- *
  * The "tree" operation's CSV: header, then one row per node (levelroot) in raster order --
  * index,x,y,z,gval,parent,flux,area and the columns of attr_idx's group, with the reference's printf
