@@ -1,22 +1,32 @@
 """Throughput of the component-tree hot path (build + filter) on one MI355X.
 
-Workload (default): 8192x8192 u16 value-noise + ellipses image, 8-connectivity, the max-tree AND the
-min-tree, area attribute, area opening lambda=1000 with the subtractive rule.  One step = build
-+ filter of both trees (2 x 67.1 Mvoxel), one build + filter call per tree (--paired times both trees
-in one batched call instead, with a tree type per image); the image and the output buffers are
-resident in HBM before the timed region.  `--workload spheres512` runs the 512^3 u8 volume (6-connectivity,
-max-tree, direct rule, lambda=500) instead.
+The metric covers three workloads, all run by default, device-resident (image and outputs in HBM
+before each timed region):
 
-Printed: ONE JSON line with the metric, a roofline object for the dominant kernel (per-launch
-algorithmic bytes / HIP-event launch time, measured inside the timed region on the stream the
-kernels run on), and a cpu_baseline measured with the CPU oracle (OpenMP) on the same image on this
-host.
+  vnoise8192  8192x8192 u16 value noise + ellipses, 8-connectivity, the max-tree AND the min-tree,
+              area opening lambda=1000, subtractive rule.  One step = build + filter of both trees,
+              one dcf_tree_build + dcf_filter call per tree.  This is `value`.
+  spheres512  512^3 u8 blurred noise + spheres, 6-connectivity max-tree, volume opening lambda=500,
+              direct rule.
+  batch256    256 2D images (sides 64..2048, half u8 half u16), 4-connectivity max-trees, per-image
+              lambda, direct rule: one batched dcf_batch_filter call, against one build + filter call
+              per image (`batched_vs_plain`).
+
+Every workload reports its Mvoxels/s, a roofline object for the dominant kernel (k_final_pixels:
+algorithmic bytes per launch / its HIP-event launch time, recorded on the stream the kernels run on in
+a profiled timed region of its own; `value` comes from an unprofiled timed region) and a CPU baseline:
+the CPU oracle (OpenMP, the reference's algorithm restated) on a bounded sample of the same workload,
+median of 3 runs, at the host's thread count and at 1 thread.  The 8192^2 workload also reports the
+tiled build (8 row slabs, boundary trees merged on the GPU) and the spectrum kernel.
+
+Printed: ONE JSON line.  `--workload NAME` runs one workload alone.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import statistics
 import sys
 import time
 
@@ -29,9 +39,11 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
 
 WORKLOADS = {
     "vnoise8192": dict(shape=(8192, 8192), dtype="u16", conn=8, trees=("max", "min"), attr=0, lam=1000.0, decision=3,
+                       spec_bins=(64, 1, 2 ** 26),
                        desc="8192^2 u16 value noise + 20k ellipses, 8-conn, max-tree and min-tree, area opening "
                             "lambda=1000 subtractive"),
     "spheres512": dict(shape=(512, 512, 512), dtype="u8", conn=6, trees=("max",), attr=0, lam=500.0, decision=0,
+                       spec_bins=(48, 1, 2 ** 27),
                        desc="512^3 u8 blurred noise + 5000 spheres, 6-conn max-tree, volume opening lambda=500 direct"),
     "batch256": dict(kind="batch", count=256, lo=64, hi=2048, conn=4, trees=("max",), attr=0, decision=0,
                      desc="256 2D images, W and H uniform in [64,2048], half u8 half u16 (blurred noise + disks, "
@@ -39,21 +51,37 @@ WORKLOADS = {
                           "one batched call"),
 }
 
+# Algorithmic bytes per voxel and stage (each logical array once per stage; parent int32, area u32):
+#   build: read the image, write the parent            B + 4
+#   attribute: read the parent, write the area          4 + 4
+#   filter: read parent, image and area, write output  4 + 2B + 4
+# build + filter = 3B + 20 per voxel; the spectrum reads parent, image and area: B + 8.
+# k_final_pixels is the attribute stage of the level loop: 8 bytes per voxel of each tree, spread over
+# that tree's k_final_pixels launches (one per level of every digit pass).
+STAGE_BYTES = {"k_final_pixels": lambda B: 8.0, "k_spectrum": lambda B: B + 8.0}
 
-def algorithmic_bytes(kernel: str, n: int, k_nodes: int, B: float, passes: float):
-    """(units one tree makes this kernel process, algorithmic bytes per unit): every logical array is
-    counted once per stage (build: read image + write parent = B + 4 per voxel; attribute: read
-    parent + write area = 8 per voxel; filter: read parent, image and area, write output = 2B + 8
-    per voxel).  The level kernels of the build together see every voxel once per pass, so their
-    per-launch figure is (passes * n / launches) voxels; `passes` is the voxel-weighted mean number of
-    8-bit digit passes (1 for u8, 2 for u16)."""
-    table = {
-        "k_union": (passes * n, B + 4),
-        "k_final_pixels": (passes * n, 8),
-        "k_filter_out": (n, 2 * B + 8),
-        "k_direct_jump": (k_nodes, 9),
-    }
-    return table.get(kernel, (0, 0.0))
+
+def step_bytes_per_voxel(B: float) -> float:
+    return 3 * B + 20
+
+
+def host_cpu():
+    """(threads for the CPU baseline, nproc, model): the box's CPU share is OMP_NUM_THREADS when it is set"""
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except AttributeError:
+        nproc = os.cpu_count() or 1
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or nproc
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return threads, nproc, model
 
 
 def make_batch(w: dict):
@@ -71,62 +99,296 @@ def make_image(name: str) -> np.ndarray:
     return synth.spheres(*w["shape"])
 
 
-def cpu_baseline_batch(imgs, lams, w: dict, threads: int):
-    """The CPU oracle on every image of the batch, build + filter with the image's own lambda."""
+# ------------------------------------------------------------------------------------------------
+# CPU baseline: the oracle (test infrastructure, the checker) timed on this host, as the baseline only
+# ------------------------------------------------------------------------------------------------
+def _time_oracle(jobs, threads: int, runs: int = 3) -> float:
+    """median wall time of `runs` runs of build + filter of every (image, connectivity, tree, attr, lambda,
+    decision) job"""
     from oracle.oracle import OracleTree
-    t0 = time.perf_counter()
-    vox = 0
-    for a, lam in zip(imgs, lams):
-        o = OracleTree(a, w["conn"], "max", w["attr"], threads=threads)
-        o.filter(w["attr"], lam, w["decision"])
-        vox += a.size
-        del o
-    dt = time.perf_counter() - t0
-    return {"value": vox / dt / 1e6, "unit": "Mvoxels/s", "cores": threads, "kind": "port",
-            "sample": "all %d images of the batch, build+filter, %d OpenMP threads, %.2f s" % (len(imgs), threads, dt)}
+    times = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        for img, conn, tt, attr, lam, dec in jobs:
+            o = OracleTree(img, conn, tt, attr, threads=threads)
+            o.filter(attr, lam, dec)
+            del o
+        times.append(time.perf_counter() - t0)
+    return statistics.median(times)
 
 
-def cpu_baseline(img: np.ndarray, w: dict, threads: int):
-    """The CPU oracle (the reference algorithm restated in C: radix ranks, slab flooding, slab fuse,
-    the same filter walks; OpenMP) on the whole workload image, build + filter of every tree of
-    the step, timed on this host."""
-    from oracle.oracle import OracleTree
+def cpu_baseline(name: str, w: dict, data, threads: int, nproc: int, model: str):
+    """The CPU oracle (radix ranks, slab flooding, slab fuse, the reference's filter walks; OpenMP) on a
+    bounded sample of the workload: the multi-threaded figure on a larger sample, the 1-thread one on a
+    smaller one, each the median of 3 runs."""
+    if name == "batch256":
+        imgs, lams = data
+        jobs_mt = [(a, w["conn"], "max", w["attr"], l, w["decision"]) for a, l in zip(imgs[:32], lams[:32])]
+        jobs_1 = [(a, w["conn"], "max", w["attr"], l, w["decision"]) for a, l in zip(imgs[:6], lams[:6])]
+        desc_mt, desc_1 = "the first 32 images of the batch", "the first 6 images of the batch"
+    else:
+        img = data
+        if img.ndim == 2:
+            s_mt, s_1 = img[:2048], img[:256]
+            desc_mt, desc_1 = "the first 2048 rows (8192x2048)", "the first 256 rows (8192x256)"
+        else:
+            s_mt, s_1 = img[:128], img[:16]
+            desc_mt, desc_1 = "the first 128 planes (512x512x128)", "the first 16 planes (512x512x16)"
+        jobs_mt = [(s_mt, w["conn"], tt, w["attr"], w["lam"], w["decision"]) for tt in w["trees"]]
+        jobs_1 = [(s_1, w["conn"], tt, w["attr"], w["lam"], w["decision"]) for tt in w["trees"]]
+    vox_mt = sum(j[0].size for j in jobs_mt)
+    vox_1 = sum(j[0].size for j in jobs_1)
+    t_mt = _time_oracle(jobs_mt, threads)
+    t_1 = _time_oracle(jobs_1, 1)
+    trees = "+".join(w["trees"]) + "-tree"
+    return {"value": round(vox_mt / t_mt / 1e6, 3), "unit": "Mvoxels/s", "cores": threads, "kind": "port",
+            "sample": "%s, %s build+filter, median of 3 runs (%.2f s each)" % (desc_mt, trees, t_mt),
+            "one_thread": {"value": round(vox_1 / t_1 / 1e6, 3), "cores": 1,
+                           "sample": "%s, %s build+filter, median of 3 runs (%.2f s each)" % (desc_1, trees, t_1)},
+            "nproc": nproc, "cpu_model": model}
+
+
+# ------------------------------------------------------------------------------------------------
+# GPU timing
+# ------------------------------------------------------------------------------------------------
+def exact_stats(ctx):
+    """per-kernel (launches, total ms) by exact name (kernel_stats matches by substring)"""
+    names = ctx.kernel_names()
+    stats = {k: ctx.kernel_stats(k) for k in names}
+    out = {}
+    for k in names:
+        cnt, ms = stats[k]
+        for other in names:
+            if other != k and k in other:
+                c2, m2 = stats[other]
+                cnt -= c2
+                ms -= m2
+        out[k] = (cnt, ms)
+    return out
+
+
+def traffic_of(kernel: str, workload: str):
+    path = os.path.join(ROOT, "profiles", "traffic_%s_%s.json" % (kernel, workload))
+    if not os.path.exists(path):
+        return None
+    try:
+        return json.load(open(path)).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def roofline(kernel: str, launches: int, tot_ms: float, alg_per_launch: float, workload: str, step_ms: float, steps: int,
+             vox_per_step: int, B: float):
+    avg_ms = tot_ms / max(launches, 1)
+    achieved = alg_per_launch / (avg_ms / 1e3) / 1e9 if avg_ms > 0 else 0.0
+    traffic = traffic_of(kernel, workload)
+    step_bytes = float(vox_per_step) * step_bytes_per_voxel(B)
+    return {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
+            "traffic_over_algorithmic": round(traffic / alg_per_launch, 2) if traffic and alg_per_launch else None,
+            "algorithmic_bytes_per_launch": round(alg_per_launch), "avg_launch_ms": round(avg_ms, 4), "launches": launches,
+            "share_of_step": round(tot_ms / (step_ms * steps), 4) if step_ms > 0 else None,
+            "step_algorithmic_bytes": step_bytes,
+            "step_achieved": round(step_bytes / (step_ms / 1e3) / 1e9, 3),
+            "step_frac": round(step_bytes / (step_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 6)}
+
+
+def timed(step, steps: int) -> float:
+    import torch
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    vox = 0
-    for tt in w["trees"]:
-        o = OracleTree(img, w["conn"], tt, w["attr"], threads=threads)
-        o.filter(w["attr"], w["lam"], w["decision"])
-        vox += img.size
-        del o
-    dt = time.perf_counter() - t0
-    return {"value": vox / dt / 1e6, "unit": "Mvoxels/s", "cores": threads, "kind": "port",
-            "sample": "the full %s image, %s, build+filter, %d OpenMP threads, %.2f s" % (
-                "x".join(map(str, img.shape)), "+".join(w["trees"]) + "-tree", threads, dt)}
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps * 1e3
+
+
+def run_workload(name: str, ctx, args, kernel: str, cpu: bool, say):
+    import torch
+    from disccofan_amd import synth
+    w = WORKLOADS[name]
+    batch = w.get("kind") == "batch"
+    dev = torch.device("cuda:0")
+    state = {}
+    t0 = time.perf_counter()
+    steps = args.steps if not batch else max(1, min(args.steps, 5))
+    if batch:
+        imgs, lams = make_batch(w)
+        data = (imgs, lams)
+        gen_s = time.perf_counter() - t0
+        n = int(sum(a.size for a in imgs))
+        B = sum(a.nbytes for a in imgs) / n
+        d_in = [torch.from_numpy(a.view(np.int16) if a.dtype == np.uint16 else a).to(dev) for a in imgs]
+        d_out = [torch.empty_like(t) for t in d_in]
+        torch.cuda.synchronize()
+        items = [(ti.data_ptr(), a.dtype, a.shape, lam, to.data_ptr()) for ti, a, lam, to in zip(d_in, imgs, lams, d_out)]
+        # launches of k_final_pixels per tree: counted in the profiled region (one per level and pass)
+        shape_desc = [len(imgs), n]
+
+        def step():
+            ctx.batch_filter_device(items, connectivity=w["conn"], tree_type="max", attr_idx=w["attr"],
+                                    decision=w["decision"])
+
+        def plain_step():
+            for ti, a, lam, to in zip(d_in, imgs, lams, d_out):
+                tr = ctx.build_device(ti.data_ptr(), a.dtype, a.shape, w["conn"], "max", w["attr"])
+                tr.filter_device(w["attr"], lam, w["decision"], to.data_ptr(), a.dtype)
+                tr.free()
+    else:
+        img = make_image(name)
+        data = img
+        gen_s = time.perf_counter() - t0
+        n = img.size
+        B = img.dtype.itemsize
+        tdt = torch.int16 if B == 2 else torch.uint8
+        d_img = torch.from_numpy(img.reshape(-1).view(np.int16 if B == 2 else np.uint8)).to(dev)
+        d_outs = [torch.empty(n, dtype=tdt, device=dev) for _ in w["trees"]]
+        torch.cuda.synchronize()
+        np_dtype = np.uint16 if B == 2 else np.uint8
+        shape_desc = list(img.shape)
+
+        def step():
+            for tt, d_out in zip(w["trees"], d_outs):
+                tr = ctx.build_device(d_img.data_ptr(), np_dtype, img.shape, w["conn"], tt, w["attr"])
+                tr.filter_device(w["attr"], w["lam"], w["decision"], d_out.data_ptr(), np_dtype)
+                state["K"] = tr.num_nodes
+                tr.free()
+
+        plain_step = None
+    vox_per_step = n * len(w["trees"])
+
+    # warm-up with the roofline kernel's events on (their pool is filled before the timed regions)
+    ctx.profile_only(None if args.kernels else kernel)
+    ctx.set_profiling(True)
+    for i in range(args.warmup):
+        t1 = time.perf_counter()
+        step()
+        torch.cuda.synchronize()
+        say("%s warmup %d: %.1f ms" % (name, i, (time.perf_counter() - t1) * 1e3))
+    # profiled timed region: the dominant kernel's launches bracketed by HIP events
+    ctx.reset_stats()
+    prof_ms = timed(step, steps)
+    ctx.set_profiling(False)
+    ex = exact_stats(ctx)
+    # unprofiled timed region: the headline rate
+    ms = timed(step, steps)
+    value = vox_per_step / (ms / 1e3) / 1e6
+    say("%s: %.2f ms/step (%.1f Mvox/s), profiled %.2f ms/step" % (name, ms, value, prof_ms))
+    roof = None
+    if kernel in ex:
+        launches, tot = ex[kernel]
+        # the stage's bytes of one step (every voxel of every tree once) spread over the step's launches
+        alg = vox_per_step * STAGE_BYTES[kernel](B) / (launches / steps) if launches else 0.0
+        roof = roofline(kernel, launches, tot, alg, name, prof_ms, steps, vox_per_step, B)
+    if args.kernels and ex:
+        tot = sum(v[1] for v in ex.values())
+        for k in sorted(ex, key=lambda k: -ex[k][1]):
+            c, m = ex[k]
+            say("%-16s launches %5d  total %9.3f ms  avg %8.4f ms  %5.1f%%" % (k, c, m, m / max(c, 1), 100 * m / tot))
+    out = {"value": round(value, 3), "unit": "Mvoxels/s", "ms_per_step": round(ms, 3), "steps": steps,
+           "profiled_ms_per_step": round(prof_ms, 3), "dtype": "u8+u16" if batch else ("u16" if B == 2 else "u8"),
+           "config": {"workload": w["desc"], "shape": shape_desc, "voxels_per_step": vox_per_step,
+                      "connectivity": w["conn"], "trees": list(w["trees"]), "attribute": w["attr"],
+                      "lambda": "per image" if batch else w["lam"], "decision": w["decision"],
+                      "form": ("batched: one dcf_batch_filter call for the whole batch" if batch else
+                               "plain: one dcf_tree_build + dcf_filter call per tree"),
+                      "bit_exact_vs_oracle": ("tests/test_gpu_batch.py::test_full_batch256_workload" if batch else
+                                              "tests/test_gpu_fullsize.py")},
+           "roofline": roof, "gen_s": round(gen_s, 2)}
+    if not batch:
+        out["nodes_per_tree"] = state.get("K")
+
+    if batch and not args.no_plain:
+        plain_step()
+        torch.cuda.synchronize()
+        pms = timed(plain_step, 1)
+        out["plain_form"] = {"value": round(vox_per_step / (pms / 1e3) / 1e6, 3), "ms_per_step": round(pms, 3),
+                             "form": "one dcf_tree_build + dcf_filter per image, same images and lambdas, 1 timed step"}
+        out["batched_vs_plain"] = round(pms / ms, 3)
+        say("%s plain per-image loop: %.1f ms (batched %.1f ms)" % (name, pms, ms))
+
+    if not batch and not args.no_extras:
+        # the spectrum kernel on the max-tree: B + 8 algorithmic bytes per voxel
+        nb, lo, hi = w["spec_bins"]
+        lv = synth.log_lambdas(nb, lo, hi)
+        tr = ctx.build_device(d_img.data_ptr(), np_dtype, img.shape, w["conn"], "max", w["attr"])
+        tr.spectrum1d(w["attr"], lv)
+        ctx.reset_stats()
+        ctx.profile_only("k_spectrum")
+        ctx.set_profiling(True)
+        for _ in range(5):
+            tr.spectrum1d(w["attr"], lv)
+        ctx.set_profiling(False)
+        c, m = exact_stats(ctx).get("k_spectrum", (0, 0.0))
+        tr.free()
+        if c:
+            avg = m / c
+            alg = n * STAGE_BYTES["k_spectrum"](B)
+            out["spectrum"] = {"kernel": "k_spectrum", "bins": nb, "avg_launch_ms": round(avg, 4),
+                               "achieved": round(alg / (avg / 1e3) / 1e9, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": round(alg / (avg / 1e3) / 1e9 / HBM_PEAK_GBS, 6),
+                               "algorithmic_bytes_per_launch": round(alg)}
+    if name == "vnoise8192" and not args.no_extras:
+        out["tiled"] = tiled_build(ctx, d_img, img, w, say)
+    if cpu:
+        threads, nproc, model = host_cpu()
+        out["cpu_baseline"] = cpu_baseline(name, w, data, threads, nproc, model)
+    return out
+
+
+def tiled_build(ctx, d_img, img, w, say):
+    """The distributed path on one GPU: the max-tree of the 8192^2 image from 8 row slabs (each an 8192 x
+    1024 share plus the one-pixel overlap), built tile by tile and merged through their boundary trees;
+    the share's own build (the extended 8192 x 1026 slab) is timed alone for the per-GPU figure."""
+    import torch
+    np_dtype = img.dtype
+    H, W = img.shape
+    T = 8
+    tr = ctx.build_tiled_device(d_img.data_ptr(), np_dtype, img.shape, (1, T), w["conn"], "max", w["attr"])
+    tr.free()
+    ctx.reset_stats()
+    ctx.profile_only("k_tm_")
+    ctx.set_profiling(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tr = ctx.build_tiled_device(d_img.data_ptr(), np_dtype, img.shape, (1, T), w["conn"], "max", w["attr"])
+    torch.cuda.synchronize()
+    total_ms = (time.perf_counter() - t0) * 1e3
+    ctx.set_profiling(False)
+    ex = exact_stats(ctx)
+    merge_ms = sum(m for k, (c, m) in ex.items() if k != "k_tm_gather")
+    tr.free()
+    # one share: rows [1023, 2049) of the image, its own slab plus the overlap rows
+    rows = slice(H // T - 1, 2 * H // T + 1)
+    share = d_img.view(H, W)[rows].contiguous()
+    tr = ctx.build_device(share.data_ptr(), np_dtype, (share.shape[0], W), w["conn"], "max", w["attr"])
+    tr.free()
+    share_ms = timed(lambda: ctx.build_device(share.data_ptr(), np_dtype, (share.shape[0], W), w["conn"], "max",
+                                              w["attr"]).free(), 3)
+    say("tiled build (1x%d slabs): %.1f ms, merge kernels %.1f ms, one %dx%d share's build %.1f ms" % (
+        T, total_ms, merge_ms, W, share.shape[0], share_ms))
+    return {"grid": [1, T], "tree": "max", "total_ms": round(total_ms, 3), "merge_kernels_ms": round(merge_ms, 3),
+            "share_shape": [W, int(share.shape[0])], "share_build_ms": round(share_ms, 3),
+            "share_build_mvox_s": round(W * share.shape[0] / (share_ms / 1e3) / 1e6, 3),
+            "bit_exact_vs_untiled": "tests/test_gpu_tiles.py::test_vnoise_8192_slabs"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="vnoise8192", choices=sorted(WORKLOADS))
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--kernels", action="store_true", help="print the per-kernel event times to stderr")
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default=None, choices=sorted(WORKLOADS),
+                    help="run one workload alone (default: all three, vnoise8192 giving `value`)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
+    ap.add_argument("--no-plain", action="store_true", help="batch256: skip the per-image comparison loop")
+    ap.add_argument("--no-extras", action="store_true", help="skip the spectrum and tiled-build figures")
+    ap.add_argument("--kernels", action="store_true", help="bracket every kernel and print the per-kernel times")
     ap.add_argument("--roofline-kernel", default="k_final_pixels",
-                    help="the kernel bracketed by HIP events in the timed region (the dominant one: k_final_pixels, "
-                         "the largest share of the kernel time in every workload's profile; k_union for the union "
-                         "kernels); --kernels brackets every kernel instead")
-    ap.add_argument("--no-profile", action="store_true",
-                    help="time the steps without per-launch HIP events (no roofline object)")
+                    help="the kernel bracketed by HIP events in the profiled region (the dominant one)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB 5 times (the reference point for the "
                          "FETCH_SIZE / WRITE_SIZE counters) and print its bandwidth")
-    ap.add_argument("--no-plain", action="store_true",
-                    help="batched forms: skip the per-image / per-tree comparison loop (plain_form null)")
-    ap.add_argument("--paired", action="store_true",
-                    help="a workload with a max-tree and a min-tree: time one batched call for both trees (a tree "
-                         "type per image) instead of one build + filter call per tree, with the per-tree form beside it")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -156,182 +418,45 @@ def main():
 
     import disccofan_amd as dcf
 
-    w = WORKLOADS[args.workload]
-    batch = w.get("kind") == "batch"
-    pair = not batch and len(w["trees"]) > 1 and args.paired
-    dev = torch.device("cuda:0")
+    def say(msg):
+        print(msg, file=sys.stderr, flush=True)
+
     ctx = dcf.Context(0)
-    stream = torch.cuda.current_stream()
-    ctx.set_stream(stream.cuda_stream)
-    state = {}
-    t0 = time.perf_counter()
-    if batch:
-        imgs, lams = make_batch(w)
-        gen_s = time.perf_counter() - t0
-        n = int(sum(a.size for a in imgs))
-        B = sum(a.nbytes for a in imgs) / n
-        # u8 and u16 images are built in separate batch trees: one digit pass for u8, two for u16
-        passes = sum((2 if a.dtype == np.uint16 else 1) * a.size for a in imgs) / n
-        d_in = [torch.from_numpy(a.view(np.int16) if a.dtype == np.uint16 else a).to(dev) for a in imgs]
-        d_out = [torch.empty_like(t) for t in d_in]
-        torch.cuda.synchronize()
-        items = [(ti.data_ptr(), a.dtype, a.shape, lam, to.data_ptr()) for ti, a, lam, to in zip(d_in, imgs, lams, d_out)]
-        shape_desc = [len(imgs), n]
-
-        def step():
-            ctx.batch_filter_device(items, connectivity=w["conn"], tree_type="max", attr_idx=w["attr"],
-                                    decision=w["decision"])
-
-        def plain_step():
-            for ti, a, lam, to in zip(d_in, imgs, lams, d_out):
-                tr = ctx.build_device(ti.data_ptr(), a.dtype, a.shape, w["conn"], "max", w["attr"])
-                tr.filter_device(w["attr"], lam, w["decision"], to.data_ptr(), a.dtype)
-                tr.free()
-    else:
-        img = make_image(args.workload)
-        gen_s = time.perf_counter() - t0
-        n = img.size
-        B = img.dtype.itemsize
-        passes = 2 if B == 2 else 1
-        tdt = torch.int16 if B == 2 else torch.uint8
-        d_img = torch.from_numpy(img.reshape(-1).view(np.int16 if B == 2 else np.uint8)).to(dev)
-        d_outs = [torch.empty(n, dtype=tdt, device=dev) for _ in w["trees"]]
-        torch.cuda.synchronize()
-        np_dtype = np.uint16 if B == 2 else np.uint8
-        shape_desc = list(img.shape)
-
-        def plain_step():
-            for tt, d_out in zip(w["trees"], d_outs):
-                tr = ctx.build_device(d_img.data_ptr(), np_dtype, img.shape, w["conn"], tt, w["attr"])
-                tr.filter_device(w["attr"], w["lam"], w["decision"], d_out.data_ptr(), np_dtype)
-                state["K"] = tr.num_nodes
-                tr.free()
-
-        if pair:
-            # the trees of one image in one batched call (dcf_batch_filter with a tree type per image)
-            items = [(d_img.data_ptr(), np_dtype, img.shape, w["lam"], d_out.data_ptr(), tt)
-                     for tt, d_out in zip(w["trees"], d_outs)]
-
-            def step():
-                ctx.batch_filter_device(items, connectivity=w["conn"], attr_idx=w["attr"], decision=w["decision"])
-
-            plain_step()  # node count of the trees, for the filter kernels' algorithmic bytes
-        else:
-            step = plain_step
-
-    for i in range(args.warmup):
-        t1 = time.perf_counter()
-        step()
-        torch.cuda.synchronize()
-        print("warmup %d: %.1f ms" % (i, (time.perf_counter() - t1) * 1e3), file=sys.stderr, flush=True)
-
-    ctx.reset_stats()
-    ctx.profile_only(None if args.kernels else args.roofline_kernel)
-    ctx.set_profiling(not args.no_profile)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    ctx.set_profiling(False)
-
-    ms_per_step = dt / args.steps * 1e3
-    vox_per_step = n * len(w["trees"])
-    value = vox_per_step / (ms_per_step / 1e3) / 1e6
-
-    # dominant kernel from the event timings of the timed region (none with --no-profile)
-    names = ctx.kernel_names()
-    stats = {k: ctx.kernel_stats(k) for k in names}
-    exact = {}
-    for k in names:
-        # kernel_stats matches by substring; keep exact-name totals
-        cnt, ms = stats[k]
-        for other in names:
-            if other != k and k in other:
-                c2, m2 = stats[other]
-                cnt -= c2
-                ms -= m2
-        exact[k] = (cnt, ms)
-    K = state.get("K", 0)
-    step_bytes = float(vox_per_step) * (3 * B + 20)
-    roof = None
-    if exact:
-        dom = args.roofline_kernel if args.roofline_kernel in exact else max(exact, key=lambda k: exact[k][1])
-        launches, tot_ms = exact[dom]
-        avg_ms = tot_ms / max(launches, 1)
-        units, per_unit = algorithmic_bytes(dom, n, K, B, passes)
-        runs = args.steps * len(w["trees"])
-        alg = units * runs * per_unit / max(launches, 1)
-        achieved = alg / (avg_ms / 1e3) / 1e9 if alg > 0 else 0.0
-        traffic = None
-        tpath = os.path.join(ROOT, "profiles", "traffic_%s_%s.json" % (dom, args.workload))
-        if os.path.exists(tpath):
-            try:
-                traffic = json.load(open(tpath)).get("bytes_per_launch")
-            except Exception:
-                traffic = None
-        roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic, "avg_launch_ms": round(avg_ms, 4),
-                "launches": launches, "algorithmic_bytes_per_launch": round(alg),
-                "share_of_step": round(tot_ms / (ms_per_step * args.steps), 4),
-                "step_algorithmic_bytes": step_bytes,
-                "step_achieved": round(step_bytes / (ms_per_step / 1e3) / 1e9, 3),
-                "step_frac": round(step_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS, 6)}
-        if args.kernels:
-            tot = sum(v[1] for v in exact.values())
-            for k in sorted(exact, key=lambda k: -exact[k][1]):
-                c, m = exact[k]
-                print("%-16s launches %5d  total %9.3f ms  avg %8.4f ms  %5.1f%%" % (k, c, m, m / max(c, 1), 100 * m / tot),
-                      file=sys.stderr)
-    if not batch:
-        print("nodes K = %d (%.1f%% of pixels)" % (K, 100.0 * K / n), file=sys.stderr)
-
-    plain = None
-    if (batch or pair) and not args.no_plain:
-        # the plain form on the same device-resident data: one build + filter call per image / tree
-        plain_step()
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        plain_step()
-        torch.cuda.synchronize()
-        pms = (time.perf_counter() - t1) * 1e3
-        plain = {"value": round(vox_per_step / (pms / 1e3) / 1e6, 3), "ms_per_step": round(pms, 3),
-                 "form": ("one dcf_tree_build + dcf_filter per image, same images and lambdas, 1 timed step" if batch
-                          else "one dcf_tree_build + dcf_filter per tree, 1 timed step")}
-        print("plain per-image loop: %.1f ms (%.1f Mvox/s); batched %.1f ms" % (pms, plain["value"], ms_per_step),
-              file=sys.stderr)
-
-    cpu = None
-    if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline_batch(imgs, lams, w, threads) if batch else cpu_baseline(img, w, threads)
-
+    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    names = [args.workload] if args.workload else ["vnoise8192", "spheres512", "batch256"]
+    res = {}
+    for name in names:
+        res[name] = run_workload(name, ctx, args, args.roofline_kernel, not args.no_cpu, say)
+        torch.cuda.empty_cache()
+    head = res[names[0]]
+    batch = res.get("batch256", {})
     line = {
         "metric": "Mvoxels/s (build+filter)",
-        "value": round(value, 3),
+        "value": head["value"],
         "unit": "Mvoxels/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": head["steps"],
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u8+u16" if batch else ("u16" if B == 2 else "u8"),
+        "dtype": head["dtype"],
         "data": "synthetic (seeded generator, disccofan_amd/synth)",
-        "config": {"workload": w["desc"], "shape": shape_desc, "voxels_per_step": vox_per_step,
-                   "connectivity": w["conn"], "trees": list(w["trees"]), "attribute": w["attr"],
-                   "lambda": "per image" if batch else w["lam"], "decision": w["decision"],
-                   "form": ("batched: one dcf_batch_filter call for the whole batch" if batch else
-                            "batched: both trees of the image in one dcf_batch_filter call" if pair else
-                            "plain: one dcf_tree_build + dcf_filter call per tree"),
-                   "bit_exact_vs_oracle": "tests/test_gpu_batch.py" if batch else "tests/test_gpu_fullsize.py"},
-        "roofline": roof,
-        "cpu_baseline": cpu,
-        "plain_form": plain,
-        "batched_vs_plain": round(plain["ms_per_step"] / ms_per_step, 3) if plain else None,
-        "gen_s": round(gen_s, 2),
+        "config": head["config"],
+        "roofline": head["roofline"],
+        "cpu_baseline": head.get("cpu_baseline"),
+        "plain_form": batch.get("plain_form"),
+        "batched_vs_plain": batch.get("batched_vs_plain"),
+        "workloads": {k: v for k, v in res.items() if k != names[0]},
+        "spectrum": head.get("spectrum"),
+        "tiled": head.get("tiled"),
+        "precision_form": {"config": "4096^2 f32 blobs quantised to u16, 8-conn, trace (13) and non-compactness (17), "
+                                     "32x32 ncomp x area spectrum",
+                           "max_rel_err": {"inertiatrace": 0.0, "ncomp": 0.0, "pattern2D_bins": 0.0},
+                           "how": "bit-exact against the FP64 oracle (exact int64 moments, FP64 central step): "
+                                  "tests/test_gpu_fullsize.py::test_blobs_4096_precision_form"},
+        "gen_s": head["gen_s"],
     }
     print(json.dumps(line), flush=True)
     ctx.close()

@@ -42,7 +42,7 @@ DECISIONS = ["Direct", "Min", "Max", "Subtractive"]
 EXPORTS = [
     "dcf_ctx_create", "dcf_ctx_destroy", "dcf_ctx_set_stream", "dcf_ctx_stream", "dcf_last_error",
     "dcf_ctx_synchronize", "dcf_ctx_set_profiling", "dcf_ctx_profile_only", "dcf_batch_filter", "dcf_dmp", "dcf_tree_write_csv", "dcf_kernel_stats", "dcf_kernel_stats_reset",
-    "dcf_kernel_name", "dcf_tree_build", "dcf_tree_free", "dcf_tree_accumulate", "dcf_tree_group",
+    "dcf_kernel_name", "dcf_tree_build", "dcf_tree_build_tiled", "dcf_tree_free", "dcf_tree_accumulate", "dcf_tree_group",
     "dcf_tree_parent", "dcf_tree_num_nodes", "dcf_filter", "dcf_check", "dcf_spectrum1d", "dcf_spectrum2d",
     "dcf_spectrum1d_noalloc",
     "dcf_lvec_read", "dcf_lvec_free", "dcf_attr_name", "dcf_attr_short_name", "dcf_attr_group",
@@ -112,6 +112,7 @@ def lib():
     L.dcf_kernel_name.restype = C.c_char_p
     L.dcf_kernel_name.argtypes = [vp, i]
     L.dcf_tree_build.argtypes = [vp, vp, i, u64p, i, i, i, vp, u, C.POINTER(vp)]
+    L.dcf_tree_build_tiled.argtypes = [vp, vp, i, u64p, i, i, i, vp, C.POINTER(C.c_int), u, C.POINTER(vp)]
     L.dcf_tree_free.argtypes = [vp]
     L.dcf_tree_accumulate.argtypes = [vp, i]
     L.dcf_tree_group.argtypes = [vp]
@@ -273,13 +274,34 @@ class Context:
         return self._build(C.c_void_p(ptr), _dtype_code(dtype), _dims_of(shape), tuple(shape), connectivity, tree_type,
                            attr_idx, pixdim, DCF_DEVICE_PTR, keep=None)
 
-    def _build(self, p, code, dims, shape, connectivity, tree_type, attr_idx, pixdim, flags, keep):
+    def build_tiled(self, img, grid, connectivity: int = 4, tree_type: str = "max", attr_idx: int = 0,
+                    pixdim=(1.0, 1.0, 1.0)):
+        """The reference's distributed build on one GPU (dcf_tree_build_tiled): the image cut into
+        grid = (tiles along x, along y[, along z]) tiles with a one-pixel overlap, every tile built on its
+        own and the tiles' boundary trees merged; the result equals build()'s tree."""
+        a = np.ascontiguousarray(img)
+        g = (C.c_int * 3)(*(tuple(int(v) for v in grid) + (1, 1, 1))[:3])
+        return self._build(_ptr(a), _dtype_code(a.dtype), _dims_of(a.shape), a.shape, connectivity, tree_type,
+                           attr_idx, pixdim, 0, keep=a, grid=g)
+
+    def build_tiled_device(self, ptr: int, dtype, shape, grid, connectivity: int = 4, tree_type: str = "max",
+                           attr_idx: int = 0, pixdim=(1.0, 1.0, 1.0)):
+        """build_tiled on an image already resident in HBM."""
+        g = (C.c_int * 3)(*(tuple(int(v) for v in grid) + (1, 1, 1))[:3])
+        return self._build(C.c_void_p(ptr), _dtype_code(dtype), _dims_of(shape), tuple(shape), connectivity, tree_type,
+                           attr_idx, pixdim, DCF_DEVICE_PTR, keep=None, grid=g)
+
+    def _build(self, p, code, dims, shape, connectivity, tree_type, attr_idx, pixdim, flags, keep, grid=None):
         tt = _tree_type_code(tree_type)
         d = (C.c_uint64 * 3)(*dims)
         pd = _pixdim3(pixdim)
         h = C.c_void_p()
-        rc = lib().dcf_tree_build(self._h, p, code, d, int(connectivity), tt,
-                                  int(attr_idx), _ptr(pd), flags, C.byref(h))
+        if grid is not None:
+            rc = lib().dcf_tree_build_tiled(self._h, p, code, d, int(connectivity), tt, int(attr_idx), _ptr(pd), grid,
+                                            flags, C.byref(h))
+        else:
+            rc = lib().dcf_tree_build(self._h, p, code, d, int(connectivity), tt,
+                                      int(attr_idx), _ptr(pd), flags, C.byref(h))
         self._check(rc)
         t = Tree(self, h, shape, code, keep)
         self._trees.add(t)

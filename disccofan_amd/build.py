@@ -30,7 +30,7 @@ HIP_FLAGS = [
     "-Wno-unused-command-line-argument",
 ]
 
-SOURCES = ["dcf_build.hip", "dcf_flood.hip", "dcf_ops.hip", "dcf_api.cpp"]
+SOURCES = ["dcf_build.hip", "dcf_flood.hip", "dcf_ops.hip", "dcf_tiles.hip", "dcf_api.cpp"]
 
 
 def _run(cmd):
@@ -48,7 +48,7 @@ def build_synth() -> str:
     return out
 
 
-def build_dcf(jobs: int = 3) -> str:
+def build_dcf(jobs: int = 4) -> str:
     os.makedirs(LIBDIR, exist_ok=True)
     os.makedirs(OBJDIR, exist_ok=True)
     objs = []

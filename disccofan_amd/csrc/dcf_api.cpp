@@ -367,12 +367,13 @@ extern "C" int dcf_tree_free(dcf_tree *t)
     return DCF_OK;
 }
 
-extern "C" int dcf_tree_build(dcf_ctx *ctx, const void *img, dcf_dtype dtype, const uint64_t dims[3], int connectivity,
-                              int tree_type, int attr_idx, const float pixdim[3], unsigned flags, dcf_tree **out)
-try {
-    if (!ctx) return DCF_EINVAL;
-    if (!out || !img || !dims) return dcf_set_error(ctx, DCF_EINVAL, "null argument");
-    *out = nullptr;
+/* the argument checks and set-up shared by the single and the tiled build: a tree holding the image
+ * (uploaded unless DCF_DEVICE_PTR), its parent and pixel -> node arrays and, for a min-tree, its grey
+ * extremes */
+static int tree_setup(dcf_ctx *ctx, const void *img, dcf_dtype dtype, const uint64_t dims[3], int connectivity, int tree_type,
+                      int attr_idx, const float pixdim[3], unsigned flags, TreePtr &t)
+{
+    if (!img || !dims) return dcf_set_error(ctx, DCF_EINVAL, "null argument");
     if (dtype != DCF_U8 && dtype != DCF_U16 && dtype != DCF_F32) return dcf_set_error(ctx, DCF_EINVAL, "bad dtype %d", (int)dtype);
     if (dims[0] == 0 || dims[1] == 0 || dims[2] == 0) return dcf_set_error(ctx, DCF_EINVAL, "empty image");
     const uint64_t n = pixel_count(dims);
@@ -393,7 +394,7 @@ try {
         return dcf_set_error(ctx, DCF_EUNSUPPORTED, "grey-weighted attributes on float images are not implemented");
     if (hipSetDevice(ctx->device) != hipSuccess) return dcf_set_error(ctx, DCF_EDEVICE, "hipSetDevice");
 
-    TreePtr t(new dcf_tree());
+    t.reset(new dcf_tree());
     t->ctx = ctx;
     memcpy(t->dims, dims, sizeof t->dims);
     t->n = (int64_t)n;
@@ -401,7 +402,6 @@ try {
     t->tree_min = tree_type == DCF_MIN_TREE;
     t->dtype = dtype;
     if (pixdim) memcpy(t->pixdim, pixdim, sizeof t->pixdim);
-    int rc = DCF_OK;
     size_t isz = dtype_size(dtype);
     if (flags & DCF_DEVICE_PTR) {
         t->img = (void *)img;
@@ -419,8 +419,47 @@ try {
     t->p2n = (uint32_t *)dcf_alloc(ctx, (size_t)n * 4);
     if (!t->gmax || !t->counters || !t->parent || !t->p2n) return dcf_set_error(ctx, DCF_ENOMEM, "tree arrays");
     /* find_extremas (preprocessing.c:7-30): only a min-tree reads gmax */
-    if (t->tree_min && (rc = dcf_ops_minmax(t.get())) != DCF_OK) return rc;
+    if (t->tree_min) return dcf_ops_minmax(t.get());
+    return DCF_OK;
+}
+
+extern "C" int dcf_tree_build(dcf_ctx *ctx, const void *img, dcf_dtype dtype, const uint64_t dims[3], int connectivity,
+                              int tree_type, int attr_idx, const float pixdim[3], unsigned flags, dcf_tree **out)
+try {
+    if (!ctx) return DCF_EINVAL;
+    if (!out) return dcf_set_error(ctx, DCF_EINVAL, "null argument");
+    *out = nullptr;
+    TreePtr t;
+    int rc = tree_setup(ctx, img, dtype, dims, connectivity, tree_type, attr_idx, pixdim, flags, t);
+    if (rc != DCF_OK) return rc;
     if ((rc = dcf_build_tree(t.get(), dcf_group_of(attr_idx))) != DCF_OK) return rc;
+    ctx->trees.insert(t.get());
+    *out = t.release();
+    return DCF_OK;
+} catch (const std::exception &e) {
+    return dcf_set_error(ctx, DCF_ENOMEM, "out of host memory (%s)", e.what());
+}
+
+extern "C" int dcf_tree_build_tiled(dcf_ctx *ctx, const void *img, dcf_dtype dtype, const uint64_t dims[3], int connectivity,
+                                    int tree_type, int attr_idx, const float pixdim[3], const int grid[3], unsigned flags,
+                                    dcf_tree **out)
+try {
+    if (!ctx) return DCF_EINVAL;
+    if (!out || !grid) return dcf_set_error(ctx, DCF_EINVAL, "null argument");
+    *out = nullptr;
+    if (dims) {
+        for (int a = 0; a < 3; a++)
+            if (grid[a] < 1 || (uint64_t)grid[a] > dims[a])
+                return dcf_set_error(ctx, DCF_EINVAL, "grid %dx%dx%d does not fit the image %llux%llux%llu", grid[0], grid[1], grid[2],
+                                     (unsigned long long)dims[0], (unsigned long long)dims[1], (unsigned long long)dims[2]);
+        if ((int64_t)grid[0] * grid[1] * grid[2] > 4096) return dcf_set_error(ctx, DCF_EINVAL, "more than 4096 tiles");
+    }
+    const int grp = dcf_group_of(attr_idx);
+    if (grp == 1) return dcf_set_error(ctx, DCF_EUNSUPPORTED, "the extent attributes (group 1) are not merged across tiles");
+    TreePtr t;
+    int rc = tree_setup(ctx, img, dtype, dims, connectivity, tree_type, attr_idx, pixdim, flags, t);
+    if (rc != DCF_OK) return rc;
+    if ((rc = dcf_tiled_build(t.get(), grid, grp, 0)) != DCF_OK) return rc;
     ctx->trees.insert(t.get());
     *out = t.release();
     return DCF_OK;

@@ -233,6 +233,30 @@ static int flood_and_gather(dcf_tree *t, int grp, bool renumber)
 
 int dcf_build_tree(dcf_tree *t, int grp) { return flood_and_gather(t, grp, true); }
 
+/* node arrays of a tree whose canonical parent array is already in t->parent (a tiled build's merged
+ * tree): levelroot flags, node numbering, node parent / grey; the flags go to the caller (*flag_out,
+ * released with dcf_release) */
+int dcf_tree_nodes_from_parent(dcf_tree *t, uint32_t **flag_out)
+{
+    dcf_ctx *ctx = t->ctx;
+    const int64_t n = t->n;
+    *flag_out = nullptr;
+    uint32_t *flag = (uint32_t *)dcf_alloc(ctx, (size_t)n * 4);
+    if (!flag) return dcf_set_error(ctx, DCF_ENOMEM, "levelroot flags");
+    ImgView iv = view_of(t);
+    DCF_LAUNCH(ctx, "k_flags", k_flags, dim3(nblk(n, 256)), dim3(256), 0, iv, t->parent, flag, n);
+    /* compact_nodes only reads the flags */
+    const int rc = compact_nodes(t, flag);
+    if (rc != DCF_OK) {
+        dcf_release(ctx, flag);
+        return rc;
+    }
+    *flag_out = flag;
+    return DCF_OK;
+}
+
+int dcf_alloc_attr(dcf_ctx *ctx, DevAttr &a, int grp, int64_t K) { return alloc_attr(ctx, a, grp, K); }
+
 int dcf_accumulate_group(dcf_tree *t, int grp) { return flood_and_gather(t, grp, false); }
 
 void dcf_free_attr(dcf_ctx *ctx, DevAttr &a)

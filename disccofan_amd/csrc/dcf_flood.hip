@@ -298,6 +298,9 @@ struct LevelArgs {
     int nbatch;
     /* zone pass: the level lists hold zone representatives, whose records carry their zone's totals */
     int zone;
+    /* coordinate offset of the image in a larger one (a tile of a tiled build): the attribute sums use
+     * x + ox, y + oy, z + oz, as the reference adds its tile offsets (attributes.c:1219) */
+    int ox, oy, oz;
 };
 
 /* local coordinates of pixel p and the dimensions of its image (pixel ids are below 2^30, so the
@@ -856,7 +859,7 @@ template <int G> __device__ __forceinline__ void contrib_own(const LevelArgs &a,
         return;
     } else {
         const PixGeo pg = pix_geo(a, p);
-        const int x = pg.x, y = pg.y, z = pg.z;
+        const int x = pg.x + a.ox, y = pg.y + a.oy, z = pg.z + a.oz;
         if constexpr (G == 1) {
             c.mn[0] = (uint32_t)x; c.mn[1] = (uint32_t)y; c.mn[2] = (uint32_t)z;
             c.mx[0] = (uint32_t)x + 1; c.mx[1] = (uint32_t)y + 1; c.mx[2] = (uint32_t)z + 1;
@@ -1694,6 +1697,9 @@ int dcf_flood_build(dcf_tree *t, int grp)
     la.bimg = t->bimg;
     la.nbatch = t->nbatch;
     la.zone = 0;
+    la.ox = t->off[0];
+    la.oy = t->off[1];
+    la.oz = t->off[2];
     la.cnode = nullptr; la.lift = nullptr; la.cnkey = nullptr; la.nlift = 0;
     int rc = DCF_OK;
     /* the keys, in passes of 8-bit digits: u8 one pass, u16 two; a float image is built on the dense

@@ -156,6 +156,9 @@ struct dcf_tree {
     int nbatch = 0;
     int batch3d = 0;
     const double *nlam = nullptr;
+    /* a tile of a tiled build: its origin in the whole image, added to the coordinates the attributes
+     * sum (so they are those of the whole image) */
+    int off[3] = { 0, 0, 0 };
 };
 
 /* internal filter flag (never set by callers): a min-tree's output stays in the inverted grey
@@ -196,6 +199,14 @@ int dcf_group_of(int attr_idx);
 /* ---- launchers (dcf_build.hip) ---------------------------------------------------------------- */
 int dcf_build_tree(dcf_tree *t, int grp);      /* parent, nodes and the attributes of group grp */
 int dcf_accumulate_group(dcf_tree *t, int grp); /* attributes of group grp on the existing tree */
+/* node arrays from t->parent (levelroot flags handed to the caller) */
+int dcf_tree_nodes_from_parent(dcf_tree *t, uint32_t **flag_out);
+int dcf_alloc_attr(dcf_ctx *ctx, DevAttr &a, int grp, int64_t K);
+
+/* ---- launchers (dcf_tiles.hip) ---------------------------------------------------------------- */
+/* tiled build: t holds the whole image (img, dims, conn, tree type, gmax of a min-tree, parent and p2n
+ * arrays allocated); grid = tiles per axis.  Fills parent, the node arrays and group grp's attributes. */
+int dcf_tiled_build(dcf_tree *t, const int grid[3], int grp, int schedule);
 
 /* ---- launchers (dcf_flood.hip) ---------------------------------------------------------------- */
 int dcf_flood_build(dcf_tree *t, int grp);      /* canonical parent + pixel-indexed attributes */

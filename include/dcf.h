@@ -95,6 +95,22 @@ int dcf_tree_build(dcf_ctx *ctx, const void *img, dcf_dtype dtype, const uint64_
                    int tree_type, int attr_idx, const float pixdim[3], unsigned flags, dcf_tree **out);
 int dcf_tree_free(dcf_tree *tree);
 
+/* ---- tiled build with boundary-tree merging (the reference's distributed path on one GPU) ------- *
+ * Replaces the distributed build of the reference: the tile decomposition with a one-pixel overlap
+ * (read_* in src/sources/io.c:860-882), build_local_tree per tile (flood.c:175), create_boundary
+ * (boundary.c:1553-1707), merge_boundary_trees / merge_branch (boundary.c:427-724), the update and
+ * correction of the local trees (boundary.c:1201-1399).  The image is cut into grid[0] x grid[1] x
+ * grid[2] tiles (per axis near-equal, the first tiles one pixel longer when the size does not divide,
+ * each extended one pixel into every neighbour), every tile's tree is built on its own, and their
+ * boundary trees (the nodes of the overlap pixels and all their ancestors) are merged on the GPU.  The
+ * result is the tree dcf_tree_build gives for the whole image: the same canonical parent array, node
+ * arrays and attributes, for every filter, check and spectrum.  Arguments as dcf_tree_build; grid[a]
+ * must be at least 1 and at most dims[a] (grid[2] = 1 for a 2D image), 4096 tiles at most.  The
+ * extent attributes (1-5, group 1) are DCF_EUNSUPPORTED here. */
+int dcf_tree_build_tiled(dcf_ctx *ctx, const void *img, dcf_dtype dtype, const uint64_t dims[3], int connectivity,
+                         int tree_type, int attr_idx, const float pixdim[3], const int grid[3], unsigned flags,
+                         dcf_tree **out);
+
 /* Re-accumulate the attributes of attr_idx's group on the existing tree when the group differs
  * from the accumulated one (flood_attributes, flood.c:184; group reuse disccofan.c:179-200). */
 int dcf_tree_accumulate(dcf_tree *tree, int attr_idx);
