@@ -188,6 +188,11 @@ try {
     if (const char *z = getenv("DCF_ZONES")) c->zones = atoi(z);
     if (const char *z = getenv("DCF_TARGETS")) c->targets = atoi(z);
     if (const char *z = getenv("DCF_RECORDS")) c->records = atoi(z);
+    if (const char *z = getenv("DCF_GRAPH")) c->graph = atoi(z) != 0;
+    if (const char *z = getenv("DCF_FIX_BLOCKS")) {
+        const int v = atoi(z);
+        if (v >= 1 && v <= 65536) c->fix_blocks = v;
+    }
     if (const char *e = getenv("DCF_ZONE_TILE2D")) {
         int a = 0, b = 0;
         if (sscanf(e, "%dx%d", &a, &b) == 2 && a > 0 && b > 0 && a <= ZONE_TILE_MAX && b <= ZONE_TILE_MAX &&
@@ -239,6 +244,13 @@ extern "C" int dcf_ctx_set_stream(dcf_ctx *ctx, void *stream)
 {
     if (!ctx) return DCF_EINVAL;
     ctx->stream = stream ? (hipStream_t)stream : ctx->own;
+    return DCF_OK;
+}
+
+extern "C" int dcf_ctx_set_graph(dcf_ctx *ctx, int on)
+{
+    if (!ctx) return DCF_EINVAL;
+    ctx->graph = on != 0;
     return DCF_OK;
 }
 

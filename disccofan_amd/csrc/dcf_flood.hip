@@ -1620,58 +1620,90 @@ static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAtt
     int rc = alloc_attr_n(ctx, &sc, P, grp, nel);
     if (rc || (rc = clear_attr(ctx, P, nel))) return rc;
     DCF_LAUNCH(ctx, "k_fill", k_fill_u32f, dim3(2), dim3(256), 0, la.fcount, 0u, (int64_t)(NLEV + 4));
-    int ord = 0;
-    for (int lev = NLEV - 1; lev >= 0; lev--) {
-        uint32_t cnt = h[lev];
-        if (!cnt) continue;
-        const uint32_t *lvl = order + off[lev];
-        if (zones) {
-            if (ecnt[lev])
-                DCF_LAUNCH(ctx, "k_union", k_union_edges, dim3(nblk(ecnt[lev], 256)), dim3(256), 0, la,
-                           (const uint2 *)(edges + bucket_h[lev]), ecnt[lev], (uint32_t)lev);
-        } else if (tgt) {
-            const uint32_t *tl = tgt + (size_t)off[lev] * la.nb.n;
-            switch (la.nb.n) {
-            case 4: DCF_LAUNCH(ctx, "k_union", k_union_tgt<4>, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, tl, cnt, (uint32_t)lev); break;
-            case 6: DCF_LAUNCH(ctx, "k_union", k_union_tgt<6>, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, tl, cnt, (uint32_t)lev); break;
-            case 8: DCF_LAUNCH(ctx, "k_union", k_union_tgt<8>, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, tl, cnt, (uint32_t)lev); break;
-            default: DCF_LAUNCH(ctx, "k_union", k_union_tgt<26>, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, tl, cnt, (uint32_t)lev); break;
+    /* the level loop: its launches are fixed by the histogram already on the host.  With ctx->graph on
+     * (and no per-kernel events) they are captured on the context's own stream into one HIP graph,
+     * which is then launched on the working stream */
+    auto level_loop = [&]() -> int {
+        int ord = 0;
+        for (int lev = NLEV - 1; lev >= 0; lev--) {
+            uint32_t cnt = h[lev];
+            if (!cnt) continue;
+            const uint32_t *lvl = order + off[lev];
+            if (zones) {
+                if (ecnt[lev])
+                    DCF_LAUNCH(ctx, "k_union", k_union_edges, dim3(nblk(ecnt[lev], 256)), dim3(256), 0, la,
+                               (const uint2 *)(edges + bucket_h[lev]), ecnt[lev], (uint32_t)lev);
+            } else if (tgt) {
+                const uint32_t *tl = tgt + (size_t)off[lev] * la.nb.n;
+                switch (la.nb.n) {
+                case 4: DCF_LAUNCH(ctx, "k_union", k_union_tgt<4>, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, tl, cnt, (uint32_t)lev); break;
+                case 6: DCF_LAUNCH(ctx, "k_union", k_union_tgt<6>, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, tl, cnt, (uint32_t)lev); break;
+                case 8: DCF_LAUNCH(ctx, "k_union", k_union_tgt<8>, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, tl, cnt, (uint32_t)lev); break;
+                default: DCF_LAUNCH(ctx, "k_union", k_union_tgt<26>, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, tl, cnt, (uint32_t)lev); break;
+                }
+            } else switch (la.nb.n) {
+            case 4: DCF_LAUNCH(ctx, "k_union", k_union<4>, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, cnt, (uint32_t)lev, (uint32_t)kc); break;
+            case 6: DCF_LAUNCH(ctx, "k_union", k_union<6>, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, cnt, (uint32_t)lev, (uint32_t)kc); break;
+            case 8: DCF_LAUNCH(ctx, "k_union", k_union<8>, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, cnt, (uint32_t)lev, (uint32_t)kc); break;
+            default: DCF_LAUNCH(ctx, "k_union", k_union<26>, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, cnt, (uint32_t)lev, (uint32_t)kc); break;
             }
-        } else switch (la.nb.n) {
-        case 4: DCF_LAUNCH(ctx, "k_union", k_union<4>, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, cnt, (uint32_t)lev, (uint32_t)kc); break;
-        case 6: DCF_LAUNCH(ctx, "k_union", k_union<6>, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, cnt, (uint32_t)lev, (uint32_t)kc); break;
-        case 8: DCF_LAUNCH(ctx, "k_union", k_union<8>, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, cnt, (uint32_t)lev, (uint32_t)kc); break;
-        default: DCF_LAUNCH(ctx, "k_union", k_union<26>, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, cnt, (uint32_t)lev, (uint32_t)kc); break;
+            /* the grid covers the level's pixels and, striding, the roots absorbed at the level */
+            unsigned rb = nblk(cnt, FB), ab = nblk((int64_t)cnt * la.nb.n / 4, FB);
+            if (ab > 1024) ab = 1024;
+            if (ab > rb) rb = ab;
+    #define DCF_FINAL(GV)                                                                                                        \
+        if (la.pass == 1 && !zones)                                                                                              \
+            DCF_LAUNCH(ctx, "k_final_pixels", (k_final_pixels<GV, true>), dim3(rb), dim3(FB), 0, la, lvl, cnt, (uint32_t)lev, A, S, P, \
+                       par, spar, nabs + 1 + ord);                                                                              \
+        else                                                                                                                     \
+            DCF_LAUNCH(ctx, "k_final_pixels", (k_final_pixels<GV, false>), dim3(rb), dim3(FB), 0, la, lvl, cnt, (uint32_t)lev, A, S, P, \
+                       par, spar, nabs + 1 + ord)
+            switch (grp) {
+            case 0: DCF_FINAL(0); break;
+            case 1: DCF_FINAL(1); break;
+            case 2: DCF_FINAL(2); break;
+            case 3: DCF_FINAL(3); break;
+            default: DCF_FINAL(4); break;
+            }
+    #undef DCF_FINAL
+            /* the sets to fix are far fewer than the level's elements: a small strided grid */
+            unsigned fb = nblk(cnt, 256);
+            if (fb > (unsigned)ctx->fix_blocks) fb = (unsigned)ctx->fix_blocks;
+            switch (grp) {
+            case 0: DCF_LAUNCH(ctx, "k_fix", k_fix<0>, dim3(fb), dim3(256), 0, la, (uint32_t)ord, (uint32_t)lev, A, S, P, par, spar); break;
+            case 1: DCF_LAUNCH(ctx, "k_fix", k_fix<1>, dim3(fb), dim3(256), 0, la, (uint32_t)ord, (uint32_t)lev, A, S, P, par, spar); break;
+            case 2: DCF_LAUNCH(ctx, "k_fix", k_fix<2>, dim3(fb), dim3(256), 0, la, (uint32_t)ord, (uint32_t)lev, A, S, P, par, spar); break;
+            case 3: DCF_LAUNCH(ctx, "k_fix", k_fix<3>, dim3(fb), dim3(256), 0, la, (uint32_t)ord, (uint32_t)lev, A, S, P, par, spar); break;
+            default: DCF_LAUNCH(ctx, "k_fix", k_fix<4>, dim3(fb), dim3(256), 0, la, (uint32_t)ord, (uint32_t)lev, A, S, P, par, spar); break;
+            }
+            ord++;
         }
-        /* the grid covers the level's pixels and, striding, the roots absorbed at the level */
-        unsigned rb = nblk(cnt, FB), ab = nblk((int64_t)cnt * la.nb.n / 4, FB);
-        if (ab > 1024) ab = 1024;
-        if (ab > rb) rb = ab;
-#define DCF_FINAL(GV)                                                                                                        \
-    if (la.pass == 1 && !zones)                                                                                              \
-        DCF_LAUNCH(ctx, "k_final_pixels", (k_final_pixels<GV, true>), dim3(rb), dim3(FB), 0, la, lvl, cnt, (uint32_t)lev, A, S, P, \
-                   par, spar, nabs + 1 + ord);                                                                              \
-    else                                                                                                                     \
-        DCF_LAUNCH(ctx, "k_final_pixels", (k_final_pixels<GV, false>), dim3(rb), dim3(FB), 0, la, lvl, cnt, (uint32_t)lev, A, S, P, \
-                   par, spar, nabs + 1 + ord)
-        switch (grp) {
-        case 0: DCF_FINAL(0); break;
-        case 1: DCF_FINAL(1); break;
-        case 2: DCF_FINAL(2); break;
-        case 3: DCF_FINAL(3); break;
-        default: DCF_FINAL(4); break;
+        return DCF_OK;
+    };
+    const bool graph = ctx->graph && !ctx->profiling;
+    if (!graph) {
+        if ((rc = level_loop()) != DCF_OK) return rc;
+    } else {
+        hipStream_t work = ctx->stream;
+        /* the capture stream is idle: nothing it records runs before the graph is launched on `work` */
+        DCF_HIP(ctx, hipStreamBeginCapture(ctx->own, hipStreamCaptureModeThreadLocal));
+        ctx->stream = ctx->own;
+        rc = level_loop();
+        ctx->stream = work;
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(ctx->own, &g);
+        if (rc != DCF_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
         }
-#undef DCF_FINAL
-        unsigned fb = nblk(cnt, 256);
-        if (fb > 1024) fb = 1024;
-        switch (grp) {
-        case 0: DCF_LAUNCH(ctx, "k_fix", k_fix<0>, dim3(fb), dim3(256), 0, la, (uint32_t)ord, (uint32_t)lev, A, S, P, par, spar); break;
-        case 1: DCF_LAUNCH(ctx, "k_fix", k_fix<1>, dim3(fb), dim3(256), 0, la, (uint32_t)ord, (uint32_t)lev, A, S, P, par, spar); break;
-        case 2: DCF_LAUNCH(ctx, "k_fix", k_fix<2>, dim3(fb), dim3(256), 0, la, (uint32_t)ord, (uint32_t)lev, A, S, P, par, spar); break;
-        case 3: DCF_LAUNCH(ctx, "k_fix", k_fix<3>, dim3(fb), dim3(256), 0, la, (uint32_t)ord, (uint32_t)lev, A, S, P, par, spar); break;
-        default: DCF_LAUNCH(ctx, "k_fix", k_fix<4>, dim3(fb), dim3(256), 0, la, (uint32_t)ord, (uint32_t)lev, A, S, P, par, spar); break;
-        }
-        ord++;
+        if (ec != hipSuccess) return dcf_set_error(ctx, DCF_EDEVICE, "level loop capture: %s", hipGetErrorString(ec));
+        hipGraphExec_t ge = nullptr;
+        hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        if (e == hipSuccess) e = hipGraphLaunch(ge, work);
+        if (e == hipSuccess) e = hipStreamSynchronize(work);
+        if (ge) (void)hipGraphExecDestroy(ge);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return dcf_set_error(ctx, DCF_EDEVICE, "level loop graph: %s", hipGetErrorString(e));
     }
     DCF_LAUNCH(ctx, "k_held_resolve", k_held_resolve, dim3(nblk(n + kc, 256)), dim3(256), 0, par, n, spar, kc);
     if (zones)

@@ -93,6 +93,10 @@ struct dcf_ctx {
     /* zone graph of the first pass (DCF_ZONES): 0 off, 1 every single-image first pass,
      * 2 the coarse pass of u16 images and 2D u8 images */
     int zones = 2;
+    /* grid cap of the per-level k_fix launch (DCF_FIX_BLOCKS) */
+    int fix_blocks = 1024;
+    /* capture each pass's level loop into a HIP graph (DCF_GRAPH, dcf_ctx_set_graph) */
+    int graph = 0;
     /* device address of the allocation-free entry points' scratch array on this context's device */
     void *noalloc_scratch = nullptr;
 };

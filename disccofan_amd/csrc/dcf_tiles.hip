@@ -197,19 +197,19 @@ __global__ __launch_bounds__(256) void k_tm_bid(const uint32_t *__restrict__ mar
     bid[k] = mark[k] ? base + bid[k] : TM_NONE;
 }
 
-/* the boundary nodes of one tile: key, levelroot index in the whole image, parent boundary node */
-__global__ __launch_bounds__(256) void k_tm_bnodes(TileDev t, GridGeo G, ImgView iv, uint32_t *__restrict__ bkey,
-                                                   uint32_t *__restrict__ blr, uint32_t *__restrict__ bpar)
+/* the boundary nodes of one tile: levelroot index in the whole image, and key << 32 | parent boundary node */
+__global__ __launch_bounds__(256) void k_tm_bnodes(TileDev t, GridGeo G, ImgView iv, uint32_t *__restrict__ blr,
+                                                   unsigned long long *__restrict__ bw)
 {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= t.K) return;
     const uint32_t v = t.bid[k];
     if (v == TM_NONE) return;
     const int64_t p = tile_global(t, G, t.n2p[k]);
-    bkey[v] = tm_key(iv, p);
+    const uint32_t key = tm_key(iv, p);
     blr[v] = (uint32_t)p;
     const int32_t q = t.npar[k];
-    bpar[v] = q >= 0 ? t.bid[q] : TM_NONE;
+    bw[v] = ((unsigned long long)key << 32) | (q >= 0 ? t.bid[q] : TM_NONE);
 }
 
 /* ------------------------------------------------------------------------------------------------ */
@@ -358,77 +358,94 @@ __global__ __launch_bounds__(256) void k_tm_pairs(const TileDev *__restrict__ ti
 }
 
 /* boundary node order: x above y when its key is higher, or equal with a larger (levelroot, id) --
- * the class of equal-key nodes keeps the smallest levelroot index on top */
+ * the class of equal-key nodes keeps the smallest levelroot index on top.  Every node's key and parent
+ * entry share one 64-bit word (key << 32 | parent), so one load brings both and the parent moves by a
+ * 64-bit compare-and-swap. */
 struct ZipView {
-    uint32_t *bpar;
-    const uint32_t *bkey;
+    unsigned long long *bw;
     const uint32_t *blr;
     uint32_t *fail;
 };
 
-__device__ __forceinline__ uint32_t zp_par(const ZipView &z, uint32_t x)
+__device__ __forceinline__ unsigned long long zp_word(const ZipView &z, uint32_t x)
 {
-    return __hip_atomic_load(&z.bpar[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(&z.bw[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ __forceinline__ bool zp_above(const ZipView &z, uint32_t x, uint32_t y)
+__device__ __forceinline__ uint32_t zw_key(unsigned long long w) { return (uint32_t)(w >> 32); }
+__device__ __forceinline__ uint32_t zw_par(unsigned long long w) { return (uint32_t)w; }
+
+__device__ __forceinline__ bool zp_above(const ZipView &z, uint32_t x, uint32_t kx, uint32_t y, uint32_t ky)
 {
-    const uint32_t kx = z.bkey[x], ky = z.bkey[y];
     if (kx != ky) return kx > ky;
     const uint32_t lx = z.blr[x], ly = z.blr[y];
     return lx != ly ? lx > ly : x > y;
 }
 
-/* the top of x's class of equal-key nodes (its levelroot in the merged tree so far) */
-__device__ __forceinline__ uint32_t zp_lr(const ZipView &z, uint32_t x, uint32_t &steps)
+/* the top of x's class of equal-key nodes (its levelroot in the merged tree so far), with its word */
+__device__ __forceinline__ uint32_t zp_lr(const ZipView &z, uint32_t x, unsigned long long &wx, uint32_t &steps)
 {
-    const uint32_t kx = z.bkey[x];
+    wx = zp_word(z, x);
+    const uint32_t k = zw_key(wx);
     for (;;) {
-        const uint32_t p = zp_par(z, x);
-        if (p == TM_NONE || z.bkey[p] != kx || ++steps > TM_SPIN_LIMIT) return x;
+        const uint32_t p = zw_par(wx);
+        if (p == TM_NONE || ++steps > TM_SPIN_LIMIT) return x;
+        const unsigned long long wp = zp_word(z, p);
+        if (zw_key(wp) != k) return x;
         x = p;
+        wx = wp;
     }
 }
 
 /* merge_branch: the ancestor chains of a and b (nodes of one pixel in two tiles, equal keys) become
  * one chain ordered by key, equal keys forming one class.  A node's parent entry only ever moves to a
- * node between it and its old parent (compare-and-swap), so concurrent zips keep a forest whose
- * chains descend in (key, levelroot, id) */
+ * node between it and its old parent (compare-and-swap on its word), so concurrent zips keep a forest
+ * whose chains descend in (key, levelroot, id) */
 __global__ __launch_bounds__(256) void k_tm_zip(ZipView z, const uint2 *__restrict__ pairs, uint32_t np)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
     uint32_t steps = 0;
-    uint32_t a = zp_lr(z, pairs[i].x, steps), b = zp_lr(z, pairs[i].y, steps);
+    unsigned long long wa, wb;
+    uint32_t a = zp_lr(z, pairs[i].x, wa, steps), b = zp_lr(z, pairs[i].y, wb, steps);
     while (a != b) {
         if (++steps > TM_SPIN_LIMIT) {
             atomicOr(z.fail, 1u);
             return;
         }
-        if (zp_above(z, b, a)) {
+        if (zp_above(z, b, zw_key(wb), a, zw_key(wa))) {
             const uint32_t t = a;
             a = b;
             b = t;
+            const unsigned long long tw = wa;
+            wa = wb;
+            wb = tw;
         }
         /* a above b: b's class belongs on a's chain, at or below a */
-        const uint32_t raw = zp_par(z, a);
+        const uint32_t raw = zw_par(wa);
+        uint32_t pa = TM_NONE;
+        unsigned long long wpa = 0;
         if (raw != TM_NONE) {
-            const uint32_t pa = zp_lr(z, raw, steps);
+            pa = zp_lr(z, raw, wpa, steps);
             if (pa == b) return;
-            if (zp_above(z, pa, b)) {
+            if (zp_above(z, pa, zw_key(wpa), b, zw_key(wb))) {
                 a = pa; /* climb */
+                wa = wpa;
                 continue;
             }
         }
         /* b goes between a and its parent */
-        uint32_t exp = raw;
-        if (__hip_atomic_compare_exchange_strong(&z.bpar[a], &exp, b, __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        unsigned long long exp = wa;
+        const unsigned long long want = (wa & 0xFFFFFFFF00000000ull) | b;
+        if (__hip_atomic_compare_exchange_strong(&z.bw[a], &exp, want, __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
             if (raw == TM_NONE) return;
             a = b;
-            b = zp_lr(z, raw, steps);
+            wa = wb;
+            b = pa;
+            wb = wpa;
         } else {
-            a = zp_lr(z, a, steps);
-            b = zp_lr(z, b, steps);
+            a = zp_lr(z, a, wa, steps);
+            b = zp_lr(z, b, wb, steps);
         }
     }
 }
@@ -437,15 +454,18 @@ __global__ __launch_bounds__(256) void k_tm_zip(ZipView z, const uint2 *__restri
 /* classes and their subtree sums                                                                   */
 /* ------------------------------------------------------------------------------------------------ */
 /* cls[v] = top of v's class; flag the tops */
-__global__ __launch_bounds__(256) void k_tm_cls(ZipView z, uint32_t *__restrict__ cls, uint32_t *__restrict__ top, int64_t V)
+__global__ __launch_bounds__(256) void k_tm_cls(ZipView z, uint32_t *__restrict__ cls, uint32_t *__restrict__ top,
+                                                uint32_t *__restrict__ bpar, int64_t V)
 {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
     uint32_t steps = 0;
-    const uint32_t c = zp_lr(z, (uint32_t)v, steps);
+    unsigned long long w;
+    const uint32_t c = zp_lr(z, (uint32_t)v, w, steps);
     if (steps > TM_SPIN_LIMIT) atomicOr(z.fail, 2u);
     cls[v] = c;
     top[v] = c == (uint32_t)v ? 1u : 0u;
+    bpar[v] = zw_par(zp_word(z, (uint32_t)v));
 }
 
 /* every boundary node's base into its class top (in place: tops receive, the others give) */
@@ -825,10 +845,11 @@ int dcf_tiled_build(dcf_tree *T, const int grid[3], int grp, int schedule)
     const int64_t V = vbase[(size_t)ntiles];
 
     /* ---- boundary nodes of all tiles ---- */
-    uint32_t *bkey = bl.get<uint32_t>((size_t)V), *blr = bl.get<uint32_t>((size_t)V), *bpar = bl.get<uint32_t>((size_t)V);
+    uint32_t *blr = bl.get<uint32_t>((size_t)V), *bpar = bl.get<uint32_t>((size_t)V);
+    unsigned long long *bw = bl.get<unsigned long long>((size_t)V);
     uint32_t *fail = bl.get<uint32_t>(4);
     TileDev *dtiles = bl.get<TileDev>((size_t)ntiles);
-    if (!bkey || !blr || !bpar || !fail || !dtiles) return dcf_set_error(ctx, DCF_ENOMEM, "boundary nodes");
+    if (!blr || !bpar || !bw || !fail || !dtiles) return dcf_set_error(ctx, DCF_ENOMEM, "boundary nodes");
     DCF_HIP(ctx, hipMemcpyAsync(dtiles, td.data(), td.size() * sizeof(TileDev), hipMemcpyHostToDevice, ctx->stream));
     DCF_LAUNCH(ctx, "k_fill", k_tm_fill32, dim3(1), dim3(64), 0, fail, 0u, (int64_t)4);
     DevAttr BA;
@@ -843,7 +864,7 @@ int dcf_tiled_build(dcf_tree *T, const int grid[3], int grp, int schedule)
     for (int i = 0; i < BA.nsum; i++) DCF_LAUNCH(ctx, "k_fill", k_tm_fill64, dim3(nblk(V1, 256)), dim3(256), 0, BA.s[i], (int64_t)0, V1);
     for (int i = 0; i < ntiles; i++) {
         const TileDev &d = td[(size_t)i];
-        DCF_LAUNCH(ctx, "k_tm_bnodes", k_tm_bnodes, dim3(nblk(d.K, 256)), dim3(256), 0, d, G, iv, bkey, blr, bpar);
+        DCF_LAUNCH(ctx, "k_tm_bnodes", k_tm_bnodes, dim3(nblk(d.K, 256)), dim3(256), 0, d, G, iv, blr, bw);
         rc = tm_group(grp, [&](auto g) -> int {
             constexpr int GV = decltype(g)::value;
             DCF_LAUNCH(ctx, "k_tm_base", k_tm_base_pix<GV>, dim3(nblk(d.n, 256)), dim3(256), 0, d, G, iv, BA);
@@ -869,13 +890,13 @@ int dcf_tiled_build(dcf_tree *T, const int grid[3], int grp, int schedule)
     DCF_HIP(ctx, hipMemcpyAsync(&np, npairs, 4, hipMemcpyDeviceToHost, ctx->stream));
     DCF_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (np > cap) return dcf_set_error(ctx, DCF_EDEVICE, "pair list overflow (%u > %u)", np, cap);
-    ZipView zv{ bpar, bkey, blr, fail };
+    ZipView zv{ bw, blr, fail };
     if (np) DCF_LAUNCH(ctx, "k_tm_zip", k_tm_zip, dim3(nblk(np, 256)), dim3(256), 0, zv, (const uint2 *)pairs, np);
 
     /* ---- classes, their parents and subtree sums ---- */
     uint32_t *cls = bl.get<uint32_t>((size_t)V), *top = bl.get<uint32_t>((size_t)V), *ci = bl.get<uint32_t>((size_t)V);
     if (!cls || !top || !ci) return dcf_set_error(ctx, DCF_ENOMEM, "classes");
-    if (V) DCF_LAUNCH(ctx, "k_tm_cls", k_tm_cls, dim3(nblk(V, 256)), dim3(256), 0, zv, cls, top, V);
+    if (V) DCF_LAUNCH(ctx, "k_tm_cls", k_tm_cls, dim3(nblk(V, 256)), dim3(256), 0, zv, cls, top, bpar, V);
     uint32_t C = 0;
     if (V) {
         size_t need = 0;

@@ -68,6 +68,11 @@ int dcf_ctx_destroy(dcf_ctx *ctx);
  * NULL restores the context's own stream. */
 int dcf_ctx_set_stream(dcf_ctx *ctx, void *stream);
 void *dcf_ctx_stream(dcf_ctx *ctx);
+/* Build-loop launch mode (no counterpart in the reference): with on != 0, every digit pass of a build
+ * captures its level loop (three launches per grey level) into one HIP graph and launches that; off
+ * (the default) launches the kernels one by one.  Results are identical.  The environment variable
+ * DCF_GRAPH sets the default of new contexts. */
+int dcf_ctx_set_graph(dcf_ctx *ctx, int on);
 const char *dcf_last_error(dcf_ctx *ctx);
 int dcf_ctx_synchronize(dcf_ctx *ctx);
 

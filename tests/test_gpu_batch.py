@@ -204,3 +204,33 @@ def test_batch_errors(ctx):
     items[0].out = out.ctypes.data
     items[0].flags = dcf.DCF_IMAGE_MAX_TREE | dcf.DCF_IMAGE_MIN_TREE
     assert dcf.lib().dcf_batch_filter(ctx._h, items, 1, 4, 0, 0, 0, 0, 0) == -1
+
+
+def test_graph_captured_level_loops(ctx):
+    """dcf_ctx_set_graph: every pass's level loop captured into one HIP graph; three calls in a row, with
+    eager calls between them, give the eager results (batch against the oracle, single trees against
+    the eager build)"""
+    shapes = synth.batch_shapes(20, 64, 400)
+    imgs = [synth.disks(H, W, is16, seed) for (W, H, is16, seed, lam) in shapes]
+    lams = [lam for (W, H, is16, seed, lam) in shapes]
+    vol = synth.spheres(40, 48, 56)
+    u16 = synth.vnoise(300, 410)
+    eager = [OracleTree(img, 4, "max", 0).filter(0, lam, 0).astype(img.dtype) for img, lam in zip(imgs, lams)]
+    e_vol = ctx.build(vol, 6, "max", 0).parent()
+    e_u16 = ctx.build(u16, 8, "min", 17).filter(17, 0.5, 3)
+    try:
+        for rep in range(3):
+            ctx.set_graph(True)
+            say("graph batch replay", rep)
+            outs = ctx.batch_filter(imgs, lams, connectivity=4, tree_type="max")
+            g_vol = ctx.build(vol, 6, "max", 0).parent()
+            g_u16 = ctx.build(u16, 8, "min", 17).filter(17, 0.5, 3)
+            ctx.set_graph(False)
+            for k, (out, want) in enumerate(zip(outs, eager)):
+                assert np.array_equal(out, want), "graph batch image %d (replay %d)" % (k, rep)
+            assert np.array_equal(g_vol, e_vol), "graph volume parent (replay %d)" % rep
+            assert np.array_equal(g_u16, e_u16), "graph u16 min-tree filter (replay %d)" % rep
+            # eager work between the replays
+            assert np.array_equal(ctx.build(vol, 6, "max", 0).parent(), e_vol)
+    finally:
+        ctx.set_graph(False)
