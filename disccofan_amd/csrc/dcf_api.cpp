@@ -189,6 +189,10 @@ try {
     if (const char *z = getenv("DCF_TARGETS")) c->targets = atoi(z);
     if (const char *z = getenv("DCF_RECORDS")) c->records = atoi(z);
     if (const char *z = getenv("DCF_GRAPH")) c->graph = atoi(z) != 0;
+    if (const char *z = getenv("DCF_ZIP_SEGMENTS")) {
+        const int v = atoi(z);
+        if (v >= 1 && v <= 4096) c->zip_segments = v;
+    }
     if (const char *z = getenv("DCF_FIX_BLOCKS")) {
         const int v = atoi(z);
         if (v >= 1 && v <= 65536) c->fix_blocks = v;
@@ -467,7 +471,6 @@ try {
         if ((int64_t)grid[0] * grid[1] * grid[2] > 4096) return dcf_set_error(ctx, DCF_EINVAL, "more than 4096 tiles");
     }
     const int grp = dcf_group_of(attr_idx);
-    if (grp == 1) return dcf_set_error(ctx, DCF_EUNSUPPORTED, "the extent attributes (group 1) are not merged across tiles");
     TreePtr t;
     int rc = tree_setup(ctx, img, dtype, dims, connectivity, tree_type, attr_idx, pixdim, flags, t);
     if (rc != DCF_OK) return rc;

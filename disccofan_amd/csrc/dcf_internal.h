@@ -97,6 +97,8 @@ struct dcf_ctx {
     int fix_blocks = 1024;
     /* capture each pass's level loop into a HIP graph (DCF_GRAPH, dcf_ctx_set_graph) */
     int graph = 0;
+    /* tiled build: zips per identification pair, spread down the key range (DCF_ZIP_SEGMENTS; 1 = one) */
+    int zip_segments = 64;
     /* device address of the allocation-free entry points' scratch array on this context's device */
     void *noalloc_scratch = nullptr;
 };

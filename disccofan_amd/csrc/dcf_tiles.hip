@@ -30,8 +30,8 @@
  *             tour of the classes (list ranking by pointer jumping, prefix sums)
  *   output    the canonical parent array of the whole image, every pixel read from its owner tile,
  *             then node arrays and attributes as dcf_tree_build gives them
- * Attribute groups with sums (area, mean, weighted mean, inertia) merge exactly; the extent group
- * (minima and maxima) is not merged here.
+ * Every attribute group merges exactly: sums through prefix sums over the tour, the extent group's
+ * minima and maxima through range queries over it.
  */
 #include <hipcub/hipcub.hpp>
 
@@ -217,12 +217,26 @@ __global__ __launch_bounds__(256) void k_tm_bnodes(TileDev t, GridGeo G, ImgView
 /* ------------------------------------------------------------------------------------------------ */
 template <int G> struct TmShape {
     static constexpr int NS = G == 2 ? 3 : (G == 3 ? 4 : (G == 4 ? 9 : 0));
+    static constexpr bool EXT = G == 1;
 };
 
-/* adds (area, sums) of every valid lane to record r: one atomic per field when the wave's valid lanes
- * share one record (a node's many owned pixels), else one per lane */
+__device__ __forceinline__ uint32_t tm_wmin(uint32_t v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ uint32_t tm_wmax(uint32_t v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor(v, o, 64));
+    return v;
+}
+
+/* adds (area, sums, extents) of every valid lane to record r: one atomic per field when the wave's valid
+ * lanes share one record (a node's many owned pixels), else one per lane */
 template <int G>
-__device__ __forceinline__ void tm_add(const DevAttr &A, bool valid, uint32_t r, uint32_t area, const int64_t (&s)[TmShape<G>::NS > 0 ? TmShape<G>::NS : 1])
+__device__ __forceinline__ void tm_add(const DevAttr &A, bool valid, uint32_t r, uint32_t area, const int64_t (&s)[TmShape<G>::NS > 0 ? TmShape<G>::NS : 1],
+                                       const uint32_t (&mn)[3], const uint32_t (&mx)[3])
 {
     constexpr int NS = TmShape<G>::NS;
     const uint64_t vm = __ballot(valid);
@@ -245,6 +259,14 @@ __device__ __forceinline__ void tm_add(const DevAttr &A, bool valid, uint32_t r,
             }
             if ((int)(threadIdx.x & 63) == leader && v) atomicAdd((unsigned long long *)&A.s[i][rl], (unsigned long long)v);
         }
+        if constexpr (TmShape<G>::EXT)
+            for (int i = 0; i < 3; i++) {
+                const uint32_t lo = tm_wmin(valid ? mn[i] : 0xFFFFFFFFu), hi = tm_wmax(valid ? mx[i] : 0u);
+                if ((int)(threadIdx.x & 63) == leader) {
+                    atomicMin(&A.mn[i][rl], lo);
+                    atomicMax(&A.mx[i][rl], hi);
+                }
+            }
         return;
     }
     if (!valid) return;
@@ -252,6 +274,11 @@ __device__ __forceinline__ void tm_add(const DevAttr &A, bool valid, uint32_t r,
 #pragma unroll
     for (int i = 0; i < NS; i++)
         if (s[i]) atomicAdd((unsigned long long *)&A.s[i][r], (unsigned long long)s[i]);
+    if constexpr (TmShape<G>::EXT)
+        for (int i = 0; i < 3; i++) {
+            atomicMin(&A.mn[i][r], mn[i]);
+            atomicMax(&A.mx[i][r], mx[i]);
+        }
 }
 
 /* the owned pixels of the tile whose node is a boundary node add their contribution to it (the
@@ -264,6 +291,7 @@ __global__ __launch_bounds__(256) void k_tm_base_pix(TileDev t, GridGeo Gg, ImgV
     bool valid = false;
     uint32_t v = 0;
     int64_t s[NS > 0 ? NS : 1] = {};
+    uint32_t mn[3] = { 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu }, mx[3] = { 0u, 0u, 0u };
     if (pl < t.n) {
         const int64_t row = pl / t.W;
         const int x = t.ox + (int)(pl - row * t.W), y = t.oy + (int)(row % t.H), z = t.oz + (int)(row / t.H);
@@ -271,7 +299,10 @@ __global__ __launch_bounds__(256) void k_tm_base_pix(TileDev t, GridGeo Gg, ImgV
         v = owned ? t.bid[t.p2n[pl]] : TM_NONE;
         valid = v != TM_NONE;
         if (valid) {
-            if constexpr (G == 2) {
+            if constexpr (G == 1) {
+                mn[0] = (uint32_t)x; mn[1] = (uint32_t)y; mn[2] = (uint32_t)z;
+                mx[0] = (uint32_t)x + 1; mx[1] = (uint32_t)y + 1; mx[2] = (uint32_t)z + 1;
+            } else if constexpr (G == 2) {
                 s[0] = x; s[1] = y; s[2] = z;
             } else if constexpr (G == 3) {
                 const int64_t p = (int64_t)x + (int64_t)y * Gg.dim[0] + (int64_t)z * Gg.dim[0] * Gg.dim[1];
@@ -284,7 +315,7 @@ __global__ __launch_bounds__(256) void k_tm_base_pix(TileDev t, GridGeo Gg, ImgV
             }
         }
     }
-    tm_add<G>(BA, valid, v, 1u, s);
+    tm_add<G>(BA, valid, v, 1u, s, mn, mx);
 }
 
 /* non-boundary nodes under a boundary node add their complete records to it (their subtrees lie in
@@ -297,6 +328,7 @@ __global__ __launch_bounds__(256) void k_tm_base_nodes(TileDev t, DevAttr BA)
     bool valid = false;
     uint32_t v = 0, area = 0;
     int64_t s[NS > 0 ? NS : 1] = {};
+    uint32_t mn[3] = { 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu }, mx[3] = { 0u, 0u, 0u };
     if (k < t.K && t.bid[k] == TM_NONE) {
         const int32_t q = t.npar[k];
         v = q >= 0 ? t.bid[q] : TM_NONE;
@@ -305,9 +337,11 @@ __global__ __launch_bounds__(256) void k_tm_base_nodes(TileDev t, DevAttr BA)
             area = t.attr.area[k];
 #pragma unroll
             for (int i = 0; i < NS; i++) s[i] = t.attr.s[i][k];
+            if constexpr (TmShape<G>::EXT)
+                for (int i = 0; i < 3; i++) { mn[i] = t.attr.mn[i][k]; mx[i] = t.attr.mx[i][k]; }
         }
     }
-    tm_add<G>(BA, valid, v, area, s);
+    tm_add<G>(BA, valid, v, area, s, mn, mx);
 }
 
 /* ------------------------------------------------------------------------------------------------ */
@@ -397,17 +431,15 @@ __device__ __forceinline__ uint32_t zp_lr(const ZipView &z, uint32_t x, unsigned
     }
 }
 
-/* merge_branch: the ancestor chains of a and b (nodes of one pixel in two tiles, equal keys) become
- * one chain ordered by key, equal keys forming one class.  A node's parent entry only ever moves to a
- * node between it and its old parent (compare-and-swap on its word), so concurrent zips keep a forest
- * whose chains descend in (key, levelroot, id) */
-__global__ __launch_bounds__(256) void k_tm_zip(ZipView z, const uint2 *__restrict__ pairs, uint32_t np)
+/* merge_branch: the ancestor chains of a and b become one chain ordered by key, equal keys forming one
+ * class.  A node's parent entry only ever moves to a node between it and its old parent (compare-and-swap
+ * on its word), so concurrent zips keep a forest whose chains descend in (key, levelroot, id); a zip stops
+ * where its two chains have already met. */
+__device__ void zip_pair(const ZipView &z, uint32_t a0, uint32_t b0)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= np) return;
     uint32_t steps = 0;
     unsigned long long wa, wb;
-    uint32_t a = zp_lr(z, pairs[i].x, wa, steps), b = zp_lr(z, pairs[i].y, wb, steps);
+    uint32_t a = zp_lr(z, a0, wa, steps), b = zp_lr(z, b0, wb, steps);
     while (a != b) {
         if (++steps > TM_SPIN_LIMIT) {
             atomicOr(z.fail, 1u);
@@ -450,6 +482,53 @@ __global__ __launch_bounds__(256) void k_tm_zip(ZipView z, const uint2 *__restri
     }
 }
 
+/* the first ancestor of x in the tiles' own trees (lift: 2^j-th ancestors, nl levels) whose key is below t */
+__device__ __forceinline__ uint32_t first_below(const ZipView &z, const uint32_t *__restrict__ lift, int nl, uint32_t V, uint32_t x,
+                                                uint32_t t)
+{
+    for (int j = nl - 1; j >= 0; j--) {
+        const uint32_t y = lift[(size_t)j * V + x];
+        if (y != TM_NONE && (uint32_t)(z.bw[y] >> 32) >= t) x = y;
+    }
+    return lift[x];
+}
+
+/* thread (pair i, segment s): s = 0 zips the pair's two nodes; s > 0 zips the first ancestors of each
+ * below the key K (S - s) / S, K the pair's key.  Both zips state facts of the merged tree (every ancestor
+ * of the pixel's nodes lies on one chain), and the segments' zips run concurrently, so a chain thousands of
+ * nodes deep is merged in about 1/S of the steps. */
+__global__ __launch_bounds__(256) void k_tm_zip(ZipView z, const uint2 *__restrict__ pairs, uint32_t np, uint32_t S,
+                                                const uint32_t *__restrict__ lift, int nl, uint32_t V)
+{
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (uint64_t)np * S) return;
+    const uint32_t i = (uint32_t)(t / S), seg = (uint32_t)(t % S);
+    uint32_t a = pairs[i].x, b = pairs[i].y;
+    if (seg) {
+        const uint32_t K = (uint32_t)(z.bw[a] >> 32);
+        const uint32_t th = (uint32_t)((uint64_t)K * (S - seg) / S);
+        a = first_below(z, lift, nl, V, a, th);
+        b = first_below(z, lift, nl, V, b, th);
+        if (a == TM_NONE || b == TM_NONE) return;
+    }
+    zip_pair(z, a, b);
+}
+
+/* lift level 0: every boundary node's parent in its tile's tree (before any zip) */
+__global__ __launch_bounds__(256) void k_tm_lift0(const unsigned long long *__restrict__ bw, uint32_t *__restrict__ lift, int64_t V)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < V) lift[v] = (uint32_t)bw[v];
+}
+
+__global__ __launch_bounds__(256) void k_tm_lift(uint32_t *__restrict__ lift, int j, int64_t V)
+{
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const uint32_t u = lift[(size_t)(j - 1) * V + v];
+    lift[(size_t)j * V + v] = u == TM_NONE ? TM_NONE : lift[(size_t)(j - 1) * V + u];
+}
+
 /* ------------------------------------------------------------------------------------------------ */
 /* classes and their subtree sums                                                                   */
 /* ------------------------------------------------------------------------------------------------ */
@@ -481,6 +560,56 @@ __global__ __launch_bounds__(256) void k_tm_class_base(const uint32_t *__restric
 #pragma unroll
     for (int i = 0; i < NS; i++)
         if (BA.s[i][v]) atomicAdd((unsigned long long *)&BA.s[i][c], (unsigned long long)BA.s[i][v]);
+    if constexpr (TmShape<G>::EXT)
+        for (int i = 0; i < 3; i++) {
+            atomicMin(&BA.mn[i][c], BA.mn[i][v]);
+            atomicMax(&BA.mx[i][c], BA.mx[i][v]);
+        }
+}
+
+/* extents of the classes' subtrees: a sparse table of minima (maxima) over the tour positions, the
+ * class bases at their entries and the neutral value at the exits.  f = i (0..2) for min i, 3 + i for
+ * max i. */
+__global__ __launch_bounds__(256) void k_tm_place_ext(const uint32_t *__restrict__ rank, const uint32_t *__restrict__ top_of,
+                                                      DevAttr BA, int f, uint32_t *__restrict__ st, uint32_t E)
+{
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E || rank[e] >= E) return;
+    const uint32_t pos = E - 1 - rank[e];
+    const uint32_t u = e >> 1;
+    const bool is_max = f >= 3;
+    uint32_t x = is_max ? 0u : 0xFFFFFFFFu;
+    if (!(e & 1) && u + 1 < E / 2) {
+        const uint32_t v = top_of[u];
+        x = is_max ? BA.mx[f - 3][v] : BA.mn[f][v];
+    }
+    st[pos] = x;
+}
+
+__global__ __launch_bounds__(256) void k_tm_st_level(uint32_t *__restrict__ st, uint32_t E, int j, int is_max)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t half = 1u << (j - 1);
+    if ((uint64_t)i + 2 * half > E) return;
+    const uint32_t *prev = st + (size_t)(j - 1) * E;
+    const uint32_t a = prev[i], b = prev[i + half];
+    st[(size_t)j * E + i] = is_max ? max(a, b) : min(a, b);
+}
+
+__global__ __launch_bounds__(256) void k_tm_st_query(const uint32_t *__restrict__ rank, const uint32_t *__restrict__ st, DevAttr MA,
+                                                     int f, uint32_t C)
+{
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= C) return;
+    const uint32_t E = 2 * (C + 1);
+    if (rank[2 * u] >= E || rank[2 * u + 1] >= E) return;
+    const uint32_t pin = E - 1 - rank[2 * u], pout = E - 1 - rank[2 * u + 1];
+    const uint32_t len = pout - pin + 1;
+    const int j = 31 - __clz(len);
+    const uint32_t *lv = st + (size_t)j * E;
+    const uint32_t a = lv[pin], b = lv[pout + 1 - (1u << j)];
+    if (f >= 3) MA.mx[f - 3][u] = max(a, b);
+    else MA.mn[f][u] = min(a, b);
 }
 
 /* per class (numbered by the exclusive scan ci of the tops): its parent class number (C for a root,
@@ -648,11 +777,15 @@ __global__ __launch_bounds__(256) void k_tm_attr_out(const TileDev *__restrict__
         A.area[node] = t.attr.area[k];
 #pragma unroll
         for (int i = 0; i < NS; i++) A.s[i][node] = t.attr.s[i][k];
+        if constexpr (TmShape<G>::EXT)
+            for (int i = 0; i < 3; i++) { A.mn[i][node] = t.attr.mn[i][k]; A.mx[i][node] = t.attr.mx[i][k]; }
     } else {
         const uint32_t u = ci[cls[b]];
         A.area[node] = MA.area[u];
 #pragma unroll
         for (int i = 0; i < NS; i++) A.s[i][node] = MA.s[i][u];
+        if constexpr (TmShape<G>::EXT)
+            for (int i = 0; i < 3; i++) { A.mn[i][node] = MA.mn[i][u]; A.mx[i][node] = MA.mx[i][u]; }
     }
 }
 
@@ -725,6 +858,7 @@ template <class F> int tm_group(int grp, F &&f)
 {
     switch (grp) {
     case 0: return f(std::integral_constant<int, 0>());
+    case 1: return f(std::integral_constant<int, 1>());
     case 2: return f(std::integral_constant<int, 2>());
     case 3: return f(std::integral_constant<int, 3>());
     default: return f(std::integral_constant<int, 4>());
@@ -738,7 +872,6 @@ int dcf_tiled_build(dcf_tree *T, const int grid[3], int grp, int schedule)
     (void)schedule;
     dcf_ctx *ctx = T->ctx;
     const int64_t n = T->n;
-    if (grp == 1) return dcf_set_error(ctx, DCF_EUNSUPPORTED, "the extent attributes (group 1) are not merged across tiles");
     GridGeo G;
     for (int a = 0; a < 3; a++) {
         G.dim[a] = (int)T->dims[a];
@@ -862,6 +995,10 @@ int dcf_tiled_build(dcf_tree *T, const int grid[3], int grp, int schedule)
     const int64_t V1 = V > 0 ? V : 1;
     DCF_LAUNCH(ctx, "k_fill", k_tm_fill32, dim3(nblk(V1, 256)), dim3(256), 0, BA.area, 0u, V1);
     for (int i = 0; i < BA.nsum; i++) DCF_LAUNCH(ctx, "k_fill", k_tm_fill64, dim3(nblk(V1, 256)), dim3(256), 0, BA.s[i], (int64_t)0, V1);
+    for (int i = 0; BA.has_ext && i < 3; i++) {
+        DCF_LAUNCH(ctx, "k_fill", k_tm_fill32, dim3(nblk(V1, 256)), dim3(256), 0, BA.mn[i], 0xFFFFFFFFu, V1);
+        DCF_LAUNCH(ctx, "k_fill", k_tm_fill32, dim3(nblk(V1, 256)), dim3(256), 0, BA.mx[i], 0u, V1);
+    }
     for (int i = 0; i < ntiles; i++) {
         const TileDev &d = td[(size_t)i];
         DCF_LAUNCH(ctx, "k_tm_bnodes", k_tm_bnodes, dim3(nblk(d.K, 256)), dim3(256), 0, d, G, iv, blr, bw);
@@ -891,7 +1028,21 @@ int dcf_tiled_build(dcf_tree *T, const int grid[3], int grp, int schedule)
     DCF_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (np > cap) return dcf_set_error(ctx, DCF_EDEVICE, "pair list overflow (%u > %u)", np, cap);
     ZipView zv{ bw, blr, fail };
-    if (np) DCF_LAUNCH(ctx, "k_tm_zip", k_tm_zip, dim3(nblk(np, 256)), dim3(256), 0, zv, (const uint2 *)pairs, np);
+    if (np) {
+        /* segment zips along the chains, from 2^j-th ancestors in the tiles' own trees */
+        uint32_t S = (uint32_t)std::max(1, ctx->zip_segments);
+        int nl = 1;
+        while (nl < 32 && ((int64_t)1 << nl) < V) nl++;
+        uint32_t *lift = S > 1 ? bl.get<uint32_t>((size_t)V * nl) : nullptr;
+        if (!lift) S = 1;
+        if ((uint64_t)np * S >= ((uint64_t)1 << 31)) S = 1;
+        if (S > 1) {
+            DCF_LAUNCH(ctx, "k_tm_lift", k_tm_lift0, dim3(nblk(V, 256)), dim3(256), 0, (const unsigned long long *)bw, lift, V);
+            for (int j = 1; j < nl; j++) DCF_LAUNCH(ctx, "k_tm_lift", k_tm_lift, dim3(nblk(V, 256)), dim3(256), 0, lift, j, V);
+        }
+        DCF_LAUNCH(ctx, "k_tm_zip", k_tm_zip, dim3(nblk((int64_t)np * S, 256)), dim3(256), 0, zv, (const uint2 *)pairs, np, S,
+                   (const uint32_t *)lift, nl, (uint32_t)V);
+    }
 
     /* ---- classes, their parents and subtree sums ---- */
     uint32_t *cls = bl.get<uint32_t>((size_t)V), *top = bl.get<uint32_t>((size_t)V), *ci = bl.get<uint32_t>((size_t)V);
@@ -958,6 +1109,21 @@ int dcf_tiled_build(dcf_tree *T, const int grid[3], int grp, int schedule)
             DCF_HIP(ctx, hipcub::DeviceScan::InclusiveSum(tmp, need_scan, vals, pre, (int)E, ctx->stream));
             DCF_LAUNCH(ctx, "k_tm_subtree", k_tm_subtree, dim3(nblk(C, 256)), dim3(256), 0, (const uint32_t *)r0, (const int64_t *)pre,
                        (const uint32_t *)top_of, BA, MA, f, C);
+        }
+        if (BA.has_ext) {
+            /* extents: range minima / maxima over each class's span of the tour */
+            int L = 1;
+            while (((uint64_t)1 << L) <= E) L++;
+            uint32_t *st = bl.get<uint32_t>((size_t)E * L);
+            if (!st) return dcf_set_error(ctx, DCF_ENOMEM, "extent table");
+            for (int f = 0; f < 6; f++) {
+                DCF_LAUNCH(ctx, "k_tm_place_ext", k_tm_place_ext, dim3(nblk(E, 256)), dim3(256), 0, (const uint32_t *)r0,
+                           (const uint32_t *)top_of, BA, f, st, E);
+                for (int j = 1; j < L; j++)
+                    DCF_LAUNCH(ctx, "k_tm_st_level", k_tm_st_level, dim3(nblk(E, 256)), dim3(256), 0, st, E, j, f >= 3 ? 1 : 0);
+                DCF_LAUNCH(ctx, "k_tm_st_query", k_tm_st_query, dim3(nblk(C, 256)), dim3(256), 0, (const uint32_t *)r0,
+                           (const uint32_t *)st, MA, f, C);
+            }
         }
     }
 

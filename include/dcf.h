@@ -110,8 +110,7 @@ int dcf_tree_free(dcf_tree *tree);
  * boundary trees (the nodes of the overlap pixels and all their ancestors) are merged on the GPU.  The
  * result is the tree dcf_tree_build gives for the whole image: the same canonical parent array, node
  * arrays and attributes, for every filter, check and spectrum.  Arguments as dcf_tree_build; grid[a]
- * must be at least 1 and at most dims[a] (grid[2] = 1 for a 2D image), 4096 tiles at most.  The
- * extent attributes (1-5, group 1) are DCF_EUNSUPPORTED here. */
+ * must be at least 1 and at most dims[a] (grid[2] = 1 for a 2D image), 4096 tiles at most. */
 int dcf_tree_build_tiled(dcf_ctx *ctx, const void *img, dcf_dtype dtype, const uint64_t dims[3], int connectivity,
                          int tree_type, int attr_idx, const float pixdim[3], const int grid[3], unsigned flags,
                          dcf_tree **out);

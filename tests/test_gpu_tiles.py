@@ -76,10 +76,10 @@ def test_random_2d(ctx, seed, conn):
         grid = GRIDS_2D[int(rng.integers(0, len(GRIDS_2D)))]
         grid = (min(grid[0], W), min(grid[1], H))
         tt = "min" if rng.random() < 0.5 else "max"
-        attr = int(rng.choice([0, 6, 9, 13, 17])) if img.dtype != np.float32 else int(rng.choice([0, 6, 13, 17]))
+        attr = int(rng.choice([0, 1, 3, 6, 9, 13, 17])) if img.dtype != np.float32 else int(rng.choice([0, 2, 6, 13, 17]))
         lv = np.array([0, 1, 2, 4, 8, 16, 64, 256, 1024], np.float32)
         lams = [3.0, 50.0] if attr == 0 else [float(rng.uniform(0.1, 30.0))]
-        compare(ctx, img, grid, conn, tt, attr, lams, lv if attr in (0, 6, 13, 17) else None,
+        compare(ctx, img, grid, conn, tt, attr, lams, lv if attr in (0, 2, 3, 6, 13, 17) else None,
                 label="2d seed %d case %d" % (seed, k))
 
 
@@ -96,7 +96,7 @@ def test_random_3d(ctx, seed, conn):
         grid = GRIDS_3D[int(rng.integers(0, len(GRIDS_3D)))]
         grid = (min(grid[0], W), min(grid[1], H), min(grid[2], D))
         tt = "min" if rng.random() < 0.5 else "max"
-        attr = int(rng.choice([0, 6, 9, 13, 17]))
+        attr = int(rng.choice([0, 1, 5, 6, 9, 13, 17]))
         lams = [4.0, 40.0] if attr == 0 else [float(rng.uniform(0.1, 30.0))]
         compare(ctx, img, grid, conn, tt, attr, lams, np.array([0, 1, 3, 9, 27, 81, 243], np.float32),
                 label="3d seed %d case %d" % (seed, k))
@@ -116,12 +116,24 @@ def test_small_tiles_against_oracle(ctx):
 
 def test_refusals(ctx):
     img = synth.disks(64, 64)
-    with pytest.raises(dcf.DcfError) as e:
-        ctx.build_tiled(img, (65, 1), 4, "max", 0)
-    assert e.value.code == -1
-    with pytest.raises(dcf.DcfError) as e:
-        ctx.build_tiled(img, (2, 2), 4, "max", 1)  # the extent group is not merged
-    assert e.value.code == -4
+    for grid in ((65, 1), (0, 2), (2, 2, 2)):
+        with pytest.raises(dcf.DcfError) as e:
+            ctx.build_tiled(img, grid, 4, "max", 0)
+        assert e.value.code == -1, grid
+
+
+@pytest.mark.parametrize("attr", [1, 2, 3, 4, 5])
+def test_extent_attributes(ctx, attr):
+    """the extent group's minima and maxima merged across tiles (range queries over the class tour)"""
+    img = synth.spheres(40, 36, 44)
+    t = ctx.build(img, 26, "max", attr)
+    s = ctx.build_tiled(img, (2, 3, 2), 26, "max", attr)
+    same(s.parent(), t.parent(), "parent")
+    same(s.check(attr), t.check(attr), "check %d" % attr)
+    o = OracleTree(img, 26, "max", attr)
+    same(s.check(attr), o.check(attr), "check %d vs oracle" % attr)
+    s.free()
+    t.free()
 
 
 @pytest.fixture(scope="module")
