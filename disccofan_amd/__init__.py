@@ -40,7 +40,8 @@ DECISIONS = ["Direct", "Min", "Max", "Subtractive"]
 
 # exported symbols of libdcf.so (kept in sync with include/dcf.h)
 EXPORTS = [
-    "dcf_ctx_create", "dcf_ctx_destroy", "dcf_ctx_set_stream", "dcf_ctx_set_graph", "dcf_ctx_stream", "dcf_last_error",
+    "dcf_ctx_create", "dcf_ctx_destroy", "dcf_ctx_set_stream", "dcf_ctx_set_graph", "dcf_ctx_set_option",
+    "dcf_ctx_stream", "dcf_last_error",
     "dcf_ctx_synchronize", "dcf_ctx_set_profiling", "dcf_ctx_profile_only", "dcf_batch_filter", "dcf_dmp", "dcf_tree_write_csv", "dcf_kernel_stats", "dcf_kernel_stats_reset",
     "dcf_kernel_name", "dcf_tree_build", "dcf_tree_build_tiled", "dcf_tree_free", "dcf_tree_accumulate", "dcf_tree_group",
     "dcf_tree_parent", "dcf_tree_num_nodes", "dcf_filter", "dcf_check", "dcf_spectrum1d", "dcf_spectrum2d",
@@ -100,6 +101,7 @@ def lib():
     L.dcf_ctx_destroy.argtypes = [vp]
     L.dcf_ctx_set_stream.argtypes = [vp, vp]
     L.dcf_ctx_set_graph.argtypes = [vp, i]
+    L.dcf_ctx_set_option.argtypes = [vp, C.c_char_p, C.c_char_p]
     L.dcf_ctx_stream.restype = vp
     L.dcf_ctx_stream.argtypes = [vp]
     L.dcf_last_error.restype = C.c_char_p
@@ -232,6 +234,10 @@ class Context:
 
     def set_stream(self, stream_handle: int | None):
         self._check(lib().dcf_ctx_set_stream(self._h, C.c_void_p(stream_handle) if stream_handle else None))
+
+    def set_option(self, name: str, value):
+        """one tunable of the build by its environment name (dcf_ctx_set_option), e.g. ("DCF_ZONE_TILE2D", "32x32")"""
+        self._check(lib().dcf_ctx_set_option(self._h, name.encode(), str(value).encode()))
 
     def set_graph(self, on: bool):
         """capture every build pass's level loop into one HIP graph (dcf_ctx_set_graph)"""

@@ -161,6 +161,65 @@ void dcf_release(dcf_ctx *ctx, void *p)
 /* ------------------------------------------------------------------------------------------------ */
 /* context                                                                                          */
 /* ------------------------------------------------------------------------------------------------ */
+/* one tunable of the build, by its environment name; DCF_EINVAL for an unknown name or a value out of range */
+static int set_option(dcf_ctx *c, const char *name, const char *value)
+{
+    if (!name || !value) return DCF_EINVAL;
+    auto shape3 = [&](int v[3]) -> int {
+        int a = 0, b = 0, d = 0;
+        if (sscanf(value, "%dx%dx%d", &a, &b, &d) != 3 || a <= 0 || b <= 0 || d <= 0 || a > (1 << 20) || b > (1 << 20) ||
+            d > (1 << 20) || (int64_t)a * b * d > (1 << 20))
+            return DCF_EINVAL;
+        v[0] = a; v[1] = b; v[2] = d;
+        return DCF_OK;
+    };
+    auto integer = [&](int lo, int hi, int &out) -> int {
+        char *end = nullptr;
+        const long v = strtol(value, &end, 10);
+        if (end == value || *end || v < lo || v > hi) return DCF_EINVAL;
+        out = (int)v;
+        return DCF_OK;
+    };
+    const std::string n(name);
+    if (n == "DCF_ZONES") return integer(0, 2, c->zones);
+    if (n == "DCF_TARGETS") return integer(0, 2, c->targets);
+    if (n == "DCF_RECORDS") return integer(0, 1, c->records);
+    if (n == "DCF_FIX_BLOCKS") return integer(1, 65536, c->fix_blocks);
+    if (n == "DCF_GRAPH") return integer(0, 1, c->graph);
+    if (n == "DCF_ZIP_SEGMENTS") return integer(1, 4096, c->zip_segments);
+    if (n == "DCF_ZONE_TILE2D") {
+        int a = 0, b = 0;
+        if (sscanf(value, "%dx%d", &a, &b) != 2 || a <= 0 || b <= 0 || a > ZONE_TILE_MAX || b > ZONE_TILE_MAX ||
+            (int64_t)a * b > ZONE_TILE_MAX)
+            return DCF_EINVAL;
+        c->zt2[0] = a;
+        c->zt2[1] = b;
+        c->zt2_set = 1;
+        return DCF_OK;
+    }
+    if (n == "DCF_ZONE_TILE3D") {
+        int a = 0;
+        if (integer(1, ZONE_TILE_MAX, a) != DCF_OK || (int64_t)a * a * a > ZONE_TILE_MAX) return DCF_EINVAL;
+        c->zt3 = a;
+        return DCF_OK;
+    }
+    if (n == "DCF_LEVEL_TILE2D") return shape3(c->lt2);
+    if (n == "DCF_LEVEL_GROUP2D") return shape3(c->lg2);
+    if (n == "DCF_LEVEL_TILE3D") return shape3(c->lt3);
+    if (n == "DCF_LEVEL_GROUP3D") return shape3(c->lg3);
+    return DCF_EINVAL;
+}
+
+extern "C" int dcf_ctx_set_option(dcf_ctx *ctx, const char *name, const char *value)
+try {
+    if (!ctx) return DCF_EINVAL;
+    const int rc = set_option(ctx, name, value);
+    if (rc != DCF_OK) return dcf_set_error(ctx, rc, "bad option %s=%s", name ? name : "(null)", value ? value : "(null)");
+    return DCF_OK;
+} catch (const std::exception &e) {
+    return dcf_set_error(ctx, DCF_ENOMEM, "out of host memory (%s)", e.what());
+}
+
 extern "C" int dcf_ctx_create(int device, unsigned flags, dcf_ctx **out)
 try {
     (void)flags;
@@ -177,42 +236,13 @@ try {
         return DCF_EDEVICE;
     }
     c->stream = c->own;
-    auto shape3 = [](const char *env, int v[3]) {
-        const char *e = getenv(env);
-        int a = 0, b = 0, d = 0;
-        if (e && sscanf(e, "%dx%dx%d", &a, &b, &d) == 3 && a > 0 && b > 0 && d > 0 && a <= (1 << 20) && b <= (1 << 20) &&
-            d <= (1 << 20) && (int64_t)a * b * d <= (1 << 20)) {
-            v[0] = a; v[1] = b; v[2] = d;
-        }
-    };
-    if (const char *z = getenv("DCF_ZONES")) c->zones = atoi(z);
-    if (const char *z = getenv("DCF_TARGETS")) c->targets = atoi(z);
-    if (const char *z = getenv("DCF_RECORDS")) c->records = atoi(z);
-    if (const char *z = getenv("DCF_GRAPH")) c->graph = atoi(z) != 0;
-    if (const char *z = getenv("DCF_ZIP_SEGMENTS")) {
-        const int v = atoi(z);
-        if (v >= 1 && v <= 4096) c->zip_segments = v;
-    }
-    if (const char *z = getenv("DCF_FIX_BLOCKS")) {
-        const int v = atoi(z);
-        if (v >= 1 && v <= 65536) c->fix_blocks = v;
-    }
-    if (const char *e = getenv("DCF_ZONE_TILE2D")) {
-        int a = 0, b = 0;
-        if (sscanf(e, "%dx%d", &a, &b) == 2 && a > 0 && b > 0 && a <= ZONE_TILE_MAX && b <= ZONE_TILE_MAX &&
-            (int64_t)a * b <= ZONE_TILE_MAX) {
-            c->zt2[0] = a;
-            c->zt2[1] = b;
-        }
-    }
-    if (const char *e = getenv("DCF_ZONE_TILE3D")) {
-        int a = 0;
-        if (sscanf(e, "%d", &a) == 1 && a > 0 && a <= ZONE_TILE_MAX && (int64_t)a * a * a <= ZONE_TILE_MAX) c->zt3 = a;
-    }
-    shape3("DCF_LEVEL_TILE2D", c->lt2);
-    shape3("DCF_LEVEL_GROUP2D", c->lg2);
-    shape3("DCF_LEVEL_TILE3D", c->lt3);
-    shape3("DCF_LEVEL_GROUP3D", c->lg3);
+    /* the build's tunables: the environment gives a new context its defaults (dcf_ctx_set_option
+     * changes them afterwards) */
+    static const char *const kOptions[] = { "DCF_ZONES", "DCF_TARGETS", "DCF_RECORDS", "DCF_FIX_BLOCKS", "DCF_GRAPH",
+                                            "DCF_ZIP_SEGMENTS", "DCF_ZONE_TILE2D", "DCF_ZONE_TILE3D", "DCF_LEVEL_TILE2D",
+                                            "DCF_LEVEL_GROUP2D", "DCF_LEVEL_TILE3D", "DCF_LEVEL_GROUP3D" };
+    for (const char *name : kOptions)
+        if (const char *v = getenv(name)) (void)set_option(c, name, v);
     /* the allocation-free entry points' scratch address on this device (their calls fail with
      * DCF_EDEVICE when it is unavailable; the context's other calls do not need it) */
     (void)dcf_noalloc_init(c);

@@ -83,6 +83,8 @@ struct dcf_ctx {
     /* zone tiles of the first pass (DCF_ZONE_TILE2D "WxH", DCF_ZONE_TILE3D "N"; at most ZONE_TILE_MAX pixels, the
      * size of the labelling's LDS tables), which are also a batch's level-list tiles */
     int zt2[2] = { ZONE_T2, ZONE_T2 }, zt3 = ZONE_T3;
+    /* 1 when the 2D zone tile was set (DCF_ZONE_TILE2D): a batch then keeps it instead of choosing its own */
+    int zt2_set = 0;
     /* refining passes keep {union-find entry, key, coarse node} in one 16-byte record per element
      * (DCF_RECORDS, 0 off: the default, since the precomputed targets already carry what the records
      * would) */

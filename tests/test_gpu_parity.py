@@ -657,3 +657,21 @@ def test_c_caller_on_gpu(tmp_path):
     assert int(got["nodes"]) == o.stats()["nodes"]
     assert int(got["filter_sum"]) == int(o.filter(0, 25.0, 0).astype(np.int64).sum())
     assert float(got["spectrum_sum"]) == float(got["grey_sum"]) == float(img.astype(np.int64).sum())
+
+
+def test_context_options():
+    """dcf_ctx_set_option: the build's tunables by their environment names; unknown names and values out of
+    range are DCF_EINVAL; a changed schedule gives the same tree"""
+    ctx = dcf.Context(0)
+    img = synth.vnoise(200, 260)
+    ref = ctx.build(img, 8, "max", 0).parent()
+    for name, value in (("DCF_ZONES", "0"), ("DCF_ZONE_TILE2D", "32x32"), ("DCF_LEVEL_TILE2D", "32x32x1"),
+                        ("DCF_FIX_BLOCKS", "64"), ("DCF_TARGETS", "0"), ("DCF_ZIP_SEGMENTS", "4")):
+        ctx.set_option(name, value)
+        assert np.array_equal(ctx.build(img, 8, "max", 0).parent(), ref), (name, value)
+    for name, value in (("DCF_NOPE", "1"), ("DCF_ZONES", "7"), ("DCF_ZONE_TILE2D", "100x100"), ("DCF_ZONE_TILE2D", "64"),
+                        ("DCF_LEVEL_TILE3D", "0x1x1"), ("DCF_FIX_BLOCKS", "12x"), ("DCF_ZIP_SEGMENTS", "0")):
+        with pytest.raises(dcf.DcfError) as e:
+            ctx.set_option(name, value)
+        assert e.value.code == -1, (name, value)
+    ctx.close()

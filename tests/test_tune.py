@@ -7,6 +7,8 @@ from disccofan_amd import synth, tune
 
 
 def test_knob_tables():
+    assert tune.knobs_for(False, False, True) is tune.KNOBS_TILED
+    assert tune.KNOBS_TILED["DCF_ZIP_SEGMENTS"][0] == "64"
     for is3d, batched in ((False, False), (True, False), (False, True)):
         knobs = tune.knobs_for(is3d, batched)
         for name, values in knobs.items():
@@ -45,6 +47,10 @@ def test_tune_single_image_and_batch():
     best, s, table = tune.tune(imgs, lams, connectivity=4, batched=True, repeats=1)
     assert len(table) == 1 + sum(len(v) - 1 for v in tune.KNOBS_BATCH.values())
     assert set(best) == set(tune.KNOBS_BATCH)
+    # the tiled build's merge schedule (every candidate's parent array equal to the default's)
+    best, s, table = tune.tune([img], [300.0], connectivity=8, repeats=1, grid=(1, 3),
+                               knobs={"DCF_ZIP_SEGMENTS": ["64", "1", "4"]})
+    assert len(table) == 3 and best["DCF_ZIP_SEGMENTS"] in ("64", "1", "4")
 
 
 def _fake_runner(monkeypatch, cost, outputs=None):
@@ -80,7 +86,8 @@ def test_cli_sample_sizes(monkeypatch):
     monkeypatch.setattr(tune.synth, "vnoise", lambda H, W: calls.append(("vnoise", H, W)) or np.zeros((1, 1), np.uint16))
     monkeypatch.setattr(tune.synth, "spheres", lambda D, H, W: calls.append(("spheres", D, H, W)) or np.zeros((1, 1, 1), np.uint8))
     monkeypatch.setattr(tune.synth, "batch_shapes", lambda n, lo, hi: calls.append(("batch", n, lo, hi)) or [])
-    for wl, size in (("vnoise", 0), ("spheres", 0), ("batch", 0), ("spheres", 64)):
-        tune._workload(argparse.Namespace(workload=wl, size=size, count=8))
-    assert calls == [("vnoise", 2048, 2048), ("spheres", 256, 256, 256), ("batch", 8, 64, 2048), ("spheres", 64, 64, 64)]
+    for wl, size in (("vnoise", 0), ("spheres", 0), ("batch", 0), ("spheres", 64), ("tiled", 0)):
+        tune._workload(argparse.Namespace(workload=wl, size=size, count=8, lo=64))
+    assert calls == [("vnoise", 2048, 2048), ("spheres", 256, 256, 256), ("batch", 8, 64, 2048), ("spheres", 64, 64, 64),
+                     ("vnoise", 4096, 4096)]
     assert 256 ** 3 < (1 << 30)
