@@ -4,13 +4,15 @@ The metric covers three workloads, all run by default, device-resident (image an
 before each timed region):
 
   vnoise8192  8192x8192 u16 value noise + ellipses, 8-connectivity, the max-tree AND the min-tree,
-              area opening lambda=1000, subtractive rule.  One step = build + filter of both trees,
-              one dcf_tree_build + dcf_filter call per tree.  This is `value`.
+              area opening lambda=1000, subtractive rule.  One step = build + filter of both trees in
+              one dcf_batch_filter call (a tree type per image), which builds images this large
+              concurrently on two worker contexts.  This is `value`; the plain form (one
+              dcf_tree_build + dcf_filter call per tree, one after the other) is timed beside it.
   spheres512  512^3 u8 blurred noise + spheres, 6-connectivity max-tree, volume opening lambda=500,
               direct rule.
   batch256    256 2D images (sides 64..2048, half u8 half u16), 4-connectivity max-trees, per-image
               lambda, direct rule: one batched dcf_batch_filter call, against one build + filter call
-              per image (`batched_vs_plain`).
+              per image (`workloads.batch256.plain_form`).
 
 Every workload reports its Mvoxels/s, a roofline object for the dominant kernel (k_final_pixels:
 algorithmic bytes per launch / its HIP-event launch time, recorded on the stream the kernels run on in
@@ -247,14 +249,26 @@ def run_workload(name: str, ctx, args, kernel: str, cpu: bool, say):
         np_dtype = np.uint16 if B == 2 else np.uint8
         shape_desc = list(img.shape)
 
-        def step():
+        def plain_step():
             for tt, d_out in zip(w["trees"], d_outs):
                 tr = ctx.build_device(d_img.data_ptr(), np_dtype, img.shape, w["conn"], tt, w["attr"])
                 tr.filter_device(w["attr"], w["lam"], w["decision"], d_out.data_ptr(), np_dtype)
                 state["K"] = tr.num_nodes
                 tr.free()
 
-        plain_step = None
+        if len(w["trees"]) > 1 and not args.sequential:
+            # every tree of the image in one batched call: images this large are built concurrently, one per
+            # worker context (own stream, own host thread)
+            items = [(d_img.data_ptr(), np_dtype, img.shape, w["lam"], d_out.data_ptr(), tt)
+                     for tt, d_out in zip(w["trees"], d_outs)]
+
+            def step():
+                ctx.batch_filter_device(items, connectivity=w["conn"], attr_idx=w["attr"], decision=w["decision"])
+
+            plain_step()  # node count, for the report
+        else:
+            step = plain_step
+            plain_step = None
     vox_per_step = n * len(w["trees"])
 
     # warm-up with the roofline kernel's events on (their pool is filled before the timed regions)
@@ -267,9 +281,9 @@ def run_workload(name: str, ctx, args, kernel: str, cpu: bool, say):
         say("%s warmup %d: %.1f ms" % (name, i, (time.perf_counter() - t1) * 1e3))
     # profiled timed region: the dominant kernel's launches bracketed by HIP events
     ctx.reset_stats()
-    prof_ms = timed(step, steps)
+    prof_ms = timed(step, steps) if not args.no_profile else 0.0
     ctx.set_profiling(False)
-    ex = exact_stats(ctx)
+    ex = exact_stats(ctx) if not args.no_profile else {}
     # unprofiled timed region: the headline rate
     ms = timed(step, steps)
     value = vox_per_step / (ms / 1e3) / 1e6
@@ -291,6 +305,8 @@ def run_workload(name: str, ctx, args, kernel: str, cpu: bool, say):
                       "connectivity": w["conn"], "trees": list(w["trees"]), "attribute": w["attr"],
                       "lambda": "per image" if batch else w["lam"], "decision": w["decision"],
                       "form": ("batched: one dcf_batch_filter call for the whole batch" if batch else
+                               "batched: both trees in one dcf_batch_filter call, built concurrently on two worker "
+                               "contexts (one HIP stream and one host thread each)" if plain_step else
                                "plain: one dcf_tree_build + dcf_filter call per tree"),
                       "bit_exact_vs_oracle": ("tests/test_gpu_batch.py::test_full_batch256_workload" if batch else
                                               "tests/test_gpu_fullsize.py")},
@@ -310,14 +326,16 @@ def run_workload(name: str, ctx, args, kernel: str, cpu: bool, say):
                         "graph_vs_no_graph": round(ms / gms, 3)}
         say("%s with graph-captured level loops: %.1f ms (without %.1f ms)" % (name, gms, ms))
 
-    if batch and not args.no_plain:
+    if plain_step is not None and not args.no_plain:
         plain_step()
         torch.cuda.synchronize()
-        pms = timed(plain_step, 1)
+        pms = timed(plain_step, 1 if batch else steps)
         out["plain_form"] = {"value": round(vox_per_step / (pms / 1e3) / 1e6, 3), "ms_per_step": round(pms, 3),
-                             "form": "one dcf_tree_build + dcf_filter per image, same images and lambdas, 1 timed step"}
+                             "form": ("one dcf_tree_build + dcf_filter per image, same images and lambdas, 1 timed step"
+                                      if batch else "one dcf_tree_build + dcf_filter call per tree, one after the other, "
+                                                    "%d timed steps" % steps)}
         out["batched_vs_plain"] = round(pms / ms, 3)
-        say("%s plain per-image loop: %.1f ms (batched %.1f ms)" % (name, pms, ms))
+        say("%s plain form: %.1f ms (batched %.1f ms)" % (name, pms, ms))
 
     if not batch and not args.no_extras:
         # the spectrum kernel on the max-tree: B + 8 algorithmic bytes per voxel
@@ -398,7 +416,9 @@ def main():
     ap.add_argument("--workload", default=None, choices=sorted(WORKLOADS),
                     help="run one workload alone (default: all three, vnoise8192 giving `value`)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
-    ap.add_argument("--no-plain", action="store_true", help="batch256: skip the per-image comparison loop")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain (per-image / per-tree) comparison")
+    ap.add_argument("--sequential", action="store_true",
+                    help="vnoise8192: time the plain form (one call per tree, one after the other) as the step")
     ap.add_argument("--no-extras", action="store_true", help="skip the spectrum and tiled-build figures")
     ap.add_argument("--kernels", action="store_true", help="bracket every kernel and print the per-kernel times")
     ap.add_argument("--roofline-kernel", default="k_final_pixels",
@@ -406,6 +426,11 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB 5 times (the reference point for the "
                          "FETCH_SIZE / WRITE_SIZE counters) and print its bandwidth")
+    ap.add_argument("--calibrate-gather", type=int, default=0, metavar="MIB",
+                    help="only run random 4-byte gathers from a table of MIB MiB (one per table element, torch.take), "
+                         "5 times, and print their rate: the random-access ceiling of the level kernels")
+    ap.add_argument("--no-profile", action="store_true",
+                    help="skip the profiled timed region (no roofline object; for counter runs)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -431,6 +456,24 @@ def main():
         ms = e0.elapsed_time(e1) / 5
         print(json.dumps({"copy_bytes_read": nb, "copy_bytes_written": nb, "ms": round(ms, 4),
                           "GBps_read_plus_write": round(2 * nb / ms / 1e6, 1)}), flush=True)
+        return
+
+    if args.calibrate_gather:
+        n = (args.calibrate_gather << 20) // 4
+        table = torch.arange(n, dtype=torch.int32, device="cuda:0")
+        idx = torch.randint(0, n, (n,), dtype=torch.int64, device="cuda:0", generator=torch.Generator("cuda:0").manual_seed(7))
+        out = torch.take(table, idx)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(5):
+            out = torch.take(table, idx)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 5
+        assert int(out[0]) == int(idx[0])
+        print(json.dumps({"gathers": n, "table_bytes": n * 4, "ms": round(ms, 4), "gathers_per_s": round(n / (ms / 1e3), 1),
+                          "index_and_output_bytes": n * 12}), flush=True)
         return
 
     import disccofan_amd as dcf
@@ -463,8 +506,8 @@ def main():
         "config": head["config"],
         "roofline": head["roofline"],
         "cpu_baseline": head.get("cpu_baseline"),
-        "plain_form": batch.get("plain_form"),
-        "batched_vs_plain": batch.get("batched_vs_plain"),
+        "plain_form": head.get("plain_form", batch.get("plain_form")),
+        "batched_vs_plain": head.get("batched_vs_plain", batch.get("batched_vs_plain")),
         "workloads": {k: v for k, v in res.items() if k != names[0]},
         "spectrum": head.get("spectrum"),
         "tiled": head.get("tiled"),

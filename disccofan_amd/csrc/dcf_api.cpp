@@ -19,7 +19,9 @@
 #include <string>
 #include <vector>
 
+#include <algorithm>
 #include <memory>
+#include <thread>
 
 #include "dcf_internal.h"
 
@@ -187,6 +189,13 @@ static int set_option(dcf_ctx *c, const char *name, const char *value)
     if (n == "DCF_FIX_BLOCKS") return integer(1, 65536, c->fix_blocks);
     if (n == "DCF_GRAPH") return integer(0, 1, c->graph);
     if (n == "DCF_ZIP_SEGMENTS") return integer(1, 4096, c->zip_segments);
+    if (n == "DCF_WORKERS") return integer(1, 16, c->max_workers);
+    if (n == "DCF_LARGE_PIXELS") {
+        int v = 0;
+        if (integer(1, 0x7FFFFFFF, v) != DCF_OK) return DCF_EINVAL;
+        c->large_pixels = v;
+        return DCF_OK;
+    }
     if (n == "DCF_ZONE_TILE2D") {
         int a = 0, b = 0;
         if (sscanf(value, "%dx%d", &a, &b) != 2 || a <= 0 || b <= 0 || a > ZONE_TILE_MAX || b > ZONE_TILE_MAX ||
@@ -239,8 +248,9 @@ try {
     /* the build's tunables: the environment gives a new context its defaults (dcf_ctx_set_option
      * changes them afterwards) */
     static const char *const kOptions[] = { "DCF_ZONES", "DCF_TARGETS", "DCF_RECORDS", "DCF_FIX_BLOCKS", "DCF_GRAPH",
-                                            "DCF_ZIP_SEGMENTS", "DCF_ZONE_TILE2D", "DCF_ZONE_TILE3D", "DCF_LEVEL_TILE2D",
-                                            "DCF_LEVEL_GROUP2D", "DCF_LEVEL_TILE3D", "DCF_LEVEL_GROUP3D" };
+                                            "DCF_ZIP_SEGMENTS", "DCF_WORKERS", "DCF_LARGE_PIXELS", "DCF_ZONE_TILE2D",
+                                            "DCF_ZONE_TILE3D", "DCF_LEVEL_TILE2D", "DCF_LEVEL_GROUP2D", "DCF_LEVEL_TILE3D",
+                                            "DCF_LEVEL_GROUP3D" };
     for (const char *name : kOptions)
         if (const char *v = getenv(name)) (void)set_option(c, name, v);
     /* the allocation-free entry points' scratch address on this device (their calls fail with
@@ -258,6 +268,8 @@ static void tree_release(dcf_tree *t);
 extern "C" int dcf_ctx_destroy(dcf_ctx *ctx)
 {
     if (!ctx) return DCF_EINVAL;
+    for (dcf_ctx *w : ctx->workers) (void)dcf_ctx_destroy(w);
+    ctx->workers.clear();
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     for (dcf_tree *t : ctx->trees) {
@@ -668,6 +680,96 @@ static int batch_filter_same(dcf_ctx *ctx, const dcf_batch_image *images, int co
     return rc;
 }
 
+/* joins every started thread when it goes out of scope, also when an exception unwinds */
+struct Joiner {
+    std::vector<std::thread> t;
+    ~Joiner()
+    {
+        for (auto &x : t)
+            if (x.joinable()) x.join();
+    }
+};
+
+/* The large images of a batch: each is one single-image build + filter (dcf_tree_build, dcf_filter) on a
+ * worker context, the workers running concurrently, each on its own stream from its own host thread.  A
+ * large image fills a level kernel by itself, so joining it to the batch's pixel space gains nothing, but
+ * the level loops of two images overlap on the GPU (their kernels leave room for each other).  The workers
+ * wait for the work already queued on ctx's stream; the call returns when all of them are done. */
+static int batch_large_concurrent(dcf_ctx *ctx, const dcf_batch_image *images, const std::vector<int> &idx, int connectivity,
+                                  int tree_type, int attr_idx, int decision, int extract, unsigned flags)
+{
+    const int K = std::min((int)idx.size(), ctx->max_workers);
+    while ((int)ctx->workers.size() < K) {
+        dcf_ctx *w = nullptr;
+        const int rc = dcf_ctx_create(ctx->device, 0, &w);
+        if (rc != DCF_OK) return dcf_set_error(ctx, rc, "worker context");
+        ctx->workers.push_back(w);
+    }
+    for (int k = 0; k < K; k++) {
+        dcf_ctx *w = ctx->workers[(size_t)k];
+        /* the caller's tunables and profiling */
+        w->zones = ctx->zones; w->targets = ctx->targets; w->records = ctx->records; w->fix_blocks = ctx->fix_blocks;
+        w->graph = ctx->graph; w->zip_segments = ctx->zip_segments;
+        memcpy(w->zt2, ctx->zt2, sizeof w->zt2); w->zt2_set = ctx->zt2_set; w->zt3 = ctx->zt3;
+        memcpy(w->lt2, ctx->lt2, sizeof w->lt2); memcpy(w->lg2, ctx->lg2, sizeof w->lg2);
+        memcpy(w->lt3, ctx->lt3, sizeof w->lt3); memcpy(w->lg3, ctx->lg3, sizeof w->lg3);
+        w->profiling = ctx->profiling;
+        w->prof_only = ctx->prof_only;
+    }
+    if (hipSetDevice(ctx->device) != hipSuccess) return dcf_set_error(ctx, DCF_EDEVICE, "hipSetDevice");
+    hipEvent_t ready = nullptr;
+    if (hipEventCreateWithFlags(&ready, hipEventDisableTiming) != hipSuccess) return dcf_set_error(ctx, DCF_EDEVICE, "event");
+    struct EventGuard {
+        hipEvent_t e;
+        ~EventGuard() { (void)hipEventDestroy(e); }
+    } eg{ ready };
+    if (hipEventRecord(ready, ctx->stream) != hipSuccess) return dcf_set_error(ctx, DCF_EDEVICE, "event record");
+    for (int k = 0; k < K; k++)
+        if (hipStreamWaitEvent(ctx->workers[(size_t)k]->stream, ready, 0) != hipSuccess)
+            return dcf_set_error(ctx, DCF_EDEVICE, "stream wait");
+    std::vector<int> rcs((size_t)K, DCF_OK);
+    {
+        Joiner j;
+        for (int k = 0; k < K; k++)
+            j.t.emplace_back([&, k]() {
+                dcf_ctx *w = ctx->workers[(size_t)k];
+                (void)hipSetDevice(w->device);
+                for (size_t i = (size_t)k; i < idx.size() && rcs[(size_t)k] == DCF_OK; i += (size_t)K) {
+                    const dcf_batch_image &b = images[idx[i]];
+                    const int tt = (b.flags & DCF_IMAGE_MIN_TREE) ? DCF_MIN_TREE : ((b.flags & DCF_IMAGE_MAX_TREE) ? DCF_MAX_TREE : tree_type);
+                    dcf_tree *t = nullptr;
+                    int rc = dcf_tree_build(w, b.img, b.dtype, b.dims, connectivity, tt, attr_idx, nullptr, flags & DCF_DEVICE_PTR, &t);
+                    if (rc == DCF_OK) rc = dcf_filter(t, attr_idx, decision, b.lambda, extract, b.out, b.dtype, flags);
+                    if (t) dcf_tree_free(t);
+                    rcs[(size_t)k] = rc;
+                }
+                if (hipStreamSynchronize(w->stream) != hipSuccess && rcs[(size_t)k] == DCF_OK) rcs[(size_t)k] = DCF_EDEVICE;
+            });
+    }
+    int rc = DCF_OK;
+    for (int k = 0; k < K; k++) {
+        dcf_ctx *w = ctx->workers[(size_t)k];
+        if (rcs[(size_t)k] != DCF_OK && rc == DCF_OK) {
+            rc = rcs[(size_t)k];
+            dcf_set_error(ctx, rc, "large image: %s", w->err.c_str());
+        }
+        /* the workers' kernel timings join the caller's */
+        if (w->profiling) {
+            (void)hipStreamSynchronize(w->stream);
+            prof_collect(w);
+            for (auto &kv : w->stats) {
+                KernelStat &st = ctx->stats[kv.first];
+                if (st.launches == 0 && kv.second.launches) ctx->names.push_back(kv.first);
+                st.launches += kv.second.launches;
+                st.ms += kv.second.ms;
+            }
+            w->stats.clear();
+            w->names.clear();
+        }
+    }
+    return rc;
+}
+
 extern "C" int dcf_batch_filter(dcf_ctx *ctx, const dcf_batch_image *images, int count, int connectivity, int tree_type,
                                 int attr_idx, int decision, int extract, unsigned flags)
 try {
@@ -691,11 +793,29 @@ try {
         if ((b.flags & ~(DCF_IMAGE_MAX_TREE | DCF_IMAGE_MIN_TREE)) || b.flags == (DCF_IMAGE_MAX_TREE | DCF_IMAGE_MIN_TREE))
             return dcf_set_error(ctx, DCF_EINVAL, "image %d: bad flags 0x%x", i, b.flags);
     }
+    if (attr_idx < 0 || attr_idx >= DCF_NUM_ATTR) return dcf_set_error(ctx, DCF_EINVAL, "bad attribute index %d", attr_idx);
+    if (decision < 0 || decision > 3) return dcf_set_error(ctx, DCF_EINVAL, "bad decision %d", decision);
+    if (tree_type != DCF_MAX_TREE && tree_type != DCF_MIN_TREE) return dcf_set_error(ctx, DCF_EINVAL, "bad tree type %d", tree_type);
+    /* two or more large images: built concurrently, one per worker (see batch_large_concurrent) */
+    std::vector<int> large;
+    for (int i = 0; i < count; i++)
+        if ((int64_t)pixel_count(images[i].dims) >= ctx->large_pixels) large.push_back(i);
+    if (large.size() < 2) large.clear();
+    int rc = DCF_OK;
+    if (!large.empty() && (rc = batch_large_concurrent(ctx, images, large, connectivity, tree_type, attr_idx, decision, extract,
+                                                       flags)) != DCF_OK)
+        return rc;
     /* u8 and u16 images in separate trees: a u8 image on 16-bit keys would spend a whole first pass
      * on one coarse level */
     std::vector<dcf_batch_image> g8, g16;
-    for (int i = 0; i < count; i++) (images[i].dtype == DCF_U8 ? g8 : g16).push_back(images[i]);
-    int rc = DCF_OK;
+    size_t li = 0;
+    for (int i = 0; i < count; i++) {
+        if (li < large.size() && large[li] == i) {
+            li++;
+            continue;
+        }
+        (images[i].dtype == DCF_U8 ? g8 : g16).push_back(images[i]);
+    }
     if (!g8.empty())
         rc = batch_filter_same(ctx, g8.data(), (int)g8.size(), connectivity, tree_type, attr_idx, decision, extract, flags);
     if (rc == DCF_OK && !g16.empty())

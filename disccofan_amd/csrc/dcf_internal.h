@@ -103,6 +103,12 @@ struct dcf_ctx {
     int zip_segments = 64;
     /* device address of the allocation-free entry points' scratch array on this context's device */
     void *noalloc_scratch = nullptr;
+    /* batched form: images of at least large_pixels pixels are built one per worker context (own stream,
+     * own host thread, at most max_workers at once) instead of joining the batch's pixel space
+     * (DCF_LARGE_PIXELS, DCF_WORKERS) */
+    int64_t large_pixels = (int64_t)1 << 24;
+    int max_workers = 2;
+    std::vector<dcf_ctx *> workers;
 };
 
 /* batched form: one entry per image of a batch laid out back to back in one pixel space */

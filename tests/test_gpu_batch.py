@@ -234,3 +234,28 @@ def test_graph_captured_level_loops(ctx):
             assert np.array_equal(ctx.build(vol, 6, "max", 0).parent(), e_vol)
     finally:
         ctx.set_graph(False)
+
+
+@pytest.mark.parametrize("workers", [2, 3])
+def test_large_images_on_worker_contexts(workers):
+    """images of at least DCF_LARGE_PIXELS pixels are built concurrently, one per worker context (own stream and
+    host thread), the others in the batch's pixel space: every output still equals the oracle's.  The threshold
+    is lowered so that small images take that path; u8 and u16, max- and min-trees, extract on"""
+    c = dcf.Context(0)
+    c.set_option("DCF_LARGE_PIXELS", "5000")
+    c.set_option("DCF_WORKERS", str(workers))
+    rng = np.random.default_rng(40 + workers)
+    imgs, lams = random_batch(rng, 14, maxside=140)
+    imgs += [synth.vnoise(130, 150), synth.disks(120, 90), synth.vnoise(100, 101)]
+    lams += [300.0, 80.0, 55.0]
+    tts = [("min" if rng.random() < 0.5 else "max") for _ in imgs]
+    n_large = sum(a.size >= 5000 for a in imgs)
+    say("worker contexts", workers, "large images", n_large, "of", len(imgs))
+    assert n_large >= 4
+    for dec in (0, 3):
+        outs = c.batch_filter(imgs, lams, connectivity=8, tree_type="max", decision=dec, extract=dec == 3, tree_types=tts)
+        for k, (img, lam, tt, out) in enumerate(zip(imgs, lams, tts, outs)):
+            want = OracleTree(img, 8, tt, 0).filter(0, lam, dec, extract=dec == 3).astype(img.dtype)
+            bad = np.nonzero(out.reshape(-1) != want.reshape(-1))[0]
+            assert len(bad) == 0, "image %d %s %s decision %d: %d pixels differ" % (k, img.shape, tt, dec, len(bad))
+    c.close()
