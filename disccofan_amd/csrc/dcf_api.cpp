@@ -20,6 +20,8 @@
 #include <vector>
 
 #include <algorithm>
+#include <atomic>
+#include <functional>
 #include <memory>
 #include <thread>
 
@@ -190,6 +192,7 @@ static int set_option(dcf_ctx *c, const char *name, const char *value)
     if (n == "DCF_GRAPH") return integer(0, 1, c->graph);
     if (n == "DCF_ZIP_SEGMENTS") return integer(1, 4096, c->zip_segments);
     if (n == "DCF_WORKERS") return integer(1, 16, c->max_workers);
+    if (n == "DCF_BATCH_PARTS") return integer(0, 64, c->batch_parts);
     if (n == "DCF_LARGE_PIXELS") {
         int v = 0;
         if (integer(1, 0x7FFFFFFF, v) != DCF_OK) return DCF_EINVAL;
@@ -248,7 +251,7 @@ try {
     /* the build's tunables: the environment gives a new context its defaults (dcf_ctx_set_option
      * changes them afterwards) */
     static const char *const kOptions[] = { "DCF_ZONES", "DCF_TARGETS", "DCF_RECORDS", "DCF_FIX_BLOCKS", "DCF_GRAPH",
-                                            "DCF_ZIP_SEGMENTS", "DCF_WORKERS", "DCF_LARGE_PIXELS", "DCF_ZONE_TILE2D",
+                                            "DCF_ZIP_SEGMENTS", "DCF_WORKERS", "DCF_BATCH_PARTS", "DCF_LARGE_PIXELS", "DCF_ZONE_TILE2D",
                                             "DCF_ZONE_TILE3D", "DCF_LEVEL_TILE2D", "DCF_LEVEL_GROUP2D", "DCF_LEVEL_TILE3D",
                                             "DCF_LEVEL_GROUP3D" };
     for (const char *name : kOptions)
@@ -690,15 +693,16 @@ struct Joiner {
     }
 };
 
-/* The large images of a batch: each is one single-image build + filter (dcf_tree_build, dcf_filter) on a
- * worker context, the workers running concurrently, each on its own stream from its own host thread.  A
- * large image fills a level kernel by itself, so joining it to the batch's pixel space gains nothing, but
- * the level loops of two images overlap on the GPU (their kernels leave room for each other).  The workers
- * wait for the work already queued on ctx's stream; the call returns when all of them are done. */
-static int batch_large_concurrent(dcf_ctx *ctx, const dcf_batch_image *images, const std::vector<int> &idx, int connectivity,
-                                  int tree_type, int attr_idx, int decision, int extract, unsigned flags)
+/* Independent parts of one batched call, run concurrently on worker contexts: job(w, j) runs part j on worker w.
+ * A part that fills the GPU level by level (a large image, or a batch tree of many images) still leaves it
+ * idle between the dependent accesses of its latency-bound level kernels, and each level loop waits once
+ * per digit pass on the host for its level histogram, so two loops overlap only from two host threads.
+ * Each worker has its own stream and host thread and takes the next part not yet taken (parts come largest
+ * first); the workers first wait for the work already queued on ctx's stream, and the call returns once
+ * every worker stream is drained. */
+static int run_on_workers(dcf_ctx *ctx, int njobs, const std::function<int(dcf_ctx *, int)> &job)
 {
-    const int K = std::min((int)idx.size(), ctx->max_workers);
+    const int K = std::min(njobs, ctx->max_workers);
     while ((int)ctx->workers.size() < K) {
         dcf_ctx *w = nullptr;
         const int rc = dcf_ctx_create(ctx->device, 0, &w);
@@ -709,7 +713,7 @@ static int batch_large_concurrent(dcf_ctx *ctx, const dcf_batch_image *images, c
         dcf_ctx *w = ctx->workers[(size_t)k];
         /* the caller's tunables and profiling */
         w->zones = ctx->zones; w->targets = ctx->targets; w->records = ctx->records; w->fix_blocks = ctx->fix_blocks;
-        w->graph = ctx->graph; w->zip_segments = ctx->zip_segments;
+        w->graph = ctx->graph; w->zip_segments = ctx->zip_segments; w->large_pixels = ctx->large_pixels;
         memcpy(w->zt2, ctx->zt2, sizeof w->zt2); w->zt2_set = ctx->zt2_set; w->zt3 = ctx->zt3;
         memcpy(w->lt2, ctx->lt2, sizeof w->lt2); memcpy(w->lg2, ctx->lg2, sizeof w->lg2);
         memcpy(w->lt3, ctx->lt3, sizeof w->lt3); memcpy(w->lg3, ctx->lg3, sizeof w->lg3);
@@ -728,21 +732,14 @@ static int batch_large_concurrent(dcf_ctx *ctx, const dcf_batch_image *images, c
         if (hipStreamWaitEvent(ctx->workers[(size_t)k]->stream, ready, 0) != hipSuccess)
             return dcf_set_error(ctx, DCF_EDEVICE, "stream wait");
     std::vector<int> rcs((size_t)K, DCF_OK);
+    std::atomic<int> next{ 0 };
     {
         Joiner j;
         for (int k = 0; k < K; k++)
             j.t.emplace_back([&, k]() {
                 dcf_ctx *w = ctx->workers[(size_t)k];
                 (void)hipSetDevice(w->device);
-                for (size_t i = (size_t)k; i < idx.size() && rcs[(size_t)k] == DCF_OK; i += (size_t)K) {
-                    const dcf_batch_image &b = images[idx[i]];
-                    const int tt = (b.flags & DCF_IMAGE_MIN_TREE) ? DCF_MIN_TREE : ((b.flags & DCF_IMAGE_MAX_TREE) ? DCF_MAX_TREE : tree_type);
-                    dcf_tree *t = nullptr;
-                    int rc = dcf_tree_build(w, b.img, b.dtype, b.dims, connectivity, tt, attr_idx, nullptr, flags & DCF_DEVICE_PTR, &t);
-                    if (rc == DCF_OK) rc = dcf_filter(t, attr_idx, decision, b.lambda, extract, b.out, b.dtype, flags);
-                    if (t) dcf_tree_free(t);
-                    rcs[(size_t)k] = rc;
-                }
+                for (int i = next++; i < njobs && rcs[(size_t)k] == DCF_OK; i = next++) rcs[(size_t)k] = job(w, i);
                 if (hipStreamSynchronize(w->stream) != hipSuccess && rcs[(size_t)k] == DCF_OK) rcs[(size_t)k] = DCF_EDEVICE;
             });
     }
@@ -751,11 +748,10 @@ static int batch_large_concurrent(dcf_ctx *ctx, const dcf_batch_image *images, c
         dcf_ctx *w = ctx->workers[(size_t)k];
         if (rcs[(size_t)k] != DCF_OK && rc == DCF_OK) {
             rc = rcs[(size_t)k];
-            dcf_set_error(ctx, rc, "large image: %s", w->err.c_str());
+            dcf_set_error(ctx, rc, "%s", w->err.c_str());
         }
         /* the workers' kernel timings join the caller's */
         if (w->profiling) {
-            (void)hipStreamSynchronize(w->stream);
             prof_collect(w);
             for (auto &kv : w->stats) {
                 KernelStat &st = ctx->stats[kv.first];
@@ -793,34 +789,83 @@ try {
         if ((b.flags & ~(DCF_IMAGE_MAX_TREE | DCF_IMAGE_MIN_TREE)) || b.flags == (DCF_IMAGE_MAX_TREE | DCF_IMAGE_MIN_TREE))
             return dcf_set_error(ctx, DCF_EINVAL, "image %d: bad flags 0x%x", i, b.flags);
     }
+    if (connectivity != 4 && connectivity != 6 && connectivity != 8 && connectivity != 26)
+        return dcf_set_error(ctx, DCF_EINVAL, "wrong connectivity %d (must be 4 or 8 for 2D, 6 or 26 for 3D)", connectivity);
     if (attr_idx < 0 || attr_idx >= DCF_NUM_ATTR) return dcf_set_error(ctx, DCF_EINVAL, "bad attribute index %d", attr_idx);
     if (decision < 0 || decision > 3) return dcf_set_error(ctx, DCF_EINVAL, "bad decision %d", decision);
     if (tree_type != DCF_MAX_TREE && tree_type != DCF_MIN_TREE) return dcf_set_error(ctx, DCF_EINVAL, "bad tree type %d", tree_type);
-    /* two or more large images: built concurrently, one per worker (see batch_large_concurrent) */
+    /* the parts: each image of at least large_pixels pixels on its own (when there are two or more of them:
+     * a single one gains nothing), then the u8 images and the u16 images as one batch tree each (a u8 image
+     * on 16-bit keys would spend a whole first pass on one coarse level) */
     std::vector<int> large;
     for (int i = 0; i < count; i++)
         if ((int64_t)pixel_count(images[i].dims) >= ctx->large_pixels) large.push_back(i);
     if (large.size() < 2) large.clear();
-    int rc = DCF_OK;
-    if (!large.empty() && (rc = batch_large_concurrent(ctx, images, large, connectivity, tree_type, attr_idx, decision, extract,
-                                                       flags)) != DCF_OK)
-        return rc;
-    /* u8 and u16 images in separate trees: a u8 image on 16-bit keys would spend a whole first pass
-     * on one coarse level */
-    std::vector<dcf_batch_image> g8, g16;
-    size_t li = 0;
-    for (int i = 0; i < count; i++) {
-        if (li < large.size() && large[li] == i) {
-            li++;
-            continue;
+    /* each dtype's images are dealt into batch trees, largest image first onto the tree with the fewest
+     * pixels: batch_parts trees per dtype, or by default (0) enough for every worker to have a part when the
+     * call has no large image (one tree per dtype when both are present and there are two workers) */
+    int ndt = 0;
+    for (int dt : { (int)DCF_U8, (int)DCF_U16 })
+        for (int i = 0; i < count; i++)
+            if (images[i].dtype == dt) {
+                ndt++;
+                break;
+            }
+    const int P = ctx->batch_parts > 0 ? ctx->batch_parts
+                                       : (large.empty() ? std::max(1, (ctx->max_workers + ndt - 1) / std::max(1, ndt)) : 1);
+    std::vector<std::vector<dcf_batch_image>> groups;
+    std::vector<uint64_t> gpix;
+    for (int dt : { (int)DCF_U8, (int)DCF_U16 }) {
+        std::vector<int> ids;
+        size_t li = 0;
+        for (int i = 0; i < count; i++) {
+            if (li < large.size() && large[li] == i) {
+                li++;
+                continue;
+            }
+            if (images[i].dtype == dt) ids.push_back(i);
         }
-        (images[i].dtype == DCF_U8 ? g8 : g16).push_back(images[i]);
+        if (ids.empty()) continue;
+        std::stable_sort(ids.begin(), ids.end(),
+                         [&](int x, int y) { return pixel_count(images[x].dims) > pixel_count(images[y].dims); });
+        const size_t g0 = groups.size();
+        const int np = std::min(P, (int)ids.size());
+        groups.resize(g0 + (size_t)np);
+        gpix.resize(g0 + (size_t)np, 0);
+        for (int i : ids) {
+            size_t best = g0;
+            for (size_t g = g0; g < groups.size(); g++)
+                if (gpix[g] < gpix[best]) best = g;
+            groups[best].push_back(images[i]);
+            gpix[best] += pixel_count(images[i].dims);
+        }
     }
-    if (!g8.empty())
-        rc = batch_filter_same(ctx, g8.data(), (int)g8.size(), connectivity, tree_type, attr_idx, decision, extract, flags);
-    if (rc == DCF_OK && !g16.empty())
-        rc = batch_filter_same(ctx, g16.data(), (int)g16.size(), connectivity, tree_type, attr_idx, decision, extract, flags);
-    return rc;
+    /* the parts, largest first: batch trees by pixel count, then the large images (also largest first) */
+    std::vector<int> gorder(groups.size());
+    for (size_t g = 0; g < groups.size(); g++) gorder[g] = (int)g;
+    std::stable_sort(gorder.begin(), gorder.end(), [&](int x, int y) { return gpix[(size_t)x] > gpix[(size_t)y]; });
+    std::stable_sort(large.begin(), large.end(),
+                     [&](int x, int y) { return pixel_count(images[x].dims) > pixel_count(images[y].dims); });
+    auto part = [&](dcf_ctx *c, int j) -> int {
+        if (j < (int)groups.size()) {
+            const std::vector<dcf_batch_image> &g = groups[(size_t)gorder[(size_t)j]];
+            return batch_filter_same(c, g.data(), (int)g.size(), connectivity, tree_type, attr_idx, decision, extract, flags);
+        }
+        const dcf_batch_image &b = images[large[(size_t)j - groups.size()]];
+        const int tt = (b.flags & DCF_IMAGE_MIN_TREE) ? DCF_MIN_TREE : ((b.flags & DCF_IMAGE_MAX_TREE) ? DCF_MAX_TREE : tree_type);
+        dcf_tree *t = nullptr;
+        int rc = dcf_tree_build(c, b.img, b.dtype, b.dims, connectivity, tt, attr_idx, nullptr, flags & DCF_DEVICE_PTR, &t);
+        if (rc == DCF_OK) rc = dcf_filter(t, attr_idx, decision, b.lambda, extract, b.out, b.dtype, flags);
+        if (t) dcf_tree_free(t);
+        return rc;
+    };
+    const int njobs = (int)(groups.size() + large.size());
+    if (njobs >= 2 && ctx->max_workers >= 2) return run_on_workers(ctx, njobs, part);
+    for (int j = 0; j < njobs; j++) {
+        const int rc = part(ctx, j);
+        if (rc != DCF_OK) return rc;
+    }
+    return DCF_OK;
 } catch (const std::exception &e) {
     return dcf_set_error(ctx, DCF_ENOMEM, "out of host memory (%s)", e.what());
 }

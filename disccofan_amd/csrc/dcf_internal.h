@@ -103,11 +103,13 @@ struct dcf_ctx {
     int zip_segments = 64;
     /* device address of the allocation-free entry points' scratch array on this context's device */
     void *noalloc_scratch = nullptr;
-    /* batched form: images of at least large_pixels pixels are built one per worker context (own stream,
-     * own host thread, at most max_workers at once) instead of joining the batch's pixel space
-     * (DCF_LARGE_PIXELS, DCF_WORKERS) */
+    /* batched form: the parts of a call (each image of at least large_pixels pixels on its own, and the
+     * u8 and u16 images in batch_parts batch trees each, 0: enough trees for every worker) run concurrently
+     * on at most max_workers worker contexts, each with its own stream and host thread (DCF_LARGE_PIXELS,
+     * DCF_BATCH_PARTS, DCF_WORKERS) */
     int64_t large_pixels = (int64_t)1 << 24;
     int max_workers = 2;
+    int batch_parts = 0;
     std::vector<dcf_ctx *> workers;
 };
 

@@ -46,6 +46,9 @@ KNOBS_BATCH = {
     "DCF_TARGETS": ["1", "2", "0"],
     "DCF_ZONE_TILE2D": ["64x64", "32x32", "16x16", "128x32", "32x128"],
     "DCF_FIX_BLOCKS": ["1024", "256"],
+    # concurrent batch trees: worker contexts, and batch trees per dtype (0: one part per worker)
+    "DCF_WORKERS": ["2", "1", "3"],
+    "DCF_BATCH_PARTS": ["0", "1", "2"],
 }
 # the tiled build: its merge schedule (zips started per identification pair) and the tiles' own build
 KNOBS_TILED = {

@@ -70,8 +70,8 @@ int dcf_ctx_set_stream(dcf_ctx *ctx, void *stream);
 void *dcf_ctx_stream(dcf_ctx *ctx);
 /* Tunables of the build (no counterpart in the reference; they change the schedule, never a result), by
  * the names of the environment variables that give new contexts their defaults: DCF_ZONES, DCF_TARGETS,
- * DCF_RECORDS, DCF_FIX_BLOCKS, DCF_GRAPH, DCF_ZIP_SEGMENTS, DCF_WORKERS, DCF_LARGE_PIXELS, DCF_ZONE_TILE2D
- * ("WxH"), DCF_ZONE_TILE3D ("N"), DCF_LEVEL_TILE2D / DCF_LEVEL_GROUP2D / DCF_LEVEL_TILE3D / DCF_LEVEL_GROUP3D
+ * DCF_RECORDS, DCF_FIX_BLOCKS, DCF_GRAPH, DCF_ZIP_SEGMENTS, DCF_WORKERS, DCF_BATCH_PARTS, DCF_LARGE_PIXELS,
+ * DCF_ZONE_TILE2D ("WxH"), DCF_ZONE_TILE3D ("N"), DCF_LEVEL_TILE2D / DCF_LEVEL_GROUP2D / DCF_LEVEL_TILE3D / DCF_LEVEL_GROUP3D
  * ("WxHxD").  DCF_EINVAL for
  * an unknown name or a value out of range.  The autotuner (disccofan_amd/tune.py) sets them here. */
 int dcf_ctx_set_option(dcf_ctx *ctx, const char *name, const char *value);
@@ -201,10 +201,12 @@ const char *dcf_decision_name(int decision);
  * 2^30 pixels.  flags: DCF_DEVICE_PTR (img and out are device pointers), DCF_ALL_RULES.
  * tree_type is every image's tree unless the image's own flags name one (DCF_IMAGE_MAX_TREE /
  * DCF_IMAGE_MIN_TREE): the max-tree and the min-tree of one image can share a call.
- * When two or more images hold at least DCF_LARGE_PIXELS pixels (default 2^24), those are built and
- * filtered one by one on DCF_WORKERS worker contexts (default 2), concurrently: each worker has its own
- * HIP stream and host thread, waits for the work already queued on ctx's stream, and the call returns
- * once they are done.  The other images form the batch trees as above.  Results are the same. */
+ * The call runs in parts: each image of at least DCF_LARGE_PIXELS pixels (default 2^24) on its own when
+ * there are two or more of them, and the other images in batch trees, u8 and u16 apart, DCF_BATCH_PARTS
+ * per dtype (default 0: enough for every worker).  With two or more parts and DCF_WORKERS > 1 (default 2)
+ * the parts run concurrently on worker contexts, each with its own HIP stream and host thread; they wait
+ * for the work already queued on ctx's stream, and the call returns once they are done.  Results do not
+ * depend on the split. */
 typedef struct {
     const void *img;  /* dims[0]*dims[1]*dims[2] values of dtype, x fastest */
     dcf_dtype dtype;  /* DCF_U8 or DCF_U16 */

@@ -236,11 +236,12 @@ def test_graph_captured_level_loops(ctx):
         ctx.set_graph(False)
 
 
-@pytest.mark.parametrize("workers", [2, 3])
+@pytest.mark.parametrize("workers", [1, 2, 3])
 def test_large_images_on_worker_contexts(workers):
-    """images of at least DCF_LARGE_PIXELS pixels are built concurrently, one per worker context (own stream and
-    host thread), the others in the batch's pixel space: every output still equals the oracle's.  The threshold
-    is lowered so that small images take that path; u8 and u16, max- and min-trees, extract on"""
+    """images of at least DCF_LARGE_PIXELS pixels are built one by one, the others in the u8 and u16 batch trees,
+    these parts spread over worker contexts (own stream and host thread each) and run concurrently; one worker
+    runs them in turn on the context's own stream.  Every output still equals the oracle's.  The threshold is
+    lowered so that small images take the large-image path; u8 and u16, max- and min-trees, extract on"""
     c = dcf.Context(0)
     c.set_option("DCF_LARGE_PIXELS", "5000")
     c.set_option("DCF_WORKERS", str(workers))
