@@ -293,6 +293,7 @@ struct LevelArgs {
     const int32_t *lift;   /* nlift x Kc: 2^j-th coarse ancestor (node id), -1 above the root */
     int nlift;             /* 2^nlift exceeds the depth of the coarse tree */
     const uint32_t *cnkey; /* coarse key of every coarse node */
+    const uint32_t *cdepth; /* depth of every coarse node in the coarse tree (a root: 0) */
     /* batched form: image table (nullptr for one image) */
     const BatchImg *bimg;
     int nbatch;
@@ -328,19 +329,39 @@ __device__ __forceinline__ PixGeo pix_geo(const LevelArgs &a, uint32_t p)
     return g;
 }
 
-/* the child of coarse node N (id nid) on the path up from its descendant e: e itself when its
- * coarse parent is N, else the highest ancestor of e whose coarse key is above N's (binary lifting) */
-__device__ __forceinline__ uint32_t child_toward(const LevelArgs &a, uint32_t e, uint32_t nid, uint32_t kc_count)
+/* the child of coarse node N (id nid, depth dn) on the path up from its descendant e: e itself when
+ * its coarse parent is N, else e's ancestor one level below N, reached by jumping up depth(e) - dn - 1
+ * levels in one 2^j step per set bit (binary lifting) */
+__device__ __forceinline__ uint32_t child_toward(const LevelArgs &a, uint32_t e, uint32_t nid, uint32_t dn, uint32_t kc_count)
 {
-    int32_t up = a.lift[e];
+    const int32_t up = a.lift[e];
+    const uint32_t de = a.cdepth[e];
     if (up == (int32_t)nid) return e;
-    uint32_t kn = a.cnkey[nid];
     e = (uint32_t)up;
-    for (int j = a.nlift - 1; j >= 0; j--) {
-        int32_t u = a.lift[(int64_t)j * kc_count + e];
-        if (u >= 0 && a.cnkey[u] > kn) e = (uint32_t)u;
-    }
+    uint32_t rest = de - dn - 2; /* levels still to climb above up */
+    for (int j = 0; rest && j < a.nlift; j++, rest >>= 1)
+        if (rest & 1u) {
+            const int32_t u = a.lift[(int64_t)j * kc_count + e];
+            if (u >= 0) e = (uint32_t)u;
+        }
     return e;
+}
+
+/* depth of every coarse node: the sum of the 2^j jumps that stay inside the tree, largest first
+ * (2^nlift exceeds the depth) */
+__global__ __launch_bounds__(256) void k_cdepth(const int32_t *__restrict__ lift, int nlift, int64_t kc, uint32_t *__restrict__ cdepth)
+{
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= kc) return;
+    uint32_t e = (uint32_t)c, d = 0;
+    for (int j = nlift - 1; j >= 0; j--) {
+        const int32_t u = lift[(int64_t)j * kc + e];
+        if (u >= 0) {
+            d += 1u << j;
+            e = (uint32_t)u;
+        }
+    }
+    cdepth[c] = d;
 }
 
 #define CH 8 /* neighbours gathered together */
@@ -358,6 +379,8 @@ __device__ __forceinline__ void pixel_targets(const LevelArgs &a, const PixGeo &
     const uint32_t kp = rec ? a.uf.p[(size_t)p * 4 + 1] : fl_key(a.iv, p);
     const uint32_t cp = a.pass == 2 ? kp >> (a.shift + 8) : 0u;
     const uint32_t nid = a.pass == 2 ? (rec ? a.uf.p[(size_t)p * 4 + 2] : a.cnode[p]) : 0u;
+    /* neighbours in one child region mostly share their coarse node: its child of nid is looked up once */
+    uint32_t dn = NONE, last_node = NONE, last_sv = NONE;
 #pragma unroll
     for (int k = 0; k < NB; k++) {
         t[k] = NONE;
@@ -374,7 +397,12 @@ __device__ __forceinline__ void pixel_targets(const LevelArgs &a, const PixGeo &
                 if (dq > lev || (dq == lev && q > p)) t[k] = q;
             } else if (cq > cp) {
                 const uint32_t cq_node = rec ? a.uf.p[(size_t)q * 4 + 2] : a.cnode[q];
-                const uint32_t sv = (uint32_t)a.g.n + child_toward(a, cq_node, nid, kc_count);
+                if (cq_node != last_node) {
+                    if (dn == NONE) dn = a.cdepth[nid];
+                    last_node = cq_node;
+                    last_sv = (uint32_t)a.g.n + child_toward(a, cq_node, nid, dn, kc_count);
+                }
+                const uint32_t sv = last_sv;
                 bool dup = false;
 #pragma unroll
                 for (int m = 0; m < k; m++) dup = dup || t[m] == sv;
@@ -1732,7 +1760,7 @@ int dcf_flood_build(dcf_tree *t, int grp)
     la.ox = t->off[0];
     la.oy = t->off[1];
     la.oz = t->off[2];
-    la.cnode = nullptr; la.lift = nullptr; la.cnkey = nullptr; la.nlift = 0;
+    la.cnode = nullptr; la.lift = nullptr; la.cnkey = nullptr; la.cdepth = nullptr; la.nlift = 0;
     int rc = DCF_OK;
     /* the keys, in passes of 8-bit digits: u8 one pass, u16 two; a float image is built on the dense
      * ranks of its float keys (the same order, the same equalities), 1 to 4 passes */
@@ -1822,6 +1850,9 @@ int dcf_flood_build(dcf_tree *t, int grp)
         DCF_LAUNCH(ctx, "k_coarse_nodes", k_coarse_nodes, dim3(nblk(n, 256)), dim3(256), 0, la.iv, (const int32_t *)par, flag, cidx,
                    cnkey, lift, cpix, A, S, pshift, n);
         for (int j = 1; j < nlift; j++) DCF_LAUNCH(ctx, "k_lift", k_lift, dim3(nblk(kc, 256)), dim3(256), 0, lift, j, kc);
+        uint32_t *cdepth = (uint32_t *)sc.get((size_t)kc * 4);
+        if (!cdepth) return dcf_set_error(ctx, DCF_ENOMEM, "coarse depths");
+        DCF_LAUNCH(ctx, "k_cdepth", k_cdepth, dim3(nblk(kc, 256)), dim3(256), 0, (const int32_t *)lift, nlift, kc, cdepth);
         DCF_LAUNCH(ctx, "k_coarse_node_of", k_coarse_node_of, dim3(nblk(n, 256)), dim3(256), 0, (const int32_t *)par, flag, cidx,
                    cnode, n);
         DCF_LAUNCH(ctx, "k_fill", k_fill_i32, dim3(nblk(kc, 256)), dim3(256), 0, spar, -1, kc);
@@ -1840,6 +1871,7 @@ int dcf_flood_build(dcf_tree *t, int grp)
         la.cnode = cnode;
         la.lift = lift;
         la.cnkey = cnkey;
+        la.cdepth = cdepth;
         la.nlift = nlift;
         if (recs)
             DCF_LAUNCH(ctx, "k_uf_init", k_rec_init, dim3(nblk(n + kc, 256)), dim3(256), 0, la.iv, la.shift, (const uint32_t *)cnode, n,
