@@ -1101,10 +1101,49 @@ __device__ __forceinline__ uint32_t block_common_root(BlockScratch &bs, bool val
     return R;
 }
 
-/* block-wide wave_add: one atomic per field when every valid lane of the block shares one root */
-template <int G, bool CANON = false>
+/* a block's totals for one root (thread 0), kept across several chunks of a level while the root stays
+ * the same, then added with one atomic per field */
+template <int G> struct BlkCarry {
+    static constexpr int NS = Contrib<G>::NS;
+    uint32_t R;
+    uint64_t area;
+    uint64_t s[NS > 0 ? NS : 1];
+    uint32_t mn[3], mx[3];
+    uint32_t pmin;
+};
+
+template <int G> __device__ __forceinline__ void carry_zero(BlkCarry<G> &k, uint32_t R)
+{
+    k.R = R;
+    k.area = 0;
+#pragma unroll
+    for (int i = 0; i < BlkCarry<G>::NS; i++) k.s[i] = 0;
+    for (int i = 0; i < 3; i++) { k.mn[i] = 0xFFFFFFFFu; k.mx[i] = 0u; }
+    k.pmin = 0xFFFFFFFFu;
+}
+
+template <int G, bool CANON>
+__device__ __forceinline__ void carry_flush(const DevAttr &A, const BlkCarry<G> &k, const LevelArgs *a)
+{
+    if (k.R == R_EMPTY) return;
+    atomicAdd(&A.area[k.R], (uint32_t)k.area);
+    if constexpr (CANON) canon_min(*a, k.R, k.pmin);
+#pragma unroll
+    for (int i = 0; i < BlkCarry<G>::NS; i++)
+        if (k.s[i]) atomicAdd((unsigned long long *)&A.s[i][k.R], (unsigned long long)k.s[i]);
+    if constexpr (Contrib<G>::EXT)
+        for (int i = 0; i < 3; i++) {
+            atomicMin(&A.mn[i][k.R], k.mn[i]);
+            atomicMax(&A.mx[i][k.R], k.mx[i]);
+        }
+}
+
+/* block-wide wave_add: when every valid lane of the block shares one root R, the block's totals go to
+ * thread 0, which adds them to its carry (CARRY: flushing the carry first when it holds another root)
+ * or straight to R's record with one atomic per field; otherwise each wave aggregates on its own */
+template <int G, bool CANON = false, bool CARRY = false>
 __device__ void block_add(const DevAttr &A, BlockScratch &bs, bool valid, uint32_t r, const Contrib<G> &c,
-                          const LevelArgs *a = nullptr, uint32_t p = 0)
+                          const LevelArgs *a = nullptr, uint32_t p = 0, BlkCarry<G> *carry = nullptr)
 {
     constexpr int NS = Contrib<G>::NS;
     const uint32_t R = block_common_root(bs, valid, r);
@@ -1132,30 +1171,35 @@ __device__ void block_add(const DevAttr &A, BlockScratch &bs, bool valid, uint32
         }
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint64_t t = 0;
-        for (int k = 0; k < FW; k++) t += bs.red[k][0];
-        atomicAdd(&A.area[R], (uint32_t)t);
-        if constexpr (CANON) {
-            uint32_t m = 0xFFFFFFFFu;
-            for (int k = 0; k < FW; k++) m = min(m, bs.pmin[k]);
-            canon_min(*a, R, m);
-        }
+        BlkCarry<G> t;
+        carry_zero(t, R);
+        for (int k = 0; k < FW; k++) t.area += bs.red[k][0];
+        if constexpr (CANON)
+            for (int k = 0; k < FW; k++) t.pmin = min(t.pmin, bs.pmin[k]);
 #pragma unroll
-        for (int i = 0; i < NS; i++) {
-            uint64_t v = 0;
-            for (int k = 0; k < FW; k++) v += bs.red[k][1 + i];
-            if (v) atomicAdd((unsigned long long *)&A.s[i][R], (unsigned long long)v);
-        }
+        for (int i = 0; i < NS; i++)
+            for (int k = 0; k < FW; k++) t.s[i] += bs.red[k][1 + i];
         if constexpr (Contrib<G>::EXT)
-            for (int i = 0; i < 3; i++) {
-                uint32_t lo = 0xFFFFFFFFu, hi = 0;
+            for (int i = 0; i < 3; i++)
                 for (int k = 0; k < FW; k++) {
-                    lo = min(lo, (uint32_t)bs.red[k][1 + DCF_MAX_SUMS + i]);
-                    hi = max(hi, (uint32_t)bs.red[k][1 + DCF_MAX_SUMS + 3 + i]);
+                    t.mn[i] = min(t.mn[i], (uint32_t)bs.red[k][1 + DCF_MAX_SUMS + i]);
+                    t.mx[i] = max(t.mx[i], (uint32_t)bs.red[k][1 + DCF_MAX_SUMS + 3 + i]);
                 }
-                atomicMin(&A.mn[i][R], lo);
-                atomicMax(&A.mx[i][R], hi);
+        if constexpr (CARRY) {
+            BlkCarry<G> &k = *carry;
+            if (k.R != R) {
+                carry_flush<G, CANON>(A, k, a);
+                k = t;
+            } else {
+                k.area += t.area;
+                k.pmin = min(k.pmin, t.pmin);
+#pragma unroll
+                for (int i = 0; i < NS; i++) k.s[i] += t.s[i];
+                for (int i = 0; i < 3; i++) { k.mn[i] = min(k.mn[i], t.mn[i]); k.mx[i] = max(k.mx[i], t.mx[i]); }
             }
+        } else {
+            carry_flush<G, CANON>(A, t, a);
+        }
     }
 }
 
@@ -1176,11 +1220,19 @@ __device__ __forceinline__ uint32_t held_target(const LevelArgs &a, uint32_t r)
 template <int G, bool BLK>
 __global__ __launch_bounds__(FB) void k_final_pixels(LevelArgs a, const uint32_t *__restrict__ lvl, uint32_t cnt, uint32_t lev,
                                                      DevAttr A, DevAttr S, DevAttr P, int32_t *__restrict__ par,
-                                                     int32_t *__restrict__ spar, uint32_t *__restrict__ marks)
+                                                     int32_t *__restrict__ spar, uint32_t *__restrict__ marks, uint32_t chunk)
 {
     __shared__ BlockScratch bs;
-    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
-    {
+    /* BLK: the block takes `chunk` consecutive FB-element chunks of the level list (one when the grid
+     * covers the list); the totals of a root shared by the whole block are carried from chunk to chunk
+     * (thread 0) and added once the root changes, so a level that is one big set sends one atomic per
+     * field and block instead of one per chunk */
+    BlkCarry<G> cf, co;
+    carry_zero(cf, R_EMPTY);
+    carry_zero(co, R_EMPTY);
+    const uint32_t c0 = BLK ? xcd_block() * chunk : xcd_block(), c1 = BLK ? c0 + chunk : c0 + 1;
+    for (uint32_t ch = c0; ch < c1 && ch * FB < cnt; ch++) {
+        const uint32_t i = ch * FB + threadIdx.x;
         const bool valid = i < cnt;
         uint32_t p = 0, r = 0;
         bool fresh = false, old = false;
@@ -1206,14 +1258,19 @@ __global__ __launch_bounds__(FB) void k_final_pixels(LevelArgs a, const uint32_t
          * adds once; otherwise (zone representatives, a refining pass's scattered pixels) the waves
          * aggregate on their own */
         if constexpr (BLK) {
-            block_add(A, bs, fresh, r, c);
+            block_add<G, false, true>(A, bs, fresh, r, c, nullptr, 0, &cf);
             __syncthreads();
             /* the held pixels: contributions pending at their old root, canonical levelroot candidates */
-            block_add<G, true>(P, bs, old, r, c, &a, p);
+            block_add<G, true, true>(P, bs, old, r, c, &a, p, &co);
+            __syncthreads();
         } else {
             wave_add(A, fresh, r, c);
             wave_add<G, true>(P, old, r, c, &a, p);
         }
+    }
+    if (BLK && threadIdx.x == 0) {
+        carry_flush<G, false>(A, cf, nullptr);
+        carry_flush<G, true>(P, co, &a);
     }
     /* the old roots linked under another root at this level: parent = the set's canonical levelroot
      * (resolved after the pass), complete record pending at the set's root */
@@ -1676,16 +1733,22 @@ static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAtt
             default: DCF_LAUNCH(ctx, "k_union", k_union<26>, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, cnt, (uint32_t)lev, (uint32_t)kc); break;
             }
             /* the grid covers the level's pixels and, striding, the roots absorbed at the level */
-            unsigned rb = nblk(cnt, FB), ab = nblk((int64_t)cnt * la.nb.n / 4, FB);
+            const bool blk = la.pass == 1 && !zones;
+            unsigned rb = nblk(cnt, FB), ab = nblk((int64_t)cnt * la.nb.n / 4, FB), chunk = 1;
+            /* block-wide aggregation: at most final_blocks blocks, each carrying its totals over chunks */
+            if (blk && ctx->final_blocks > 0 && rb > (unsigned)ctx->final_blocks) {
+                chunk = (rb + (unsigned)ctx->final_blocks - 1) / (unsigned)ctx->final_blocks;
+                rb = (rb + chunk - 1) / chunk;
+            }
             if (ab > 1024) ab = 1024;
             if (ab > rb) rb = ab;
     #define DCF_FINAL(GV)                                                                                                        \
-        if (la.pass == 1 && !zones)                                                                                              \
+        if (blk)                                                                                                                 \
             DCF_LAUNCH(ctx, "k_final_pixels", (k_final_pixels<GV, true>), dim3(rb), dim3(FB), 0, la, lvl, cnt, (uint32_t)lev, A, S, P, \
-                       par, spar, nabs + 1 + ord);                                                                              \
+                       par, spar, nabs + 1 + ord, chunk);                                                                       \
         else                                                                                                                     \
             DCF_LAUNCH(ctx, "k_final_pixels", (k_final_pixels<GV, false>), dim3(rb), dim3(FB), 0, la, lvl, cnt, (uint32_t)lev, A, S, P, \
-                       par, spar, nabs + 1 + ord)
+                       par, spar, nabs + 1 + ord, 1u)
             switch (grp) {
             case 0: DCF_FINAL(0); break;
             case 1: DCF_FINAL(1); break;

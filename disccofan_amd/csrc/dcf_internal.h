@@ -110,6 +110,9 @@ struct dcf_ctx {
     int64_t large_pixels = (int64_t)1 << 24;
     int max_workers = 2;
     int batch_parts = 0;
+    /* first-pass pixel levels: grid cap of k_final_pixels, whose blocks then carry their totals over
+     * several chunks of the level (0: one chunk per block) (DCF_FINAL_BLOCKS) */
+    int final_blocks = 1024;
     std::vector<dcf_ctx *> workers;
 };
 

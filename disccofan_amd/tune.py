@@ -38,6 +38,7 @@ KNOBS_3D = {
     "DCF_LEVEL_TILE3D": ["16x16x16", "32x32x8", "64x16x4", "8x8x8"],
     "DCF_LEVEL_GROUP3D": ["1x1x1", "2x2x2"],
     "DCF_ZONE_TILE3D": ["16", "8", "12"],
+    "DCF_FINAL_BLOCKS": ["1024", "0", "512", "2048"],
 }
 # the batched form orders its level lists by each image's zone tiles: the zone tiles are its tile shape
 KNOBS_BATCH = {
@@ -46,6 +47,7 @@ KNOBS_BATCH = {
     "DCF_TARGETS": ["1", "2", "0"],
     "DCF_ZONE_TILE2D": ["64x64", "32x32", "16x16", "128x32", "32x128"],
     "DCF_FIX_BLOCKS": ["1024", "256"],
+    "DCF_FINAL_BLOCKS": ["1024", "0", "512"],
     # concurrent batch trees: worker contexts, and batch trees per dtype (0: one part per worker)
     "DCF_WORKERS": ["2", "1", "3"],
     "DCF_BATCH_PARTS": ["0", "1", "2"],
