@@ -265,12 +265,13 @@ struct ScatterGeo {
 /* Blocks are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8), and each XCD has its own
  * L2.  This renumbering gives every XCD one contiguous eighth of the level list, i.e. one compact
  * part of the image, so neighbouring pixels share that XCD's L2. */
-__device__ __forceinline__ uint32_t xcd_block()
+__device__ __forceinline__ uint32_t xcd_map(uint32_t b, uint32_t nb)
 {
-    const uint32_t nb = gridDim.x, b = blockIdx.x;
     const uint32_t q = nb >> 3, r = nb & 7u, x = b & 7u, k = b >> 3;
     return x * q + min(x, r) + k;
 }
+
+__device__ __forceinline__ uint32_t xcd_block() { return xcd_map(blockIdx.x, gridDim.x); }
 
 /* ------------------------------------------------------------------------------------------------ */
 /* per-level kernels                                                                                */
@@ -1220,9 +1221,13 @@ __device__ __forceinline__ uint32_t held_target(const LevelArgs &a, uint32_t r)
 template <int G, bool BLK>
 __global__ __launch_bounds__(FB) void k_final_pixels(LevelArgs a, const uint32_t *__restrict__ lvl, uint32_t cnt, uint32_t lev,
                                                      DevAttr A, DevAttr S, DevAttr P, int32_t *__restrict__ par,
-                                                     int32_t *__restrict__ spar, uint32_t *__restrict__ marks, uint32_t chunk)
+                                                     int32_t *__restrict__ spar, uint32_t *__restrict__ marks, uint32_t chunk,
+                                                     uint32_t nmain)
 {
     __shared__ BlockScratch bs;
+    /* blocks [0, nmain) take the level's elements, the others the roots absorbed at the level: the two
+     * chains of dependent loads run side by side instead of one after the other in each thread */
+    const bool main_part = blockIdx.x < nmain;
     /* BLK: the block takes `chunk` consecutive FB-element chunks of the level list (one when the grid
      * covers the list); the totals of a root shared by the whole block are carried from chunk to chunk
      * (thread 0) and added once the root changes, so a level that is one big set sends one atomic per
@@ -1230,7 +1235,8 @@ __global__ __launch_bounds__(FB) void k_final_pixels(LevelArgs a, const uint32_t
     BlkCarry<G> cf, co;
     carry_zero(cf, R_EMPTY);
     carry_zero(co, R_EMPTY);
-    const uint32_t c0 = BLK ? xcd_block() * chunk : xcd_block(), c1 = BLK ? c0 + chunk : c0 + 1;
+    const uint32_t L = xcd_map(blockIdx.x, nmain);
+    const uint32_t c0 = main_part ? (BLK ? L * chunk : L) : 0u, c1 = main_part ? (BLK ? c0 + chunk : c0 + 1) : 0u;
     for (uint32_t ch = c0; ch < c1 && ch * FB < cnt; ch++) {
         const uint32_t i = ch * FB + threadIdx.x;
         const bool valid = i < cnt;
@@ -1272,13 +1278,14 @@ __global__ __launch_bounds__(FB) void k_final_pixels(LevelArgs a, const uint32_t
         carry_flush<G, false>(A, cf, nullptr);
         carry_flush<G, true>(P, co, &a);
     }
+    if (main_part) return;
     /* the old roots linked under another root at this level: parent = the set's canonical levelroot
      * (resolved after the pass), complete record pending at the set's root */
     const uint32_t start = marks[0], end = *a.nabs;
-    if (blockIdx.x == 0 && threadIdx.x == 0) marks[1] = end;
-    const uint32_t stride = gridDim.x * blockDim.x;
+    if (blockIdx.x == nmain && threadIdx.x == 0) marks[1] = end;
+    const uint32_t stride = (gridDim.x - nmain) * blockDim.x;
     /* every lane of a wave runs the same number of iterations so wave_add sees the full wave */
-    for (uint32_t base = start + blockIdx.x * blockDim.x; base < end; base += stride) {
+    for (uint32_t base = start + (blockIdx.x - nmain) * blockDim.x; base < end; base += stride) {
         const uint32_t j = base + threadIdx.x;
         const bool valid = j < end;
         uint32_t r = 0;
@@ -1741,14 +1748,15 @@ static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAtt
                 rb = (rb + chunk - 1) / chunk;
             }
             if (ab > 1024) ab = 1024;
-            if (ab > rb) rb = ab;
+            if (ab < 1) ab = 1;
+            /* rb blocks for the level's elements, then ab blocks striding over the absorbed roots */
     #define DCF_FINAL(GV)                                                                                                        \
         if (blk)                                                                                                                 \
-            DCF_LAUNCH(ctx, "k_final_pixels", (k_final_pixels<GV, true>), dim3(rb), dim3(FB), 0, la, lvl, cnt, (uint32_t)lev, A, S, P, \
-                       par, spar, nabs + 1 + ord, chunk);                                                                       \
+            DCF_LAUNCH(ctx, "k_final_pixels", (k_final_pixels<GV, true>), dim3(rb + ab), dim3(FB), 0, la, lvl, cnt, (uint32_t)lev, A, \
+                       S, P, par, spar, nabs + 1 + ord, chunk, rb);                                                              \
         else                                                                                                                     \
-            DCF_LAUNCH(ctx, "k_final_pixels", (k_final_pixels<GV, false>), dim3(rb), dim3(FB), 0, la, lvl, cnt, (uint32_t)lev, A, S, P, \
-                       par, spar, nabs + 1 + ord, 1u)
+            DCF_LAUNCH(ctx, "k_final_pixels", (k_final_pixels<GV, false>), dim3(rb + ab), dim3(FB), 0, la, lvl, cnt, (uint32_t)lev, A, \
+                       S, P, par, spar, nabs + 1 + ord, 1u, rb)
             switch (grp) {
             case 0: DCF_FINAL(0); break;
             case 1: DCF_FINAL(1); break;
