@@ -415,15 +415,26 @@ __device__ __forceinline__ void pixel_targets(const LevelArgs &a, const PixGeo &
 
 /* p's set (root rp, tag tp) is united with the set of every target in cur: the targets' roots are
  * found together (independent loads in flight, path splitting), duplicates dropped */
+/* the first entries of a chunk of targets, loaded before they are needed (their finds and CASes
+ * tolerate entries that have changed since, as they must for any concurrent update) */
+__device__ __forceinline__ void chunk_entries(const LevelArgs &a, const uint32_t (&cur)[CH], uint32_t (&val)[CH])
+{
+    const uint32_t NONE = 0xFFFFFFFFu;
+#pragma unroll
+    for (int j = 0; j < CH; j++) val[j] = cur[j] != NONE ? ldc(a.uf, cur[j]) : UF_ROOT;
+}
+
+template <bool PRE = false>
 __device__ __forceinline__ uint32_t unite_chunk(const LevelArgs &a, const uint32_t (&cur)[CH], uint32_t rp, uint32_t &tp, uint32_t lev,
-                                                const AbsBuf &ab)
+                                                const AbsBuf &ab, const uint32_t *pre = nullptr)
 {
     const uint32_t NONE = 0xFFFFFFFFu;
     uint32_t val[CH], node[CH];
 #pragma unroll
     for (int j = 0; j < CH; j++) {
         node[j] = cur[j];
-        val[j] = cur[j] != NONE ? ldc(a.uf, cur[j]) : UF_ROOT;
+        if constexpr (PRE) val[j] = pre[j];
+        else val[j] = cur[j] != NONE ? ldc(a.uf, cur[j]) : UF_ROOT;
     }
     bool more = true;
     while (more) {
@@ -517,9 +528,15 @@ __global__ __launch_bounds__(256) void k_union_tgt(LevelArgs a, const uint32_t *
                 t[k] = v.x; t[k + 1] = v.y;
             }
         }
-        uint32_t tp = 0, rp = uf_find(a.uf, p, tp);
+        /* the first chunk's entries are loaded together with p's own */
+        uint32_t cur0[CH], pre[CH];
 #pragma unroll
-        for (int k0 = 0; k0 < NB; k0 += CH) {
+        for (int j = 0; j < CH; j++) cur0[j] = j < NB ? t[j] : NONE;
+        chunk_entries(a, cur0, pre);
+        uint32_t tp = 0, rp = uf_find(a.uf, p, tp);
+        rp = unite_chunk<true>(a, cur0, rp, tp, lev, ab, pre);
+#pragma unroll
+        for (int k0 = CH; k0 < NB; k0 += CH) {
             uint32_t cur[CH];
 #pragma unroll
             for (int j = 0; j < CH; j++) cur[j] = k0 + j < NB ? t[k0 + j] : NONE;
