@@ -169,6 +169,76 @@ def exact_stats(ctx):
     return out
 
 
+def _pmc_rows(path: str, counter: str):
+    """per dispatch of a rocprofv3 --pmc run (its -d folder or counter CSV): (kernel name, grid size, KiB)"""
+    import csv
+    import glob
+    files = [path] if path.endswith(".csv") else sorted(glob.glob(os.path.join(path, "**", "*counter_collection.csv"),
+                                                                      recursive=True))
+    if not files:
+        raise SystemExit("no counter_collection.csv under %s" % path)
+    per = {}
+    for f in files:
+        for r in csv.DictReader(open(f)):
+            if r["Counter_Name"] != counter:
+                continue
+            key = (f, int(r["Dispatch_Id"]))
+            name, grid, v = per.get(key, (r["Kernel_Name"], int(r["Grid_Size"]), 0.0))
+            per[key] = (name, grid, v + float(r["Counter_Value"]))
+    return list(per.values())
+
+
+def traffic_summary(args):
+    """--traffic FETCH_RUN WRITE_RUN --copy COPY_FETCH_RUN COPY_WRITE_RUN: the HBM bytes per launch of
+    --roofline-kernel in --workload, from two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of the workload, each
+    scaled by what the same counter reads for the plain copy of --calibrate-copy MiB; writes
+    profiles/traffic_<kernel>_<workload>.json (read back as roofline.traffic) and prints the table"""
+    kernel, workload = args.roofline_kernel, args.workload
+    if not workload or not args.copy or not args.calibrate_copy:
+        raise SystemExit("--traffic needs --workload, --copy and --calibrate-copy MIB")
+    copy_kib = float(args.calibrate_copy << 10)
+    scale = []
+    for run, counter in zip(args.copy, ("FETCH_SIZE", "WRITE_SIZE")):
+        vals = [v for (name, g, v) in _pmc_rows(run, counter) if "copyBuffer" in name]
+        if not vals:
+            raise SystemExit("no copy dispatch in %s" % run)
+        scale.append(copy_kib / max(vals))
+    rows = {}
+    for run, counter, sc in zip(args.traffic, ("FETCH_SIZE", "WRITE_SIZE"), scale):
+        for name, grid, v in _pmc_rows(run, counter):
+            if kernel not in name:
+                continue
+            short = name.split("(")[0].replace("void ", "")
+            ent = rows.setdefault(short, {"n": 0, "grid": 0, "FETCH_SIZE": 0.0, "WRITE_SIZE": 0.0})
+            if counter == "FETCH_SIZE":
+                ent["n"] += 1
+                ent["grid"] += grid
+            ent[counter] += v * 1024.0 * sc
+    if not rows:
+        raise SystemExit("no dispatch of %s in the runs" % kernel)
+    print("| kernel | dispatches | fetch MB/launch | write MB/launch | total MB/launch | fetch B per grid thread | "
+          "write B per grid thread |")
+    print("|---|---|---|---|---|---|---|")
+    tot = {"n": 0, "grid": 0, "FETCH_SIZE": 0.0, "WRITE_SIZE": 0.0}
+    for short, e in sorted(rows.items(), key=lambda kv: -kv[1]["n"]):
+        for k in tot:
+            tot[k] += e[k]
+        print("| %s | %d | %.2f | %.2f | %.2f | %.0f | %.0f |" % (
+            short, e["n"], e["FETCH_SIZE"] / e["n"] / 1e6, e["WRITE_SIZE"] / e["n"] / 1e6,
+            (e["FETCH_SIZE"] + e["WRITE_SIZE"]) / e["n"] / 1e6, e["FETCH_SIZE"] / max(e["grid"], 1),
+            e["WRITE_SIZE"] / max(e["grid"], 1)))
+    out = {"kernel": kernel, "workload": workload,
+           "bytes_per_launch": round((tot["FETCH_SIZE"] + tot["WRITE_SIZE"]) / tot["n"]),
+           "fetch_bytes_per_launch": round(tot["FETCH_SIZE"] / tot["n"]),
+           "write_bytes_per_launch": round(tot["WRITE_SIZE"] / tot["n"]),
+           "launches_counted": tot["n"], "fetch_scale": scale[0], "write_scale": scale[1]}
+    path = os.path.join(ROOT, "profiles", "traffic_%s_%s.json" % (kernel, workload))
+    with open(path, "w") as f:
+        json.dump(out, f, indent=1)
+    print("\nFETCH_SIZE scale %.4f, WRITE_SIZE scale %.4f; all listed kernels: %.2f MB per launch -> %s" % (
+        scale[0], scale[1], out["bytes_per_launch"] / 1e6, os.path.relpath(path, ROOT)))
+
+
 def traffic_of(kernel: str, workload: str):
     path = os.path.join(ROOT, "profiles", "traffic_%s_%s.json" % (kernel, workload))
     if not os.path.exists(path):
@@ -265,7 +335,7 @@ def run_workload(name: str, ctx, args, kernel: str, cpu: bool, say):
             def step():
                 ctx.batch_filter_device(items, connectivity=w["conn"], attr_idx=w["attr"], decision=w["decision"])
 
-            plain_step()  # node count, for the report
+            # (nodes_per_tree comes from the plain form, timed after the batched one)
         else:
             step = plain_step
             plain_step = None
@@ -311,8 +381,6 @@ def run_workload(name: str, ctx, args, kernel: str, cpu: bool, say):
                       "bit_exact_vs_oracle": ("tests/test_gpu_batch.py::test_full_batch256_workload" if batch else
                                               "tests/test_gpu_fullsize.py")},
            "roofline": roof, "gen_s": round(gen_s, 2)}
-    if not batch:
-        out["nodes_per_tree"] = state.get("K")
 
     if batch and not args.no_extras:
         # the same batched call with every pass's level loop captured into one HIP graph
@@ -336,6 +404,8 @@ def run_workload(name: str, ctx, args, kernel: str, cpu: bool, say):
                                                     "%d timed steps" % steps)}
         out["batched_vs_plain"] = round(pms / ms, 3)
         say("%s plain form: %.1f ms (batched %.1f ms)" % (name, pms, ms))
+    if not batch:
+        out["nodes_per_tree"] = state.get("K")
 
     if not batch and not args.no_extras:
         # the spectrum kernel on the max-tree: B + 8 algorithmic bytes per voxel
@@ -429,10 +499,18 @@ def main():
     ap.add_argument("--calibrate-gather", type=int, default=0, metavar="MIB",
                     help="only run random 4-byte gathers from a table of MIB MiB (one per table element, torch.take), "
                          "5 times, and print their rate: the random-access ceiling of the level kernels")
+    ap.add_argument("--traffic", nargs=2, metavar=("FETCH_RUN", "WRITE_RUN"),
+                    help="no GPU work: summarise two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of --workload into "
+                         "profiles/traffic_<--roofline-kernel>_<workload>.json, scaled by the --copy runs")
+    ap.add_argument("--copy", nargs=2, metavar=("FETCH_RUN", "WRITE_RUN"),
+                    help="the two --pmc runs of `bench.py --calibrate-copy MIB` that scale --traffic")
     ap.add_argument("--no-profile", action="store_true",
                     help="skip the profiled timed region (no roofline object; for counter runs)")
     args = ap.parse_args()
 
+    if args.traffic:
+        traffic_summary(args)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
