@@ -20,6 +20,7 @@
 #include <vector>
 
 #include <algorithm>
+#include <cstdint>
 #include <atomic>
 #include <functional>
 #include <memory>
@@ -194,6 +195,12 @@ static int set_option(dcf_ctx *c, const char *name, const char *value)
     if (n == "DCF_WORKERS") return integer(1, 16, c->max_workers);
     if (n == "DCF_BATCH_PARTS") return integer(0, 64, c->batch_parts);
     if (n == "DCF_FINAL_BLOCKS") return integer(0, 1 << 20, c->final_blocks);
+    if (n == "DCF_SMALL_PIXELS") {
+        int v = 0;
+        if (integer(0, DCF_SMALL_MAX, v) != DCF_OK) return DCF_EINVAL;
+        c->small_pixels = v;
+        return DCF_OK;
+    }
     if (n == "DCF_LARGE_PIXELS") {
         int v = 0;
         if (integer(1, 0x7FFFFFFF, v) != DCF_OK) return DCF_EINVAL;
@@ -252,7 +259,7 @@ try {
     /* the build's tunables: the environment gives a new context its defaults (dcf_ctx_set_option
      * changes them afterwards) */
     static const char *const kOptions[] = { "DCF_ZONES", "DCF_TARGETS", "DCF_RECORDS", "DCF_FIX_BLOCKS", "DCF_GRAPH",
-                                            "DCF_ZIP_SEGMENTS", "DCF_WORKERS", "DCF_BATCH_PARTS", "DCF_FINAL_BLOCKS", "DCF_LARGE_PIXELS", "DCF_ZONE_TILE2D",
+                                            "DCF_ZIP_SEGMENTS", "DCF_WORKERS", "DCF_BATCH_PARTS", "DCF_FINAL_BLOCKS", "DCF_SMALL_PIXELS", "DCF_LARGE_PIXELS", "DCF_ZONE_TILE2D",
                                             "DCF_ZONE_TILE3D", "DCF_LEVEL_TILE2D", "DCF_LEVEL_GROUP2D", "DCF_LEVEL_TILE3D",
                                             "DCF_LEVEL_GROUP3D" };
     for (const char *name : kOptions)
@@ -541,7 +548,7 @@ struct StreamWait {
 /* one batch tree over images that are all u8 or all u16 (keys of that width: one digit pass for u8,
  * two for u16) */
 static int batch_filter_same(dcf_ctx *ctx, const dcf_batch_image *images, int count, int connectivity, int tree_type,
-                             int attr_idx, int decision, int extract, unsigned flags)
+                             int attr_idx, int decision, int extract, unsigned flags, bool small = false)
 {
     if (!ctx) return DCF_EINVAL;
     if (!images || count <= 0) return dcf_set_error(ctx, DCF_EINVAL, "empty batch");
@@ -610,6 +617,7 @@ static int batch_filter_same(dcf_ctx *ctx, const dcf_batch_image *images, int co
     t->nbatch = count;
     t->batch3d = is3d;
     t->batch_tiles = (int64_t)tiles;
+    t->small_batch = small ? 1 : 0;
     int rc = DCF_OK;
     t->img = dcf_alloc(ctx, (size_t)N * csz);
     t->own_img = 1;
@@ -715,7 +723,7 @@ static int run_on_workers(dcf_ctx *ctx, int njobs, const std::function<int(dcf_c
         /* the caller's tunables and profiling */
         w->zones = ctx->zones; w->targets = ctx->targets; w->records = ctx->records; w->fix_blocks = ctx->fix_blocks;
         w->graph = ctx->graph; w->zip_segments = ctx->zip_segments; w->large_pixels = ctx->large_pixels;
-        w->final_blocks = ctx->final_blocks;
+        w->final_blocks = ctx->final_blocks; w->small_pixels = ctx->small_pixels;
         memcpy(w->zt2, ctx->zt2, sizeof w->zt2); w->zt2_set = ctx->zt2_set; w->zt3 = ctx->zt3;
         memcpy(w->lt2, ctx->lt2, sizeof w->lt2); memcpy(w->lg2, ctx->lg2, sizeof w->lg2);
         memcpy(w->lt3, ctx->lt3, sizeof w->lt3); memcpy(w->lg3, ctx->lg3, sizeof w->lg3);
@@ -803,6 +811,18 @@ try {
     for (int i = 0; i < count; i++)
         if ((int64_t)pixel_count(images[i].dims) >= ctx->large_pixels) large.push_back(i);
     if (large.size() < 2) large.clear();
+    /* small images (the area group only): one part built in LDS, one workgroup per image */
+    const int64_t small_max = std::min<int64_t>(ctx->small_pixels, DCF_SMALL_MAX);
+    std::vector<char> is_small((size_t)count, 0);
+    std::vector<dcf_batch_image> gsmall;
+    uint64_t small_pix = 0;
+    if (dcf_group_of(attr_idx) == 0)
+        for (int i = 0; i < count; i++)
+            if ((int64_t)pixel_count(images[i].dims) <= small_max) {
+                is_small[(size_t)i] = 1;
+                gsmall.push_back(images[i]);
+                small_pix += pixel_count(images[i].dims);
+            }
     /* each dtype's images are dealt into batch trees, largest image first onto the tree with the fewest
      * pixels: batch_parts trees per dtype, or by default (0) enough for every worker to have a part when the
      * call has no large image (one tree per dtype when both are present and there are two workers) */
@@ -825,7 +845,7 @@ try {
                 li++;
                 continue;
             }
-            if (images[i].dtype == dt) ids.push_back(i);
+            if (images[i].dtype == dt && !is_small[(size_t)i]) ids.push_back(i);
         }
         if (ids.empty()) continue;
         std::stable_sort(ids.begin(), ids.end(),
@@ -842,6 +862,11 @@ try {
             gpix[best] += pixel_count(images[i].dims);
         }
     }
+    if (!gsmall.empty()) {
+        groups.push_back(gsmall);
+        gpix.push_back(small_pix);
+    }
+    const size_t small_group = gsmall.empty() ? SIZE_MAX : groups.size() - 1;
     /* the parts, largest first: batch trees by pixel count, then the large images (also largest first) */
     std::vector<int> gorder(groups.size());
     for (size_t g = 0; g < groups.size(); g++) gorder[g] = (int)g;
@@ -850,8 +875,10 @@ try {
                      [&](int x, int y) { return pixel_count(images[x].dims) > pixel_count(images[y].dims); });
     auto part = [&](dcf_ctx *c, int j) -> int {
         if (j < (int)groups.size()) {
-            const std::vector<dcf_batch_image> &g = groups[(size_t)gorder[(size_t)j]];
-            return batch_filter_same(c, g.data(), (int)g.size(), connectivity, tree_type, attr_idx, decision, extract, flags);
+            const size_t gi = (size_t)gorder[(size_t)j];
+            const std::vector<dcf_batch_image> &g = groups[gi];
+            return batch_filter_same(c, g.data(), (int)g.size(), connectivity, tree_type, attr_idx, decision, extract, flags,
+                                     gi == small_group);
         }
         const dcf_batch_image &b = images[large[(size_t)j - groups.size()]];
         const int tt = (b.flags & DCF_IMAGE_MIN_TREE) ? DCF_MIN_TREE : ((b.flags & DCF_IMAGE_MAX_TREE) ? DCF_MAX_TREE : tree_type);

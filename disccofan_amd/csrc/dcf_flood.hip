@@ -42,6 +42,9 @@
  * key prefixes) as the coarse tree.  Pixels with equal keys are merged by the union-find itself
  * (no chains), so flat zones cost nothing extra.
  */
+#include <algorithm>
+#include <vector>
+
 #include <hipcub/hipcub.hpp>
 
 #include "dcf_internal.h"
@@ -1502,6 +1505,194 @@ __global__ __launch_bounds__(256) void k_fine_roots(UFV uf, const uint32_t *__re
     par[fr] = spar[c];
 }
 
+
+/* ------------------------------------------------------------------------------------------------ */
+/* small images of a batch: each one's whole tree built in one workgroup's LDS                      */
+/* ------------------------------------------------------------------------------------------------ */
+/* A persistent kernel fed by a work queue: every 64-lane workgroup takes the next image of its list
+ * from a counter and builds that image's canonical parent array and levelroot areas in LDS, with no
+ * launch per grey level.  For one image of N <= SMAX pixels:
+ *  - sort: the pixels by key, highest first, equal keys by index, highest first (bitonic, in LDS);
+ *  - flood: in that order each pixel p becomes a set of its own, the lanes find the set of each of its
+ *    processed neighbours, and every such set hangs below p in the tree (parent[rep] = p for the set's
+ *    representative pixel, its most recent one) while the sets merge in a union-find by rank whose
+ *    representative becomes p: one pixel at a time;
+ *  - levelroots: the last pixel of a level component to be processed is its root in this parent
+ *    forest, and with equal keys taken by falling index that is the component's minimum pixel, the
+ *    canonical levelroot; every pixel finds its levelroot by pointer jumping over same-key parents,
+ *    and the canonical parent of p is the levelroot of parent[p];
+ *  - areas: each pixel counts 1 at its levelroot, then the levelroots add their totals to their
+ *    parents, one run of equal keys after the other (a levelroot's parent lies at a lower key).
+ * The result is the level loop's: the canonical parent array (-1 at the image's root) and the area at
+ * every levelroot. */
+/* p / d for p < 2^16 and 1 <= d <= 2^16, with m = floor((2^32 - 1) / d) + 1 (exact: the error of the
+ * product is below 2^-16 and a quotient's fraction is at most 1 - 1/d) */
+__device__ __forceinline__ uint32_t small_div(uint32_t p, uint32_t d, uint32_t m) { return d == 1 ? p : __umulhi(p, m); }
+
+template <int SMAX>
+__global__ __launch_bounds__(64) void k_small_build(ImgView iv, const BatchImg *__restrict__ bimg, const uint32_t *__restrict__ list,
+                                                    uint32_t nlist, Nbr nb, int32_t *__restrict__ par, uint32_t *__restrict__ area,
+                                                    uint32_t *__restrict__ queue)
+{
+    constexpr uint32_t NONE = 0xFFFFu;
+    constexpr uint32_t IMASK = (uint32_t)SMAX - 1u; /* SMAX is a power of two */
+    constexpr int IBITS = __builtin_ctz(SMAX);
+    __shared__ uint32_t s32[SMAX]; /* sort keys; then low half: processing order, high half: areas */
+    __shared__ uint16_t kk[SMAX];  /* grey keys */
+    __shared__ uint16_t zp[SMAX];  /* union-find while flooding (NONE: not processed yet), then levelroots */
+    __shared__ uint16_t pa[SMAX];  /* parent while flooding, then canonical parent */
+    __shared__ uint16_t rep[SMAX]; /* a union-find root's representative: the set's most recent pixel */
+    __shared__ uint8_t rk[SMAX];   /* union-find rank */
+    const int lane = threadIdx.x;
+    /* this lane's neighbour offset (lanes beyond the neighbourhood have none) */
+    const bool has_nb = lane < nb.n;
+    int ndx = 0, ndy = 0, ndz = 0;
+    for (int k = 0; k < nb.n; k++)
+        if (k == lane) {
+            ndx = nb.dx[k];
+            ndy = nb.dy[k];
+            ndz = nb.dz[k];
+        }
+    for (;;) {
+        uint32_t j = 0;
+        if (lane == 0) j = atomicAdd(queue, 1u);
+        j = __shfl(j, 0, 64);
+        if (j >= nlist) break;
+        const BatchImg b = bimg[list[j]];
+        const int W = (int)b.W, H = (int)b.H, D = (int)b.D, WH = W * H;
+        const int N = WH * D;
+        const uint32_t mW = 0xFFFFFFFFu / (uint32_t)W + 1u, mWH = 0xFFFFFFFFu / (uint32_t)WH + 1u;
+        const int64_t off = (int64_t)b.off;
+        int Np = 1;
+        while (Np < N) Np <<= 1;
+        for (int l = lane; l < Np; l += 64) {
+            uint32_t v = 0xFFFFFFFFu;
+            if (l < N) {
+                const uint32_t k = fl_key(iv, off + l);
+                kk[l] = (uint16_t)k;
+                zp[l] = (uint16_t)NONE;
+                pa[l] = (uint16_t)l;
+                v = ((65535u - k) << IBITS) | (IMASK - (uint32_t)l);
+            }
+            s32[l] = v;
+        }
+        __syncthreads();
+        for (int k = 2; k <= Np; k <<= 1)
+            for (int h = k >> 1; h > 0; h >>= 1) {
+                for (int i = lane; i < Np; i += 64) {
+                    const int x = i ^ h;
+                    if (x > i) {
+                        const uint32_t a = s32[i], c = s32[x];
+                        if ((a > c) == ((i & k) == 0)) {
+                            s32[i] = c;
+                            s32[x] = a;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        for (int i = lane; i < N; i += 64) s32[i] = IMASK - (s32[i] & IMASK);
+        __syncthreads();
+        /* flood */
+        for (int i = 0; i < N; i++) {
+            const uint32_t p = s32[i] & 0xFFFFu;
+            if (lane == 0) {
+                zp[p] = (uint16_t)p;
+                rep[p] = (uint16_t)p;
+                rk[p] = 0;
+            }
+            uint32_t r = NONE;
+            if (has_nb) {
+                const uint32_t z = small_div(p, (uint32_t)WH, mWH), pr = p - z * (uint32_t)WH;
+                const uint32_t y = small_div(pr, (uint32_t)W, mW), x = pr - y * (uint32_t)W;
+                const int nx = (int)x + ndx, ny = (int)y + ndy, nz = (int)z + ndz;
+                if (nx >= 0 && nx < W && ny >= 0 && ny < H && nz >= 0 && nz < D) {
+                    uint32_t q = (uint32_t)(nx + ny * W + nz * WH);
+                    if (zp[q] != NONE) {
+                        while (zp[q] != q) {
+                            const uint32_t g = zp[zp[q]];
+                            zp[q] = (uint16_t)g;
+                            q = g;
+                        }
+                        r = q;
+                    }
+                }
+            }
+            __syncthreads();
+            /* the neighbours' sets, in lane order: each distinct one hangs below p and joins p's set (a set
+             * merged earlier in this step is no longer a root, or is p's set itself) */
+            uint32_t cur = p;
+            for (int k = 0; k < nb.n; k++) {
+                const uint32_t rr = (uint32_t)__shfl((int)r, k, 64);
+                if (lane == 0 && rr != NONE && rr != cur && zp[rr] == rr) {
+                    pa[rep[rr]] = (uint16_t)p;
+                    const uint32_t ka = rk[cur], kb = rk[rr];
+                    const uint32_t a = ka >= kb ? cur : rr, c = ka >= kb ? rr : cur;
+                    zp[c] = (uint16_t)a;
+                    if (ka == kb) rk[a] = (uint8_t)(ka + 1);
+                    rep[a] = (uint16_t)p;
+                    cur = a;
+                }
+            }
+            __syncthreads();
+        }
+        /* levelroots: pointer jumping over same-key parents (zp is free now) */
+        for (int l = lane; l < N; l += 64) {
+            const int q = pa[l];
+            zp[l] = (uint16_t)((q != l && kk[q] == kk[l]) ? q : l);
+        }
+        __syncthreads();
+        for (;;) {
+            bool changed = false;
+            for (int l = lane; l < N; l += 64) {
+                const uint32_t a = zp[l], c = zp[a];
+                if (c != a) {
+                    zp[l] = (uint16_t)c;
+                    changed = true;
+                }
+            }
+            __syncthreads();
+            if (!__ballot(changed)) break;
+        }
+        /* canonical parents; area counts in the high halves of s32 */
+        for (int l = lane; l < N; l += 64) {
+            const int q = pa[l];
+            pa[l] = (uint16_t)(q != l ? zp[q] : l);
+            s32[l] &= 0xFFFFu;
+        }
+        __syncthreads();
+        for (int l = lane; l < N; l += 64) atomicAdd(&s32[zp[l]], 1u << 16);
+        __syncthreads();
+        /* levelroot totals into their parents, one run of equal keys after the other */
+        for (int i = 0; i < N;) {
+            const uint32_t k0 = kk[s32[i] & 0xFFFFu];
+            int e = i;
+            for (;;) {
+                const int at = e + lane;
+                const bool same = at < N && kk[s32[at] & 0xFFFFu] == k0;
+                const uint64_t m = __ballot(!same);
+                if (m) {
+                    e += __ffsll((unsigned long long)m) - 1;
+                    break;
+                }
+                e += 64;
+            }
+            for (int at = i + lane; at < e; at += 64) {
+                const int p = (int)(s32[at] & 0xFFFFu);
+                if (zp[p] == p && pa[p] != p) atomicAdd(&s32[pa[p]], s32[p] & 0xFFFF0000u);
+            }
+            __syncthreads();
+            i = e;
+        }
+        for (int l = lane; l < N; l += 64) {
+            const int c = pa[l];
+            par[off + l] = c == l ? -1 : (int32_t)(off + c);
+            area[off + l] = s32[l] >> 16;
+        }
+        __syncthreads();
+    }
+}
+
 /* ------------------------------------------------------------------------------------------------ */
 /* host side                                                                                        */
 /* ------------------------------------------------------------------------------------------------ */
@@ -1828,8 +2019,52 @@ static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAtt
     return DCF_OK;
 }
 
+
+/* a batch of small images (every one at most DCF_SMALL_MAX pixels, attribute group 0): the LDS build
+ * (k_small_build, about 52 KB of LDS per workgroup, three per CU); the canonical parent array into
+ * t->parent, the areas into t->attr_pix */
+static int small_build(dcf_tree *t)
+{
+    dcf_ctx *ctx = t->ctx;
+    const int64_t n = t->n;
+    const int nb_img = t->nbatch;
+    std::vector<BatchImg> tab((size_t)nb_img);
+    DCF_HIP(ctx, hipMemcpyAsync(tab.data(), t->bimg, tab.size() * sizeof(BatchImg), hipMemcpyDeviceToHost, ctx->stream));
+    DCF_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::vector<uint32_t> list((size_t)nb_img);
+    for (int i = 0; i < nb_img; i++) {
+        if ((uint64_t)tab[i].W * tab[i].H * tab[i].D > DCF_SMALL_MAX)
+            return dcf_set_error(ctx, DCF_EINVAL, "image %d: too large for the LDS build", i);
+        list[(size_t)i] = (uint32_t)i;
+    }
+    Scratch sc{ ctx, {} };
+    uint32_t *dl = (uint32_t *)sc.get(list.size() * 4);
+    uint32_t *queue = (uint32_t *)sc.get(4);
+    if (!dl || !queue) return dcf_set_error(ctx, DCF_ENOMEM, "small-image list");
+    DevAttr A;
+    int rc = dcf_alloc_attr(ctx, A, 0, n);
+    if (rc != DCF_OK) return rc;
+    t->attr_pix = A;
+    DCF_HIP(ctx, hipMemcpyAsync(dl, list.data(), list.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    DCF_HIP(ctx, hipStreamSynchronize(ctx->stream)); /* the host list is freed on return */
+    DCF_LAUNCH(ctx, "k_fill", k_fill_u32f, dim3(1), dim3(256), 0, queue, 0u, (int64_t)1);
+    ImgView iv;
+    iv.img = t->img;
+    iv.dtype = t->dtype;
+    iv.invert = 0;
+    iv.gmax = t->gmax;
+    const Nbr nb = make_nbr(t->conn, t->batch3d);
+    DCF_LAUNCH(ctx, "k_small_build", k_small_build<DCF_SMALL_MAX>, dim3(std::min(nb_img, 768)), dim3(64), 0, iv, t->bimg, dl,
+               (uint32_t)nb_img, nb, t->parent, A.area, queue);
+    return DCF_OK;
+}
+
 int dcf_flood_build(dcf_tree *t, int grp)
 {
+    if (t->small_batch) {
+        if (grp != 0) return dcf_set_error(t->ctx, DCF_EINVAL, "the LDS build accumulates the area only");
+        return small_build(t);
+    }
     dcf_ctx *ctx = t->ctx;
     const int64_t n = t->n;
     const int is3d = t->bimg ? t->batch3d : t->dims[2] > 1;

@@ -30,6 +30,9 @@
 
 #define DCF_MAX_SUMS 9
 
+/* the largest image the LDS build takes (a power of two: its LDS arrays hold 13 bytes per pixel) */
+#define DCF_SMALL_MAX 4096
+
 struct DevAttr {
     uint32_t *area = nullptr;
     int64_t *s[DCF_MAX_SUMS] = {};
@@ -113,6 +116,9 @@ struct dcf_ctx {
     /* first-pass pixel levels: grid cap of k_final_pixels, whose blocks then carry their totals over
      * several chunks of the level (0: one chunk per block) (DCF_FINAL_BLOCKS) */
     int final_blocks = 1024;
+    /* batched form: images of at most small_pixels pixels (and at most DCF_SMALL_MAX) are built in LDS,
+     * one workgroup per image, with the area attribute group (DCF_SMALL_PIXELS; 0, the default: never) */
+    int64_t small_pixels = 0;
     std::vector<dcf_ctx *> workers;
 };
 
@@ -175,6 +181,8 @@ struct dcf_tree {
     int nbatch = 0;
     int batch3d = 0;
     const double *nlam = nullptr;
+    /* a batch of small images (at most DCF_SMALL_MAX pixels each): built in LDS, one image per workgroup */
+    int small_batch = 0;
     /* a tile of a tiled build: its origin in the whole image, added to the coordinates the attributes
      * sum (so they are those of the whole image) */
     int off[3] = { 0, 0, 0 };

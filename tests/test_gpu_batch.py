@@ -260,3 +260,77 @@ def test_large_images_on_worker_contexts(workers):
             bad = np.nonzero(out.reshape(-1) != want.reshape(-1))[0]
             assert len(bad) == 0, "image %d %s %s decision %d: %d pixels differ" % (k, img.shape, tt, dec, len(bad))
     c.close()
+
+
+def _small_images(rng, is3d):
+    """images on both sides of the LDS build's 4096-pixel limit (from one pixel up), flat ones, u8 and u16, few
+    and many grey levels"""
+    if is3d:
+        shapes = [(2, 1, 1), (2, 2, 2), (16, 16, 16), (16, 16, 17), (4, 64, 64), (8, 8, 37), (3, 5, 7)]
+    else:
+        shapes = [(1, 1), (1, 37), (37, 1), (64, 64), (64, 65), (128, 128), (17, 240), (100, 3), (90, 90)]
+    imgs = []
+    for k, shape in enumerate(shapes):
+        levels = [2, 7, 256, 4000, 65536][k % 5]
+        dt = np.uint16 if levels > 256 or k % 3 == 0 else np.uint8
+        img = rng.integers(0, levels, size=shape).astype(dt)
+        if k % 4 == 3:
+            img[:] = img.reshape(-1)[0]
+        imgs.append(img)
+    imgs.append(synth.disks(96, 120, True, 7) if not is3d else synth.spheres(12, 20, 24))
+    return imgs
+
+
+@pytest.mark.parametrize("is3d,conn", [(False, 4), (False, 8), (True, 6), (True, 26)])
+def test_small_images_built_in_lds(is3d, conn):
+    """images of at most 4096 pixels are built whole in one workgroup's LDS (k_small_build): sorted by key,
+    flooded with a union-find in that order, levelroots by pointer jumping; the larger ones take the level loop
+    in the same call.  Every output equals the oracle's, for the max- and the min-tree of each image (one call,
+    a tree type per image) and every rule, and equals the level loop's (DCF_SMALL_PIXELS=0)"""
+    rng = np.random.default_rng(90 + conn)
+    imgs = _small_images(rng, is3d)
+    imgs = imgs + imgs
+    tts = ["max"] * (len(imgs) // 2) + ["min"] * (len(imgs) // 2)
+    lams = [float(rng.choice([1.0, 3.0, 40.0, 300.0, 1e9])) for _ in imgs]
+    lds, loop = dcf.Context(0), dcf.Context(0)
+    lds.set_option("DCF_SMALL_PIXELS", "4096")
+    loop.set_option("DCF_SMALL_PIXELS", "0")
+    for dec in range(4):
+        say("LDS build", "3D" if is3d else "2D", "conn", conn, "decision", dec, len(imgs), "images")
+        a = lds.batch_filter(imgs, lams, connectivity=conn, decision=dec, extract=dec == 1, tree_types=tts)
+        b = loop.batch_filter(imgs, lams, connectivity=conn, decision=dec, extract=dec == 1, tree_types=tts)
+        for k, (img, lam, tt) in enumerate(zip(imgs, lams, tts)):
+            want = OracleTree(img, conn, tt, 0).filter(0, lam, dec, extract=dec == 1).astype(img.dtype)
+            assert np.array_equal(a[k], want), "image %d %s %s %s lam %g decision %d" % (k, img.shape, img.dtype, tt, lam, dec)
+            assert np.array_equal(b[k], want)
+    lds.close()
+    loop.close()
+
+
+def test_small_images_with_other_attributes():
+    """the LDS build accumulates the area only: with another attribute the small images take the level loop"""
+    rng = np.random.default_rng(77)
+    imgs = _small_images(rng, False)[:6]
+    lams = [5.0] * len(imgs)
+    c = dcf.Context(0)
+    c.set_option("DCF_SMALL_PIXELS", "4096")
+    for attr in (1, 13):
+        check_batch(c, imgs, lams, 8, "max", decision=0, attr=attr, label="attr %d" % attr)
+    c.close()
+
+
+def test_many_small_images_queue():
+    """the LDS build with more images than workgroups: every workgroup takes image after image from the queue"""
+    rng = np.random.default_rng(5)
+    imgs, lams = [], []
+    for k in range(1500):
+        side = int(rng.integers(1, 40)), int(rng.integers(1, 40))
+        imgs.append(rng.integers(0, [3, 256, 65536][k % 3], size=side).astype(np.uint16 if k % 3 == 2 else np.uint8))
+        lams.append(float(rng.choice([1.0, 4.0, 20.0])))
+    c = dcf.Context(0)
+    c.set_option("DCF_SMALL_PIXELS", "4096")
+    outs = c.batch_filter(imgs, lams, connectivity=4, decision=0)
+    for k in range(0, 1500, 7):
+        want = OracleTree(imgs[k], 4, "max", 0).filter(0, lams[k], 0).astype(imgs[k].dtype)
+        assert np.array_equal(outs[k], want), "image %d %s" % (k, imgs[k].shape)
+    c.close()
