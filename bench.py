@@ -232,7 +232,7 @@ def traffic_summary(args):
            "fetch_bytes_per_launch": round(tot["FETCH_SIZE"] / tot["n"]),
            "write_bytes_per_launch": round(tot["WRITE_SIZE"] / tot["n"]),
            "launches_counted": tot["n"], "fetch_scale": scale[0], "write_scale": scale[1]}
-    path = os.path.join(ROOT, "profiles", "traffic_%s_%s.json" % (kernel, workload))
+    path = args.traffic_out or os.path.join(ROOT, "profiles", "traffic_%s_%s.json" % (kernel, workload))
     with open(path, "w") as f:
         json.dump(out, f, indent=1)
     print("\nFETCH_SIZE scale %.4f, WRITE_SIZE scale %.4f; all listed kernels: %.2f MB per launch -> %s" % (
@@ -504,6 +504,8 @@ def main():
                          "profiles/traffic_<--roofline-kernel>_<workload>.json, scaled by the --copy runs")
     ap.add_argument("--copy", nargs=2, metavar=("FETCH_RUN", "WRITE_RUN"),
                     help="the two --pmc runs of `bench.py --calibrate-copy MIB` that scale --traffic")
+    ap.add_argument("--traffic-out", default=None, metavar="JSON",
+                    help="where --traffic writes (default profiles/traffic_<kernel>_<workload>.json)")
     ap.add_argument("--no-profile", action="store_true",
                     help="skip the profiled timed region (no roofline object; for counter runs)")
     args = ap.parse_args()
