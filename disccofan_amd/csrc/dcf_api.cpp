@@ -195,6 +195,7 @@ static int set_option(dcf_ctx *c, const char *name, const char *value)
     if (n == "DCF_WORKERS") return integer(1, 16, c->max_workers);
     if (n == "DCF_BATCH_PARTS") return integer(0, 64, c->batch_parts);
     if (n == "DCF_FINAL_BLOCKS") return integer(0, 1 << 20, c->final_blocks);
+    if (n == "DCF_FINE_LOOPS") return integer(1, 4, c->fine_loops);
     if (n == "DCF_SMALL_PIXELS") {
         int v = 0;
         if (integer(0, DCF_SMALL_MAX, v) != DCF_OK) return DCF_EINVAL;
@@ -259,7 +260,7 @@ try {
     /* the build's tunables: the environment gives a new context its defaults (dcf_ctx_set_option
      * changes them afterwards) */
     static const char *const kOptions[] = { "DCF_ZONES", "DCF_TARGETS", "DCF_RECORDS", "DCF_FIX_BLOCKS", "DCF_GRAPH",
-                                            "DCF_ZIP_SEGMENTS", "DCF_WORKERS", "DCF_BATCH_PARTS", "DCF_FINAL_BLOCKS", "DCF_SMALL_PIXELS", "DCF_LARGE_PIXELS", "DCF_ZONE_TILE2D",
+                                            "DCF_ZIP_SEGMENTS", "DCF_WORKERS", "DCF_BATCH_PARTS", "DCF_FINAL_BLOCKS", "DCF_SMALL_PIXELS", "DCF_FINE_LOOPS", "DCF_LARGE_PIXELS", "DCF_ZONE_TILE2D",
                                             "DCF_ZONE_TILE3D", "DCF_LEVEL_TILE2D", "DCF_LEVEL_GROUP2D", "DCF_LEVEL_TILE3D",
                                             "DCF_LEVEL_GROUP3D" };
     for (const char *name : kOptions)
@@ -293,6 +294,7 @@ extern "C" int dcf_ctx_destroy(dcf_ctx *ctx)
     for (auto &f : ctx->free_blocks) (void)hipFree(f.second);
     for (auto &l : ctx->live_blocks) (void)hipFree(l.first);
     if (ctx->own) (void)hipStreamDestroy(ctx->own);
+    for (hipStream_t st : ctx->aux) (void)hipStreamDestroy(st);
     delete ctx;
     return DCF_OK;
 }
@@ -724,6 +726,8 @@ static int run_on_workers(dcf_ctx *ctx, int njobs, const std::function<int(dcf_c
         w->zones = ctx->zones; w->targets = ctx->targets; w->records = ctx->records; w->fix_blocks = ctx->fix_blocks;
         w->graph = ctx->graph; w->zip_segments = ctx->zip_segments; w->large_pixels = ctx->large_pixels;
         w->final_blocks = ctx->final_blocks; w->small_pixels = ctx->small_pixels;
+        /* concurrent parts already keep the GPU busy: one level loop each (more loops measured slower) */
+        w->fine_loops = K > 1 ? 1 : ctx->fine_loops;
         memcpy(w->zt2, ctx->zt2, sizeof w->zt2); w->zt2_set = ctx->zt2_set; w->zt3 = ctx->zt3;
         memcpy(w->lt2, ctx->lt2, sizeof w->lt2); memcpy(w->lg2, ctx->lg2, sizeof w->lg2);
         memcpy(w->lt3, ctx->lt3, sizeof w->lt3); memcpy(w->lg3, ctx->lg3, sizeof w->lg3);

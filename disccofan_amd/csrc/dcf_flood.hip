@@ -191,21 +191,33 @@ __device__ __forceinline__ uint32_t uf_unite(const UFV &uf, uint32_t a, uint32_t
 /* ------------------------------------------------------------------------------------------------ */
 /* counting sort of the pixels by level digit                                                       */
 /* ------------------------------------------------------------------------------------------------ */
+/* Refining passes may run as K independent level loops (1 <= K <= FINE_LOOPS_MAX): a coarse region's sets
+ * hold its own pixels and its children's super-vertices only, so the regions refine independently, and a
+ * pixel belongs to loop cnode[p] % K.  Its bin in the level lists is then loop * NLEV + level digit. */
+#define FINE_LOOPS_MAX 4
+#define FINE_LOOPS_MIN_PIXELS ((int64_t)1 << 22)
+__device__ __forceinline__ uint32_t lvl_bin(uint32_t d, int64_t p, const uint32_t *cnode, uint32_t K)
+{
+    return K > 1 ? (cnode[p] % K) * NLEV + d : d;
+}
+
 /* only != nullptr: count only the pixels with only[p] == p (zone representatives) */
 __global__ __launch_bounds__(256) void k_lvl_hist(ImgView iv, int pass, int shift, int64_t n, const uint32_t *__restrict__ only,
-                                                  uint32_t *__restrict__ hist)
+                                                  uint32_t *__restrict__ hist, const uint32_t *__restrict__ cnode, uint32_t K)
 {
-    __shared__ uint32_t sh[NLEV];
-    sh[threadIdx.x] = 0;
+    __shared__ uint32_t sh[NLEV * FINE_LOOPS_MAX];
+    const uint32_t nbins = NLEV * K;
+    for (uint32_t b = threadIdx.x; b < nbins; b += blockDim.x) sh[b] = 0;
     __syncthreads();
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
         if (only && only[p] != (uint32_t)p) continue;
         uint32_t k = fl_key(iv, p);
         uint32_t d = (k >> shift) & 255u;
-        atomicAdd(&sh[d], 1u);
+        atomicAdd(&sh[lvl_bin(d, p, cnode, K)], 1u);
     }
     __syncthreads();
-    if (sh[threadIdx.x]) atomicAdd(&hist[threadIdx.x], sh[threadIdx.x]);
+    for (uint32_t b = threadIdx.x; b < nbins; b += blockDim.x)
+        if (sh[b]) atomicAdd(&hist[b], sh[b]);
 }
 
 struct TileGeo {
@@ -554,10 +566,11 @@ template <int NB>
 __global__ __launch_bounds__(256) void k_lvl_scatter(ImgView iv, int pass, int shift, int64_t n, ScatterGeo sg,
                                                      const uint32_t *__restrict__ only, uint32_t *__restrict__ cursor,
                                                      uint32_t *__restrict__ order, LevelArgs a, uint32_t kc_count,
-                                                     uint32_t *__restrict__ tgt)
+                                                     uint32_t *__restrict__ tgt, uint32_t K)
 {
-    __shared__ uint32_t cnt[NLEV], base[NLEV];
-    cnt[threadIdx.x] = 0;
+    __shared__ uint32_t cnt[NLEV * FINE_LOOPS_MAX], base[NLEV * FINE_LOOPS_MAX];
+    const uint32_t nbins = NLEV * K;
+    for (uint32_t b = threadIdx.x; b < nbins; b += blockDim.x) cnt[b] = 0;
     __syncthreads();
     const int tsz = sg.TW * sg.TH * sg.TD;
     int64_t p0 = 0;
@@ -592,19 +605,22 @@ __global__ __launch_bounds__(256) void k_lvl_scatter(ImgView iv, int pass, int s
         const int64_t p = pix(l);
         if (p < 0) continue;
         uint32_t k = fl_key(iv, p);
-        atomicAdd(&cnt[(k >> shift) & 255u], 1u);
+        atomicAdd(&cnt[lvl_bin((k >> shift) & 255u, p, a.cnode, K)], 1u);
     }
     __syncthreads();
-    if (cnt[threadIdx.x]) base[threadIdx.x] = atomicAdd(&cursor[threadIdx.x], cnt[threadIdx.x]);
-    cnt[threadIdx.x] = 0;
+    for (uint32_t b = threadIdx.x; b < nbins; b += blockDim.x) {
+        if (cnt[b]) base[b] = atomicAdd(&cursor[b], cnt[b]);
+        cnt[b] = 0;
+    }
     __syncthreads();
     for (int l = threadIdx.x; l < tsz; l += blockDim.x) {
         const int64_t p = pix(l);
         if (p < 0) continue;
         uint32_t k = fl_key(iv, p);
         uint32_t d = (k >> shift) & 255u;
-        uint32_t r = atomicAdd(&cnt[d], 1u);
-        order[base[d] + r] = (uint32_t)p;
+        const uint32_t bin = lvl_bin(d, p, a.cnode, K);
+        uint32_t r = atomicAdd(&cnt[bin], 1u);
+        order[base[bin] + r] = (uint32_t)p;
         if constexpr (NB > 0) {
             uint32_t t[NB];
             /* the block's tile knows its image: no lookup in the image table */
@@ -612,7 +628,7 @@ __global__ __launch_bounds__(256) void k_lvl_scatter(ImgView iv, int pass, int s
             const int zz = (int)(((int64_t)p - p0) / ((int64_t)iW * iH));
             const PixGeo pg{ l % iW, l / iW, zz, iW, iH, iD };
             pixel_targets<NB>(a, pg, (uint32_t)p, d, kc_count, t);
-            uint32_t *dst = tgt + (size_t)(base[d] + r) * NB;
+            uint32_t *dst = tgt + (size_t)(base[bin] + r) * NB;
             if (NB % 4 == 0) {
 #pragma unroll
                 for (int k = 0; k < NB; k += 4) *(uint4 *)(dst + k) = make_uint4(t[k], t[k + 1], t[k + 2], t[k + 3]);
@@ -1766,11 +1782,17 @@ static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAtt
                     int64_t kc, bool zones, uint32_t *tgt, int is3d, int64_t batch_tiles)
 {
     const int64_t n = la.g.n;
-    uint32_t *hist = (uint32_t *)sc.get(3 * NLEV * 4);
+    /* a refining pass of a large image runs as K independent level loops, each on a stream of its own
+     * (not under graph capture): level lists, absorbed roots and fix queues per loop, bin = loop * NLEV +
+     * level.  Small images keep one loop: their levels are launch-bound already */
+    const uint32_t K = (la.pass == 2 && !ctx->graph && ctx->fine_loops > 1 && n >= FINE_LOOPS_MIN_PIXELS)
+                           ? (uint32_t)std::min(ctx->fine_loops, FINE_LOOPS_MAX) : 1u;
+    const uint32_t NB = K * NLEV;
+    uint32_t *hist = (uint32_t *)sc.get((size_t)(2 * NB + NLEV) * 4);
     uint32_t *order = (uint32_t *)sc.get((size_t)n * 4);
     if (!hist || !order) return dcf_set_error(ctx, DCF_ENOMEM, "level lists");
-    uint32_t *cursor = hist + NLEV, *hist_all = hist + 2 * NLEV;
-    DCF_LAUNCH(ctx, "k_fill", k_fill_u32f, dim3(3), dim3(256), 0, hist, 0u, (int64_t)3 * NLEV);
+    uint32_t *cursor = hist + NB, *hist_all = hist + 2 * NB;
+    DCF_LAUNCH(ctx, "k_fill", k_fill_u32f, dim3(nblk(2 * NB + NLEV, 256)), dim3(256), 0, hist, 0u, (int64_t)(2 * NB + NLEV));
     unsigned hb = nblk(n, 256);
     if (hb > 2048) hb = 2048;
     /* zone pass: label the tile zones first (pixels pre-linked to their representative, zone pixel
@@ -1817,11 +1839,12 @@ static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAtt
             default: DCF_LAUNCH(ctx, "k_zone_totals", k_zone_totals<4>, dim3(nblk(n, 256)), dim3(256), 0, la, (const uint32_t *)zrep, A, n); break;
             }
         DCF_LAUNCH(ctx, "k_lvl_hist", k_lvl_hist, dim3(hb), dim3(256), 0, la.iv, la.pass, la.shift, n, (const uint32_t *)nullptr,
-                   hist_all);
+                   hist_all, (const uint32_t *)nullptr, 1u);
     }
-    DCF_LAUNCH(ctx, "k_lvl_hist", k_lvl_hist, dim3(hb), dim3(256), 0, la.iv, la.pass, la.shift, n, (const uint32_t *)zrep, hist);
-    uint32_t h[NLEV], hall[NLEV];
-    DCF_HIP(ctx, hipMemcpyAsync(h, hist, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+    DCF_LAUNCH(ctx, "k_lvl_hist", k_lvl_hist, dim3(hb), dim3(256), 0, la.iv, la.pass, la.shift, n, (const uint32_t *)zrep, hist,
+               la.cnode, K);
+    uint32_t h[NLEV * FINE_LOOPS_MAX], hall[NLEV];
+    DCF_HIP(ctx, hipMemcpyAsync(h, hist, (size_t)NB * 4, hipMemcpyDeviceToHost, ctx->stream));
     if (zones) DCF_HIP(ctx, hipMemcpyAsync(hall, hist_all, sizeof hall, hipMemcpyDeviceToHost, ctx->stream));
     DCF_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (zones) {
@@ -1833,16 +1856,16 @@ static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAtt
             zones = false;
             la.zone = 0;
             zrep = nullptr;
-            memcpy(h, hall, sizeof h);
+            memcpy(h, hall, sizeof hall);
             DCF_LAUNCH(ctx, "k_uf_init", k_uf_init, dim3(nblk(n, 256)), dim3(256), 0, la.iv, la.pass, la.shift, n, n, la.uf);
             int rc = clear_attr(ctx, A, n);
             if (rc) return rc;
         }
     }
-    uint32_t off[NLEV + 1];
+    uint32_t off[NLEV * FINE_LOOPS_MAX + 1];
     off[0] = 0;
-    for (int i = 0; i < NLEV; i++) off[i + 1] = off[i] + h[i];
-    DCF_HIP(ctx, hipMemcpyAsync(cursor, off, NLEV * 4, hipMemcpyHostToDevice, ctx->stream));
+    for (uint32_t i = 0; i < NB; i++) off[i + 1] = off[i] + h[i];
+    DCF_HIP(ctx, hipMemcpyAsync(cursor, off, (size_t)NB * 4, hipMemcpyHostToDevice, ctx->stream));
     ScatterGeo sg;
     sg.batch = la.bimg != nullptr;
     sg.bimg = la.bimg;
@@ -1868,7 +1891,7 @@ static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAtt
     if (zones) tgt = nullptr;
 #define DCF_SCATTER(NBV)                                                                                                    \
     DCF_LAUNCH(ctx, "k_lvl_scatter", k_lvl_scatter<NBV>, dim3((unsigned)sblocks), dim3(256), 0, la.iv, la.pass, la.shift, n, sg, \
-               (const uint32_t *)zrep, cursor, order, la, (uint32_t)kc, tgt)
+               (const uint32_t *)zrep, cursor, order, la, (uint32_t)kc, tgt, K)
     if (!tgt) DCF_SCATTER(0);
     else switch (la.nb.n) {
         case 4: DCF_SCATTER(4); break;
@@ -1920,21 +1943,63 @@ static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAtt
     int rc = alloc_attr_n(ctx, &sc, P, grp, nel);
     if (rc || (rc = clear_attr(ctx, P, nel))) return rc;
     DCF_LAUNCH(ctx, "k_fill", k_fill_u32f, dim3(2), dim3(256), 0, la.fcount, 0u, (int64_t)(NLEV + 4));
+    /* every further loop: its own absorbed roots and fix queue, its own stream */
+    LevelArgs lg[FINE_LOOPS_MAX];
+    hipStream_t strm[FINE_LOOPS_MAX];
+    lg[0] = la;
+    strm[0] = ctx->stream;
+    while (ctx->aux.size() + 1 < K) {
+        hipStream_t st = nullptr;
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return dcf_set_error(ctx, DCF_EDEVICE, "loop stream");
+        ctx->aux.push_back(st);
+    }
+    for (uint32_t g = 1; g < K; g++) {
+        lg[g] = la;
+        lg[g].absorbed = (uint32_t *)sc.get((size_t)(nel + 16) * 4);
+        lg[g].nabs = (uint32_t *)sc.get((NLEV + 4) * 4);
+        lg[g].fix = (uint32_t *)sc.get((size_t)(nel + 16) * 4);
+        lg[g].fcount = (uint32_t *)sc.get((NLEV + 4) * 4);
+        if (!lg[g].absorbed || !lg[g].nabs || !lg[g].fix || !lg[g].fcount) return dcf_set_error(ctx, DCF_ENOMEM, "loop queues");
+        DCF_LAUNCH(ctx, "k_fill", k_fill_u32f, dim3(2), dim3(256), 0, lg[g].nabs, 0u, (int64_t)(NLEV + 2));
+        DCF_LAUNCH(ctx, "k_fill", k_fill_u32f, dim3(2), dim3(256), 0, lg[g].fcount, 0u, (int64_t)(NLEV + 4));
+        strm[g] = ctx->aux[g - 1];
+    }
+    /* fork: the further loops start after everything queued so far */
+    struct Events {
+        std::vector<hipEvent_t> e;
+        ~Events()
+        {
+            for (hipEvent_t x : e) (void)hipEventDestroy(x);
+        }
+    } ev;
+    if (K > 1) {
+        hipEvent_t f = nullptr;
+        if (hipEventCreateWithFlags(&f, hipEventDisableTiming) != hipSuccess) return dcf_set_error(ctx, DCF_EDEVICE, "event");
+        ev.e.push_back(f);
+        DCF_HIP(ctx, hipEventRecord(f, strm[0]));
+        for (uint32_t g = 1; g < K; g++) DCF_HIP(ctx, hipStreamWaitEvent(strm[g], f, 0));
+    }
     /* the level loop: its launches are fixed by the histogram already on the host.  With ctx->graph on
      * (and no per-kernel events) they are captured on the context's own stream into one HIP graph,
      * which is then launched on the working stream */
+    int ordg[FINE_LOOPS_MAX] = { 0 };
     auto level_loop = [&]() -> int {
-        int ord = 0;
-        for (int lev = NLEV - 1; lev >= 0; lev--) {
-            uint32_t cnt = h[lev];
+        for (int lev = NLEV - 1; lev >= 0; lev--)
+        for (uint32_t g = 0; g < K; g++) {
+            const uint32_t bin = g * NLEV + (uint32_t)lev;
+            uint32_t cnt = h[bin];
             if (!cnt) continue;
-            const uint32_t *lvl = order + off[lev];
+            const uint32_t *lvl = order + off[bin];
+            LevelArgs &la = lg[g];
+            uint32_t *nabs = la.nabs;
+            int &ord = ordg[g];
+            if (K > 1) ctx->stream = strm[g];
             if (zones) {
                 if (ecnt[lev])
                     DCF_LAUNCH(ctx, "k_union", k_union_edges, dim3(nblk(ecnt[lev], 256)), dim3(256), 0, la,
                                (const uint2 *)(edges + bucket_h[lev]), ecnt[lev], (uint32_t)lev);
             } else if (tgt) {
-                const uint32_t *tl = tgt + (size_t)off[lev] * la.nb.n;
+                const uint32_t *tl = tgt + (size_t)off[bin] * la.nb.n;
                 switch (la.nb.n) {
                 case 4: DCF_LAUNCH(ctx, "k_union", k_union_tgt<4>, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, tl, cnt, (uint32_t)lev); break;
                 case 6: DCF_LAUNCH(ctx, "k_union", k_union_tgt<6>, dim3(nblk(cnt, 256)), dim3(256), 0, la, lvl, tl, cnt, (uint32_t)lev); break;
@@ -1989,7 +2054,17 @@ static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAtt
     };
     const bool graph = ctx->graph && !ctx->profiling;
     if (!graph) {
-        if ((rc = level_loop()) != DCF_OK) return rc;
+        rc = level_loop();
+        ctx->stream = strm[0];
+        if (rc != DCF_OK) return rc;
+        /* join: the pass's last kernels wait for every loop */
+        for (uint32_t g = 1; g < K; g++) {
+            hipEvent_t j = nullptr;
+            if (hipEventCreateWithFlags(&j, hipEventDisableTiming) != hipSuccess) return dcf_set_error(ctx, DCF_EDEVICE, "event");
+            ev.e.push_back(j);
+            DCF_HIP(ctx, hipEventRecord(j, strm[g]));
+            DCF_HIP(ctx, hipStreamWaitEvent(strm[0], j, 0));
+        }
     } else {
         hipStream_t work = ctx->stream;
         /* the capture stream is idle: nothing it records runs before the graph is launched on `work` */

@@ -68,6 +68,10 @@ struct dcf_ctx {
     int device = 0;
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
+    /* a refining pass of an image of at least 2^22 pixels runs as fine_loops independent level loops
+     * (DCF_FINE_LOOPS, 1..4); the streams of the further loops, created on first use */
+    std::vector<hipStream_t> aux;
+    int fine_loops = 3;
     std::string err;
     int profiling = 0;
     std::string prof_only; /* empty: every kernel */

@@ -71,7 +71,7 @@ void *dcf_ctx_stream(dcf_ctx *ctx);
 /* Tunables of the build (no counterpart in the reference; they change the schedule, never a result), by
  * the names of the environment variables that give new contexts their defaults: DCF_ZONES, DCF_TARGETS,
  * DCF_RECORDS, DCF_FIX_BLOCKS, DCF_FINAL_BLOCKS, DCF_GRAPH, DCF_ZIP_SEGMENTS, DCF_WORKERS, DCF_BATCH_PARTS,
- * DCF_LARGE_PIXELS, DCF_SMALL_PIXELS, DCF_ZONE_TILE2D ("WxH"), DCF_ZONE_TILE3D ("N"), DCF_LEVEL_TILE2D / DCF_LEVEL_GROUP2D / DCF_LEVEL_TILE3D / DCF_LEVEL_GROUP3D
+ * DCF_LARGE_PIXELS, DCF_SMALL_PIXELS, DCF_FINE_LOOPS, DCF_ZONE_TILE2D ("WxH"), DCF_ZONE_TILE3D ("N"), DCF_LEVEL_TILE2D / DCF_LEVEL_GROUP2D / DCF_LEVEL_TILE3D / DCF_LEVEL_GROUP3D
  * ("WxHxD").  DCF_EINVAL for
  * an unknown name or a value out of range.  The autotuner (disccofan_amd/tune.py) sets them here. */
 int dcf_ctx_set_option(dcf_ctx *ctx, const char *name, const char *value);

@@ -675,3 +675,26 @@ def test_context_options():
             ctx.set_option(name, value)
         assert e.value.code == -1, (name, value)
     ctx.close()
+
+
+@pytest.mark.parametrize("tree_type", ["max", "min"])
+def test_refining_pass_loops(tree_type):
+    """the refining pass of an image of at least 2^22 pixels runs as DCF_FINE_LOOPS independent level loops (coarse
+    regions dealt by node id, each loop on its own stream): 1 to 4 loops give the oracle's parent array, areas
+    and subtractive filter; out-of-range loop counts are refused"""
+    img = synth.vnoise(2048, 2048)
+    o = OracleTree(img, 8, tree_type, 0, threads=16)
+    want_par = o.parent().astype(np.int32)
+    want_out = o.filter(0, 300.0, 3)
+    ctx = dcf.Context(0)
+    for loops in (1, 2, 3, 4):
+        ctx.set_option("DCF_FINE_LOOPS", str(loops))
+        say("refining pass in", loops, "loops,", tree_type, "tree")
+        t = ctx.build(img, 8, tree_type, 0)
+        assert np.array_equal(t.parent(), want_par), loops
+        assert np.array_equal(t.filter(0, 300.0, 3), want_out), loops
+        t.free()
+    for bad in ("0", "5"):
+        with pytest.raises(dcf.DcfError):
+            ctx.set_option("DCF_FINE_LOOPS", bad)
+    ctx.close()
