@@ -30,6 +30,7 @@ KNOBS_2D = {
     "DCF_LEVEL_GROUP2D": ["1x1x1", "2x2x1", "4x1x1"],
     "DCF_ZONE_TILE2D": ["64x64", "32x32", "128x32", "32x128"],
     "DCF_FIX_BLOCKS": ["1024", "256"],
+    "DCF_FINE_LOOPS": ["3", "1", "2", "4"],
 }
 KNOBS_3D = {
     "DCF_ZONES": ["2", "1", "0"],
@@ -39,6 +40,7 @@ KNOBS_3D = {
     "DCF_LEVEL_GROUP3D": ["1x1x1", "2x2x2"],
     "DCF_ZONE_TILE3D": ["16", "8", "12"],
     "DCF_FINAL_BLOCKS": ["1024", "0", "512", "2048"],
+    "DCF_FINE_LOOPS": ["3", "1", "2"],
 }
 # the batched form orders its level lists by each image's zone tiles: the zone tiles are its tile shape
 KNOBS_BATCH = {
