@@ -18,7 +18,7 @@ Every workload reports its Mvoxels/s, a roofline object for the dominant kernel 
 algorithmic bytes per launch / its HIP-event launch time, recorded on the stream the kernels run on in
 a profiled timed region of its own; `value` comes from an unprofiled timed region) and a CPU baseline:
 the CPU oracle (OpenMP, the reference's algorithm restated) on a bounded sample of the same workload,
-median of 3 runs, at the host's thread count and at 1 thread.  The 8192^2 workload also reports the
+1 warm-up run then the median of 5, at the host's thread count and at 1 thread.  The 8192^2 workload also reports the
 tiled build (8 row slabs, boundary trees merged on the GPU) and the spectrum kernel.
 
 Printed: ONE JSON line.  `--workload NAME` runs one workload alone.
@@ -104,10 +104,13 @@ def make_image(name: str) -> np.ndarray:
 # ------------------------------------------------------------------------------------------------
 # CPU baseline: the oracle (test infrastructure, the checker) timed on this host, as the baseline only
 # ------------------------------------------------------------------------------------------------
-def _time_oracle(jobs, threads: int, runs: int = 3) -> float:
-    """median wall time of `runs` runs of build + filter of every (image, connectivity, tree, attr, lambda,
-    decision) job"""
+def _time_oracle(jobs, threads: int, runs: int = 5, warmup: int = 1) -> float:
+    """median wall time of `runs` runs, after `warmup` untimed ones, of build + filter of every (image,
+    connectivity, tree, attr, lambda, decision) job"""
     from oracle.oracle import OracleTree
+    for _ in range(warmup):
+        for img, conn, tt, attr, lam, dec in jobs:
+            OracleTree(img, conn, tt, attr, threads=threads).filter(attr, lam, dec)
     times = []
     for _ in range(runs):
         t0 = time.perf_counter()
@@ -122,7 +125,7 @@ def _time_oracle(jobs, threads: int, runs: int = 3) -> float:
 def cpu_baseline(name: str, w: dict, data, threads: int, nproc: int, model: str):
     """The CPU oracle (radix ranks, slab flooding, slab fuse, the reference's filter walks; OpenMP) on a
     bounded sample of the workload: the multi-threaded figure on a larger sample, the 1-thread one on a
-    smaller one, each the median of 3 runs."""
+    smaller one, each 1 warm-up run and then the median of 5 runs."""
     if name == "batch256":
         imgs, lams = data
         jobs_mt = [(a, w["conn"], "max", w["attr"], l, w["decision"]) for a, l in zip(imgs[:32], lams[:32])]
@@ -144,9 +147,9 @@ def cpu_baseline(name: str, w: dict, data, threads: int, nproc: int, model: str)
     t_1 = _time_oracle(jobs_1, 1)
     trees = "+".join(w["trees"]) + "-tree"
     return {"value": round(vox_mt / t_mt / 1e6, 3), "unit": "Mvoxels/s", "cores": threads, "kind": "port",
-            "sample": "%s, %s build+filter, median of 3 runs (%.2f s each)" % (desc_mt, trees, t_mt),
+            "sample": "%s, %s build+filter, 1 warm-up then the median of 5 runs (%.2f s each)" % (desc_mt, trees, t_mt),
             "one_thread": {"value": round(vox_1 / t_1 / 1e6, 3), "cores": 1,
-                           "sample": "%s, %s build+filter, median of 3 runs (%.2f s each)" % (desc_1, trees, t_1)},
+                           "sample": "%s, %s build+filter, 1 warm-up then the median of 5 runs (%.2f s each)" % (desc_1, trees, t_1)},
             "nproc": nproc, "cpu_model": model}
 
 
