@@ -410,6 +410,31 @@ def run_workload(name: str, ctx, args, kernel: str, cpu: bool, say):
     if not batch:
         out["nodes_per_tree"] = state.get("K")
 
+    if not args.no_host:
+        # the same calls on host numpy buffers: the image goes to the GPU and the output comes back in every step
+        if batch:
+            def host_step():
+                ctx.batch_filter(imgs, lams, connectivity=w["conn"], tree_type="max", attr_idx=w["attr"],
+                                 decision=w["decision"])
+            hform = "one dcf_batch_filter call on host buffers"
+        elif plain_step is not None:
+            def host_step():
+                ctx.batch_filter([img] * len(w["trees"]), [w["lam"]] * len(w["trees"]), connectivity=w["conn"],
+                                 attr_idx=w["attr"], decision=w["decision"], tree_types=list(w["trees"]))
+            hform = "both trees in one dcf_batch_filter call on host buffers"
+        else:
+            def host_step():
+                for tt in w["trees"]:
+                    tr = ctx.build(img, w["conn"], tt, w["attr"])
+                    tr.filter(w["attr"], w["lam"], w["decision"], out_dtype=img.dtype)
+                    tr.free()
+            hform = "dcf_tree_build + dcf_filter on host buffers"
+        host_step()
+        hms = timed(host_step, 1 if batch else min(steps, 3))
+        out["host_buffers"] = {"value": round(vox_per_step / (hms / 1e3) / 1e6, 3), "ms_per_step": round(hms, 3),
+                               "form": hform + " (image in, output out over PCIe in every step)"}
+        say("%s on host buffers: %.1f ms (device-resident %.1f ms)" % (name, hms, ms))
+
     if not batch and not args.no_extras:
         # the spectrum kernel on the max-tree: B + 8 algorithmic bytes per voxel
         nb, lo, hi = w["spec_bins"]
@@ -490,6 +515,7 @@ def main():
                     help="run one workload alone (default: all three, vnoise8192 giving `value`)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain (per-image / per-tree) comparison")
+    ap.add_argument("--no-host", action="store_true", help="skip the host-buffer (PCIe-inclusive) figure")
     ap.add_argument("--sequential", action="store_true",
                     help="vnoise8192: time the plain form (one call per tree, one after the other) as the step")
     ap.add_argument("--no-extras", action="store_true", help="skip the spectrum and tiled-build figures")
@@ -589,6 +615,7 @@ def main():
         "config": head["config"],
         "roofline": head["roofline"],
         "cpu_baseline": head.get("cpu_baseline"),
+        "host_buffers": head.get("host_buffers"),
         "plain_form": head.get("plain_form", batch.get("plain_form")),
         "batched_vs_plain": head.get("batched_vs_plain", batch.get("batched_vs_plain")),
         "workloads": {k: v for k, v in res.items() if k != names[0]},
