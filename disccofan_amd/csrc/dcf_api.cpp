@@ -753,7 +753,14 @@ static int run_on_workers(dcf_ctx *ctx, int njobs, const std::function<int(dcf_c
             j.t.emplace_back([&, k]() {
                 dcf_ctx *w = ctx->workers[(size_t)k];
                 (void)hipSetDevice(w->device);
-                for (int i = next++; i < njobs && rcs[(size_t)k] == DCF_OK; i = next++) rcs[(size_t)k] = job(w, i);
+                for (int i = next++; i < njobs && rcs[(size_t)k] == DCF_OK; i = next++) {
+                    /* no exception may leave a thread (it would end the process) */
+                    try {
+                        rcs[(size_t)k] = job(w, i);
+                    } catch (const std::exception &e) {
+                        rcs[(size_t)k] = dcf_set_error(w, DCF_ENOMEM, "out of host memory (%s)", e.what());
+                    }
+                }
                 if (hipStreamSynchronize(w->stream) != hipSuccess && rcs[(size_t)k] == DCF_OK) rcs[(size_t)k] = DCF_EDEVICE;
             });
     }
