@@ -37,10 +37,16 @@
  *           LR(N) adjacent to a pixel q of a child region is united with S_C, C being the
  *           ancestor of cl(q) whose coarse parent is N (binary lifting on the coarse tree).
  *           When S_C is absorbed at fine level d, the fine node it joins is the parent of C's
- *           lowest fine node (the final root of LR(C)).
+ *           lowest fine node (the final root of LR(C)).  The child is found by depth: it is cl(q)'s
+ *           ancestor at depth(N) + 1.  Since a region's sets hold only its own pixels and its
+ *           children's super-vertices, the regions refine independently: a large image's pass 2
+ *           runs as several level loops (regions dealt by node id), each on a stream of its own.
  * A further pass repeats pass 2 one digit lower, with the tree of the previous pass (levels =
  * key prefixes) as the coarse tree.  Pixels with equal keys are merged by the union-find itself
  * (no chains), so flat zones cost nothing extra.
+ *
+ * Batches of small images can instead be built whole in LDS, one workgroup per image
+ * (k_small_build, near the end of the kernels).
  */
 #include <algorithm>
 #include <vector>
