@@ -711,7 +711,7 @@ struct Joiner {
  * Each worker has its own stream and host thread and takes the next part not yet taken (parts come largest
  * first); the workers first wait for the work already queued on ctx's stream, and the call returns once
  * every worker stream is drained. */
-static int run_on_workers(dcf_ctx *ctx, int njobs, const std::function<int(dcf_ctx *, int)> &job)
+int dcf_run_on_workers(dcf_ctx *ctx, int njobs, const std::function<int(dcf_ctx *, int)> &job)
 {
     const int K = std::min(njobs, ctx->max_workers);
     while ((int)ctx->workers.size() < K) {
@@ -900,7 +900,7 @@ try {
         return rc;
     };
     const int njobs = (int)(groups.size() + large.size());
-    if (njobs >= 2 && ctx->max_workers >= 2) return run_on_workers(ctx, njobs, part);
+    if (njobs >= 2 && ctx->max_workers >= 2) return dcf_run_on_workers(ctx, njobs, part);
     for (int j = 0; j < njobs; j++) {
         const int rc = part(ctx, j);
         if (rc != DCF_OK) return rc;

@@ -24,6 +24,7 @@
 #include <map>
 #include <set>
 #include <string>
+#include <functional>
 #include <vector>
 
 #include "../../include/dcf.h"
@@ -238,6 +239,9 @@ int dcf_alloc_attr(dcf_ctx *ctx, DevAttr &a, int grp, int64_t K);
 /* tiled build: t holds the whole image (img, dims, conn, tree type, gmax of a min-tree, parent and p2n
  * arrays allocated); grid = tiles per axis.  Fills parent, the node arrays and group grp's attributes. */
 int dcf_tiled_build(dcf_tree *t, const int grid[3], int grp, int schedule);
+/* independent jobs on the context's worker contexts (own stream and host thread each), after the work
+ * queued on ctx's stream; returns once every worker stream is drained (dcf_api.cpp) */
+int dcf_run_on_workers(dcf_ctx *ctx, int njobs, const std::function<int(dcf_ctx *, int)> &job);
 
 /* ---- launchers (dcf_flood.hip) ---------------------------------------------------------------- */
 int dcf_flood_build(dcf_tree *t, int grp);      /* canonical parent + pixel-indexed attributes */

@@ -814,6 +814,9 @@ struct TileTree {
     uint32_t *n2p = nullptr, *bid = nullptr, *mark = nullptr;
 };
 
+/* a tile's pixels, parent array and node numbering come from the caller's context; what its build
+ * allocates (node arrays, attributes) from the context that built it (t.ctx: a worker when the tiles are
+ * built concurrently), and each block goes back to the pool it came from */
 struct TileTrees {
     dcf_ctx *ctx;
     std::vector<std::unique_ptr<TileTree>> v;
@@ -821,15 +824,16 @@ struct TileTrees {
     {
         for (auto &tt : v) {
             dcf_tree &t = tt->t;
+            dcf_ctx *b = t.ctx ? t.ctx : ctx;
             dcf_release(ctx, t.img);
             dcf_release(ctx, t.gmax);
             dcf_release(ctx, t.parent);
             dcf_release(ctx, t.p2n);
-            dcf_release(ctx, t.npar);
-            dcf_release(ctx, t.ngrey);
             dcf_release(ctx, t.counters);
-            dcf_free_attr(ctx, t.attr);
-            dcf_free_attr(ctx, t.attr_pix);
+            dcf_release(b, t.npar);
+            dcf_release(b, t.ngrey);
+            dcf_free_attr(b, t.attr);
+            dcf_free_attr(b, t.attr_pix);
             dcf_release(ctx, tt->n2p);
             dcf_release(ctx, tt->bid);
             dcf_release(ctx, tt->mark);
@@ -893,7 +897,7 @@ int dcf_tiled_build(dcf_tree *T, const int grid[3], int grp, int schedule)
     std::vector<uint32_t> vbase((size_t)ntiles + 1, 0);
     int rc = DCF_OK;
 
-    /* ---- every tile's own tree, its boundary nodes and their numbering ---- */
+    /* ---- every tile's own tree ---- */
     for (int rz = 0; rz < grid[2]; rz++)
         for (int ry = 0; ry < grid[1]; ry++)
             for (int rx = 0; rx < grid[0]; rx++) {
@@ -913,8 +917,7 @@ int dcf_tiled_build(dcf_tree *T, const int grid[3], int grp, int schedule)
                 d.W = e1[0] - e0[0]; d.H = e1[1] - e0[1]; d.D = e1[2] - e0[2];
                 d.n = (int64_t)d.W * d.H * d.D;
                 tt.v.emplace_back(new TileTree());
-                TileTree &tr = *tt.v.back();
-                dcf_tree &t = tr.t;
+                dcf_tree &t = tt.v.back()->t;
                 t.ctx = ctx;
                 t.dims[0] = (uint64_t)d.W; t.dims[1] = (uint64_t)d.H; t.dims[2] = (uint64_t)d.D;
                 t.n = d.n;
@@ -930,11 +933,32 @@ int dcf_tiled_build(dcf_tree *T, const int grid[3], int grp, int schedule)
                 t.parent = (int32_t *)dcf_alloc(ctx, (size_t)d.n * 4);
                 t.p2n = (uint32_t *)dcf_alloc(ctx, (size_t)d.n * 4);
                 if (!t.img || !t.gmax || !t.counters || !t.parent || !t.p2n) return dcf_set_error(ctx, DCF_ENOMEM, "tile %d arrays", i);
-                DCF_LAUNCH(ctx, "k_tm_gather", k_tm_gather, dim3(nblk(d.n, 256)), dim3(256), 0, (const uint8_t *)T->img, (int)esz, G,
-                           d.ox, d.oy, d.oz, d.W, d.H, d.n, (uint8_t *)t.img);
-                /* the whole image's grey extremes: a min-tree tile inverts against the image's maximum */
-                DCF_HIP(ctx, hipMemcpyAsync(t.gmax, T->gmax, 16, hipMemcpyDeviceToDevice, ctx->stream));
-                if ((rc = dcf_build_tree(&t, grp)) != DCF_OK) return rc;
+            }
+    /* one tile's pixels gathered and its tree built on context c (its stream) */
+    auto build_tile = [&](dcf_ctx *c, int i) -> int {
+        const TileDev &d = td[(size_t)i];
+        dcf_tree &t = tt.v[(size_t)i]->t;
+        t.ctx = c;
+        DCF_LAUNCH(c, "k_tm_gather", k_tm_gather, dim3(nblk(d.n, 256)), dim3(256), 0, (const uint8_t *)T->img, (int)esz, G,
+                   d.ox, d.oy, d.oz, d.W, d.H, d.n, (uint8_t *)t.img);
+        /* the whole image's grey extremes: a min-tree tile inverts against the image's maximum */
+        DCF_HIP(c, hipMemcpyAsync(t.gmax, T->gmax, 16, hipMemcpyDeviceToDevice, c->stream));
+        return dcf_build_tree(&t, grp);
+    };
+    /* the tiles are independent: built concurrently on the worker contexts (each on its own stream,
+     * after the work queued so far; the call returns with every tile built) */
+    if (ntiles >= 2 && ctx->max_workers >= 2) {
+        if ((rc = dcf_run_on_workers(ctx, ntiles, build_tile)) != DCF_OK) return rc;
+    } else {
+        for (int i = 0; i < ntiles; i++)
+            if ((rc = build_tile(ctx, i)) != DCF_OK) return rc;
+    }
+
+    /* ---- every tile's boundary nodes and their numbering ---- */
+    for (int i = 0; i < ntiles; i++) {
+                TileDev &d = td[(size_t)i];
+                TileTree &tr = *tt.v[(size_t)i];
+                dcf_tree &t = tr.t;
                 /* the tile's pixels are no longer needed: only its tree */
                 dcf_release(ctx, t.img);
                 t.img = nullptr;
@@ -974,7 +998,7 @@ int dcf_tiled_build(dcf_tree *T, const int grid[3], int grp, int schedule)
                 DCF_LAUNCH(ctx, "k_tm_bid", k_tm_bid, dim3(nblk(t.K, 256)), dim3(256), 0, (const uint32_t *)tr.mark, tr.bid,
                            vbase[(size_t)i], t.K);
                 vbase[(size_t)i + 1] = vbase[(size_t)i] + (uint32_t)vi;
-            }
+    }
     const int64_t V = vbase[(size_t)ntiles];
 
     /* ---- boundary nodes of all tiles ---- */

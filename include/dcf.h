@@ -113,7 +113,8 @@ int dcf_tree_free(dcf_tree *tree);
  * (boundary.c:1553-1707), merge_boundary_trees / merge_branch (boundary.c:427-724), the update and
  * correction of the local trees (boundary.c:1201-1399).  The image is cut into grid[0] x grid[1] x
  * grid[2] tiles (per axis near-equal, the first tiles one pixel longer when the size does not divide,
- * each extended one pixel into every neighbour), every tile's tree is built on its own, and their
+ * each extended one pixel into every neighbour), every tile's tree is built on its own (the tiles
+ * concurrently on DCF_WORKERS worker contexts, like the parts of dcf_batch_filter), and their
  * boundary trees (the nodes of the overlap pixels and all their ancestors) are merged on the GPU.  The
  * result is the tree dcf_tree_build gives for the whole image: the same canonical parent array, node
  * arrays and attributes, for every filter, check and spectrum.  Arguments as dcf_tree_build; grid[a]
