@@ -835,8 +835,8 @@ try {
                 small_pix += pixel_count(images[i].dims);
             }
     /* each dtype's images are dealt into batch trees, largest image first onto the tree with the fewest
-     * pixels: batch_parts trees per dtype, or by default (0) enough for every worker to have a part when the
-     * call has no large image (one tree per dtype when both are present and there are two workers) */
+     * pixels: batch_parts trees per dtype, or by default (0), when the call has no large image, workers / dtypes
+     * trees per dtype (u8 and u16 with two or three workers: one tree each; one dtype: one per worker) */
     int ndt = 0;
     for (int dt : { (int)DCF_U8, (int)DCF_U16 })
         for (int i = 0; i < count; i++)
@@ -845,7 +845,7 @@ try {
                 break;
             }
     const int P = ctx->batch_parts > 0 ? ctx->batch_parts
-                                       : (large.empty() ? std::max(1, (ctx->max_workers + ndt - 1) / std::max(1, ndt)) : 1);
+                                       : (large.empty() ? std::max(1, ctx->max_workers / std::max(1, ndt)) : 1);
     std::vector<std::vector<dcf_batch_image>> groups;
     std::vector<uint64_t> gpix;
     for (int dt : { (int)DCF_U8, (int)DCF_U16 }) {

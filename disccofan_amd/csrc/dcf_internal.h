@@ -116,7 +116,7 @@ struct dcf_ctx {
      * on at most max_workers worker contexts, each with its own stream and host thread (DCF_LARGE_PIXELS,
      * DCF_BATCH_PARTS, DCF_WORKERS) */
     int64_t large_pixels = (int64_t)1 << 24;
-    int max_workers = 2;
+    int max_workers = 3;
     int batch_parts = 0;
     /* first-pass pixel levels: grid cap of k_final_pixels, whose blocks then carry their totals over
      * several chunks of the level (0: one chunk per block) (DCF_FINAL_BLOCKS) */

@@ -204,7 +204,7 @@ const char *dcf_decision_name(int decision);
  * DCF_IMAGE_MIN_TREE): the max-tree and the min-tree of one image can share a call.
  * The call runs in parts: each image of at least DCF_LARGE_PIXELS pixels (default 2^24) on its own when
  * there are two or more of them, and the other images in batch trees, u8 and u16 apart, DCF_BATCH_PARTS
- * per dtype (default 0: enough for every worker).  With two or more parts and DCF_WORKERS > 1 (default 2)
+ * per dtype (default 0: workers / dtypes).  With two or more parts and DCF_WORKERS > 1 (default 3)
  * the parts run concurrently on worker contexts, each with its own HIP stream and host thread; they wait
  * for the work already queued on ctx's stream, and the call returns once they are done.  Results do not
  * depend on the split. */
