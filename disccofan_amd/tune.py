@@ -51,7 +51,7 @@ KNOBS_BATCH = {
     "DCF_FIX_BLOCKS": ["1024", "256"],
     "DCF_FINAL_BLOCKS": ["1024", "0", "512"],
     # concurrent batch trees: worker contexts, and batch trees per dtype (0: one part per worker)
-    "DCF_WORKERS": ["2", "1", "3"],
+    "DCF_WORKERS": ["3", "1", "2"],
     "DCF_BATCH_PARTS": ["0", "1", "2"],
 }
 # the tiled build: its merge schedule (zips started per identification pair) and the tiles' own build
