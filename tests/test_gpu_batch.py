@@ -334,3 +334,23 @@ def test_many_small_images_queue():
         want = OracleTree(imgs[k], 4, "max", 0).filter(0, lams[k], 0).astype(imgs[k].dtype)
         assert np.array_equal(outs[k], want), "image %d %s" % (k, imgs[k].shape)
     c.close()
+
+
+def test_batch_tree_refining_loops():
+    """one worker: the batch trees run in turn on the context itself, and a u16 batch tree of at least 2^22 pixels
+    refines in three independent level loops (coarse regions dealt by node id); every output equals the oracle's"""
+    rng = np.random.default_rng(31)
+    imgs = [synth.vnoise(1400, 1300), synth.disks(1100, 1300, True, 5), synth.vnoise(900, 1500),
+            rng.integers(0, 65536, size=(600, 700)).astype(np.uint16), synth.disks(500, 400, False, 8)]
+    assert sum(a.size for a in imgs if a.dtype == np.uint16) >= 1 << 22
+    lams = [300.0, 50.0, 1000.0, 20.0, 80.0]
+    tts = ["max", "min", "max", "min", "max"]
+    c = dcf.Context(0)
+    c.set_option("DCF_WORKERS", "1")
+    for dec in (0, 3):
+        say("batch trees on the context itself, decision", dec)
+        outs = c.batch_filter(imgs, lams, connectivity=8, decision=dec, tree_types=tts)
+        for k, (img, lam, tt) in enumerate(zip(imgs, lams, tts)):
+            want = OracleTree(img, 8, tt, 0, threads=16).filter(0, lam, dec).astype(img.dtype)
+            assert np.array_equal(outs[k], want), "image %d %s %s decision %d" % (k, img.shape, tt, dec)
+    c.close()
