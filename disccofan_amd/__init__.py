@@ -23,6 +23,26 @@ import weakref
 
 import numpy as np
 
+
+
+def _ask_for_hw_queues(n: int = 16) -> None:
+    """The HIP runtime gives a process 4 hardware queues unless GPU_MAX_HW_QUEUES says otherwise.  A
+    stream takes its queue when it is created (once all are taken, the one with the fewest streams), and
+    busy streams that share a queue run one after the other.  The library runs the parts of a batched
+    call and the tiles of a tiled build on worker streams (up to 12 streams with the null stream), so it
+    asks for 16 queues before the runtime starts (torch starts it on its first GPU call).  A value
+    already set is kept.  The library reads the variable when a context is created to choose how many
+    workers and level loops to run."""
+    if "GPU_MAX_HW_QUEUES" in os.environ:
+        return
+    t = sys.modules.get("torch")
+    if t is not None and t.cuda.is_initialized():
+        return
+    os.environ["GPU_MAX_HW_QUEUES"] = str(n)
+
+
+_ask_for_hw_queues()
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libdcf.so")
 SYNTH_PATH = os.path.join(_HERE, "lib", "libdcf_synth.so")

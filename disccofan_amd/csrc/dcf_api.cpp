@@ -196,6 +196,7 @@ static int set_option(dcf_ctx *c, const char *name, const char *value)
     if (n == "DCF_BATCH_PARTS") return integer(0, 64, c->batch_parts);
     if (n == "DCF_FINAL_BLOCKS") return integer(0, 1 << 20, c->final_blocks);
     if (n == "DCF_FINE_LOOPS") return integer(1, 4, c->fine_loops);
+    if (n == "DCF_WORKER_LOOPS") return integer(0, 4, c->worker_loops);
     if (n == "DCF_SMALL_PIXELS") {
         int v = 0;
         if (integer(0, DCF_SMALL_MAX, v) != DCF_OK) return DCF_EINVAL;
@@ -241,9 +242,20 @@ try {
     return dcf_set_error(ctx, DCF_ENOMEM, "out of host memory (%s)", e.what());
 }
 
-extern "C" int dcf_ctx_create(int device, unsigned flags, dcf_ctx **out)
+/* the streams of a context's further level loops, created up to n */
+int dcf_add_aux_streams(dcf_ctx *c, size_t n)
+{
+    while (c->aux.size() < n) {
+        hipStream_t st = nullptr;
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return DCF_EDEVICE;
+        c->aux.push_back(st);
+    }
+    return DCF_OK;
+}
+
+/* a caller's context, or a worker context (no loop streams of its own yet: dcf_run_on_workers creates them) */
+static int ctx_create(int device, bool worker, dcf_ctx **out)
 try {
-    (void)flags;
     if (!out) return DCF_EINVAL;
     *out = nullptr;
     int ndev = 0;
@@ -257,14 +269,28 @@ try {
         return DCF_EDEVICE;
     }
     c->stream = c->own;
+    /* concurrent worker streams run side by side only on hardware queues of their own */
+    if (const char *q = getenv("GPU_MAX_HW_QUEUES")) {
+        const long v = strtol(q, nullptr, 10);
+        if (v > 0) c->hw_queues = (int)v;
+    }
+    if (c->hw_queues >= 8) c->max_workers = 4;
     /* the build's tunables: the environment gives a new context its defaults (dcf_ctx_set_option
      * changes them afterwards) */
     static const char *const kOptions[] = { "DCF_ZONES", "DCF_TARGETS", "DCF_RECORDS", "DCF_FIX_BLOCKS", "DCF_GRAPH",
-                                            "DCF_ZIP_SEGMENTS", "DCF_WORKERS", "DCF_BATCH_PARTS", "DCF_FINAL_BLOCKS", "DCF_SMALL_PIXELS", "DCF_FINE_LOOPS", "DCF_LARGE_PIXELS", "DCF_ZONE_TILE2D",
+                                            "DCF_ZIP_SEGMENTS", "DCF_WORKERS", "DCF_BATCH_PARTS", "DCF_FINAL_BLOCKS", "DCF_SMALL_PIXELS", "DCF_FINE_LOOPS", "DCF_WORKER_LOOPS", "DCF_LARGE_PIXELS", "DCF_ZONE_TILE2D",
                                             "DCF_ZONE_TILE3D", "DCF_LEVEL_TILE2D", "DCF_LEVEL_GROUP2D", "DCF_LEVEL_TILE3D",
                                             "DCF_LEVEL_GROUP3D" };
     for (const char *name : kOptions)
         if (const char *v = getenv(name)) (void)set_option(c, name, v);
+    /* the streams of the refining pass's further loops, right after the context's own.  The queues of a
+     * process are spread over the command processor's pipes in the order they are created (in turn over
+     * four, as measured), and kernels of queues that share a pipe launch one after the other: streams
+     * created together get pipes of their own whatever the process created before */
+    if (!worker && c->fine_loops > 1 && dcf_add_aux_streams(c, (size_t)std::min(c->fine_loops, 4) - 1) != DCF_OK) {
+        (void)dcf_ctx_destroy(c);
+        return DCF_EDEVICE;
+    }
     /* the allocation-free entry points' scratch address on this device (their calls fail with
      * DCF_EDEVICE when it is unavailable; the context's other calls do not need it) */
     (void)dcf_noalloc_init(c);
@@ -273,6 +299,12 @@ try {
     return DCF_OK;
 } catch (const std::exception &) {
     return DCF_ENOMEM;
+}
+
+extern "C" int dcf_ctx_create(int device, unsigned flags, dcf_ctx **out)
+{
+    (void)flags;
+    return ctx_create(device, false, out);
 }
 
 static void tree_release(dcf_tree *t);
@@ -714,20 +746,42 @@ struct Joiner {
 int dcf_run_on_workers(dcf_ctx *ctx, int njobs, const std::function<int(dcf_ctx *, int)> &job)
 {
     const int K = std::min(njobs, ctx->max_workers);
-    while ((int)ctx->workers.size() < K) {
+    /* the level loops of each of k concurrent workers: up to two for an image of at least 2^25 pixels (the
+     * loops of smaller ones are launch-bound), as many as the hardware queues left over by the null stream
+     * and the caller's own loops give every worker stream a queue of its own (two busy streams on one queue
+     * run one after the other) */
+    const int spare = ctx->hw_queues - 1 - ctx->fine_loops;
+    auto loops_for = [&](int k) { return ctx->worker_loops ? ctx->worker_loops : std::max(1, std::min(2, spare / k)); };
+    auto add_worker = [&]() -> int {
         dcf_ctx *w = nullptr;
-        const int rc = dcf_ctx_create(ctx->device, 0, &w);
-        if (rc != DCF_OK) return dcf_set_error(ctx, rc, "worker context");
+        const int rc = ctx_create(ctx->device, true, &w);
+        if (rc != DCF_OK) return rc;
         ctx->workers.push_back(w);
+        return DCF_OK;
+    };
+    if (ctx->workers.empty()) {
+        /* every worker's stream at once, then their further loop streams in a rotated order: queues created
+         * in turn take the pipes in turn, so W workers' streams get pipes of their own, and so do two
+         * workers' two loops each (worker 0 and 1, then the loop streams of workers W/2 and W/2 + 1) */
+        const int W = std::max(K, ctx->max_workers);
+        for (int k = 0; k < W; k++)
+            if (const int rc = add_worker()) return dcf_set_error(ctx, rc, "worker context");
+        const int L = loops_for(2);
+        for (int j = 1; j < L; j++)
+            for (int i = 0; i < W; i++)
+                if (dcf_add_aux_streams(ctx->workers[(size_t)((i + j * (W / 2)) % W)], (size_t)j) != DCF_OK)
+                    return dcf_set_error(ctx, DCF_EDEVICE, "loop stream");
     }
+    while ((int)ctx->workers.size() < K)
+        if (const int rc = add_worker()) return dcf_set_error(ctx, rc, "worker context");
     for (int k = 0; k < K; k++) {
         dcf_ctx *w = ctx->workers[(size_t)k];
         /* the caller's tunables and profiling */
         w->zones = ctx->zones; w->targets = ctx->targets; w->records = ctx->records; w->fix_blocks = ctx->fix_blocks;
         w->graph = ctx->graph; w->zip_segments = ctx->zip_segments; w->large_pixels = ctx->large_pixels;
         w->final_blocks = ctx->final_blocks; w->small_pixels = ctx->small_pixels;
-        /* concurrent parts already keep the GPU busy: one level loop each (more loops measured slower) */
-        w->fine_loops = K > 1 ? 1 : ctx->fine_loops;
+        w->fine_loops = loops_for(K);
+        w->loops_min_pixels = (int64_t)1 << 25;
         memcpy(w->zt2, ctx->zt2, sizeof w->zt2); w->zt2_set = ctx->zt2_set; w->zt3 = ctx->zt3;
         memcpy(w->lt2, ctx->lt2, sizeof w->lt2); memcpy(w->lg2, ctx->lg2, sizeof w->lg2);
         memcpy(w->lt3, ctx->lt3, sizeof w->lt3); memcpy(w->lg3, ctx->lg3, sizeof w->lg3);

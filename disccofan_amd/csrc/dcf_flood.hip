@@ -201,7 +201,6 @@ __device__ __forceinline__ uint32_t uf_unite(const UFV &uf, uint32_t a, uint32_t
  * hold its own pixels and its children's super-vertices only, so the regions refine independently, and a
  * pixel belongs to loop cnode[p] % K.  Its bin in the level lists is then loop * NLEV + level digit. */
 #define FINE_LOOPS_MAX 4
-#define FINE_LOOPS_MIN_PIXELS ((int64_t)1 << 22)
 __device__ __forceinline__ uint32_t lvl_bin(uint32_t d, int64_t p, const uint32_t *cnode, uint32_t K)
 {
     return K > 1 ? (cnode[p] % K) * NLEV + d : d;
@@ -1791,7 +1790,7 @@ static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAtt
     /* a refining pass of a large image runs as K independent level loops, each on a stream of its own
      * (not under graph capture): level lists, absorbed roots and fix queues per loop, bin = loop * NLEV +
      * level.  Small images keep one loop: their levels are launch-bound already */
-    const uint32_t K = (la.pass == 2 && !ctx->graph && ctx->fine_loops > 1 && n >= FINE_LOOPS_MIN_PIXELS)
+    const uint32_t K = (la.pass == 2 && !ctx->graph && ctx->fine_loops > 1 && n >= ctx->loops_min_pixels)
                            ? (uint32_t)std::min(ctx->fine_loops, FINE_LOOPS_MAX) : 1u;
     const uint32_t NB = K * NLEV;
     uint32_t *hist = (uint32_t *)sc.get((size_t)(2 * NB + NLEV) * 4);
@@ -1954,11 +1953,7 @@ static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAtt
     hipStream_t strm[FINE_LOOPS_MAX];
     lg[0] = la;
     strm[0] = ctx->stream;
-    while (ctx->aux.size() + 1 < K) {
-        hipStream_t st = nullptr;
-        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return dcf_set_error(ctx, DCF_EDEVICE, "loop stream");
-        ctx->aux.push_back(st);
-    }
+    if (dcf_add_aux_streams(ctx, K - 1) != DCF_OK) return dcf_set_error(ctx, DCF_EDEVICE, "loop stream");
     for (uint32_t g = 1; g < K; g++) {
         lg[g] = la;
         lg[g].absorbed = (uint32_t *)sc.get((size_t)(nel + 16) * 4);

@@ -73,6 +73,14 @@ struct dcf_ctx {
      * (DCF_FINE_LOOPS, 1..4); the streams of the further loops, created on first use */
     std::vector<hipStream_t> aux;
     int fine_loops = 3;
+    /* smallest image whose refining pass is split into loops */
+    int64_t loops_min_pixels = (int64_t)1 << 22;
+    /* the loops of each worker context when several run at once (DCF_WORKER_LOOPS, 1..4; 0: up to two, as
+     * many as give every worker stream a hardware queue of its own), for images of at least 2^25 pixels */
+    int worker_loops = 0;
+    /* the HIP runtime's hardware queues per process (GPU_MAX_HW_QUEUES, 4 when unset), read when the
+     * context is created: streams beyond that number share queues and run one after the other */
+    int hw_queues = 4;
     std::string err;
     int profiling = 0;
     std::string prof_only; /* empty: every kernel */
@@ -116,7 +124,7 @@ struct dcf_ctx {
      * on at most max_workers worker contexts, each with its own stream and host thread (DCF_LARGE_PIXELS,
      * DCF_BATCH_PARTS, DCF_WORKERS) */
     int64_t large_pixels = (int64_t)1 << 24;
-    int max_workers = 3;
+    int max_workers = 3; /* 4 when the process has at least 8 hardware queues */
     int batch_parts = 0;
     /* first-pass pixel levels: grid cap of k_final_pixels, whose blocks then carry their totals over
      * several chunks of the level (0: one chunk per block) (DCF_FINAL_BLOCKS) */
@@ -242,6 +250,8 @@ int dcf_tiled_build(dcf_tree *t, const int grid[3], int grp, int schedule);
 /* independent jobs on the context's worker contexts (own stream and host thread each), after the work
  * queued on ctx's stream; returns once every worker stream is drained (dcf_api.cpp) */
 int dcf_run_on_workers(dcf_ctx *ctx, int njobs, const std::function<int(dcf_ctx *, int)> &job);
+/* creates the context's further level-loop streams up to n (dcf_api.cpp) */
+int dcf_add_aux_streams(dcf_ctx *c, size_t n);
 
 /* ---- launchers (dcf_flood.hip) ---------------------------------------------------------------- */
 int dcf_flood_build(dcf_tree *t, int grp);      /* canonical parent + pixel-indexed attributes */

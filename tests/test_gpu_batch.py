@@ -111,13 +111,20 @@ def test_full_batch256_workload(ctx):
     imgs = [synth.disks(H, W, is16, seed) for (W, H, is16, seed, lam) in shapes]
     lams = [lam for (W, H, is16, seed, lam) in shapes]
     say("batch256: %d images, %d pixels" % (len(imgs), sum(a.size for a in imgs)))
-    outs = ctx.batch_filter(imgs, lams, connectivity=4, tree_type="max", attr_idx=0, decision=0)
-    for k, (img, lam, out) in enumerate(zip(imgs, lams, outs)):
+    runs = {"context defaults": ctx.batch_filter(imgs, lams, connectivity=4, tree_type="max", attr_idx=0, decision=0)}
+    # four workers (two batch trees per key width) with two refining loops each
+    c4 = dcf.Context(0)
+    c4.set_option("DCF_WORKERS", "4")
+    c4.set_option("DCF_WORKER_LOOPS", "2")
+    runs["4 workers, 2 loops"] = c4.batch_filter(imgs, lams, connectivity=4, tree_type="max", attr_idx=0, decision=0)
+    c4.close()
+    for k, (img, lam) in enumerate(zip(imgs, lams)):
         want = OracleTree(img, 4, "max", 0, threads=16).filter(0, lam, 0).astype(img.dtype)
-        bad = np.nonzero(out.reshape(-1) != want.reshape(-1))[0]
-        assert len(bad) == 0, "batch256 image %d %s %s lam %g: %d pixels differ, first %s" % (
-            k, img.shape, img.dtype, lam, len(bad), bad[:5])
-    say("batch256: 256 images bit-exact")
+        for how, outs in runs.items():
+            bad = np.nonzero(outs[k].reshape(-1) != want.reshape(-1))[0]
+            assert len(bad) == 0, "batch256 (%s) image %d %s %s lam %g: %d pixels differ, first %s" % (
+                how, k, img.shape, img.dtype, lam, len(bad), bad[:5])
+    say("batch256: 256 images bit-exact in both runs")
 
 
 def test_uneven_workload_batch(ctx):

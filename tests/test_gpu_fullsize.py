@@ -48,9 +48,32 @@ def test_vnoise_8192_u16(ctx, vnoise8192, tree_type):
     o = OracleTree(img, 8, tree_type, 0, threads=THREADS)
     same(t.parent(), o.parent().astype(np.int32), "parent")
     f = t.filter(0, 1000, 3, out_dtype=np.uint16)
-    same(f, o.filter(0, 1000, 3).astype(np.uint16), "subtractive filter")
+    want = o.filter(0, 1000, 3).astype(np.uint16)
+    _VNOISE_FILTERED[tree_type] = want
+    same(f, want, "subtractive filter")
     lv = synth.log_lambdas(64, 1, 2 ** 26)
     same(t.spectrum1d(0, lv), o.spectrum1d(0, lv), "spectrum")
+
+
+_VNOISE_FILTERED = {}
+
+
+@pytest.mark.parametrize("loops", [2, 3])
+def test_vnoise_8192_pair_on_workers(vnoise8192, loops):
+    """the bench's step: the max-tree and the min-tree of the 8192^2 image in one batched call, the two trees on
+    two worker contexts at once, each refining pass split into `loops` level loops on streams of their own"""
+    img = vnoise8192
+    for tt in ("max", "min"):
+        if tt not in _VNOISE_FILTERED:
+            _VNOISE_FILTERED[tt] = OracleTree(img, 8, tt, 0, threads=THREADS).filter(0, 1000, 3).astype(np.uint16)
+    c = dcf.Context(0)
+    c.set_option("DCF_WORKERS", "2")
+    c.set_option("DCF_WORKER_LOOPS", str(loops))
+    say("vnoise 8192^2 pair on two workers,", loops, "loops each")
+    outs = c.batch_filter([img, img], [1000.0, 1000.0], connectivity=8, decision=3, tree_types=["max", "min"])
+    c.close()
+    for tt, out in zip(("max", "min"), outs):
+        same(out, _VNOISE_FILTERED[tt], "%s-tree subtractive filter" % tt)
 
 
 def test_spheres_256_u8(ctx):
