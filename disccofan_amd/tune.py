@@ -50,9 +50,11 @@ KNOBS_BATCH = {
     "DCF_ZONE_TILE2D": ["64x64", "32x32", "16x16", "128x32", "32x128"],
     "DCF_FIX_BLOCKS": ["1024", "256"],
     "DCF_FINAL_BLOCKS": ["1024", "0", "512"],
-    # concurrent batch trees: worker contexts, and batch trees per dtype (0: one part per worker)
-    "DCF_WORKERS": ["3", "1", "2"],
+    # concurrent batch trees: worker contexts (the default with 16 hardware queues first), batch trees per dtype
+    # (0: one part per worker), refining loops per worker (0: as the hardware queues allow)
+    "DCF_WORKERS": ["4", "1", "2", "3"],
     "DCF_BATCH_PARTS": ["0", "1", "2"],
+    "DCF_WORKER_LOOPS": ["0", "1", "2"],
 }
 # the tiled build: its merge schedule (zips started per identification pair) and the tiles' own build
 KNOBS_TILED = {

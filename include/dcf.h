@@ -71,7 +71,7 @@ void *dcf_ctx_stream(dcf_ctx *ctx);
 /* Tunables of the build (no counterpart in the reference; they change the schedule, never a result), by
  * the names of the environment variables that give new contexts their defaults: DCF_ZONES, DCF_TARGETS,
  * DCF_RECORDS, DCF_FIX_BLOCKS, DCF_FINAL_BLOCKS, DCF_GRAPH, DCF_ZIP_SEGMENTS, DCF_WORKERS, DCF_BATCH_PARTS,
- * DCF_LARGE_PIXELS, DCF_SMALL_PIXELS, DCF_FINE_LOOPS, DCF_ZONE_TILE2D ("WxH"), DCF_ZONE_TILE3D ("N"), DCF_LEVEL_TILE2D / DCF_LEVEL_GROUP2D / DCF_LEVEL_TILE3D / DCF_LEVEL_GROUP3D
+ * DCF_LARGE_PIXELS, DCF_SMALL_PIXELS, DCF_FINE_LOOPS, DCF_WORKER_LOOPS, DCF_ZONE_TILE2D ("WxH"), DCF_ZONE_TILE3D ("N"), DCF_LEVEL_TILE2D / DCF_LEVEL_GROUP2D / DCF_LEVEL_TILE3D / DCF_LEVEL_GROUP3D
  * ("WxHxD").  DCF_EINVAL for
  * an unknown name or a value out of range.  The autotuner (disccofan_amd/tune.py) sets them here. */
 int dcf_ctx_set_option(dcf_ctx *ctx, const char *name, const char *value);
@@ -204,10 +204,11 @@ const char *dcf_decision_name(int decision);
  * DCF_IMAGE_MIN_TREE): the max-tree and the min-tree of one image can share a call.
  * The call runs in parts: each image of at least DCF_LARGE_PIXELS pixels (default 2^24) on its own when
  * there are two or more of them, and the other images in batch trees, u8 and u16 apart, DCF_BATCH_PARTS
- * per dtype (default 0: workers / dtypes).  With two or more parts and DCF_WORKERS > 1 (default 3)
- * the parts run concurrently on worker contexts, each with its own HIP stream and host thread; they wait
- * for the work already queued on ctx's stream, and the call returns once they are done.  Results do not
- * depend on the split. */
+ * per dtype (default 0: workers / dtypes).  With two or more parts and DCF_WORKERS > 1 (default 4 when
+ * GPU_MAX_HW_QUEUES was at least 8 at context creation, else 3) the parts run concurrently on worker
+ * contexts, each with its own HIP stream and host thread; they wait for the work already queued on ctx's
+ * stream, and the call returns once they are done.  The streams overlap only on hardware queues of their
+ * own: run the process with GPU_MAX_HW_QUEUES=16.  Results do not depend on the split. */
 typedef struct {
     const void *img;  /* dims[0]*dims[1]*dims[2] values of dtype, x fastest */
     dcf_dtype dtype;  /* DCF_U8 or DCF_U16 */
