@@ -709,9 +709,23 @@ __global__ __launch_bounds__(256) void k_tile_zones(ImgView iv, int shift, TileG
         zc[l] = 0;
     }
     __syncthreads();
+    /* 2D 8-connectivity: a diagonal neighbour of the same digit is reached through the pixel below (or, for the
+     * lower right one, the pixel to the right) when that one has the digit too, by unions made anyway (the
+     * right and lower neighbours are always linked), so only the other diagonal unions are needed */
+    const bool nb8 = nb.n == 8;
     for (int l = threadIdx.x; l < tsz; l += blockDim.x) {
         if (dg[l] == 0xFFFFu) continue;
         const int lx = l % tg.TW, ly = (l / tg.TW) % tg.TH, lz = l / (tg.TW * tg.TH);
+        if (nb8) {
+            const uint16_t d0 = dg[l];
+            const bool hr = lx + 1 < tg.TW, hd = ly + 1 < tg.TH;
+            const bool sr = hr && dg[l + 1] == d0, sd = hd && dg[l + tg.TW] == d0;
+            if (sr) lds_union(par, (uint32_t)l, (uint32_t)(l + 1));
+            if (sd) lds_union(par, (uint32_t)l, (uint32_t)(l + tg.TW));
+            if (hd && lx > 0 && !sd && dg[l + tg.TW - 1] == d0) lds_union(par, (uint32_t)l, (uint32_t)(l + tg.TW - 1));
+            if (hd && hr && !sd && !sr && dg[l + tg.TW + 1] == d0) lds_union(par, (uint32_t)l, (uint32_t)(l + tg.TW + 1));
+            continue;
+        }
         for (int k = 0; k < nb.n; k++) {
             /* the forward half of the neighbourhood: every pair once */
             const int dx = nb.dx[k], dy = nb.dy[k], dz = nb.dz[k];
@@ -787,7 +801,10 @@ __device__ __forceinline__ uint32_t zh_hash(unsigned long long k)
 /* distinct (zone, processed neighbour zone) pairs of every tile, appended to the bucket of the
  * zone's level: a neighbour q of p (zone rp, digit d) is processed when its digit is higher, or
  * equal in another zone (each such pair once, from the zone of smaller representative).  A level's
- * bucket holds at most NB edges per pixel of the level, which bounds every append. */
+ * bucket holds at most NB edges per pixel of the level, which bounds every append.  NMAX: the
+ * neighbourhood's size rounded up to 4, 6, 8 or 26; a pixel loads every neighbour's digit and zone
+ * before it looks at any, so the loads are in flight together instead of one round trip each. */
+template <int NMAX>
 __global__ __launch_bounds__(256) void k_zone_edges(ImgView iv, int shift, TileGeo tg, Nbr nb, const uint32_t *__restrict__ zrep,
                                                     const uint64_t *__restrict__ bucket, uint32_t *__restrict__ cursor,
                                                     uint2 *__restrict__ edges)
@@ -807,15 +824,24 @@ __global__ __launch_bounds__(256) void k_zone_edges(ImgView iv, int shift, TileG
         const int64_t p = ta.off + x + (int64_t)y * ta.W + (int64_t)z * WH;
         const uint32_t dp = fl_key(iv, p) >> shift;
         const uint32_t rp = zrep[p];
-        for (int k = 0; k < nb.n; k++) {
-            const int nx = x + nb.dx[k], ny = y + nb.dy[k], nz = z + nb.dz[k];
-            if (nx < 0 || nx >= ta.W || ny < 0 || ny >= ta.H || nz < 0 || nz >= ta.D) continue;
-            const int64_t q = ta.off + nx + (int64_t)ny * ta.W + (int64_t)nz * WH;
-            const uint32_t dq = fl_key(iv, q) >> shift;
-            if (dq < dp) continue;
-            const uint32_t rq = zrep[q];
-            if (dq == dp && rq <= rp) continue;
-            const unsigned long long key = ((unsigned long long)rp << 32) | rq;
+        uint32_t dq[NMAX], rq[NMAX];
+#pragma unroll
+        for (int k = 0; k < NMAX; k++) {
+            /* a neighbour outside the image (or beyond the neighbourhood) reads the pixel itself, whose
+             * pair is skipped below (same digit, same zone) */
+            int64_t q = p;
+            if (k < nb.n) {
+                const int nx = x + nb.dx[k], ny = y + nb.dy[k], nz = z + nb.dz[k];
+                if (nx >= 0 && nx < ta.W && ny >= 0 && ny < ta.H && nz >= 0 && nz < ta.D)
+                    q = ta.off + nx + (int64_t)ny * ta.W + (int64_t)nz * WH;
+            }
+            dq[k] = fl_key(iv, q) >> shift;
+            rq[k] = zrep[q];
+        }
+#pragma unroll
+        for (int k = 0; k < NMAX; k++) {
+            if (dq[k] < dp || (dq[k] == dp && rq[k] <= rp)) continue;
+            const unsigned long long key = ((unsigned long long)rp << 32) | rq[k];
             uint32_t h = zh_hash(key) & (ZH_CAP - 1);
             bool placed = false;
             for (int probe = 0; probe < 64; probe++) {
@@ -834,7 +860,7 @@ __global__ __launch_bounds__(256) void k_zone_edges(ImgView iv, int shift, TileG
             if (!placed) {
                 /* table crowded: append this pair directly */
                 const uint32_t pos = atomicAdd(&cursor[dp], 1u);
-                edges[bucket[dp] + pos] = make_uint2(rp, rq);
+                edges[bucket[dp] + pos] = make_uint2(rp, rq[k]);
             }
         }
     }
@@ -1922,8 +1948,17 @@ static int run_pass(dcf_ctx *ctx, LevelArgs &la, Scratch &sc, DevAttr &A, DevAtt
         }
         DCF_HIP(ctx, hipMemcpyAsync(bucket, bucket_h, sizeof bucket_h, hipMemcpyHostToDevice, ctx->stream));
         DCF_LAUNCH(ctx, "k_fill", k_fill_u32f, dim3(1), dim3(256), 0, ecur, 0u, (int64_t)NLEV);
-        DCF_LAUNCH(ctx, "k_zone_edges", k_zone_edges, dim3((unsigned)ntiles), dim3(256), 0, la.iv, la.shift, tg, la.nb,
-                   (const uint32_t *)zrep, (const uint64_t *)bucket, ecur, edges);
+        const dim3 zg((unsigned)ntiles), zb(256);
+        const uint32_t *zr = zrep;
+        const uint64_t *bk = bucket;
+        if (la.nb.n <= 4)
+            DCF_LAUNCH(ctx, "k_zone_edges", k_zone_edges<4>, zg, zb, 0, la.iv, la.shift, tg, la.nb, zr, bk, ecur, edges);
+        else if (la.nb.n <= 6)
+            DCF_LAUNCH(ctx, "k_zone_edges", k_zone_edges<6>, zg, zb, 0, la.iv, la.shift, tg, la.nb, zr, bk, ecur, edges);
+        else if (la.nb.n <= 8)
+            DCF_LAUNCH(ctx, "k_zone_edges", k_zone_edges<8>, zg, zb, 0, la.iv, la.shift, tg, la.nb, zr, bk, ecur, edges);
+        else
+            DCF_LAUNCH(ctx, "k_zone_edges", k_zone_edges<26>, zg, zb, 0, la.iv, la.shift, tg, la.nb, zr, bk, ecur, edges);
         DCF_HIP(ctx, hipMemcpyAsync(ecnt, ecur, sizeof ecnt, hipMemcpyDeviceToHost, ctx->stream));
         DCF_HIP(ctx, hipStreamSynchronize(ctx->stream));
     }
