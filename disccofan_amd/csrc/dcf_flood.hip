@@ -400,15 +400,26 @@ __device__ __forceinline__ void pixel_targets(const LevelArgs &a, const PixGeo &
     const uint32_t kp = rec ? a.uf.p[(size_t)p * 4 + 1] : fl_key(a.iv, p);
     const uint32_t cp = a.pass == 2 ? kp >> (a.shift + 8) : 0u;
     const uint32_t nid = a.pass == 2 ? (rec ? a.uf.p[(size_t)p * 4 + 2] : a.cnode[p]) : 0u;
+    /* every neighbour's key (and in a refining pass its coarse node) is loaded before any is looked at, so the
+     * loads are in flight together; a neighbour outside the image reads the pixel itself and is skipped */
+    uint32_t qs[NB], kqs[NB], cns[NB];
+    bool inb[NB];
+#pragma unroll
+    for (int k = 0; k < NB; k++) {
+        const int nx = pg.x + a.nb.dx[k], ny = pg.y + a.nb.dy[k], nz = pg.z + a.nb.dz[k];
+        inb[k] = nx >= 0 && nx < pg.W && ny >= 0 && ny < pg.H && nz >= 0 && nz < pg.D;
+        qs[k] = inb[k] ? (uint32_t)((int)p + a.nb.dx[k] + a.nb.dy[k] * pg.W + a.nb.dz[k] * WH) : p;
+        kqs[k] = rec ? a.uf.p[(size_t)qs[k] * 4 + 1] : fl_key(a.iv, qs[k]);
+        cns[k] = a.pass == 2 ? (rec ? a.uf.p[(size_t)qs[k] * 4 + 2] : a.cnode[qs[k]]) : 0u;
+    }
     /* neighbours in one child region mostly share their coarse node: its child of nid is looked up once */
     uint32_t dn = NONE, last_node = NONE, last_sv = NONE;
 #pragma unroll
     for (int k = 0; k < NB; k++) {
         t[k] = NONE;
-        const int nx = pg.x + a.nb.dx[k], ny = pg.y + a.nb.dy[k], nz = pg.z + a.nb.dz[k];
-        if (nx < 0 || nx >= pg.W || ny < 0 || ny >= pg.H || nz < 0 || nz >= pg.D) continue;
-        const uint32_t q = (uint32_t)((int)p + a.nb.dx[k] + a.nb.dy[k] * pg.W + a.nb.dz[k] * WH);
-        const uint32_t kq = rec ? a.uf.p[(size_t)q * 4 + 1] : fl_key(a.iv, q);
+        if (!inb[k]) continue;
+        const uint32_t q = qs[k];
+        const uint32_t kq = kqs[k];
         const uint32_t dq = (kq >> a.shift) & 255u;
         if (a.pass == 1) {
             if (dq > lev || (dq == lev && q > p)) t[k] = q;
@@ -417,7 +428,7 @@ __device__ __forceinline__ void pixel_targets(const LevelArgs &a, const PixGeo &
             if (cq == cp) {
                 if (dq > lev || (dq == lev && q > p)) t[k] = q;
             } else if (cq > cp) {
-                const uint32_t cq_node = rec ? a.uf.p[(size_t)q * 4 + 2] : a.cnode[q];
+                const uint32_t cq_node = cns[k];
                 if (cq_node != last_node) {
                     if (dn == NONE) dn = a.cdepth[nid];
                     last_node = cq_node;
