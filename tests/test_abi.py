@@ -125,3 +125,20 @@ def test_c_caller_links_and_reads_tables(tmp_path):
     r = subprocess.run([exe, "lvec", lv], capture_output=True, text=True, check=True)
     vals = [float(x) for x in r.stdout.split()[2:]]
     assert vals == [float(x) for x in d.lvec_read(lv)]
+
+
+def test_package_asks_for_hardware_queues():
+    """importing the package before any HIP runtime has started asks for 16 hardware queues, so that the worker
+    streams get queues of their own; a value already in the environment is kept"""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = "import os, disccofan_amd; print(os.environ.get('GPU_MAX_HW_QUEUES'))"
+    env = {k: v for k, v in os.environ.items() if k != "GPU_MAX_HW_QUEUES"}
+    r = subprocess.run([sys.executable, "-c", code], cwd=root, env=env, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.strip() == "16"
+    env["GPU_MAX_HW_QUEUES"] = "5"
+    r = subprocess.run([sys.executable, "-c", code], cwd=root, env=env, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.strip() == "5"

@@ -30,6 +30,14 @@ def test_step_counts_are_checked():
         assert "--steps must be at least 1" in r.stderr
 
 
+def test_hw_queue_count_is_bounded():
+    """the HIP runtime's hardware queues are requested before it starts, 0..32 (0 keeps the environment's)"""
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--hw-queues", "40"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "--hw-queues must be between 0 and 32" in r.stderr
+
+
 def _pmc_csv(path, counter, rows):
     """a rocprofv3 --pmc counter_collection.csv with the columns the summary reads"""
     os.makedirs(path, exist_ok=True)
