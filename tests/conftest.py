@@ -8,8 +8,9 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 # the GPU tests run with the hardware queue count bench.py measures with (16; HIP's default is 4), so that the
-# library's worker streams really run side by side; set before any HIP runtime starts
-os.environ["GPU_MAX_HW_QUEUES"] = "16"
+# library's worker streams really run side by side; set before any HIP runtime starts.  DCF_TEST_HW_QUEUES runs
+# them with another count (4: the library's defaults for a process that keeps HIP's)
+os.environ["GPU_MAX_HW_QUEUES"] = os.environ.get("DCF_TEST_HW_QUEUES", "16")
 
 
 def pytest_configure(config):
