@@ -14,9 +14,11 @@ before each timed region):
               lambda, direct rule: one batched dcf_batch_filter call, against one build + filter call
               per image (`workloads.batch256.plain_form`).
 
-Every workload reports its Mvoxels/s, a roofline object for the dominant kernel (k_final_pixels:
-algorithmic bytes per launch / its HIP-event launch time, recorded on the stream the kernels run on in
-a profiled timed region of its own; `value` comes from an unprofiled timed region) and a CPU baseline:
+Every workload reports its Mvoxels/s, roofline objects for the level loop's two largest kernels (`rooflines`:
+the unions, k_union, the build stage, and k_final_pixels, the attribute stage; each is algorithmic bytes per
+launch / its HIP-event launch time, recorded on the stream the kernels run on, in a profiled timed region of
+its own), `roofline` being the one with more time in its launches (`value` comes from an unprofiled timed
+region), and a CPU baseline:
 the CPU oracle (OpenMP, the reference's algorithm restated) on a bounded sample of the same workload,
 1 warm-up run then the median of 5, at the host's thread count and at 1 thread.  The 8192^2 workload also reports the
 tiled build (8 row slabs, boundary trees merged on the GPU) and the spectrum kernel.
@@ -58,9 +60,12 @@ WORKLOADS = {
 #   attribute: read the parent, write the area          4 + 4
 #   filter: read parent, image and area, write output  4 + 2B + 4
 # build + filter = 3B + 20 per voxel; the spectrum reads parent, image and area: B + 8.
-# k_final_pixels is the attribute stage of the level loop: 8 bytes per voxel of each tree, spread over
-# that tree's k_final_pixels launches (one per level of every digit pass).
-STAGE_BYTES = {"k_final_pixels": lambda B: 8.0, "k_spectrum": lambda B: B + 8.0}
+# The level loop's two largest kernels carry two stages: the unions (k_union: k_union_edges over the zone graph,
+# k_union_tgt over pixels) link every voxel into its component, the build stage, B + 4 bytes per voxel; k_final_pixels
+# gives every voxel its parent and accumulates the areas, the attribute stage, 8 bytes per voxel.  Each stage's bytes
+# of a tree are spread over that tree's launches of the kernel (one per level, loop and digit pass).
+STAGE_BYTES = {"k_union": lambda B: B + 4.0, "k_final_pixels": lambda B: 8.0, "k_spectrum": lambda B: B + 8.0}
+ROOFLINE_KERNELS = ("k_union", "k_final_pixels")
 
 
 def step_bytes_per_voxel(B: float) -> float:
@@ -197,8 +202,8 @@ def traffic_summary(args):
     scaled by what the same counter reads for the plain copy of --calibrate-copy MiB; writes
     profiles/traffic_<kernel>_<workload>.json (read back as roofline.traffic) and prints the table"""
     kernel, workload = args.roofline_kernel, args.workload
-    if not workload or not args.copy or not args.calibrate_copy:
-        raise SystemExit("--traffic needs --workload, --copy and --calibrate-copy MIB")
+    if not workload or not kernel or not args.copy or not args.calibrate_copy:
+        raise SystemExit("--traffic needs --workload, --roofline-kernel, --copy and --calibrate-copy MIB")
     copy_kib = float(args.calibrate_copy << 10)
     scale = []
     for run, counter in zip(args.copy, ("FETCH_SIZE", "WRITE_SIZE")):
@@ -344,33 +349,56 @@ def run_workload(name: str, ctx, args, kernel: str, cpu: bool, say):
             plain_step = None
     vox_per_step = n * len(w["trees"])
 
-    # warm-up with the roofline kernel's events on (their pool is filled before the timed regions)
-    ctx.profile_only(None if args.kernels else kernel)
+    # the kernels whose launches are bracketed by HIP events, each in a profiled region of its own (no other
+    # launch pays for events); --kernels brackets every launch in one region
+    kernels = [kernel] if kernel else list(ROOFLINE_KERNELS)
+    regions = [None] if args.kernels else kernels
+    # warm-up with the first region's events on (the event pools are filled before the timed regions)
+    ctx.profile_only(regions[0])
     ctx.set_profiling(True)
     for i in range(args.warmup):
         t1 = time.perf_counter()
         step()
         torch.cuda.synchronize()
         say("%s warmup %d: %.1f ms" % (name, i, (time.perf_counter() - t1) * 1e3))
-    # profiled timed region: the dominant kernel's launches bracketed by HIP events
-    ctx.reset_stats()
-    prof_ms = timed(step, steps) if not args.no_profile else 0.0
     ctx.set_profiling(False)
-    ex = exact_stats(ctx) if not args.no_profile else {}
+    prof_ms, ex, region_ms, ex_all = 0.0, {}, {}, {}
+    if not args.no_profile:
+        for r in regions:
+            ctx.profile_only(r)
+            ctx.set_profiling(True)
+            if r is not regions[0]:
+                step()
+                torch.cuda.synchronize()
+            ctx.reset_stats()
+            t_ms = timed(step, steps)
+            ctx.set_profiling(False)
+            stats = exact_stats(ctx)
+            for k in kernels if r is None else [r]:
+                if k in stats:
+                    ex[k] = stats[k]
+                    region_ms[k] = t_ms
+            if r is regions[0]:
+                prof_ms = t_ms
+                if args.kernels:
+                    ex_all = stats
     # unprofiled timed region: the headline rate
     ms = timed(step, steps)
     value = vox_per_step / (ms / 1e3) / 1e6
     say("%s: %.2f ms/step (%.1f Mvox/s), profiled %.2f ms/step" % (name, ms, value, prof_ms))
-    roof = None
-    if kernel in ex:
-        launches, tot = ex[kernel]
-        # the stage's bytes of one step (every voxel of every tree once) spread over the step's launches
-        alg = vox_per_step * STAGE_BYTES[kernel](B) / (launches / steps) if launches else 0.0
-        roof = roofline(kernel, launches, tot, alg, name, prof_ms, steps, vox_per_step, B)
-    if args.kernels and ex:
-        tot = sum(v[1] for v in ex.values())
-        for k in sorted(ex, key=lambda k: -ex[k][1]):
-            c, m = ex[k]
+    # the stage's bytes of one step (every voxel of every tree once) spread over the step's launches; `roofline` is
+    # the kernel with the most time in its launches, `rooflines` holds every bracketed one
+    roofs = {}
+    for k in kernels:
+        if k in ex:
+            launches, tot = ex[k]
+            alg = vox_per_step * STAGE_BYTES[k](B) / (launches / steps) if launches else 0.0
+            roofs[k] = roofline(k, launches, tot, alg, name, region_ms[k], steps, vox_per_step, B)
+    roof = max(roofs.values(), key=lambda rf: rf["avg_launch_ms"] * rf["launches"]) if roofs else None
+    if args.kernels and not args.no_profile and ex_all:
+        tot = sum(v[1] for v in ex_all.values())
+        for k in sorted(ex_all, key=lambda k: -ex_all[k][1]):
+            c, m = ex_all[k]
             say("%-16s launches %5d  total %9.3f ms  avg %8.4f ms  %5.1f%%" % (k, c, m, m / max(c, 1), 100 * m / tot))
     out = {"value": round(value, 3), "unit": "Mvoxels/s", "ms_per_step": round(ms, 3), "steps": steps,
            "profiled_ms_per_step": round(prof_ms, 3), "dtype": "u8+u16" if batch else ("u16" if B == 2 else "u8"),
@@ -384,7 +412,7 @@ def run_workload(name: str, ctx, args, kernel: str, cpu: bool, say):
                       "bit_exact_vs_oracle": ("tests/test_gpu_batch.py::test_full_batch256_workload" if batch else
                                               "tests/test_gpu_fullsize.py"),
                       "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))},
-           "roofline": roof, "gen_s": round(gen_s, 2)}
+           "roofline": roof, "rooflines": roofs, "gen_s": round(gen_s, 2)}
 
     if batch and not args.no_extras:
         # the same batched call with every pass's level loop captured into one HIP graph
@@ -529,8 +557,10 @@ def main():
                     help="vnoise8192: time the plain form (one call per tree, one after the other) as the step")
     ap.add_argument("--no-extras", action="store_true", help="skip the spectrum and tiled-build figures")
     ap.add_argument("--kernels", action="store_true", help="bracket every kernel and print the per-kernel times")
-    ap.add_argument("--roofline-kernel", default="k_final_pixels",
-                    help="the kernel bracketed by HIP events in the profiled region (the dominant one)")
+    ap.add_argument("--roofline-kernel", default=None, choices=ROOFLINE_KERNELS,
+                    help="bracket only this kernel with HIP events (default: k_union and k_final_pixels, each in a "
+                         "profiled region of its own; the one with more time is `roofline`); with --traffic, the "
+                         "kernel whose counters are summarised")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB 5 times (the reference point for the "
                          "FETCH_SIZE / WRITE_SIZE counters) and print its bandwidth")
@@ -631,6 +661,7 @@ def main():
         "data": "synthetic (seeded generator, disccofan_amd/synth)",
         "config": head["config"],
         "roofline": head["roofline"],
+        "rooflines": head.get("rooflines"),
         "cpu_baseline": head.get("cpu_baseline"),
         "host_buffers": head.get("host_buffers"),
         "plain_form": head.get("plain_form", batch.get("plain_form")),
