@@ -314,7 +314,8 @@ struct LevelArgs {
     const int32_t *lift;   /* nlift x Kc: 2^j-th coarse ancestor (node id), -1 above the root */
     int nlift;             /* 2^nlift exceeds the depth of the coarse tree */
     const uint32_t *cnkey; /* coarse key of every coarse node */
-    const uint32_t *cdepth; /* depth of every coarse node in the coarse tree (a root: 0) */
+    const uint2 *cpd;       /* every coarse node's parent (x, -1 at a root) and depth (y, a root: 0), side by side so a
+                             * lookup's first step fetches one line */
     /* batched form: image table (nullptr for one image) */
     const BatchImg *bimg;
     int nbatch;
@@ -355,8 +356,9 @@ __device__ __forceinline__ PixGeo pix_geo(const LevelArgs &a, uint32_t p)
  * levels in one 2^j step per set bit (binary lifting) */
 __device__ __forceinline__ uint32_t child_toward(const LevelArgs &a, uint32_t e, uint32_t nid, uint32_t dn, uint32_t kc_count)
 {
-    const int32_t up = a.lift[e];
-    const uint32_t de = a.cdepth[e];
+    const uint2 pd = a.cpd[e];
+    const int32_t up = (int32_t)pd.x;
+    const uint32_t de = pd.y;
     if (up == (int32_t)nid) return e;
     e = (uint32_t)up;
     uint32_t rest = de - dn - 2; /* levels still to climb above up */
@@ -370,7 +372,7 @@ __device__ __forceinline__ uint32_t child_toward(const LevelArgs &a, uint32_t e,
 
 /* depth of every coarse node: the sum of the 2^j jumps that stay inside the tree, largest first
  * (2^nlift exceeds the depth) */
-__global__ __launch_bounds__(256) void k_cdepth(const int32_t *__restrict__ lift, int nlift, int64_t kc, uint32_t *__restrict__ cdepth)
+__global__ __launch_bounds__(256) void k_cdepth(const int32_t *__restrict__ lift, int nlift, int64_t kc, uint2 *__restrict__ cpd)
 {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= kc) return;
@@ -382,7 +384,7 @@ __global__ __launch_bounds__(256) void k_cdepth(const int32_t *__restrict__ lift
             e = (uint32_t)u;
         }
     }
-    cdepth[c] = d;
+    cpd[c] = make_uint2((uint32_t)lift[c], d);
 }
 
 #define CH 8 /* neighbours gathered together */
@@ -430,7 +432,7 @@ __device__ __forceinline__ void pixel_targets(const LevelArgs &a, const PixGeo &
             } else if (cq > cp) {
                 const uint32_t cq_node = cns[k];
                 if (cq_node != last_node) {
-                    if (dn == NONE) dn = a.cdepth[nid];
+                    if (dn == NONE) dn = a.cpd[nid].y;
                     last_node = cq_node;
                     last_sv = (uint32_t)a.g.n + child_toward(a, cq_node, nid, dn, kc_count);
                 }
@@ -2205,7 +2207,7 @@ int dcf_flood_build(dcf_tree *t, int grp)
     la.ox = t->off[0];
     la.oy = t->off[1];
     la.oz = t->off[2];
-    la.cnode = nullptr; la.lift = nullptr; la.cnkey = nullptr; la.cdepth = nullptr; la.nlift = 0;
+    la.cnode = nullptr; la.lift = nullptr; la.cnkey = nullptr; la.cpd = nullptr; la.nlift = 0;
     int rc = DCF_OK;
     /* the keys, in passes of 8-bit digits: u8 one pass, u16 two; a float image is built on the dense
      * ranks of its float keys (the same order, the same equalities), 1 to 4 passes */
@@ -2295,9 +2297,9 @@ int dcf_flood_build(dcf_tree *t, int grp)
         DCF_LAUNCH(ctx, "k_coarse_nodes", k_coarse_nodes, dim3(nblk(n, 256)), dim3(256), 0, la.iv, (const int32_t *)par, flag, cidx,
                    cnkey, lift, cpix, A, S, pshift, n);
         for (int j = 1; j < nlift; j++) DCF_LAUNCH(ctx, "k_lift", k_lift, dim3(nblk(kc, 256)), dim3(256), 0, lift, j, kc);
-        uint32_t *cdepth = (uint32_t *)sc.get((size_t)kc * 4);
-        if (!cdepth) return dcf_set_error(ctx, DCF_ENOMEM, "coarse depths");
-        DCF_LAUNCH(ctx, "k_cdepth", k_cdepth, dim3(nblk(kc, 256)), dim3(256), 0, (const int32_t *)lift, nlift, kc, cdepth);
+        uint2 *cpd = (uint2 *)sc.get((size_t)kc * 8);
+        if (!cpd) return dcf_set_error(ctx, DCF_ENOMEM, "coarse depths");
+        DCF_LAUNCH(ctx, "k_cdepth", k_cdepth, dim3(nblk(kc, 256)), dim3(256), 0, (const int32_t *)lift, nlift, kc, cpd);
         DCF_LAUNCH(ctx, "k_coarse_node_of", k_coarse_node_of, dim3(nblk(n, 256)), dim3(256), 0, (const int32_t *)par, flag, cidx,
                    cnode, n);
         DCF_LAUNCH(ctx, "k_fill", k_fill_i32, dim3(nblk(kc, 256)), dim3(256), 0, spar, -1, kc);
@@ -2316,7 +2318,7 @@ int dcf_flood_build(dcf_tree *t, int grp)
         la.cnode = cnode;
         la.lift = lift;
         la.cnkey = cnkey;
-        la.cdepth = cdepth;
+        la.cpd = cpd;
         la.nlift = nlift;
         if (recs)
             DCF_LAUNCH(ctx, "k_uf_init", k_rec_init, dim3(nblk(n + kc, 256)), dim3(256), 0, la.iv, la.shift, (const uint32_t *)cnode, n,
